@@ -1,0 +1,97 @@
+"""ctypes binding of libdrx.so (the C ABI declared in include/drx.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdrx.so")
+
+P = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+F32 = C.c_float
+
+DRX_OK = 0
+SPLITS_PER_ELEMENT = 33
+MAX_ORDER = 4
+
+
+class DrxMesh(C.Structure):
+    _fields_ = [("data", P), ("elem_offset", P), ("elem_order", P), ("n_el", I32), ("order", I32),
+                ("ncomp", I32), ("stride", I32)]
+
+
+class DrxBVH(C.Structure):
+    _fields_ = [("inner_nodes", P), ("leaf_nodes", P), ("aabb_ids", P), ("subrefs", P), ("leaf_pack", P),
+                ("n_leaves", I32), ("bounds", F32 * 6)]
+
+
+class DrxLocateOpts(C.Structure):
+    _fields_ = [("init_mode", I32), ("fp64_polish", I32), ("cond_threshold", F32), ("newton_steps", P),
+                ("candidates", P)]
+
+
+class DrxIntegrateStats(C.Structure):
+    _fields_ = [("located_samples", I64), ("newton_steps", I64), ("candidates", I64), ("polished", I64)]
+
+
+class DrxCamera(C.Structure):
+    _fields_ = [("width", I32), ("height", I32), ("fov_y", F32), ("pos", F32 * 3), ("look_at", F32 * 3),
+                ("up", F32 * 3), ("zoom", F32)]
+
+
+_SIGS = {
+    "drx_init": (I32, [I32]),
+    "drx_last_error": (C.c_char_p, []),
+    "drx_version": (C.c_char_p, []),
+    "drx_release": (I32, []),
+    "drx_mesh_stride": (I32, [I32, I32]),
+    "drx_mesh_pack_layout": (I64, [P, I32, I32, P]),
+    "drx_mesh_pack": (I32, [P, P, I32, P, P, P, I32, I32, P, P]),
+    "drx_bvh_split_elements": (I32, [P, P, P, P, I32, I32, P, P, P, P]),
+    "drx_bvh_build": (I32, [P, P, I32, P, P, P, P, P, P]),
+    "drx_bvh_pack_leaves": (I32, [C.POINTER(DrxBVH), P, P]),
+    "drx_locate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I64, P, C.POINTER(DrxLocateOpts), P]),
+    "drx_field_eval": (I32, [C.POINTER(DrxMesh), P, I64, P, P]),
+    "drx_camera_rays": (I32, [C.POINTER(DrxCamera), P, P]),
+    "drx_remove_missed_rays": (I32, [P, I32, P, P, C.POINTER(I32), P]),
+    "drx_ray_tile_order": (I32, [P, I32, I32, I32, P, P]),
+    "drx_volume_integrate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I32, P, I32, F32,
+                                   F32, F32, P, C.POINTER(DrxLocateOpts), P, C.POINTER(I32),
+                                   C.POINTER(DrxIntegrateStats), P]),
+    "drx_composite": (I32, [P, I32, I32, I32, P, P]),
+    "drx_color_table_sample": (I32, [P, I32, P, I32, I32, F32, I32, F32, P]),
+}
+
+EXPORTED = sorted(_SIGS)
+
+_lib = None
+
+
+class DrxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libdrx.so (built by devil_ray_amd/build.py); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DrxError(f"{LIB_PATH} is missing: run devil_ray_amd/build.py (or __graft_entry__.build()) first")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(handle, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != DRX_OK:
+        msg = lib().drx_last_error().decode(errors="replace")
+        raise DrxError(f"{what}: status {rc}: {msg}")

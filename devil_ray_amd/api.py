@@ -1,0 +1,372 @@
+"""Python mirror of dray's host API for this path, over the libdrx C ABI.
+
+Names and argument meaning follow the reference (src/dray/...): ``LinearBVHBuilder.construct``
+(linear_bvh_builder.hpp:20-21), ``UnstructuredMesh.locate`` (data_model/unstructured_mesh.cpp:123-148),
+``UnstructuredField.eval`` (data_model/unstructured_field.cpp:213-246), ``Camera.create_rays``
+(rendering/camera.cpp:208-243), ``ColorTable`` (color_table.cpp), ``Volume.integrate`` / ``Volume.save``
+(rendering/volume.cpp:274-416).  Device memory comes from torch (ROCm); every computation runs in the
+HIP kernels of libdrx.so and raises ``DrxError`` when the library or the GPU is unavailable.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .synthetic import LOCATION_DTYPE, PARTIAL_DTYPE, RAY_DTYPE, HexMesh
+
+_initialised = set()
+
+
+def _dev() -> torch.device:
+    if not torch.cuda.is_available():
+        raise L.DrxError("no GPU visible: libdrx runs on gfx950 only")
+    d = torch.cuda.current_device()
+    if d not in _initialised:
+        L.check(L.lib().drx_init(d), "drx_init")
+        _initialised.add(d)
+    return torch.device("cuda", d)
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def to_device(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to(_dev())
+
+
+def as_bytes(a: np.ndarray) -> torch.Tensor:
+    """A structured numpy array as a uint8 device tensor."""
+    return to_device(np.ascontiguousarray(a).view(np.uint8).reshape(-1))
+
+
+def from_bytes(t: torch.Tensor, dtype, n: int) -> np.ndarray:
+    return t[: n * np.dtype(dtype).itemsize].cpu().numpy().view(dtype)[:n].copy()
+
+
+# ---------------------------------------------------------------------------------------------
+class BVH:
+    """dray::BVH (bvh.hpp:15-39) in device memory, plus the fused leaf records used for traversal."""
+
+    def __init__(self, inner, leaf_nodes, aabb_ids, bounds, subrefs=None, mcodes=None):
+        self.inner_nodes = inner
+        self.leaf_nodes = leaf_nodes
+        self.aabb_ids = aabb_ids
+        self.bounds = np.asarray(bounds, np.float32)
+        self.subrefs = subrefs
+        self.mcodes = mcodes
+        self.leaf_pack = None
+
+    @property
+    def n_leaves(self) -> int:
+        return int(self.leaf_nodes.numel())
+
+    def view(self) -> L.DrxBVH:
+        v = L.DrxBVH()
+        v.inner_nodes = self.inner_nodes.data_ptr()
+        v.leaf_nodes = self.leaf_nodes.data_ptr()
+        v.aabb_ids = self.aabb_ids.data_ptr()
+        v.subrefs = self.subrefs.data_ptr() if self.subrefs is not None else None
+        v.leaf_pack = self.leaf_pack.data_ptr() if self.leaf_pack is not None else None
+        v.n_leaves = self.n_leaves
+        for q in range(6):
+            v.bounds[q] = float(self.bounds[q])
+        return v
+
+    def pack_leaves(self) -> None:
+        dev = _dev()
+        self.leaf_pack = torch.empty(self.n_leaves * 8, dtype=torch.float32, device=dev)
+        v = self.view()
+        L.check(L.lib().drx_bvh_pack_leaves(C.byref(v), _ptr(self.leaf_pack), _stream()), "drx_bvh_pack_leaves")
+
+
+class LinearBVHBuilder:
+    """LinearBVHBuilder::construct(aabbs, prim_ids) (linear_bvh_builder.cpp:399-518)."""
+
+    def construct(self, aabbs: torch.Tensor, prim_ids: Optional[torch.Tensor] = None, want_mcodes=False) -> BVH:
+        dev = _dev()
+        n = int(aabbs.numel() // 6)
+        if prim_ids is None:
+            prim_ids = torch.arange(n, dtype=torch.int32, device=dev)
+        m = max(n, 2)
+        inner = torch.empty((m - 1) * 16, dtype=torch.float32, device=dev)
+        leaf_nodes = torch.empty(m, dtype=torch.int32, device=dev)
+        aabb_ids = torch.empty(m, dtype=torch.int32, device=dev)
+        mcodes = torch.empty(m, dtype=torch.int32, device=dev) if want_mcodes else None
+        bounds = (C.c_float * 6)()
+        L.check(L.lib().drx_bvh_build(_ptr(aabbs) if n else None, _ptr(prim_ids) if n else None, n, _ptr(inner),
+                                      _ptr(leaf_nodes), _ptr(aabb_ids), C.cast(bounds, C.c_void_p), _ptr(mcodes),
+                                      _stream()), "drx_bvh_build")
+        return BVH(inner, leaf_nodes, aabb_ids, list(bounds), mcodes=mcodes)
+
+
+# ---------------------------------------------------------------------------------------------
+class DeviceGridFunction:
+    """dray GridFunction (ctrl_idx + values) uploaded, plus the element-local packed blocks."""
+
+    def __init__(self, ctrl_idx: np.ndarray, values: np.ndarray, ncomp: int, order: int,
+                 orders: Optional[np.ndarray] = None, offsets: Optional[np.ndarray] = None):
+        dev = _dev()
+        self.ncomp = ncomp
+        self.order = int(order) if orders is None else int(orders.max())
+        self.n_el = int(len(orders)) if orders is not None else int(len(ctrl_idx) // (order + 1) ** 3)
+        self.ctrl_idx = to_device(ctrl_idx.astype(np.int32))
+        self.values = to_device(values.astype(np.float32).reshape(-1))
+        self.orders = to_device(orders.astype(np.int32)) if orders is not None else None
+        self.offsets = to_device(offsets.astype(np.int64)) if offsets is not None else None
+        lib = L.lib()
+        if orders is None:
+            if not (1 <= order <= L.MAX_ORDER):
+                raise L.DrxError(f"order {order} outside 1..{L.MAX_ORDER}")
+            self.stride = lib.drx_mesh_stride(order, ncomp)
+            total = self.stride * self.n_el
+            self.pack_offsets = None
+        else:
+            h_off = np.zeros(self.n_el, np.int64)
+            o32 = np.ascontiguousarray(orders.astype(np.int32))
+            total = lib.drx_mesh_pack_layout(o32.ctypes.data, self.n_el, ncomp, h_off.ctypes.data)
+            self.stride = 0
+            self.pack_offsets = to_device(h_off)
+        self.packed = torch.empty(max(int(total), 1), dtype=torch.float32, device=dev)
+        L.check(lib.drx_mesh_pack(_ptr(self.ctrl_idx), _ptr(self.values), ncomp, _ptr(self.orders), _ptr(self.offsets),
+                                  _ptr(self.pack_offsets), int(order), self.n_el, _ptr(self.packed), _stream()),
+                "drx_mesh_pack")
+
+    def view(self) -> L.DrxMesh:
+        v = L.DrxMesh()
+        v.data = self.packed.data_ptr()
+        v.elem_offset = self.pack_offsets.data_ptr() if self.pack_offsets is not None else None
+        v.elem_order = self.orders.data_ptr() if self.orders is not None else None
+        v.n_el = self.n_el
+        v.order = self.order
+        v.ncomp = self.ncomp
+        v.stride = self.stride
+        return v
+
+
+@dataclass
+class LocateOptions:
+    fp64_polish: bool = False
+    cond_threshold: float = 1e4
+
+    def c(self, steps=None, cands=None) -> L.DrxLocateOpts:
+        o = L.DrxLocateOpts()
+        o.init_mode = 0
+        o.fp64_polish = 1 if self.fp64_polish else 0
+        o.cond_threshold = float(self.cond_threshold)
+        o.newton_steps = steps.data_ptr() if steps is not None else None
+        o.candidates = cands.data_ptr() if cands is not None else None
+        return o
+
+
+class UnstructuredMesh:
+    """UnstructuredMesh<Hex> (data_model/unstructured_mesh.hpp:117-135): geometry + lazily built BVH."""
+
+    def __init__(self, mesh: HexMesh):
+        self.host = mesh
+        self.gf = DeviceGridFunction(mesh.ctrl_idx, mesh.values, 3, mesh.order, mesh.orders, mesh.offsets)
+        self._bvh: Optional[BVH] = None
+
+    @property
+    def n_el(self) -> int:
+        return self.gf.n_el
+
+    def split_elements(self):
+        """construct_bvh's per-element split: (aabbs [33 n_el, 6], prim_ids, subrefs [33 n_el, 6])."""
+        dev = _dev()
+        k = self.n_el * L.SPLITS_PER_ELEMENT
+        aabbs = torch.empty(k * 6, dtype=torch.float32, device=dev)
+        prims = torch.empty(k, dtype=torch.int32, device=dev)
+        subrefs = torch.empty(k * 6, dtype=torch.float32, device=dev)
+        L.check(L.lib().drx_bvh_split_elements(_ptr(self.gf.ctrl_idx), _ptr(self.gf.values), _ptr(self.gf.orders),
+                                               _ptr(self.gf.offsets), self.host.order, self.n_el, _ptr(aabbs),
+                                               _ptr(prims), _ptr(subrefs), _stream()), "drx_bvh_split_elements")
+        return aabbs, prims, subrefs
+
+    def get_bvh(self) -> BVH:
+        """UnstructuredMesh::get_bvh (unstructured_mesh.cpp:23-31): built on first use, then cached."""
+        if self._bvh is None:
+            aabbs, prims, subrefs = self.split_elements()
+            bvh = LinearBVHBuilder().construct(aabbs, prims)
+            bvh.subrefs = subrefs
+            bvh.pack_leaves()
+            self._bvh = bvh
+        return self._bvh
+
+    def bounds(self) -> np.ndarray:
+        return self.get_bvh().bounds
+
+    def locate(self, points: torch.Tensor, opts: Optional[LocateOptions] = None, stats: bool = False):
+        """Mesh::locate: points float32 [n,3] (device) -> Location records (uint8 tensor, 16 B each)."""
+        bvh = self.get_bvh()
+        n = int(points.numel() // 3)
+        dev = _dev()
+        out = torch.empty(max(n, 1) * 16, dtype=torch.uint8, device=dev)
+        steps = torch.zeros(max(n, 1), dtype=torch.int32, device=dev) if stats else None
+        cands = torch.zeros(max(n, 1), dtype=torch.int32, device=dev) if stats else None
+        o = (opts or LocateOptions()).c(steps, cands)
+        mv, bv = self.gf.view(), bvh.view()
+        L.check(L.lib().drx_locate(C.byref(mv), C.byref(bv), _ptr(points), n, _ptr(out), C.byref(o), _stream()),
+                "drx_locate")
+        return (out, steps, cands) if stats else out
+
+
+class UnstructuredField:
+    """UnstructuredField<Hex scalar> (data_model/unstructured_field.cpp)."""
+
+    def __init__(self, mesh: HexMesh, values: Optional[np.ndarray] = None):
+        vals = mesh.field if values is None else values
+        self.host_values = vals
+        self.gf = DeviceGridFunction(mesh.ctrl_idx, vals, 1, mesh.order, mesh.orders, mesh.offsets)
+
+    def range(self):
+        """get_range (unstructured_field.cpp:35-90): min/max over the control values."""
+        v = self.host_values.astype(np.float32)
+        return float(v.min()), float(v.max())
+
+    def eval(self, locations: torch.Tensor, values: Optional[torch.Tensor] = None) -> torch.Tensor:
+        n = int(locations.numel() // 16)
+        if values is None or values.numel() != n:
+            values = torch.zeros(max(n, 1), dtype=torch.float32, device=_dev())
+        fv = self.gf.view()
+        L.check(L.lib().drx_field_eval(C.byref(fv), _ptr(locations), n, _ptr(values), _stream()), "drx_field_eval")
+        return values
+
+
+# ---------------------------------------------------------------------------------------------
+class ColorTable:
+    """dray::ColorTable (color_table.cpp): rgb + alpha control points, smooth flag (default False)."""
+
+    def __init__(self):
+        self.rgb = []      # (pos, r, g, b, a)
+        self.alpha = []    # (pos, alpha)
+        self.smooth = False
+
+    def add_point(self, pos, rgb):
+        c = list(rgb) + ([1.0] if len(rgb) == 3 else [])
+        self.rgb.append([float(pos)] + [float(x) for x in c])
+
+    def add_alpha(self, pos, a):
+        self.alpha.append([float(pos), float(a)])
+
+    def set_smooth(self, s: bool):
+        self.smooth = bool(s)
+
+    def sample(self, num_samples=1024, opacity_factor=-1.0, alpha_scale=1.0) -> np.ndarray:
+        """correct_opacity(opacity_factor) (<=0: none) then sample(num_samples) -> RGBA [n,4] float32."""
+        rgb = np.ascontiguousarray(np.asarray(self.rgb, np.float32).reshape(-1, 5))
+        al = np.ascontiguousarray(np.asarray(self.alpha, np.float32).reshape(-1, 2))
+        out = np.zeros((num_samples, 4), np.float32)
+        L.check(L.lib().drx_color_table_sample(rgb.ctypes.data if rgb.size else None, rgb.shape[0],
+                                               al.ctypes.data if al.size else None, al.shape[0],
+                                               1 if self.smooth else 0, float(opacity_factor), num_samples,
+                                               float(alpha_scale), out.ctypes.data), "drx_color_table_sample")
+        return out
+
+
+class Camera:
+    """dray::Camera (rendering/camera.cpp): perspective rays via gen_perspective."""
+
+    def __init__(self, width=500, height=500, fov=30.0, pos=(0, 0, 0), look_at=(0, 0, -1), up=(0, 1, 0), zoom=1.0):
+        self.width, self.height, self.fov = int(width), int(height), float(fov)
+        self.pos, self.look_at, self.up, self.zoom = tuple(pos), tuple(look_at), tuple(up), float(zoom)
+
+    def c(self) -> L.DrxCamera:
+        c = L.DrxCamera()
+        c.width, c.height, c.fov_y, c.zoom = self.width, self.height, self.fov, self.zoom
+        for q in range(3):
+            c.pos[q], c.look_at[q], c.up[q] = self.pos[q], self.look_at[q], self.up[q]
+        return c
+
+    def create_rays(self) -> torch.Tensor:
+        out = torch.empty(self.width * self.height * RAY_DTYPE.itemsize, dtype=torch.uint8, device=_dev())
+        cc = self.c()
+        L.check(L.lib().drx_camera_rays(C.byref(cc), _ptr(out), _stream()), "drx_camera_rays")
+        return out
+
+
+def remove_missed_rays(rays: torch.Tensor, bounds6) -> tuple:
+    """remove_missed_rays (ray.cpp:102-115) -> (active rays uint8 tensor, count)."""
+    n = int(rays.numel() // RAY_DTYPE.itemsize)
+    out = torch.empty(max(n, 1) * RAY_DTYPE.itemsize, dtype=torch.uint8, device=_dev())
+    b = np.ascontiguousarray(np.asarray(bounds6, np.float32))
+    cnt = C.c_int32(0)
+    L.check(L.lib().drx_remove_missed_rays(_ptr(rays), n, b.ctypes.data, _ptr(out), C.byref(cnt), _stream()),
+            "drx_remove_missed_rays")
+    return out, int(cnt.value)
+
+
+def ray_tile_order(rays: torch.Tensor, n: int, width: int, height: int) -> torch.Tensor:
+    order = torch.empty(max(n, 1), dtype=torch.int32, device=_dev())
+    L.check(L.lib().drx_ray_tile_order(_ptr(rays), n, width, height, _ptr(order), _stream()), "drx_ray_tile_order")
+    return order
+
+
+def sample_distance(bounds6, samples: int) -> float:
+    """|bounds diagonal| / samples in float32 (volume.cpp:89-91)."""
+    b = np.asarray(bounds6, np.float32)
+    d = np.array([b[1] - b[0], b[3] - b[2], b[5] - b[4]], np.float32)
+    s = np.float32(0)
+    for x in d:
+        s = np.float32(s + np.float32(x * x))
+    return float(np.float32(np.sqrt(s)) / np.float32(samples))
+
+
+class Volume:
+    """dray::Volume (rendering/volume.hpp:25-60) for one hex-mesh domain and one scalar field."""
+
+    def __init__(self, mesh: UnstructuredMesh, field: UnstructuredField):
+        self.mesh = mesh
+        self.field_ = field
+        self.m_samples = 100
+        self.use_lighting = True   # dray default; only unlit shading is implemented (raise otherwise)
+        self.table = ColorTable()
+        self.range = None
+        self.opts = LocateOptions()
+        self.last_stats = None
+
+    def samples(self, n: int):
+        self.m_samples = int(n)
+
+    def color_map_range(self, lo: float, hi: float):
+        self.range = (float(lo), float(hi))
+
+    def integrate(self, rays: torch.Tensor, tile_width: int = 0, tile_height: int = 0, active=None):
+        """Volume::integrate (volume.cpp:323-351) -> (compacted partials uint8 tensor, count)."""
+        if self.use_lighting:
+            raise L.DrxError("Volume::integrate: lit shading is not implemented; call use_lighting=False")
+        bounds = self.mesh.bounds()
+        if active is None:
+            active, n_act = remove_missed_rays(rays, bounds)
+        else:
+            active, n_act = active
+        lo, hi = self.range if self.range is not None else self.field_.range()
+        lut = to_device(self.table.sample(1024, opacity_factor=10.0 / self.m_samples).reshape(-1))
+        sd = sample_distance(bounds, self.m_samples)
+        order = ray_tile_order(active, n_act, tile_width, tile_height) if (tile_width and n_act) else None
+        out = torch.empty(max(5 * n_act, 1) * PARTIAL_DTYPE.itemsize, dtype=torch.uint8, device=_dev())
+        cnt = C.c_int32(0)
+        st = L.DrxIntegrateStats()
+        o = self.opts.c()
+        mv, fv, bv = self.mesh.gf.view(), self.field_.gf.view(), self.mesh.get_bvh().view()
+        L.check(L.lib().drx_volume_integrate(C.byref(mv), C.byref(fv), C.byref(bv), _ptr(active), n_act, _ptr(lut),
+                                             1024, lo, hi, sd, _ptr(order), C.byref(o), _ptr(out), C.byref(cnt),
+                                             C.byref(st), _stream()), "drx_volume_integrate")
+        self.last_stats = st
+        return out, int(cnt.value)
+
+
+def composite(partials: torch.Tensor, n: int, width: int, height: int) -> torch.Tensor:
+    """Volume::save's compositing + Framebuffer::composite_background -> RGBA float32 [w*h*4]."""
+    img = torch.empty(width * height * 4, dtype=torch.float32, device=_dev())
+    L.check(L.lib().drx_composite(_ptr(partials), n, width, height, _ptr(img), _stream()), "drx_composite")
+    return img

@@ -1,0 +1,92 @@
+// Shared host/device definitions of libdrx (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/drx.h"
+
+namespace drx
+{
+
+// ---- error state (per host thread) ----------------------------------------------------------
+void set_error (const char *fmt, ...);
+int fail (int status, const char *fmt, ...);
+
+#define DRX_HIP(call)                                                                           \
+  do                                                                                            \
+  {                                                                                             \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return ::drx::fail (DRX_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString (e_),  \
+                          __FILE__, __LINE__);                                                  \
+  } while (0)
+
+#define DRX_LAUNCH_CHECK()                                                                      \
+  do                                                                                            \
+  {                                                                                             \
+    hipError_t e_ = hipGetLastError ();                                                         \
+    if (e_ != hipSuccess)                                                                       \
+      return ::drx::fail (DRX_ERR_HIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString (e_), \
+                          __FILE__, __LINE__);                                                  \
+  } while (0)
+
+// ---- per-device arena of growable scratch slots ------------------------------------------------
+// Slots are independent buffers; a call may hold several at once.  Growing a slot synchronises the
+// device (hipFree).  Contents are not preserved across growth.
+enum ArenaSlot
+{
+  SLOT_SORT_KEYS_A = 0,
+  SLOT_SORT_KEYS_B,
+  SLOT_SORT_VALS_A,
+  SLOT_SORT_VALS_B,
+  SLOT_SORT_HIST,
+  SLOT_TREE,
+  SLOT_TREE2,
+  SLOT_SCAN,
+  SLOT_PARTIALS,
+  SLOT_FLAGS,
+  SLOT_STATS,
+  SLOT_COUNT
+};
+void *arena (int slot, size_t bytes);   // nullptr + error set on failure
+int arena_release ();
+
+inline hipStream_t as_stream (void *s) { return reinterpret_cast<hipStream_t> (s); }
+
+// ---- POD records shared with the reference layout ---------------------------------------------
+struct Ray
+{
+  float dir[3];
+  float near_;
+  float orig[3];
+  float far_;
+  int32_t pixel_id;
+};
+static_assert (sizeof (Ray) == 36, "Ray layout");
+
+struct Partial
+{
+  int32_t pixel_id;
+  float depth;
+  float rgba[4];
+};
+static_assert (sizeof (Partial) == 24, "VolumePartial layout");
+
+struct Location
+{
+  int32_t cell;
+  float ref[3];
+};
+static_assert (sizeof (Location) == 16, "Location layout");
+
+inline int npe_of (int p) { return (p + 1) * (p + 1) * (p + 1); }
+inline int npe_pad_of (int p) { return (npe_of (p) + 3) & ~3; }
+
+// Exclusive scan of int32 flags (device), n elements; writes scan[n] and *total (host).
+int exclusive_scan_i32 (const int32_t *in, int32_t *out, int32_t n, int32_t *total_host, hipStream_t st);
+
+} // namespace drx

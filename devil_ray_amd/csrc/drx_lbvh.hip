@@ -1,0 +1,769 @@
+// LBVH build for gfx950: per-element de Casteljau split into 33 sub-boxes, Morton codes, stable LSD
+// radix sort, Karras tree, bottom-up refit, dray-layout emit.
+//
+// This file is compiled with -ffp-contract=off: every float operation here rounds exactly like dray's
+// float32 CPU path, so the boxes, codes and the emitted tree are bit-identical to the reference's.
+#include "drx_common.hpp"
+#include "drx_wave.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace drx
+{
+
+// =============================================================================================
+// Split: one 64-lane wavefront per element, the 33 control nets of the element in LDS.
+// Semantics of construct_bvh (data_model/mesh_utils.cpp:557-670): 32 times pick the box of largest
+// volume (surface area when the volume is 0; ties -> lowest index), split its SubRef at 0.5 along the
+// longest reference axis (ties -> lowest axis, :482-506), de Casteljau-split the net (mother keeps the
+// lower half, the copy at index `count` keeps the upper half; elem_ops.hpp:1346-1406), re-hull both.
+// Then scale every box by 1.000001f about its centre (range.hpp:161-173).
+// =============================================================================================
+constexpr int kSplitBoxes = 33;
+
+__device__ __forceinline__ float box_measure (const float *b)   // b: xmin,xmax,ymin,ymax,zmin,zmax
+{
+  const float l0 = b[1] - b[0], l1 = b[3] - b[2], l2 = b[5] - b[4];
+  float v = 1.f;
+  v *= l0;
+  v *= l1;
+  v *= l2;
+  if (v == 0.f)
+  {
+    float a = 2.f * l0 * l1;
+    a += 2.f * l1 * l2;
+    a += 2.f * l2 * l0;
+    v = a;
+  }
+  return v;
+}
+
+template <int PMAX>
+__global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl_idx, const float *__restrict__ values,
+                                                 const int32_t *__restrict__ orders, const int64_t *__restrict__ offsets,
+                                                 int order, int n_el, float *__restrict__ out_aabbs,
+                                                 int32_t *__restrict__ out_prims, float *__restrict__ out_subrefs)
+{
+  constexpr int NPE = (PMAX + 1) * (PMAX + 1) * (PMAX + 1);
+  constexpr int DPL = (NPE + 63) / 64;   // dofs per lane
+  __shared__ float nets[kSplitBoxes * 3 * NPE];
+  __shared__ float boxes[kSplitBoxes * 6];
+  __shared__ float refs[kSplitBoxes * 6];
+  __shared__ float measure[kSplitBoxes];
+
+  const int e = blockIdx.x;
+  if (e >= n_el) return;
+  const int lane = threadIdx.x;
+  const int p = orders ? orders[e] : order;
+  const int npe = (p + 1) * (p + 1) * (p + 1);
+  const int64_t off = offsets ? offsets[e] : (int64_t)e * npe;
+
+  // net 0 <- element control points; box 0 = hull
+  float mn[3] = { INFINITY, INFINITY, INFINITY }, mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+#pragma unroll
+  for (int k = 0; k < DPL; ++k)
+  {
+    const int d = lane + 64 * k;
+    if (d < npe)
+    {
+      const int64_t g = (int64_t)ctrl_idx[off + d] * 3;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+      {
+        const float v = values[g + c];
+        nets[c * NPE + d] = v;
+        mn[c] = fminf (mn[c], v);
+        mx[c] = fmaxf (mx[c], v);
+      }
+    }
+  }
+  float b0[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    b0[2 * c] = wave_reduce_min (mn[c]);
+    b0[2 * c + 1] = wave_reduce_max (mx[c]);
+  }
+  if (lane == 0)
+  {
+    for (int q = 0; q < 6; ++q) boxes[q] = b0[q];
+    measure[0] = box_measure (b0);
+    for (int d = 0; d < 3; ++d)
+    {
+      refs[d] = 0.f;
+      refs[3 + d] = 1.f;
+    }
+  }
+  __syncthreads ();
+
+  for (int count = 1; count <= kSplitBoxes - 1; ++count)
+  {
+    // pick the box (uniform across the wave)
+    int max_id = 0;
+    float max_m = measure[0];
+    for (int b = 1; b < count; ++b)
+    {
+      const float m = measure[b];
+      if (m > max_m)
+      {
+        max_id = b;
+        max_m = m;
+      }
+    }
+    const float *sr = refs + 6 * max_id;
+    int axis = 0;
+    float maxlen = sr[3] - sr[0];
+    for (int d = 1; d < 3; ++d)
+    {
+      const float len = sr[3 + d] - sr[d];
+      if (len > maxlen)
+      {
+        axis = d;
+        maxlen = len;
+      }
+    }
+    const float split_pt = sr[axis] * (1.0f - 0.5f) + sr[3 + axis] * 0.5f;
+    __syncthreads ();
+    if (lane < 6)
+    {
+      const float v = refs[6 * max_id + lane];
+      refs[6 * count + lane] = (lane == axis) ? split_pt : v;        // complement: lo[axis] = split
+      if (lane == 3 + axis) refs[6 * max_id + lane] = split_pt;      // in place: hi[axis] = split
+    }
+    // copy mother -> daughter
+    float *mom = nets + max_id * 3 * NPE;
+    float *dau = nets + count * 3 * NPE;
+#pragma unroll
+    for (int k = 0; k < DPL; ++k)
+    {
+      const int d = lane + 64 * k;
+      if (d < npe)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dau[c * NPE + d] = mom[c * NPE + d];
+    }
+    __syncthreads ();
+    // de Casteljau rounds: mother keeps [lo, mid] (new[ii] = .5 old[ii-1] + .5 old[ii], ii >= r),
+    // daughter keeps [mid, hi] (new[ii] = .5 old[ii] + .5 old[ii+1], ii <= p - r)
+    const int stride = (axis == 0) ? 1 : ((axis == 1) ? (p + 1) : (p + 1) * (p + 1));
+    for (int r = 1; r <= p; ++r)
+    {
+      float nm[DPL][3], nd[DPL][3];
+#pragma unroll
+      for (int k = 0; k < DPL; ++k)
+      {
+        const int d = lane + 64 * k;
+        if (d < npe)
+        {
+          const int ii = (d / stride) % (p + 1);
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+          {
+            const float sm = mom[c * NPE + d];
+            nm[k][c] = (ii >= r) ? (mom[c * NPE + d - stride] * (1.0f - 0.5f) + sm * 0.5f) : sm;
+            const float sd = dau[c * NPE + d];
+            nd[k][c] = (ii <= p - r) ? (sd * (1.0f - 0.5f) + dau[c * NPE + d + stride] * 0.5f) : sd;
+          }
+        }
+      }
+      __syncthreads ();
+#pragma unroll
+      for (int k = 0; k < DPL; ++k)
+      {
+        const int d = lane + 64 * k;
+        if (d < npe)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+          {
+            mom[c * NPE + d] = nm[k][c];
+            dau[c * NPE + d] = nd[k][c];
+          }
+      }
+      __syncthreads ();
+    }
+    // hulls
+    float amn[3] = { INFINITY, INFINITY, INFINITY }, amx[3] = { -INFINITY, -INFINITY, -INFINITY };
+    float bmn[3] = { INFINITY, INFINITY, INFINITY }, bmx[3] = { -INFINITY, -INFINITY, -INFINITY };
+#pragma unroll
+    for (int k = 0; k < DPL; ++k)
+    {
+      const int d = lane + 64 * k;
+      if (d < npe)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+          const float a = mom[c * NPE + d], b = dau[c * NPE + d];
+          amn[c] = fminf (amn[c], a);
+          amx[c] = fmaxf (amx[c], a);
+          bmn[c] = fminf (bmn[c], b);
+          bmx[c] = fmaxf (bmx[c], b);
+        }
+    }
+    float ba[6], bb[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+      ba[2 * c] = wave_reduce_min (amn[c]);
+      ba[2 * c + 1] = wave_reduce_max (amx[c]);
+      bb[2 * c] = wave_reduce_min (bmn[c]);
+      bb[2 * c + 1] = wave_reduce_max (bmx[c]);
+    }
+    if (lane == 0)
+    {
+      for (int q = 0; q < 6; ++q)
+      {
+        boxes[6 * max_id + q] = ba[q];
+        boxes[6 * count + q] = bb[q];
+      }
+      measure[max_id] = box_measure (ba);
+      measure[count] = box_measure (bb);
+    }
+    __syncthreads ();
+  }
+
+  if (lane < kSplitBoxes)
+  {
+    float b[6];
+    for (int q = 0; q < 6; ++q) b[q] = boxes[6 * lane + q];
+    const float s = (float)1.000001;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      float lo = b[2 * d], hi = b[2 * d + 1];
+      if (!(lo > hi))
+      {
+        const float c = 0.5f * (lo + hi);
+        const float delta = s * 0.5f * (hi - lo);
+        const float v0 = c - delta, v1 = c + delta;
+        lo = fminf (lo, v0);
+        hi = fmaxf (hi, v0);
+        lo = fminf (lo, v1);
+        hi = fmaxf (hi, v1);
+      }
+      b[2 * d] = lo;
+      b[2 * d + 1] = hi;
+    }
+    const int64_t k = (int64_t)e * kSplitBoxes + lane;
+    for (int q = 0; q < 6; ++q)
+    {
+      out_aabbs[6 * k + q] = b[q];
+      out_subrefs[6 * k + q] = refs[6 * lane + q];
+    }
+    out_prims[k] = e;
+  }
+}
+
+// =============================================================================================
+// Bounds reduction (linear_bvh_builder.cpp:19-58)
+// =============================================================================================
+constexpr int kRedBlock = 256;
+
+__global__ void __launch_bounds__ (kRedBlock) k_bounds_partial (const float *__restrict__ aabbs, int64_t n,
+                                                                float *__restrict__ partial)
+{
+  float mn[3] = { INFINITY, INFINITY, INFINITY }, mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+  {
+    const float *a = aabbs + 6 * i;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      mn[d] = fminf (mn[d], a[2 * d]);
+      mx[d] = fmaxf (mx[d], a[2 * d + 1]);
+    }
+  }
+  __shared__ float s[kRedBlock / 64][6];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    mn[d] = wave_reduce_min (mn[d]);
+    mx[d] = wave_reduce_max (mx[d]);
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 3; ++d)
+    {
+      s[threadIdx.x >> 6][2 * d] = mn[d];
+      s[threadIdx.x >> 6][2 * d + 1] = mx[d];
+    }
+  __syncthreads ();
+  if (threadIdx.x < 6)
+  {
+    float v = s[0][threadIdx.x];
+    for (int w = 1; w < kRedBlock / 64; ++w)
+      v = (threadIdx.x & 1) ? fmaxf (v, s[w][threadIdx.x]) : fminf (v, s[w][threadIdx.x]);
+    partial[6 * blockIdx.x + threadIdx.x] = v;
+  }
+}
+
+// =============================================================================================
+// Morton codes (linear_bvh_builder.cpp:60-90, morton_codes.hpp:16-51)
+// =============================================================================================
+__device__ __forceinline__ uint32_t expand_bits32 (uint32_t x)
+{
+  x = (x | (x << 16)) & 0x030000FF;
+  x = (x | (x << 8)) & 0x0300F00F;
+  x = (x | (x << 4)) & 0x030C30C3;
+  x = (x | (x << 2)) & 0x09249249;
+  return x;
+}
+
+__global__ void k_morton (const float *__restrict__ aabbs, int n, float minx, float miny, float minz, float ix,
+                          float iy, float iz, uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float *a = aabbs + 6 * (int64_t)i;
+  const float mins[3] = { minx, miny, minz }, inv[3] = { ix, iy, iz };
+  uint32_t q[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    const float lo = a[2 * d], hi = a[2 * d + 1];
+    const float c = (lo > hi) ? __builtin_nanf ("") : 0.5f * (lo + hi);
+    float x = (c - mins[d]) * inv[d];
+    x = fminf (fmaxf (x * 1024.0f, 0.0f), 1023.0f);
+    q[d] = (uint32_t)x;
+  }
+  keys[i] = (expand_bits32 (q[2]) << 2) | (expand_bits32 (q[1]) << 1) | expand_bits32 (q[0]);
+  vals[i] = i;
+}
+
+// =============================================================================================
+// Stable LSD radix sort of (uint32 key, int32 value), 8-bit digits.
+// Pass = per-tile histograms -> exclusive scan over (digit, tile) -> stable per-tile scatter.
+// =============================================================================================
+constexpr int kSortBlock = 256;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortBlock * kSortRounds;   // keys per tile
+
+__global__ void __launch_bounds__ (kSortBlock) k_radix_hist (const uint32_t *__restrict__ keys, int n, int shift,
+                                                              int n_tiles, int32_t *__restrict__ hist)
+{
+  __shared__ int32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads ();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortRounds; ++r)
+  {
+    const int64_t i = base + r * kSortBlock + threadIdx.x;
+    if (i < n) atomicAdd (&h[(keys[i] >> shift) & 0xFF], 1);
+  }
+  __syncthreads ();
+  hist[threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__ (kSortBlock) k_radix_scatter (const uint32_t *__restrict__ keys_in,
+                                                                 const int32_t *__restrict__ vals_in, int n, int shift,
+                                                                 int n_tiles, const int32_t *__restrict__ offs,
+                                                                 uint32_t *__restrict__ keys_out,
+                                                                 int32_t *__restrict__ vals_out)
+{
+  constexpr int W = kSortBlock / 64;
+  __shared__ int32_t running[256];
+  __shared__ int32_t wcnt[W][256];
+  running[threadIdx.x] = offs[threadIdx.x * n_tiles + blockIdx.x];
+  const int w = threadIdx.x >> 6;
+  const uint64_t lt = lanemask_lt ();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortRounds; ++r)
+  {
+    const int64_t i = base + r * kSortBlock + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t key = 0;
+    int32_t val = 0;
+    if (valid)
+    {
+      key = keys_in[i];
+      val = vals_in[i];
+    }
+    const uint32_t digit = (key >> shift) & 0xFF;
+    // lanes of this wave with the same digit (invalid lanes form their own group)
+    uint64_t peers = __ballot (valid);
+    if (!valid) peers = ~peers;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+    {
+      const uint64_t m = __ballot ((digit >> b) & 1u);
+      peers &= ((digit >> b) & 1u) ? m : ~m;
+    }
+    const int rank = __popcll (peers & lt);
+    const int cnt = __popcll (peers);
+#pragma unroll
+    for (int q = 0; q < W; ++q) wcnt[q][threadIdx.x] = 0;
+    __syncthreads ();
+    if (valid && rank == 0) wcnt[w][digit] = cnt;
+    __syncthreads ();
+    if (valid)
+    {
+      int32_t pos = running[digit] + rank;
+      for (int q = 0; q < w; ++q) pos += wcnt[q][digit];
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads ();
+    int32_t add = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) add += wcnt[q][threadIdx.x];
+    running[threadIdx.x] += add;
+    __syncthreads ();
+  }
+}
+
+// =============================================================================================
+// Gathers, Karras tree, refit, emit
+// =============================================================================================
+__global__ void k_gather_boxes (const int32_t *__restrict__ ids, const float *__restrict__ aabbs,
+                                const int32_t *__restrict__ prims, int n, float *__restrict__ s_aabbs,
+                                int32_t *__restrict__ s_prims)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = ids[i];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) s_aabbs[6 * (int64_t)i + q] = aabbs[6 * j + q];
+  s_prims[i] = prims[j];
+}
+
+__device__ __forceinline__ int clz_u32 (uint32_t x) { return x == 0 ? 32 : __clz ((int)x); }
+
+__device__ __forceinline__ int karras_delta (int a, int b, int inner_size, const uint32_t *__restrict__ mc)
+{
+  const bool oor = (b < 0 || b > inner_size);
+  const int bb = oor ? 0 : b;
+  uint32_t ex = mc[a] ^ mc[bb];
+  const bool tie = (ex == 0);
+  ex = tie ? ((uint32_t)a ^ (uint32_t)bb) : ex;
+  int count = clz_u32 (ex);
+  if (tie) count += 32;
+  return oor ? -1 : count;
+}
+
+// Karras 2012 (linear_bvh_builder.cpp:168-251), including the float ceil of the split search.
+__global__ void k_build_tree (const uint32_t *__restrict__ mc, int inner, int32_t *__restrict__ left,
+                              int32_t *__restrict__ right, int32_t *__restrict__ parent)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= inner) return;
+  const int d = 0 > (karras_delta (i, i + 1, inner, mc) - karras_delta (i, i - 1, inner, mc)) ? -1 : 1;
+  const int min_delta = karras_delta (i, i - d, inner, mc);
+  int lmax = 2;
+  while (karras_delta (i, i + lmax * d, inner, mc) > min_delta) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (karras_delta (i, i + (l + t) * d, inner, mc) > min_delta) l += t;
+  const int j = i + l * d;
+  const int delta_node = karras_delta (i, j, inner, mc);
+  int s = 0;
+  float div = 2.f;
+  for (int t = (int)ceilf ((float)l / div);; div *= 2, t = (int)ceilf ((float)l / div))
+  {
+    if (karras_delta (i, i + (s + t) * d, inner, mc) > delta_node) s += t;
+    if (t == 1) break;
+  }
+  const int split = i + s * d + min (d, 0);
+  if (min (i, j) == split)
+  {
+    parent[split + inner] = i;
+    left[i] = split + inner;
+  }
+  else
+  {
+    parent[split] = i;
+    left[i] = split;
+  }
+  if (max (i, j) == split + 1)
+  {
+    parent[split + inner + 1] = i;
+    right[i] = split + inner + 1;
+  }
+  else
+  {
+    parent[split + 1] = i;
+    right[i] = split + 1;
+  }
+  if (i == 0) parent[0] = -1;
+}
+
+__global__ void k_fill_i32 (int32_t *p, int64_t n, int32_t v)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ void box_union (float *o, const float *a, const float *b)
+{
+  // AABB::include of two non-empty ranges per axis (aabb.hpp:46-51): min/max are order independent.
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    float lo = INFINITY, hi = -INFINITY;
+    if (!(a[2 * d] > a[2 * d + 1]))
+    {
+      lo = fminf (lo, a[2 * d]);
+      hi = fmaxf (hi, a[2 * d]);
+      lo = fminf (lo, a[2 * d + 1]);
+      hi = fmaxf (hi, a[2 * d + 1]);
+    }
+    if (!(b[2 * d] > b[2 * d + 1]))
+    {
+      lo = fminf (lo, b[2 * d]);
+      hi = fmaxf (hi, b[2 * d]);
+      lo = fminf (lo, b[2 * d + 1]);
+      hi = fmaxf (hi, b[2 * d + 1]);
+    }
+    o[2 * d] = lo;
+    o[2 * d + 1] = hi;
+  }
+}
+
+// propagate_aabbs (linear_bvh_builder.cpp:253-317): the first thread to reach a node stops, the second
+// merges both children.  The counter update is acq_rel at agent scope so that the second thread sees the
+// first one's child box on any XCD.
+__global__ void k_refit (const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                         const int32_t *__restrict__ parent, const float *__restrict__ leaf_aabbs, float *inner_aabbs,
+                         int32_t *counters, int n_leaves)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_leaves) return;
+  const int inner = n_leaves - 1;
+  int cur = parent[inner + i];
+  while (cur != -1)
+  {
+    const int old = __hip_atomic_fetch_add (&counters[cur], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    const int lc = left[cur], rc = right[cur];
+    float a[6], b[6], o[6];
+    const float *pa = (lc >= inner) ? leaf_aabbs + 6 * (int64_t)(lc - inner) : inner_aabbs + 6 * (int64_t)lc;
+    const float *pb = (rc >= inner) ? leaf_aabbs + 6 * (int64_t)(rc - inner) : inner_aabbs + 6 * (int64_t)rc;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+    {
+      a[q] = pa[q];
+      b[q] = pb[q];
+    }
+    box_union (o, a, b);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) inner_aabbs[6 * (int64_t)cur + q] = o[q];
+    cur = parent[cur];
+  }
+}
+
+// emit (linear_bvh_builder.cpp:319-397)
+__global__ void k_emit (const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+                        const float *__restrict__ leaf_aabbs, const float *__restrict__ inner_aabbs, int inner,
+                        float4 *__restrict__ out)
+{
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= inner) return;
+  int lc = left[node], rc = right[node];
+  const float *la, *ra;
+  if (lc >= inner)
+  {
+    la = leaf_aabbs + 6 * (int64_t)(lc - inner);
+    lc = -(lc - inner + 1);
+  }
+  else
+  {
+    la = inner_aabbs + 6 * (int64_t)lc;
+    lc *= 4;
+  }
+  if (rc >= inner)
+  {
+    ra = leaf_aabbs + 6 * (int64_t)(rc - inner);
+    rc = -(rc - inner + 1);
+  }
+  else
+  {
+    ra = inner_aabbs + 6 * (int64_t)rc;
+    rc *= 4;
+  }
+  float4 *o = out + 4 * (int64_t)node;
+  o[0] = make_float4 (la[0], la[2], la[4], la[1]);
+  o[1] = make_float4 (la[3], la[5], ra[0], ra[2]);
+  o[2] = make_float4 (ra[4], ra[1], ra[3], ra[5]);
+  o[3] = make_float4 (__int_as_float (lc), __int_as_float (rc), 0.f, 0.f);
+}
+
+__global__ void k_pack_leaves (const int32_t *__restrict__ leaf_nodes, const int32_t *__restrict__ aabb_ids,
+                               const float *__restrict__ subrefs, int n, float4 *__restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float *s = subrefs + 6 * (int64_t)aabb_ids[i];
+  out[2 * (int64_t)i] = make_float4 (__int_as_float (leaf_nodes[i]), s[0], s[1], s[2]);
+  out[2 * (int64_t)i + 1] = make_float4 (s[3], s[4], s[5], 0.f);
+}
+
+static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+} // namespace drx
+
+using namespace drx;
+
+extern "C" {
+
+int drx_bvh_split_elements (const int32_t *ctrl_idx, const float *values, const int32_t *orders, const int64_t *offsets,
+                            int32_t order, int32_t n_el, float *out_aabbs, int32_t *out_prims, float *out_subrefs,
+                            void *stream)
+{
+  if (n_el < 0) return fail (DRX_ERR_ARG, "drx_bvh_split_elements: n_el < 0");
+  if (n_el == 0) return DRX_OK;
+  if (!ctrl_idx || !values || !out_aabbs || !out_prims || !out_subrefs)
+    return fail (DRX_ERR_ARG, "drx_bvh_split_elements: null pointer");
+  if ((orders == nullptr) != (offsets == nullptr))
+    return fail (DRX_ERR_ARG, "drx_bvh_split_elements: orders and offsets go together");
+  const int pmax = orders ? DRX_MAX_ORDER : order;
+  if (pmax < 1 || pmax > DRX_MAX_ORDER)
+    return fail (DRX_ERR_ARG, "drx_bvh_split_elements: order %d outside 1..%d", order, DRX_MAX_ORDER);
+  hipStream_t st = as_stream (stream);
+  switch (pmax)
+  {
+  case 1: hipLaunchKernelGGL (k_split<1>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
+  case 2: hipLaunchKernelGGL (k_split<2>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
+  case 3: hipLaunchKernelGGL (k_split<3>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
+  default: hipLaunchKernelGGL (k_split<4>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
+  }
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in, float *out_inner,
+                   int32_t *out_leaf_nodes, int32_t *out_aabb_ids, float *out_bounds, uint32_t *out_mcodes,
+                   void *stream)
+{
+  if (n_in < 0) return fail (DRX_ERR_ARG, "drx_bvh_build: n < 0");
+  if (!out_inner || !out_leaf_nodes || !out_aabb_ids || !out_bounds)
+    return fail (DRX_ERR_ARG, "drx_bvh_build: null output");
+  if (n_in > 0 && (!aabbs_in || !prims_in)) return fail (DRX_ERR_ARG, "drx_bvh_build: null input");
+  if (n_in > (1 << 30)) return fail (DRX_ERR_ARG, "drx_bvh_build: %d boxes exceed the int32 tree index range", n_in);
+  hipStream_t st = as_stream (stream);
+  const float *aabbs = aabbs_in;
+  const int32_t *prims = prims_in;
+  const bool zero_case = (n_in == 0);
+  const int n = n_in < 2 ? 2 : n_in;
+  // dray pads 0 and 1 boxes to two (linear_bvh_builder.cpp:411-456)
+  float hb[12] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+  int32_t hp[2] = { 0, 0 };
+  if (n_in < 2)
+  {
+    if (n_in == 1)
+    {
+      DRX_HIP (hipMemcpyAsync (hb, aabbs_in, 6 * sizeof (float), hipMemcpyDeviceToHost, st));
+      DRX_HIP (hipMemcpyAsync (hp, prims_in, sizeof (int32_t), hipMemcpyDeviceToHost, st));
+      DRX_HIP (hipStreamSynchronize (st));
+      hp[1] = hp[0];
+    }
+    float *pa = (float *)arena (SLOT_TREE2, 12 * sizeof (float) + 2 * sizeof (int32_t));
+    if (!pa) return DRX_ERR_HIP;
+    DRX_HIP (hipMemcpyAsync (pa, hb, 12 * sizeof (float), hipMemcpyHostToDevice, st));
+    DRX_HIP (hipMemcpyAsync (pa + 12, hp, 2 * sizeof (int32_t), hipMemcpyHostToDevice, st));
+    DRX_HIP (hipStreamSynchronize (st));
+    aabbs = pa;
+    prims = (const int32_t *)(pa + 12);
+  }
+  const int inner = n - 1;
+  // scratch: keys/vals ping-pong, histogram, tree arrays
+  const int n_tiles = (n + kSortTile - 1) / kSortTile;
+  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * n);
+  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * n);
+  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * n);
+  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * n);
+  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8));
+  // tree slot: sorted aabbs (6n) | inner aabbs (6 inner) | left | right | parent (2n-1) | counters | partial bounds
+  const int red_blocks = std::min (1024, (int)nblk (n, kRedBlock));
+  const size_t tree_floats = 6 * (size_t)n + 6 * (size_t)inner;
+  const size_t tree_ints = (size_t)inner * 3 + (2 * (size_t)n - 1) + (size_t)n;
+  char *tree = (char *)arena (SLOT_TREE, tree_floats * 4 + tree_ints * 4 + 6 * 4 * (size_t)red_blocks + 64);
+  if (!kA || !kB || !vA || !vB || !hist || !tree) return DRX_ERR_HIP;
+  float *s_aabbs = (float *)tree;
+  float *i_aabbs = s_aabbs + 6 * (size_t)n;
+  int32_t *left = (int32_t *)(i_aabbs + 6 * (size_t)inner);
+  int32_t *right = left + inner;
+  int32_t *counters = right + inner;
+  int32_t *parent = counters + inner;
+  int32_t *s_prims = parent + (2 * (size_t)n - 1);
+  float *partial = (float *)(s_prims + n);
+
+  // bounds
+  float bounds[6] = { INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY };
+  if (!zero_case)
+  {
+    hipLaunchKernelGGL (k_bounds_partial, dim3 (red_blocks), dim3 (kRedBlock), 0, st, aabbs, (int64_t)n, partial);
+    DRX_LAUNCH_CHECK ();
+    std::vector<float> hpart (6 * (size_t)red_blocks);
+    DRX_HIP (hipMemcpyAsync (hpart.data (), partial, sizeof (float) * hpart.size (), hipMemcpyDeviceToHost, st));
+    DRX_HIP (hipStreamSynchronize (st));
+    float mn[3] = { INFINITY, INFINITY, INFINITY }, mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int b = 0; b < red_blocks; ++b)
+      for (int d = 0; d < 3; ++d)
+      {
+        mn[d] = std::min (mn[d], hpart[6 * b + 2 * d]);
+        mx[d] = std::max (mx[d], hpart[6 * b + 2 * d + 1]);
+      }
+    // res.include(mins); res.include(maxs)
+    for (int d = 0; d < 3; ++d)
+    {
+      float lo = INFINITY, hi = -INFINITY;
+      lo = std::fmin (lo, mn[d]);
+      hi = std::fmax (hi, mn[d]);
+      lo = std::fmin (lo, mx[d]);
+      hi = std::fmax (hi, mx[d]);
+      bounds[2 * d] = lo;
+      bounds[2 * d + 1] = hi;
+    }
+  }
+  for (int q = 0; q < 6; ++q) out_bounds[q] = bounds[q];
+  float minc[3], inv[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    minc[d] = bounds[2 * d];
+    const float ext = bounds[2 * d + 1] - bounds[2 * d];
+    inv[d] = (ext == .0f) ? 0.f : 1.f / ext;
+  }
+  hipLaunchKernelGGL (k_morton, dim3 (nblk (n, 256)), dim3 (256), 0, st, aabbs, n, minc[0], minc[1], minc[2], inv[0],
+                      inv[1], inv[2], kA, vA);
+  DRX_LAUNCH_CHECK ();
+  // stable radix sort, 8-bit digits over 30-bit keys
+  uint32_t *ks = kA, *kd = kB;
+  int32_t *vs = vA, *vd = vB;
+  int32_t *offs = hist + 256 * (size_t)n_tiles;
+  for (int shift = 0; shift < 30; shift += 8)
+  {
+    hipLaunchKernelGGL (k_radix_hist, dim3 (n_tiles), dim3 (kSortBlock), 0, st, ks, n, shift, n_tiles, hist);
+    DRX_LAUNCH_CHECK ();
+    int rc = exclusive_scan_i32 (hist, offs, 256 * n_tiles, nullptr, st);
+    if (rc != DRX_OK) return rc;
+    hipLaunchKernelGGL (k_radix_scatter, dim3 (n_tiles), dim3 (kSortBlock), 0, st, ks, vs, n, shift, n_tiles, offs, kd,
+                        vd);
+    DRX_LAUNCH_CHECK ();
+    std::swap (ks, kd);
+    std::swap (vs, vd);
+  }
+  // ks = sorted codes, vs = ids
+  DRX_HIP (hipMemcpyAsync (out_aabb_ids, vs, sizeof (int32_t) * n, hipMemcpyDeviceToDevice, st));
+  if (out_mcodes) DRX_HIP (hipMemcpyAsync (out_mcodes, ks, sizeof (uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL (k_gather_boxes, dim3 (nblk (n, 256)), dim3 (256), 0, st, vs, aabbs, prims, n, s_aabbs, s_prims);
+  DRX_LAUNCH_CHECK ();
+  DRX_HIP (hipMemcpyAsync (out_leaf_nodes, s_prims, sizeof (int32_t) * n, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL (k_build_tree, dim3 (nblk (inner, 256)), dim3 (256), 0, st, ks, inner, left, right, parent);
+  hipLaunchKernelGGL (k_fill_i32, dim3 (nblk (inner, 256)), dim3 (256), 0, st, counters, (int64_t)inner, 0);
+  hipLaunchKernelGGL (k_refit, dim3 (nblk (n, 256)), dim3 (256), 0, st, left, right, parent, s_aabbs, i_aabbs, counters, n);
+  hipLaunchKernelGGL (k_emit, dim3 (nblk (inner, 256)), dim3 (256), 0, st, left, right, s_aabbs, i_aabbs, inner,
+                      (float4 *)out_inner);
+  DRX_LAUNCH_CHECK ();
+  DRX_HIP (hipStreamSynchronize (st));
+  return DRX_OK;
+}
+
+int drx_bvh_pack_leaves (const drx_bvh *bvh, float *out_leaf_pack, void *stream)
+{
+  if (!bvh || !out_leaf_pack || !bvh->leaf_nodes || !bvh->aabb_ids || !bvh->subrefs)
+    return fail (DRX_ERR_ARG, "drx_bvh_pack_leaves: null pointer");
+  hipLaunchKernelGGL (k_pack_leaves, dim3 (nblk (bvh->n_leaves, 256)), dim3 (256), 0, as_stream (stream), bvh->leaf_nodes,
+                      bvh->aabb_ids, bvh->subrefs, bvh->n_leaves, (float4 *)out_leaf_pack);
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,793 @@
+// Point location, field evaluation, camera rays, ray set-up, volume integration, partial compaction and
+// compositing for gfx950.
+#include "drx_common.hpp"
+#include "drx_element.hpp"
+#include "drx_wave.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace drx
+{
+
+// ---- arithmetic that must round like dray's float32 CPU path (no FMA contraction) --------------
+__device__ __forceinline__ float ray_point (float o, float t, float d)
+{
+#pragma clang fp contract(off)
+  return o + t * d;
+}
+
+__device__ __forceinline__ float mul_sub_nc (float a, float b, float c)
+{
+#pragma clang fp contract(off)
+  return a * b - c;
+}
+
+__device__ __forceinline__ void blend_nc (float f[4], const float4 b)   // rendering/colors.hpp:15-23
+{
+#pragma clang fp contract(off)
+  const float a = b.w * (1.f - f[3]);
+  f[0] = f[0] + b.x * a;
+  f[1] = f[1] + b.y * a;
+  f[2] = f[2] + b.z * a;
+  f[3] = a + f[3];
+}
+
+// DeviceColorMap::color (device_color_map.hpp:61-77)
+__device__ __forceinline__ float4 color_lookup (const float4 *__restrict__ lut, int size, float mn, float mx, float inv,
+                                                float scalar)
+{
+#pragma clang fp contract(off)
+  const float s = fminf (mx, fmaxf (mn, scalar));
+  const float s2 = (scalar != scalar) ? mn : s;   // std::max(min, NaN) keeps min
+  const float normalized = (s2 - mn) * inv;
+  int idx = (int)(normalized * (float)(size - 1));
+  idx = min (max (idx, 0), size - 1);
+  return lut[idx];
+}
+
+static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#ifndef DRX_WAVES_PER_EU
+#define DRX_WAVES_PER_EU 4
+#endif
+
+// ---------------------------------------------------------------------------------------------
+// Batched locate (UnstructuredMesh::locate, unstructured_mesh.cpp:123-148): one lane per point.
+// ---------------------------------------------------------------------------------------------
+template <int P, bool POLISH>
+__global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate (LocateCfg cfg, const float *__restrict__ pts, int64_t n,
+                                                  Location *__restrict__ out, int32_t *__restrict__ steps_out,
+                                                  int32_t *__restrict__ cand_out, int32_t *__restrict__ err)
+{
+  __shared__ int32_t stack_lds[kStackLds * 256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TravStack todo;
+  todo.lds = stack_lds;
+  todo.bdim = blockDim.x;
+  todo.tid = threadIdx.x;
+  const float pt[3] = { pts[3 * i], pts[3 * i + 1], pts[3 * i + 2] };
+  float ref[3] = { -1.f, -1.f, -1.f };
+  LocateStats st { 0, 0, 0 };
+  bool overflow = false;
+  const int cell = locate_point<P, POLISH> (cfg, todo, pt, ref, st, overflow);
+  if (overflow) atomicOr (err, 1);
+  Location loc;
+  loc.cell = cell;
+  loc.ref[0] = cell >= 0 ? ref[0] : -1.f;
+  loc.ref[1] = cell >= 0 ? ref[1] : -1.f;
+  loc.ref[2] = cell >= 0 ? ref[2] : -1.f;
+  out[i] = loc;
+  if (steps_out) steps_out[i] = st.newton;
+  if (cand_out) cand_out[i] = st.candidates;
+}
+
+template <int P>
+__global__ void __launch_bounds__ (256) k_field_eval (const float *__restrict__ field, const int64_t *__restrict__ offs,
+                                                      const int32_t *__restrict__ orders, const Location *__restrict__ locs,
+                                                      int64_t n, float *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Location l = locs[i];
+  if (l.cell == -1) return;   // Field::eval leaves misses untouched
+  if constexpr (P == 0)
+  {
+    const float *f = field + offs[l.cell];
+    switch (orders[l.cell])
+    {
+    case 1: out[i] = eval_scalar<1> (f, l.ref); break;
+    case 2: out[i] = eval_scalar<2> (f, l.ref); break;
+    case 3: out[i] = eval_scalar<3> (f, l.ref); break;
+    default: out[i] = eval_scalar<4> (f, l.ref); break;
+    }
+  }
+  else
+    out[i] = eval_scalar<P> (field + (int64_t)l.cell * Ord<P>::FSTRIDE, l.ref);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Camera::gen_perspective per-pixel part (rendering/camera.cpp:457-481)
+// ---------------------------------------------------------------------------------------------
+struct CamSetup
+{
+  float nlook[3], dx[3], dy[3], pos[3];
+  int w, h;
+};
+
+__global__ void k_camera (CamSetup cs, Ray *__restrict__ out)
+{
+#pragma clang fp contract(off)
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cs.w * cs.h) return;
+  Ray r;
+  const int i = idx % cs.w, j = idx / cs.w;
+  r.orig[0] = cs.pos[0];
+  r.orig[1] = cs.pos[1];
+  r.orig[2] = cs.pos[2];
+  r.near_ = 0.f;
+  r.far_ = INFINITY;
+  r.pixel_id = j * cs.w + i;
+  const float a = (2.f * (float)i - (float)cs.w) / 2.0f;
+  const float b = (2.f * (float)j - (float)cs.w) / 2.0f;   // dray uses w for both axes (camera.cpp:469-470)
+  float s = 0.f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    const float t1 = cs.dx[d] * a;
+    const float t2 = cs.dy[d] * b;
+    float v = (cs.nlook[d] + t1) + t2;
+    if (v == 0.f) v += 0.0000001f;
+    r.dir[d] = v;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) s += r.dir[d] * r.dir[d];
+  const float m = sqrtf (s);   // correctly rounded (HIP default); __fsqrt_rn is the native approximation
+#pragma unroll
+  for (int d = 0; d < 3; ++d) r.dir[d] = r.dir[d] / m;
+  out[idx] = r;
+}
+
+static void vnormalize_host (float v[3])
+{
+  float s = 0.f;
+  for (int i = 0; i < 3; ++i) s += v[i] * v[i];
+  const float m = sqrtf (s);
+  for (int i = 0; i < 3; ++i) v[i] /= m;
+}
+
+// Camera host set-up: set_fov (camera.cpp:132-163), gen_perspective (:402-440), create_rays (:219-243).
+static CamSetup camera_setup (const drx_camera &c)
+{
+  const double pi = 3.14159265358979323846264338327950288;
+  const int w = c.width, h = c.height;
+  const float fov_y = c.fov_y;
+  const float fov_y_rad = (fov_y * static_cast<float> (pi)) / 180.0f;
+  const float vertical = std::tan (0.5f * fov_y_rad);
+  const float aspect = float (w) / float (h);
+  const float horizontal = aspect * vertical;
+  const float fov_x_rad = 2.0f * std::atan (horizontal);
+  const float fov_x = 180.0f * (fov_x_rad / static_cast<float> (pi));
+  float up[3] = { c.up[0], c.up[1], c.up[2] };
+  vnormalize_host (up);
+  float look[3];
+  for (int i = 0; i < 3; ++i) look[i] = c.look_at[i] - c.pos[i];
+  const float thx = tanf ((fov_x * float (pi) / 180.f) * .5f);
+  const float thy = tanf ((fov_y * float (pi) / 180.f) * .5f);
+  float ru[3] = { look[1] * up[2] - look[2] * up[1], look[2] * up[0] - look[0] * up[2], look[0] * up[1] - look[1] * up[0] };
+  const float ruf[3] = { ru[0], ru[1], ru[2] };
+  vnormalize_host (ru);
+  float rv[3] = { ruf[1] * look[2] - ruf[2] * look[1], ruf[2] * look[0] - ruf[0] * look[2],
+                  ruf[0] * look[1] - ruf[1] * look[0] };
+  vnormalize_host (rv);
+  CamSetup cs;
+  for (int i = 0; i < 3; ++i)
+  {
+    cs.dx[i] = ru[i] * (2 * thx / (float)w);
+    cs.dy[i] = rv[i] * (2 * thy / (float)h);
+    if (c.zoom > 0)
+    {
+      cs.dx[i] = cs.dx[i] / c.zoom;
+      cs.dy[i] = cs.dy[i] / c.zoom;
+    }
+    cs.nlook[i] = look[i];
+    cs.pos[i] = c.pos[i];
+  }
+  vnormalize_host (cs.nlook);
+  cs.w = w;
+  cs.h = h;
+  return cs;
+}
+
+// ---------------------------------------------------------------------------------------------
+// remove_missed_rays (ray.cpp:102-241): slab test against bounds scaled x1.001, compaction, near/far.
+// ---------------------------------------------------------------------------------------------
+struct Bounds6
+{
+  float b[6];
+};
+
+__device__ __forceinline__ float rcp_safe (float f)
+{
+  return 1.0f / ((fabsf (f) < 1e-8f) ? (signbit (f) ? -1e-8f : 1e-8f) : f);
+}
+
+__device__ __forceinline__ bool ray_slab (const Ray &ray, const Bounds6 &mb, float &near_o, float &far_o)
+{
+  const float idx = rcp_safe (ray.dir[0]), idy = rcp_safe (ray.dir[1]), idz = rcp_safe (ray.dir[2]);
+  float ox, oy, oz;
+  {
+#pragma clang fp contract(off)
+    ox = ray.orig[0] * idx;
+    oy = ray.orig[1] * idy;
+    oz = ray.orig[2] * idz;
+  }
+  const float xmin = mul_sub_nc (mb.b[0], idx, ox), ymin = mul_sub_nc (mb.b[2], idy, oy), zmin = mul_sub_nc (mb.b[4], idz, oz);
+  const float xmax = mul_sub_nc (mb.b[1], idx, ox), ymax = mul_sub_nc (mb.b[3], idy, oy), zmax = mul_sub_nc (mb.b[5], idz, oz);
+  // std::min/std::max semantics: (a < b) ? a : b
+  auto mn = [] (float a, float b) { return (b < a) ? b : a; };
+  auto mx = [] (float a, float b) { return (a < b) ? b : a; };
+  const float mind = mx (mx (mx (mn (ymin, ymax), mn (xmin, xmax)), mn (zmin, zmax)), ray.near_);
+  float maxd = mn (mn (mx (ymin, ymax), mx (xmin, xmax)), mx (zmin, zmax));
+  maxd = mn (maxd, ray.far_);
+  near_o = mind;
+  far_o = maxd;
+  return maxd > mind;
+}
+
+__global__ void k_mark_active (const Ray *__restrict__ rays, int n, Bounds6 mb, int32_t *__restrict__ flags)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float a, b;
+  flags[i] = ray_slab (rays[i], mb, a, b) ? 1 : 0;
+}
+
+__global__ void k_gather_active (const Ray *__restrict__ rays, int n, Bounds6 mb, const int32_t *__restrict__ flags,
+                                 const int32_t *__restrict__ pos, Ray *__restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flags[i]) return;
+  Ray r = rays[i];
+  float nr, fr;
+  ray_slab (r, mb, nr, fr);
+  r.near_ = nr;
+  r.far_ = fr;
+  out[pos[i]] = r;
+}
+
+// 8x8 tile-major rank of a pixel in a w x h image.
+__device__ __forceinline__ int64_t tile_rank (int pid, int w, int h)
+{
+  const int i = pid % w, j = pid / w;
+  const int tx = i >> 3, ty = j >> 3;
+  const int tiles_x = (w + 7) >> 3;
+  return ((int64_t)(ty * tiles_x + tx) << 6) + ((j & 7) << 3) + (i & 7);
+}
+
+__global__ void k_fill_i32q (int32_t *p, int64_t n, int32_t v)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_tile_scatter (const Ray *__restrict__ rays, int n, int w, int h, int32_t *__restrict__ slots)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int pid = rays[k].pixel_id;
+  if (pid >= 0 && pid < w * h) slots[tile_rank (pid, w, h)] = k;
+}
+
+__global__ void k_flag_nonneg (const int32_t *__restrict__ v, int64_t n, int32_t *__restrict__ flags)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = v[i] >= 0 ? 1 : 0;
+}
+
+__global__ void k_compact_i32 (const int32_t *__restrict__ v, int64_t n, const int32_t *__restrict__ pos,
+                               int32_t *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && v[i] >= 0) out[pos[i]] = v[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Volume integration: integrate_partials' per-ray loop (rendering/volume.cpp:134-217).
+// ---------------------------------------------------------------------------------------------
+struct IntegrateArgs
+{
+  LocateCfg cfg;
+  const float *field;
+  const int64_t *field_offsets;
+  const Ray *rays;
+  const int32_t *order;
+  int n_rays;
+  const float4 *lut;
+  int lut_size;
+  float fmin, fmax, inv_range;
+  float sample_dist;
+  Partial *slots;
+  unsigned long long *stats;   // located, newton, candidates, polished
+  int32_t *err;
+};
+
+template <int P>
+__device__ __forceinline__ float field_at (const IntegrateArgs &a, int cell, const float ref[3])
+{
+  if constexpr (P == 0)
+  {
+    const float *f = a.field + a.field_offsets[cell];
+    switch (a.cfg.orders[cell])
+    {
+    case 1: return eval_scalar<1> (f, ref);
+    case 2: return eval_scalar<2> (f, ref);
+    case 3: return eval_scalar<3> (f, ref);
+    default: return eval_scalar<4> (f, ref);
+    }
+  }
+  else
+    return eval_scalar<P> (a.field + (int64_t)cell * Ord<P>::FSTRIDE, ref);
+}
+
+template <int P, bool POLISH>
+__global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (IntegrateArgs a)
+{
+  __shared__ int32_t stack_lds[kStackLds * 256];
+  TravStack todo;
+  todo.lds = stack_lds;
+  todo.bdim = blockDim.x;
+  todo.tid = threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long located = 0;
+  LocateStats st { 0, 0, 0 };
+  bool overflow = false;
+  if (t < a.n_rays)
+  {
+    const int ri = a.order ? a.order[t] : t;
+    const Ray ray = a.rays[ri];
+    Partial *out5 = a.slots + 5 * (int64_t)ri;
+#pragma unroll
+    for (int s = 0; s < 5; ++s)
+    {
+      Partial p0;
+      p0.pixel_id = -1;
+      p0.depth = INFINITY;
+      p0.rgba[0] = p0.rgba[1] = p0.rgba[2] = p0.rgba[3] = 0.f;
+      out5[s] = p0;
+    }
+    float distance = ray.near_ + a.sample_dist;
+    int segment = 0;
+    float rgba[4] = { 0.f, 0.f, 0.f, 0.f };
+    for (int s = 0; s < 5; ++s)
+    {
+      bool found = false;
+      int cell = -1;
+      float ref[3];
+      while (distance < ray.far_ && !found)
+      {
+        const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
+                              ray_point (ray.orig[2], distance, ray.dir[2]) };
+        cell = locate_point<P, POLISH> (a.cfg, todo, pt, ref, st, overflow);
+        if (cell != -1)
+        {
+          found = true;
+          ++located;
+        }
+        else
+          distance += a.sample_dist;
+      }
+      if (distance >= ray.far_ || overflow) break;
+      const float depth = distance;
+      rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
+      do
+      {
+        const float scalar = field_at<P> (a, cell, ref);
+        blend_nc (rgba, color_lookup (a.lut, a.lut_size, a.fmin, a.fmax, a.inv_range, scalar));
+        distance += a.sample_dist;
+        const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
+                              ray_point (ray.orig[2], distance, ray.dir[2]) };
+        cell = locate_point<P, POLISH> (a.cfg, todo, pt, ref, st, overflow);
+        found = cell != -1;
+        if (found) ++located;
+      } while (distance < ray.far_ && found && rgba[3] < 0.95f);
+      Partial pp;
+      pp.pixel_id = ray.pixel_id;
+      pp.depth = depth;
+      pp.rgba[0] = rgba[0];
+      pp.rgba[1] = rgba[1];
+      pp.rgba[2] = rgba[2];
+      pp.rgba[3] = rgba[3];
+      out5[segment] = pp;
+      ++segment;
+      if (distance >= ray.far_ || rgba[3] > 0.95f || overflow) break;
+    }
+  }
+  if (overflow) atomicOr (a.err, 1);
+  // wave-aggregated statistics
+  unsigned long long v[4] = { located, (unsigned long long)st.newton, (unsigned long long)st.candidates,
+                              (unsigned long long)st.polished };
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+  {
+    unsigned long long x = v[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor (x, o, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd (&a.stats[q], x);
+  }
+}
+
+__global__ void k_flag_partials (const Partial *__restrict__ p, int64_t n, int32_t *__restrict__ flags)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = (p[i].depth == INFINITY) ? 0 : 1;
+}
+
+__global__ void k_compact_partials (const Partial *__restrict__ p, int64_t n, const int32_t *__restrict__ flags,
+                                    const int32_t *__restrict__ pos, Partial *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flags[i]) out[pos[i]] = p[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Volume::save compositing (volume.cpp:369-416) and Framebuffer::composite_background
+// (framebuffer.cpp:146-162).  One lane per run of equal pixel ids.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_fill_f32 (float *p, int64_t n, float v)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_composite_runs (const Partial *__restrict__ p, int n, int npix, float4 *__restrict__ img)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int pid = p[i].pixel_id;
+  if (i > 0 && p[i - 1].pixel_id == pid) return;
+  float c[4] = { p[i].rgba[0], p[i].rgba[1], p[i].rgba[2], p[i].rgba[3] };
+  for (int k = i + 1; k < n && p[k].pixel_id == pid; ++k)
+  {
+    const float a = 1.f - c[3];
+    c[0] = c[0] + p[k].rgba[0] * a;
+    c[1] = c[1] + p[k].rgba[1] * a;
+    c[2] = c[2] + p[k].rgba[2] * a;
+    c[3] = c[3] + p[k].rgba[3] * a;
+  }
+  if (pid >= 0 && pid < npix) img[pid] = make_float4 (c[0], c[1], c[2], c[3]);
+}
+
+__global__ void k_background (float4 *__restrict__ img, int npix)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  float4 c = img[i];
+  if (c.w < 1.f)
+  {
+    const float a = 1.f - c.w;
+    c.x = c.x + 1.f * a;
+    c.y = c.y + 1.f * a;
+    c.z = c.z + 1.f * a;
+    c.w = c.w + 1.f * a;
+    img[i] = c;
+  }
+}
+
+static int check_mesh (const drx_mesh *m, int ncomp, const char *what)
+{
+  if (!m || !m->data) return fail (DRX_ERR_ARG, "%s: null mesh", what);
+  if (m->ncomp != ncomp) return fail (DRX_ERR_ARG, "%s: expected ncomp %d, got %d", what, ncomp, m->ncomp);
+  if ((m->elem_order == nullptr) != (m->elem_offset == nullptr))
+    return fail (DRX_ERR_ARG, "%s: mixed-order meshes need elem_order and elem_offset", what);
+  if (!m->elem_order)
+  {
+    if (m->order < 1 || m->order > DRX_MAX_ORDER)
+      return fail (DRX_ERR_ARG, "%s: order %d outside 1..%d", what, m->order, DRX_MAX_ORDER);
+    if (m->stride != drx_mesh_stride (m->order, ncomp))
+      return fail (DRX_ERR_ARG, "%s: stride %d does not match order %d", what, m->stride, m->order);
+  }
+  return DRX_OK;
+}
+
+static int check_bvh (const drx_bvh *b, const char *what)
+{
+  if (!b || !b->inner_nodes || !b->leaf_pack) return fail (DRX_ERR_ARG, "%s: bvh needs inner_nodes and leaf_pack", what);
+  if (b->n_leaves < 2) return fail (DRX_ERR_ARG, "%s: bvh has %d leaves (dray pads to >= 2)", what, b->n_leaves);
+  return DRX_OK;
+}
+
+static LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx_locate_opts *o)
+{
+  LocateCfg c;
+  c.nodes = (const float4 *)b->inner_nodes;
+  c.leaves = (const float4 *)b->leaf_pack;
+  c.mesh = m->data;
+  c.offsets = m->elem_offset;
+  c.orders = m->elem_order;
+  c.stride = m->stride;
+  c.polish = o ? (o->fp64_polish != 0) : false;
+  c.cond_threshold = o ? o->cond_threshold : 1e4f;
+  return c;
+}
+
+} // namespace drx
+
+using namespace drx;
+
+extern "C" {
+
+int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n, void *out_locations,
+                const drx_locate_opts *opts, void *stream)
+{
+  int rc = check_mesh (mesh, 3, "drx_locate");
+  if (rc) return rc;
+  if ((rc = check_bvh (bvh, "drx_locate"))) return rc;
+  if (n < 0) return fail (DRX_ERR_ARG, "drx_locate: n < 0");
+  if (n == 0) return DRX_OK;
+  if (!points || !out_locations) return fail (DRX_ERR_ARG, "drx_locate: null points/output");
+  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE)
+    return fail (DRX_ERR_UNSUPPORTED, "drx_locate: init_mode %d not available", opts->init_mode);
+  hipStream_t st = as_stream (stream);
+  int32_t *err = (int32_t *)arena (SLOT_FLAGS, 64);
+  if (!err) return DRX_ERR_HIP;
+  hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, err, (int64_t)1, 0);
+  const LocateCfg cfg = make_cfg (mesh, bvh, opts);
+  int32_t *steps = opts ? opts->newton_steps : nullptr;
+  int32_t *cands = opts ? opts->candidates : nullptr;
+  const int P = mesh->elem_order ? 0 : mesh->order;
+  const dim3 g (nblk (n, 256)), b (256);
+  Location *out = (Location *)out_locations;
+#define DRX_LOC(PP, PO) hipLaunchKernelGGL ((k_locate<PP, PO>), g, b, 0, st, cfg, points, n, out, steps, cands, err)
+  if (cfg.polish)
+    switch (P)
+    {
+    case 0: DRX_LOC (0, true); break;
+    case 1: DRX_LOC (1, true); break;
+    case 2: DRX_LOC (2, true); break;
+    case 3: DRX_LOC (3, true); break;
+    default: DRX_LOC (4, true); break;
+    }
+  else
+    switch (P)
+    {
+    case 0: DRX_LOC (0, false); break;
+    case 1: DRX_LOC (1, false); break;
+    case 2: DRX_LOC (2, false); break;
+    case 3: DRX_LOC (3, false); break;
+    default: DRX_LOC (4, false); break;
+    }
+#undef DRX_LOC
+  DRX_LAUNCH_CHECK ();
+  int32_t herr = 0;
+  DRX_HIP (hipMemcpyAsync (&herr, err, sizeof (int32_t), hipMemcpyDeviceToHost, st));
+  DRX_HIP (hipStreamSynchronize (st));
+  if (herr) return fail (DRX_ERR_OVERFLOW, "drx_locate: BVH traversal stack (%d entries) overflowed", kStackDepth);
+  return DRX_OK;
+}
+
+int drx_field_eval (const drx_mesh *field, const void *locations, int64_t n, float *out_values, void *stream)
+{
+  int rc = check_mesh (field, 1, "drx_field_eval");
+  if (rc) return rc;
+  if (n < 0) return fail (DRX_ERR_ARG, "drx_field_eval: n < 0");
+  if (n == 0) return DRX_OK;
+  if (!locations || !out_values) return fail (DRX_ERR_ARG, "drx_field_eval: null pointer");
+  hipStream_t st = as_stream (stream);
+  const Location *l = (const Location *)locations;
+  const dim3 g (nblk (n, 256)), b (256);
+  switch (field->elem_order ? 0 : field->order)
+  {
+  case 0: hipLaunchKernelGGL (k_field_eval<0>, g, b, 0, st, field->data, field->elem_offset, field->elem_order, l, n, out_values); break;
+  case 1: hipLaunchKernelGGL (k_field_eval<1>, g, b, 0, st, field->data, nullptr, nullptr, l, n, out_values); break;
+  case 2: hipLaunchKernelGGL (k_field_eval<2>, g, b, 0, st, field->data, nullptr, nullptr, l, n, out_values); break;
+  case 3: hipLaunchKernelGGL (k_field_eval<3>, g, b, 0, st, field->data, nullptr, nullptr, l, n, out_values); break;
+  default: hipLaunchKernelGGL (k_field_eval<4>, g, b, 0, st, field->data, nullptr, nullptr, l, n, out_values); break;
+  }
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_camera_rays (const drx_camera *cam, void *out_rays, void *stream)
+{
+  if (!cam || !out_rays) return fail (DRX_ERR_ARG, "drx_camera_rays: null pointer");
+  if (cam->width <= 0 || cam->height <= 0) return fail (DRX_ERR_ARG, "Camera width/height must be greater than zero.");
+  if (cam->fov_y <= 0 || cam->fov_y > 180) return fail (DRX_ERR_ARG, "Camera field of view must be in (0, 180].");
+  const CamSetup cs = camera_setup (*cam);
+  const int64_t npix = (int64_t)cam->width * cam->height;
+  hipLaunchKernelGGL (k_camera, dim3 (nblk (npix, 256)), dim3 (256), 0, as_stream (stream), cs, (Ray *)out_rays);
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_remove_missed_rays (const void *rays, int32_t n, const float *bounds6, void *out_rays, int32_t *out_count,
+                            void *stream)
+{
+  if (n < 0 || !bounds6 || !out_count) return fail (DRX_ERR_ARG, "drx_remove_missed_rays: bad arguments");
+  *out_count = 0;
+  if (n == 0) return DRX_OK;
+  if (!rays || !out_rays) return fail (DRX_ERR_ARG, "drx_remove_missed_rays: null rays");
+  // an empty bounds box yields no rays (ray.cpp:105)
+  bool empty = true;
+  for (int d = 0; d < 3; ++d) empty = empty && (bounds6[2 * d] > bounds6[2 * d + 1]);
+  if (empty) return DRX_OK;
+  // mesh_bounds.scale(1.001f) on the host, float32 like Range::scale
+  Bounds6 mb;
+  for (int d = 0; d < 3; ++d)
+  {
+    float lo = bounds6[2 * d], hi = bounds6[2 * d + 1];
+    if (!(lo > hi))
+    {
+      volatile float c = 0.5f * (lo + hi);
+      volatile float len = hi - lo;
+      volatile float half = 1.001f * 0.5f;
+      volatile float delta = half * len;
+      volatile float v0 = c - delta, v1 = c + delta;
+      lo = std::fmin (lo, (float)v0);
+      hi = std::fmax (hi, (float)v0);
+      lo = std::fmin (lo, (float)v1);
+      hi = std::fmax (hi, (float)v1);
+    }
+    mb.b[2 * d] = lo;
+    mb.b[2 * d + 1] = hi;
+  }
+  hipStream_t st = as_stream (stream);
+  int32_t *flags = (int32_t *)arena (SLOT_FLAGS, sizeof (int32_t) * 2 * (size_t)n + 64);
+  if (!flags) return DRX_ERR_HIP;
+  int32_t *pos = flags + n;
+  hipLaunchKernelGGL (k_mark_active, dim3 (nblk (n, 256)), dim3 (256), 0, st, (const Ray *)rays, n, mb, flags);
+  DRX_LAUNCH_CHECK ();
+  int32_t total = 0;
+  int rc = exclusive_scan_i32 (flags, pos, n, &total, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL (k_gather_active, dim3 (nblk (n, 256)), dim3 (256), 0, st, (const Ray *)rays, n, mb, flags, pos,
+                      (Ray *)out_rays);
+  DRX_LAUNCH_CHECK ();
+  DRX_HIP (hipStreamSynchronize (st));
+  *out_count = total;
+  return DRX_OK;
+}
+
+int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t height, int32_t *out_order, void *stream)
+{
+  if (n < 0 || width <= 0 || height <= 0) return fail (DRX_ERR_ARG, "drx_ray_tile_order: bad sizes");
+  if (n == 0) return DRX_OK;
+  if (!rays || !out_order) return fail (DRX_ERR_ARG, "drx_ray_tile_order: null pointer");
+  hipStream_t st = as_stream (stream);
+  const int tiles_x = (width + 7) / 8, tiles_y = (height + 7) / 8;
+  const int64_t slots_n = (int64_t)tiles_x * tiles_y * 64;
+  if (slots_n > (1ll << 30)) return fail (DRX_ERR_ARG, "drx_ray_tile_order: image too large");
+  int32_t *slots = (int32_t *)arena (SLOT_TREE2, sizeof (int32_t) * 3 * (size_t)slots_n + 64);
+  if (!slots) return DRX_ERR_HIP;
+  int32_t *flags = slots + slots_n, *pos = flags + slots_n;
+  hipLaunchKernelGGL (k_fill_i32q, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots, slots_n, -1);
+  hipLaunchKernelGGL (k_tile_scatter, dim3 (nblk (n, 256)), dim3 (256), 0, st, (const Ray *)rays, n, width, height, slots);
+  hipLaunchKernelGGL (k_flag_nonneg, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots, slots_n, flags);
+  DRX_LAUNCH_CHECK ();
+  int32_t total = 0;
+  int rc = exclusive_scan_i32 (flags, pos, (int32_t)slots_n, &total, st);
+  if (rc) return rc;
+  if (total != n)
+    return fail (DRX_ERR_ARG, "drx_ray_tile_order: rays are not distinct pixels of a %dx%d image", width, height);
+  hipLaunchKernelGGL (k_compact_i32, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots, slots_n, pos, out_order);
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx_bvh *bvh, const void *active_rays,
+                          int32_t n_rays, const float *lut, int32_t lut_size, float field_min, float field_max,
+                          float sample_dist, const int32_t *ray_order, const drx_locate_opts *opts, void *out_partials,
+                          int32_t *out_count, drx_integrate_stats *stats, void *stream)
+{
+  int rc = check_mesh (mesh, 3, "drx_volume_integrate(mesh)");
+  if (rc) return rc;
+  if ((rc = check_mesh (field, 1, "drx_volume_integrate(field)"))) return rc;
+  if ((rc = check_bvh (bvh, "drx_volume_integrate"))) return rc;
+  if ((mesh->elem_order == nullptr) != (field->elem_order == nullptr) ||
+      (!mesh->elem_order && mesh->order != field->order))
+    return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: mesh and field orders must match");
+  if (!out_count) return fail (DRX_ERR_ARG, "drx_volume_integrate: null out_count");
+  if (n_rays < 0 || lut_size < 1 || !lut) return fail (DRX_ERR_ARG, "drx_volume_integrate: bad rays/lut");
+  if (!(field_min <= field_max)) return fail (DRX_ERR_ARG, "ColorMap scalar range never set");
+  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE)
+    return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: init_mode %d not available", opts->init_mode);
+  *out_count = 0;
+  if (stats) *stats = drx_integrate_stats { 0, 0, 0, 0 };
+  if (n_rays == 0) return DRX_OK;
+  if (!active_rays || !out_partials) return fail (DRX_ERR_ARG, "drx_volume_integrate: null rays/output");
+  if ((int64_t)n_rays * 5 > (1ll << 31) - 1) return fail (DRX_ERR_ARG, "drx_volume_integrate: too many rays");
+  hipStream_t st = as_stream (stream);
+  const int64_t n_slots = 5 * (int64_t)n_rays;
+  Partial *slots = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n_slots);
+  int32_t *flags = (int32_t *)arena (SLOT_FLAGS, sizeof (int32_t) * 2 * (size_t)n_slots + 64);
+  unsigned long long *dstats = (unsigned long long *)arena (SLOT_STATS, 8 * sizeof (unsigned long long));
+  if (!slots || !flags || !dstats) return DRX_ERR_HIP;
+  int32_t *pos = flags + n_slots;
+  int32_t *err = (int32_t *)(dstats + 4);
+  hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, (int32_t *)dstats, (int64_t)16, 0);
+  IntegrateArgs a;
+  a.cfg = make_cfg (mesh, bvh, opts);
+  a.field = field->data;
+  a.field_offsets = field->elem_offset;
+  a.rays = (const Ray *)active_rays;
+  a.order = ray_order;
+  a.n_rays = n_rays;
+  a.lut = (const float4 *)lut;
+  a.lut_size = lut_size;
+  a.fmin = field_min;
+  a.fmax = field_max;
+  {
+    // rcp_safe(range length) on the host, float32 (device_color_map.hpp:45, math.hpp:213-217)
+    const float len = field_max - field_min;
+    a.inv_range = 1.0f / ((std::fabs (len) < 1e-8f) ? (std::signbit (len) ? -1e-8f : 1e-8f) : len);
+  }
+  a.sample_dist = sample_dist;
+  a.slots = slots;
+  a.stats = dstats;
+  a.err = err;
+  const dim3 g (nblk (n_rays, 256)), b (256);
+  const int P = mesh->elem_order ? 0 : mesh->order;
+#define DRX_INT(PP, PO) hipLaunchKernelGGL ((k_integrate<PP, PO>), g, b, 0, st, a)
+  if (a.cfg.polish)
+    switch (P)
+    {
+    case 0: DRX_INT (0, true); break;
+    case 1: DRX_INT (1, true); break;
+    case 2: DRX_INT (2, true); break;
+    case 3: DRX_INT (3, true); break;
+    default: DRX_INT (4, true); break;
+    }
+  else
+    switch (P)
+    {
+    case 0: DRX_INT (0, false); break;
+    case 1: DRX_INT (1, false); break;
+    case 2: DRX_INT (2, false); break;
+    case 3: DRX_INT (3, false); break;
+    default: DRX_INT (4, false); break;
+    }
+#undef DRX_INT
+  DRX_LAUNCH_CHECK ();
+  hipLaunchKernelGGL (k_flag_partials, dim3 (nblk (n_slots, 256)), dim3 (256), 0, st, slots, n_slots, flags);
+  DRX_LAUNCH_CHECK ();
+  int32_t total = 0;
+  rc = exclusive_scan_i32 (flags, pos, (int32_t)n_slots, &total, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL (k_compact_partials, dim3 (nblk (n_slots, 256)), dim3 (256), 0, st, slots, n_slots, flags, pos,
+                      (Partial *)out_partials);
+  DRX_LAUNCH_CHECK ();
+  unsigned long long hs[8];
+  DRX_HIP (hipMemcpyAsync (hs, dstats, sizeof hs, hipMemcpyDeviceToHost, st));
+  DRX_HIP (hipStreamSynchronize (st));
+  if (((int32_t *)(hs + 4))[0])
+    return fail (DRX_ERR_OVERFLOW, "drx_volume_integrate: BVH traversal stack (%d entries) overflowed", kStackDepth);
+  *out_count = total;
+  if (stats)
+  {
+    stats->located_samples = (int64_t)hs[0];
+    stats->newton_steps = (int64_t)hs[1];
+    stats->candidates = (int64_t)hs[2];
+    stats->polished = (int64_t)hs[3];
+  }
+  return DRX_OK;
+}
+
+int drx_composite (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba, void *stream)
+{
+  if (n < 0 || width <= 0 || height <= 0 || !out_rgba || (n > 0 && !partials))
+    return fail (DRX_ERR_ARG, "drx_composite: bad arguments");
+  hipStream_t st = as_stream (stream);
+  const int npix = width * height;
+  hipLaunchKernelGGL (k_fill_f32, dim3 (nblk (4 * (int64_t)npix, 256)), dim3 (256), 0, st, out_rgba, 4 * (int64_t)npix, 0.f);
+  if (n > 0)
+    hipLaunchKernelGGL (k_composite_runs, dim3 (nblk (n, 256)), dim3 (256), 0, st, (const Partial *)partials, n, npix,
+                        (float4 *)out_rgba);
+  hipLaunchKernelGGL (k_background, dim3 (nblk (npix, 256)), dim3 (256), 0, st, (float4 *)out_rgba, npix);
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+} // extern "C"

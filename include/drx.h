@@ -1,0 +1,193 @@
+/*
+ * drx.h -- C ABI of the MI355X (gfx950) volume-render / point-location library (libdrx.so).
+ *
+ * Plain pointers and sizes only.  Every pointer argument is DEVICE memory unless its comment says
+ * "host".  Every function returns DRX_OK (0) or a non-zero status; drx_last_error() returns the
+ * message of the last failure on the calling host thread.  Functions enqueue on `stream` (a
+ * hipStream_t passed as void*, NULL = the null stream) and, unless noted, return without waiting;
+ * functions that hand back a host-visible count synchronise the stream first.  The library never frees
+ * caller memory; temporaries live in a per-device arena owned by the library (drx_release frees it).
+ *
+ * Reference interfaces replaced (LLNL/devil_ray, paths relative to its src/dray/):
+ *   drx_bvh_split_elements  construct_bvh per-element split          data_model/mesh_utils.cpp:514-670
+ *   drx_bvh_build           LinearBVHBuilder::construct               linear_bvh_builder.hpp:20-21,
+ *                                                                     linear_bvh_builder.cpp:406-518
+ *   drx_locate              Mesh::locate / UnstructuredMesh::locate   data_model/mesh.hpp:34,
+ *                           (DeviceMesh::locate per point)            data_model/unstructured_mesh.cpp:123-148,
+ *                                                                     data_model/device_mesh.hpp:247-354
+ *   drx_field_eval          Field::eval                               data_model/field.hpp:53,
+ *                                                                     data_model/unstructured_field.cpp:213-246
+ *   drx_camera_rays         Camera::create_rays / gen_perspective     rendering/camera.cpp:208-243,402-483
+ *   drx_remove_missed_rays  remove_missed_rays                        ray.cpp:102-241
+ *   drx_volume_integrate    Volume::integrate -> integrate_partials   rendering/volume.hpp:40,
+ *                                                                     rendering/volume.cpp:69-228,323-351
+ *   drx_composite           Volume::save compositing + Framebuffer::  rendering/volume.cpp:369-416,
+ *                           composite_background                      rendering/framebuffer.cpp:146-162
+ *   drx_color_table_sample  ColorTable::correct_opacity + sample      color_table.cpp:152-324
+ *   drx_mesh_pack           (no counterpart: GridFunction -> element-local SoA conversion done once at
+ *                           upload; replaces DeviceMesh::get_elem's indirect reads, device_mesh.hpp:79-96)
+ */
+#ifndef DRX_H
+#define DRX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum drx_status
+{
+  DRX_OK = 0,
+  DRX_ERR_ARG = 1,         /* bad argument (null pointer, size, unsupported order) */
+  DRX_ERR_HIP = 2,         /* a HIP runtime call failed */
+  DRX_ERR_UNSUPPORTED = 3, /* valid request this build does not implement */
+  DRX_ERR_OVERFLOW = 4,    /* a fixed-capacity structure overflowed (traversal stack) */
+  DRX_ERR_STATE = 5        /* library not initialised / wrong device */
+};
+
+/* Highest element order the GPU kernels accept. */
+#define DRX_MAX_ORDER 4
+/* BVH leaves per element: 32 binary de Casteljau splits -> 33 sub-boxes (mesh_utils.cpp:528). */
+#define DRX_SPLITS_PER_ELEMENT 33
+
+/* Element-local packed mesh (output of drx_mesh_pack).  Geometry (ncomp == 3) blocks are
+ *   [ox, oy, oz, 0, X[npe_pad], Y[npe_pad], Z[npe_pad]]
+ * with X = x - ox etc. (control points relative to the element's box centre, computed in double and
+ * rounded once), npe = (p+1)^3 in dray dof order i + (p+1)(j + (p+1)k), npe_pad = npe rounded up to 4.
+ * Scalar-field (ncomp == 1) blocks are [F[npe_pad]].  Uniform-order meshes place element e at
+ * e * stride floats; mixed-order meshes give per-element orders and float offsets. */
+typedef struct drx_mesh
+{
+  const float *data;
+  const int64_t *elem_offset; /* device, NULL for uniform order */
+  const int32_t *elem_order;  /* device, NULL for uniform order */
+  int32_t n_el;
+  int32_t order;              /* uniform order; for mixed meshes the maximum order */
+  int32_t ncomp;              /* 3 = geometry, 1 = scalar field */
+  int32_t stride;             /* floats per element block (uniform order) */
+} drx_mesh;
+
+/* BVH in dray's layout (bvh.hpp:15-39): inner node k = 16 floats
+ *   [L.min xyz, L.max x | L.max yz, R.min xy | R.min z, R.max xyz | lchild, rchild (int32 bits), 0, 0]
+ * child >= 0: float4 offset (4 * inner index) of an inner node; child < 0: leaf -(child) - 1. */
+typedef struct drx_bvh
+{
+  const float *inner_nodes;   /* (n_leaves - 1) * 16 floats */
+  const int32_t *leaf_nodes;  /* element id per leaf, sorted (m_leaf_nodes) */
+  const int32_t *aabb_ids;    /* original aabb index per leaf (m_aabb_ids) */
+  const float *subrefs;       /* 6 floats (lo xyz, hi xyz) per ORIGINAL aabb (mesh m_ref_aabbs) */
+  const float *leaf_pack;     /* optional, 8 floats per leaf: {el_id bits, lo xyz, hi xyz, 0}; see drx_bvh_pack_leaves */
+  int32_t n_leaves;
+  float bounds[6];            /* host copy of m_bounds: xmin, xmax, ymin, ymax, zmin, zmax */
+} drx_bvh;
+
+/* Location (location.hpp:16-21): {int32 cell (-1 = not found), float ref[3]} = 16 bytes. */
+/* Ray (ray.hpp:20-28): {float dir[3], near, orig[3], far; int32 pixel_id} = 36 bytes.   */
+/* VolumePartial (volume_partial.hpp:16-21): {int32 pixel_id; float depth; float rgba[4]} = 24 bytes. */
+
+enum drx_init_mode
+{
+  DRX_INIT_SUBREF_CENTRE = 0, /* Newton starts at the candidate leaf's SubRef centre */
+  DRX_INIT_MLP = 1            /* learned initial guess (not available in this build: DRX_ERR_UNSUPPORTED) */
+};
+
+typedef struct drx_locate_opts
+{
+  int32_t init_mode;          /* drx_init_mode */
+  int32_t fp64_polish;        /* 1: FP64 Newton correction for accepted points with cond(J) > cond_threshold */
+  float cond_threshold;       /* e.g. 1e4 */
+  int32_t *newton_steps;      /* device, nullable: Newton steps summed over candidates, per point */
+  int32_t *candidates;        /* device, nullable: candidate leaves tried, per point */
+} drx_locate_opts;
+
+typedef struct drx_integrate_stats
+{
+  int64_t located_samples;    /* successful locate() calls inside the march */
+  int64_t newton_steps;       /* Newton iterations over all candidates */
+  int64_t candidates;         /* candidate leaves tried */
+  int64_t polished;           /* samples that took the FP64 correction */
+} drx_integrate_stats;
+
+typedef struct drx_camera
+{
+  int32_t width, height;
+  float fov_y;                /* degrees */
+  float pos[3];
+  float look_at[3];
+  float up[3];
+  float zoom;
+} drx_camera;
+
+/* ---- library state ------------------------------------------------------------------------- */
+int drx_init (int device);                 /* select device, check it is gfx950 */
+const char *drx_last_error (void);
+const char *drx_version (void);
+int drx_release (void);                    /* free the library's arena on the current device */
+
+/* ---- mesh packing -------------------------------------------------------------------------- */
+/* Host helper: floats per element block for order p and ncomp. */
+int32_t drx_mesh_stride (int32_t order, int32_t ncomp);
+/* Host helper for mixed-order meshes: fills h_offsets[n_el] (host) and returns total floats. */
+int64_t drx_mesh_pack_layout (const int32_t *h_orders, int32_t n_el, int32_t ncomp, int64_t *h_offsets);
+/* dray GridFunction (ctrl_idx + ncomp values per control point) -> packed blocks in `out`.
+ * orders/offsets (device) are NULL for uniform order; offsets index ctrl_idx per element and
+ * out_offsets (device) index `out` per element. */
+int drx_mesh_pack (const int32_t *ctrl_idx, const float *values, int32_t ncomp, const int32_t *orders,
+                   const int64_t *offsets, const int64_t *out_offsets, int32_t order, int32_t n_el,
+                   float *out, void *stream);
+
+/* ---- LBVH ---------------------------------------------------------------------------------- */
+/* 33 sub-boxes per element, bit-exact with dray's float32 split: out_aabbs 6*33*n_el floats,
+ * out_prims 33*n_el, out_subrefs 6*33*n_el.  Reads dray's GridFunction (ctrl_idx + Vec3 values). */
+int drx_bvh_split_elements (const int32_t *ctrl_idx, const float *values, const int32_t *orders,
+                            const int64_t *offsets, int32_t order, int32_t n_el, float *out_aabbs,
+                            int32_t *out_prims, float *out_subrefs, void *stream);
+/* LinearBVHBuilder::construct(aabbs, prim_ids).  m = max(n, 2) leaves (dray pads 0/1 boxes):
+ * out_inner (m-1)*16 floats, out_leaf_nodes m, out_aabb_ids m; out_bounds (host) 6 floats;
+ * out_mcodes (nullable) m sorted Morton codes.  Synchronises `stream`. */
+int drx_bvh_build (const float *aabbs, const int32_t *prims, int32_t n, float *out_inner,
+                   int32_t *out_leaf_nodes, int32_t *out_aabb_ids, float *out_bounds, uint32_t *out_mcodes,
+                   void *stream);
+/* Fused leaf records for traversal: 8 floats per leaf {el_id bits, subref lo xyz, hi xyz, 0}. */
+int drx_bvh_pack_leaves (const drx_bvh *bvh, float *out_leaf_pack, void *stream);
+
+/* ---- queries ------------------------------------------------------------------------------- */
+int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
+                void *out_locations, const drx_locate_opts *opts, void *stream);
+int drx_field_eval (const drx_mesh *field, const void *locations, int64_t n, float *out_values, void *stream);
+
+/* ---- rendering ----------------------------------------------------------------------------- */
+int drx_camera_rays (const drx_camera *cam, void *out_rays, void *stream);
+/* Compacts rays hitting bounds (scaled x1.001) and sets near/far.  out_count is host; synchronises. */
+int drx_remove_missed_rays (const void *rays, int32_t n, const float *bounds6_host, void *out_rays,
+                            int32_t *out_count, void *stream);
+/* Tile-major processing order for rays that came from a width x height camera (pixel_id = j*w+i):
+ * fills out_order[n] with a permutation of 0..n-1 that walks 8x8 pixel tiles. */
+int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t height, int32_t *out_order,
+                        void *stream);
+/* Volume::integrate on active rays: samples = |bounds diagonal|/sample_dist steps, field colour from
+ * the (already opacity-corrected) LUT of lut_size RGBA entries over [field_min, field_max], front-to-back
+ * blend, <= 5 partials per ray.  out_partials must hold 5*n_rays partials; on return it holds the
+ * compacted partials (depth != inf, ray order) and *out_count (host) their number.  ray_order may be
+ * NULL.  stats (host, nullable).  Synchronises. */
+int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx_bvh *bvh,
+                          const void *active_rays, int32_t n_rays, const float *lut, int32_t lut_size,
+                          float field_min, float field_max, float sample_dist, const int32_t *ray_order,
+                          const drx_locate_opts *opts, void *out_partials, int32_t *out_count,
+                          drx_integrate_stats *stats, void *stream);
+/* Partials (grouped per pixel, as integrate emits them) -> RGBA image over a white background. */
+int drx_composite (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba,
+                   void *stream);
+
+/* ---- host-side colour table ---------------------------------------------------------------- */
+/* ColorTable with rgb points (pos, r, g, b, a) and alpha points (pos, alpha), host arrays; opacity
+ * corrected with factor (<= 0: none) then sampled to num_samples RGBA entries (host out). */
+int drx_color_table_sample (const float *rgb_pts, int32_t n_rgb, const float *alpha_pts, int32_t n_alpha,
+                            int32_t smooth, float factor, int32_t num_samples, float alpha_scale,
+                            float *out_rgba_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DRX_H */

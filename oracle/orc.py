@@ -1,0 +1,258 @@
+"""ctypes bindings for the CPU oracle (oracle/_build/liborc.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+always as the checker.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
+_lib = None
+
+P = C.c_void_p
+I32 = C.c_int
+I64 = C.c_int64
+F32 = C.c_float
+F64 = C.c_double
+
+_SIGS = {
+    "orc_num_threads_max": (I32, []),
+    "orc_split_elements": (I32, [P, P, P, P, I32, I32, P, P, P, I32]),
+    "orc_bvh_build": (I32, [P, P, I32, P, P, P, P, P, P, P, P, P]),
+    "orc_morton_3d": (C.c_uint32, [F32, F32, F32]),
+    "orc_clz": (I32, [C.c_uint32]),
+    "orc_eval_d_f32": (None, [P, I32, I32, P, P, P]),
+    "orc_eval_d_f64": (None, [P, I32, I32, P, P, P]),
+    "orc_lup_solve_f32": (I32, [P, P, P]),
+    "orc_lup_solve_f64": (I32, [P, P, P]),
+    "orc_newton_f32": (I32, [P, I32, P, P, P]),
+    "orc_newton_f64": (I32, [P, I32, P, P, P]),
+    "orc_locate_f32": (I32, [P, P, P, P, I32, I32, P, P, P, P, P, I64, P, P, P, P, I32]),
+    "orc_locate_f64": (I32, [P, P, P, P, I32, I32, P, P, P, P, P, I64, P, P, P, P, I32]),
+    "orc_locate_brute_f64": (I32, [P, P, P, P, I32, I32, P, I64, P, P]),
+    "orc_refine_f64": (I32, [P, P, P, P, I32, I32, P, P, P, I64, P, I32]),
+    "orc_field_eval_f32": (I32, [P, P, P, P, I32, I32, P, P, I64, P]),
+    "orc_camera_perspective": (None, [I32, I32, F32, P, P, P, F32, P]),
+    "orc_remove_missed_rays": (I32, [P, I32, P, P]),
+    "orc_sample_dist": (F32, [P, I32]),
+    "orc_integrate_f32": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P, P, I32]),
+    "orc_integrate_f64": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P, P, I32]),
+    "orc_compact_partials": (I32, [P, I64, P]),
+    "orc_composite": (None, [P, I32, I32, I32, P]),
+    "orc_color_table_sample": (None, [P, I32, P, I32, I32, F32, I32, F32, P]),
+}
+
+
+def build() -> None:
+    subprocess.check_call(["bash", os.path.join(_HERE, "build_oracle.sh")])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the oracle must be contiguous"
+    return a.ctypes.data
+
+
+def max_threads() -> int:
+    return lib().orc_num_threads_max()
+
+
+# ---- convenience wrappers ---------------------------------------------------------------------
+def _mesh_args(mesh, dtype=np.float32, field=False):
+    vals = np.ascontiguousarray((mesh.field if field else mesh.values).astype(dtype))
+    ctrl = np.ascontiguousarray(mesh.ctrl_idx)
+    orders = None if mesh.orders is None else np.ascontiguousarray(mesh.orders.astype(np.int32))
+    offs = None if mesh.offsets is None else np.ascontiguousarray(mesh.offsets.astype(np.int64))
+    return ctrl, vals, orders, offs
+
+
+def split_elements(mesh, nthreads=0):
+    n_el = mesh.n_el
+    aabbs = np.zeros((n_el * 33, 6), np.float32)
+    prims = np.zeros(n_el * 33, np.int32)
+    subrefs = np.zeros((n_el * 33, 6), np.float32)
+    ctrl, vals, orders, offs = _mesh_args(mesh)
+    lib().orc_split_elements(ptr(ctrl), ptr(vals), ptr(orders), ptr(offs), mesh.order, n_el,
+                             ptr(aabbs), ptr(prims), ptr(subrefs), nthreads)
+    return aabbs, prims, subrefs
+
+
+class OracleBVH:
+    pass
+
+
+def bvh_build(aabbs: np.ndarray, prims: np.ndarray) -> OracleBVH:
+    n = int(aabbs.shape[0])
+    m = max(n, 2)
+    b = OracleBVH()
+    b.inner = np.zeros(((m - 1), 16), np.float32)
+    b.leaf_nodes = np.zeros(m, np.int32)
+    b.aabb_ids = np.zeros(m, np.int32)
+    b.bounds = np.zeros(6, np.float32)
+    b.mcodes_sorted = np.zeros(m, np.uint32)
+    b.mcodes_unsorted = np.zeros(m, np.uint32)
+    b.left = np.zeros(m - 1, np.int32)
+    b.right = np.zeros(m - 1, np.int32)
+    b.parent = np.zeros(2 * m - 1, np.int32)
+    a = np.ascontiguousarray(aabbs, dtype=np.float32)
+    p = np.ascontiguousarray(prims, dtype=np.int32)
+    lib().orc_bvh_build(ptr(a), ptr(p), n, ptr(b.inner), ptr(b.leaf_nodes), ptr(b.aabb_ids), ptr(b.bounds),
+                        ptr(b.mcodes_sorted), ptr(b.mcodes_unsorted), ptr(b.left), ptr(b.right), ptr(b.parent))
+    return b
+
+
+def build_mesh_bvh(mesh, nthreads=0):
+    aabbs, prims, subrefs = split_elements(mesh, nthreads)
+    b = bvh_build(aabbs, prims)
+    b.aabbs, b.prims, b.subrefs = aabbs, prims, subrefs
+    return b
+
+
+def locate(mesh, bvh, pts, dtype=np.float32, nthreads=0, want_steps=False):
+    npts = int(pts.shape[0])
+    ctrl, vals, orders, offs = _mesh_args(mesh, dtype)
+    P_ = np.ascontiguousarray(pts.astype(dtype))
+    cells = np.zeros(npts, np.int32)
+    refs = np.zeros((npts, 3), dtype)
+    steps = np.zeros(npts, np.int32) if want_steps else None
+    stats = np.zeros(2, np.int64)
+    fn = lib().orc_locate_f32 if dtype == np.float32 else lib().orc_locate_f64
+    fn(ptr(ctrl), ptr(vals), ptr(orders), ptr(offs), mesh.order, mesh.n_el, ptr(bvh.inner), ptr(bvh.leaf_nodes),
+       ptr(bvh.aabb_ids), ptr(bvh.subrefs), ptr(P_), npts, ptr(cells), ptr(refs), ptr(steps), ptr(stats), nthreads)
+    return (cells, refs, steps, stats) if want_steps else (cells, refs)
+
+
+def locate_brute_f64(mesh, pts):
+    npts = int(pts.shape[0])
+    ctrl, vals, orders, offs = _mesh_args(mesh, np.float64)
+    P_ = np.ascontiguousarray(pts.astype(np.float64))
+    cells = np.zeros(npts, np.int32)
+    refs = np.zeros((npts, 3), np.float64)
+    lib().orc_locate_brute_f64(ptr(ctrl), ptr(vals), ptr(orders), ptr(offs), mesh.order, mesh.n_el, ptr(P_), npts,
+                               ptr(cells), ptr(refs))
+    return cells, refs
+
+
+def refine_f64(mesh, pts, cells, refs, nthreads=0):
+    npts = int(pts.shape[0])
+    ctrl, vals, orders, offs = _mesh_args(mesh, np.float64)
+    P_ = np.ascontiguousarray(pts.astype(np.float64))
+    c = np.ascontiguousarray(cells.astype(np.int32))
+    r = np.ascontiguousarray(refs.astype(np.float64))
+    out = np.zeros((npts, 3), np.float64)
+    lib().orc_refine_f64(ptr(ctrl), ptr(vals), ptr(orders), ptr(offs), mesh.order, mesh.n_el, ptr(P_), ptr(c), ptr(r),
+                         npts, ptr(out), nthreads)
+    return out
+
+
+def eval_d(dofs: np.ndarray, p: int, ref, dtype=np.float64):
+    """dofs: [npe, ncomp] in dray dof order.  Returns (val[ncomp], deriv[3, ncomp])."""
+    ncomp = dofs.shape[1]
+    d = np.ascontiguousarray(dofs.astype(dtype))
+    r = np.ascontiguousarray(np.asarray(ref, dtype=dtype))
+    val = np.zeros(ncomp, dtype)
+    der = np.zeros((3, ncomp), dtype)
+    fn = lib().orc_eval_d_f64 if dtype == np.float64 else lib().orc_eval_d_f32
+    fn(ptr(d), p, ncomp, ptr(r), ptr(val), ptr(der))
+    return val, der
+
+
+def lup_solve(A, b, dtype=np.float64):
+    A_ = np.ascontiguousarray(np.asarray(A, dtype=dtype))
+    b_ = np.ascontiguousarray(np.asarray(b, dtype=dtype))
+    x = np.zeros(3, dtype)
+    fn = lib().orc_lup_solve_f64 if dtype == np.float64 else lib().orc_lup_solve_f32
+    valid = fn(ptr(A_), ptr(b_), ptr(x))
+    return x, bool(valid)
+
+
+def newton(dofs, p, target, guess, dtype=np.float64):
+    d = np.ascontiguousarray(np.asarray(dofs, dtype=dtype))
+    t = np.ascontiguousarray(np.asarray(target, dtype=dtype))
+    x = np.ascontiguousarray(np.asarray(guess, dtype=dtype)).copy()
+    steps = np.zeros(1, np.int32)
+    fn = lib().orc_newton_f64 if dtype == np.float64 else lib().orc_newton_f32
+    ok = fn(ptr(d), p, ptr(t), ptr(x), ptr(steps))
+    return bool(ok), x, int(steps[0])
+
+
+def camera_perspective(w, h, fov_y, pos, look_at, up=(0.0, 1.0, 0.0), zoom=1.0):
+    from devil_ray_amd.synthetic import RAY_DTYPE
+    out = np.zeros(w * h, dtype=RAY_DTYPE)
+    pos_ = np.asarray(pos, np.float32)
+    la = np.asarray(look_at, np.float32)
+    up_ = np.asarray(up, np.float32)
+    lib().orc_camera_perspective(w, h, float(fov_y), ptr(pos_), ptr(la), ptr(up_), float(zoom), ptr(out))
+    return out
+
+
+def remove_missed_rays(rays, bounds6):
+    out = np.zeros_like(rays)
+    b = np.ascontiguousarray(np.asarray(bounds6, np.float32))
+    k = lib().orc_remove_missed_rays(ptr(rays), int(rays.shape[0]), ptr(b), ptr(out))
+    return out[:k].copy()
+
+
+def sample_dist(bounds6, samples):
+    b = np.ascontiguousarray(np.asarray(bounds6, np.float32))
+    return float(lib().orc_sample_dist(ptr(b), int(samples)))
+
+
+def color_table_sample(rgb_pts, alpha_pts, smooth=True, factor=-1.0, num_samples=1024, alpha_scale=1.0):
+    rgb = np.ascontiguousarray(np.asarray(rgb_pts, np.float32).reshape(-1, 5))
+    al = np.ascontiguousarray(np.asarray(alpha_pts, np.float32).reshape(-1, 2))
+    out = np.zeros((num_samples, 4), np.float32)
+    lib().orc_color_table_sample(ptr(rgb), rgb.shape[0], ptr(al), al.shape[0], int(bool(smooth)), float(factor),
+                                 num_samples, float(alpha_scale), ptr(out))
+    return out
+
+
+def integrate(mesh, bvh, active_rays, sample_dist_, lut, fmin, fmax, dtype=np.float32, nthreads=0):
+    """Returns (partials[5*n] uncompacted, per-ray located counts, stats[located, newton, candidates])."""
+    from devil_ray_amd.synthetic import PARTIAL_DTYPE
+    n = int(active_rays.shape[0])
+    ctrl, vals, orders, offs = _mesh_args(mesh, dtype)
+    fctrl, fvals, _, foffs = _mesh_args(mesh, dtype, field=True)
+    parts = np.zeros(5 * n, dtype=PARTIAL_DTYPE)
+    located = np.zeros(n, np.int32)
+    stats = np.zeros(3, np.int64)
+    lut_ = np.ascontiguousarray(lut.astype(np.float32))
+    rays = np.ascontiguousarray(active_rays)
+    fn = lib().orc_integrate_f32 if dtype == np.float32 else lib().orc_integrate_f64
+    fn(ptr(ctrl), ptr(vals), ptr(fctrl), ptr(fvals), ptr(orders), ptr(offs), ptr(foffs), mesh.order, mesh.n_el,
+       ptr(bvh.inner), ptr(bvh.leaf_nodes), ptr(bvh.aabb_ids), ptr(bvh.subrefs), ptr(rays), n, float(sample_dist_),
+       ptr(lut_), lut_.shape[0], float(fmin), float(fmax), ptr(parts), ptr(located), ptr(stats), nthreads)
+    return parts, located, stats
+
+
+def compact_partials(parts):
+    out = np.zeros_like(parts)
+    k = lib().orc_compact_partials(ptr(parts), int(parts.shape[0]), ptr(out))
+    return out[:k].copy()
+
+
+def composite(parts, w, h):
+    out = np.zeros((w * h, 4), np.float32)
+    p = np.ascontiguousarray(parts)
+    lib().orc_composite(ptr(p), int(p.shape[0]), w, h, ptr(out))
+    return out
