@@ -36,7 +36,8 @@ class DrxLocateOpts(C.Structure):
 
 
 class DrxIntegrateStats(C.Structure):
-    _fields_ = [("located_samples", I64), ("newton_steps", I64), ("candidates", I64), ("polished", I64)]
+    _fields_ = [("located_samples", I64), ("newton_steps", I64), ("candidates", I64), ("polished", I64),
+                ("kernel_ms", F32), ("total_ms", F32)]
 
 
 class DrxCamera(C.Structure):

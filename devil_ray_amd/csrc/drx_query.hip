@@ -695,7 +695,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE)
     return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: init_mode %d not available", opts->init_mode);
   *out_count = 0;
-  if (stats) *stats = drx_integrate_stats { 0, 0, 0, 0 };
+  if (stats) *stats = drx_integrate_stats { 0, 0, 0, 0, 0.f, 0.f };
   if (n_rays == 0) return DRX_OK;
   if (!active_rays || !out_partials) return fail (DRX_ERR_ARG, "drx_volume_integrate: null rays/output");
   if ((int64_t)n_rays * 5 > (1ll << 31) - 1) return fail (DRX_ERR_ARG, "drx_volume_integrate: too many rays");
@@ -730,6 +730,9 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   a.err = err;
   const dim3 g (nblk (n_rays, 256)), b (256);
   const int P = mesh->elem_order ? 0 : mesh->order;
+  hipEvent_t ev[3];
+  for (int q = 0; q < 3; ++q) DRX_HIP (hipEventCreate (&ev[q]));
+  DRX_HIP (hipEventRecord (ev[0], st));
 #define DRX_INT(PP, PO) hipLaunchKernelGGL ((k_integrate<PP, PO>), g, b, 0, st, a)
   if (a.cfg.polish)
     switch (P)
@@ -751,6 +754,8 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     }
 #undef DRX_INT
   DRX_LAUNCH_CHECK ();
+  DRX_HIP (hipEventRecord (ev[1], st));
+  DRX_LAUNCH_CHECK ();
   hipLaunchKernelGGL (k_flag_partials, dim3 (nblk (n_slots, 256)), dim3 (256), 0, st, slots, n_slots, flags);
   DRX_LAUNCH_CHECK ();
   int32_t total = 0;
@@ -759,9 +764,14 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   hipLaunchKernelGGL (k_compact_partials, dim3 (nblk (n_slots, 256)), dim3 (256), 0, st, slots, n_slots, flags, pos,
                       (Partial *)out_partials);
   DRX_LAUNCH_CHECK ();
+  DRX_HIP (hipEventRecord (ev[2], st));
   unsigned long long hs[8];
   DRX_HIP (hipMemcpyAsync (hs, dstats, sizeof hs, hipMemcpyDeviceToHost, st));
   DRX_HIP (hipStreamSynchronize (st));
+  float k_ms = 0.f, t_ms = 0.f;
+  DRX_HIP (hipEventElapsedTime (&k_ms, ev[0], ev[1]));
+  DRX_HIP (hipEventElapsedTime (&t_ms, ev[0], ev[2]));
+  for (int q = 0; q < 3; ++q) hipEventDestroy (ev[q]);
   if (((int32_t *)(hs + 4))[0])
     return fail (DRX_ERR_OVERFLOW, "drx_volume_integrate: BVH traversal stack (%d entries) overflowed", kStackDepth);
   *out_count = total;
@@ -771,6 +781,8 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     stats->newton_steps = (int64_t)hs[1];
     stats->candidates = (int64_t)hs[2];
     stats->polished = (int64_t)hs[3];
+    stats->kernel_ms = k_ms;
+    stats->total_ms = t_ms;
   }
   return DRX_OK;
 }

@@ -107,6 +107,8 @@ typedef struct drx_integrate_stats
   int64_t newton_steps;       /* Newton iterations over all candidates */
   int64_t candidates;         /* candidate leaves tried */
   int64_t polished;           /* samples that took the FP64 correction */
+  float kernel_ms;            /* duration of the integrate kernel alone (HIP events on `stream`) */
+  float total_ms;             /* the whole call: integrate + partial compaction (HIP events) */
 } drx_integrate_stats;
 
 typedef struct drx_camera
