@@ -1,0 +1,369 @@
+"""CPU tests that pin the oracle (oracle/drx_oracle.cpp) before it is trusted as the parity checker.
+
+Each check is chosen so that a plausible mistake (dropped term, swapped index, wrong sign, transposed
+operand, wrong tie rule) fails: the reference's own known answers (t_dray_external_evals.cpp,
+t_dray_matrix.cpp), closed forms, independent re-derivations and brute force on tiny inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from devil_ray_amd import synthetic as syn
+
+
+# ---------------------------------------------------------------- element evaluation (a13)
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_eval_d_iota_known_answer(orc, p):
+    """t_dray_external_evals.cpp:48 control values iota(100...): the element is the affine map
+    100 + p u + p(p+1) v + p(p+1)^2 w (sum_i i B_i^p(t) = p t), so value and derivatives are exact."""
+    npe = (p + 1) ** 3
+    dofs = np.arange(100, 100 + npe, dtype=np.float64).reshape(-1, 1)
+    g = np.linspace(0, 1, p + 1)
+    pts = [(a, b, c) for a in g for b in g for c in g] + [(0.3, 0.4, 0.9), (0.123, 0.77, 0.5)]
+    for r in pts:
+        v, d = orc.eval_d(dofs, p, r)
+        exp = 100 + p * r[0] + p * (p + 1) * r[1] + p * (p + 1) ** 2 * r[2]
+        assert abs(v[0] - exp) < 1e-9
+        assert np.allclose(d[:, 0], [p, p * (p + 1), p * (p + 1) ** 2], atol=1e-9)
+        v32, d32 = orc.eval_d(dofs, p, r, dtype=np.float32)
+        # reference tolerances 1e-4 (value) / 5e-4 (derivative), plus 8 float32 ulps of the magnitude
+        assert abs(v32[0] - exp) < 1e-4 + 8 * np.spacing(np.float32(exp))
+        assert np.allclose(d32[:, 0], d[:, 0], atol=5e-4)
+
+
+def _bernstein(p, t):
+    return np.array([math.comb(p, i) * t ** i * (1 - t) ** (p - i) for i in range(p + 1)])
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_eval_d_matches_direct_bernstein_sum(orc, p):
+    """Independent tensor Bernstein sum with dof index i + (p+1)(j + (p+1)k) (u fastest) and
+    central finite differences for the derivatives."""
+    rng = np.random.default_rng(p)
+    npe = (p + 1) ** 3
+    dofs = rng.normal(size=(npe, 3))
+    for _ in range(20):
+        r = rng.random(3)
+        v, d = orc.eval_d(dofs, p, r)
+        bu, bv, bw = _bernstein(p, r[0]), _bernstein(p, r[1]), _bernstein(p, r[2])
+        W = np.einsum("k,j,i->kji", bw, bv, bu).reshape(-1)
+        assert np.allclose(v, W @ dofs, atol=1e-12)
+        h = 1e-6
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            fd = (orc.eval_d(dofs, p, r + e)[0] - orc.eval_d(dofs, p, r - e)[0]) / (2 * h)
+            assert np.allclose(d[a], fd, atol=1e-6)
+
+
+# ---------------------------------------------------------------- LUP (a14)
+def test_lup_known_answer(orc):
+    """t_dray_matrix.cpp:31-41: inverse of [[1,0,5],[2,1,6],[3,4,0]] = [[-24,20,-5],[18,-15,4],[5,-4,1]]."""
+    A = np.array([[1, 0, 5], [2, 1, 6], [3, 4, 0]], float)
+    inv = np.array([[-24, 20, -5], [18, -15, 4], [5, -4, 1]], float)
+    for dt, tol in ((np.float64, 1e-12), (np.float32, 1e-4)):
+        cols = [orc.lup_solve(A, e, dtype=dt)[0] for e in np.eye(3)]
+        assert np.allclose(np.stack(cols, axis=1), inv, atol=tol)
+
+
+def test_lup_random_and_pivoting(orc):
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        A = rng.normal(size=(3, 3)) + 3 * np.eye(3)
+        b = rng.normal(size=3)
+        x, valid = orc.lup_solve(A, b)
+        assert valid and np.allclose(x, np.linalg.solve(A, b), atol=1e-10)
+    A = np.array([[0, 2, 1], [1, 0, 0], [0, 0, 3.0]])   # zero leading pivot: needs the row swap
+    x, valid = orc.lup_solve(A, [1, 2, 3])
+    assert valid and np.allclose(A @ x, [1, 2, 3])
+    x, valid = orc.lup_solve(np.zeros((3, 3)), [1, 2, 3])
+    assert not valid                                     # pivot below epsilon<T>() flags the solve
+
+
+# ---------------------------------------------------------------- Morton / clz (a4, a6)
+def _interleave(x, y, z):
+    c = 0
+    for b in range(10):
+        c |= ((x >> b) & 1) << (3 * b) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b + 2)
+    return c
+
+
+def test_morton_known_answers(orc):
+    assert orc.lib().orc_morton_3d(0.0, 0.0, 0.0) == 0
+    assert orc.lib().orc_morton_3d(1.0, 1.0, 1.0) == 0x3FFFFFFF          # clamps to 1023 per axis
+    assert orc.lib().orc_morton_3d(2.0, -1.0, float("nan")) == _interleave(1023, 0, 0)
+    rng = np.random.default_rng(0)
+    for _ in range(500):
+        x, y, z = rng.random(3).astype(np.float32)
+        q = [min(max(int(np.float32(v) * np.float32(1024)), 0), 1023) for v in (x, y, z)]
+        assert orc.lib().orc_morton_3d(float(x), float(y), float(z)) == _interleave(*q)
+
+
+def test_clz(orc):
+    rng = np.random.default_rng(1)
+    for x in [0, 1, 2, 3, 0x80000000, 0xFFFFFFFF] + [int(v) for v in rng.integers(0, 2 ** 32, 200)]:
+        assert orc.lib().orc_clz(x) == 32 - int(x).bit_length()
+
+
+# ---------------------------------------------------------------- LBVH (a3-a9)
+def _rand_boxes(n, seed, dup=True):
+    rng = np.random.default_rng(seed)
+    lo = rng.random((n, 3), dtype=np.float32)
+    hi = lo + rng.random((n, 3), dtype=np.float32) * np.float32(0.05)
+    a = np.zeros((n, 6), np.float32)
+    a[:, 0::2], a[:, 1::2] = lo, hi
+    if dup and n > 6:
+        a[4] = a[2]
+        a[6] = a[2]
+    return a
+
+
+def _karras_bruteforce(codes):
+    """Recursive split of sorted keys at the highest differing bit of (code, index) -> set of
+    (first, last, split) per inner node; Karras' construction yields the same partition."""
+    keys = [(int(c) << 32) | i for i, c in enumerate(codes)]
+    out = []
+
+    def rec(a, b):
+        if a == b:
+            return
+        hb = (keys[a] ^ keys[b]).bit_length()
+        s = a
+        for k in range(a, b):
+            if (keys[k + 1] ^ keys[a]).bit_length() == hb:
+                s = k
+                break
+        out.append((a, b, s))
+        rec(a, s)
+        rec(s + 1, b)
+
+    rec(0, len(keys) - 1)
+    return sorted(out)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 17, 64, 300])
+def test_bvh_invariants_and_bruteforce(orc, n):
+    a = _rand_boxes(n, n)
+    prims = np.arange(n, dtype=np.int32) * 3
+    b = orc.bvh_build(a, prims)
+    codes = b.mcodes_unsorted
+    # sort: non-decreasing, ties in index order (stable), permutation
+    assert np.all(np.diff(b.mcodes_sorted.astype(np.int64)) >= 0)
+    assert np.array_equal(b.aabb_ids, np.lexsort((np.arange(n), codes)))
+    assert np.array_equal(b.leaf_nodes, prims[b.aabb_ids])
+    # bounds
+    assert np.array_equal(b.bounds[0::2], a[:, 0::2].min(axis=0)) and np.array_equal(b.bounds[1::2], a[:, 1::2].max(axis=0))
+    # topology: every node but the root has exactly one parent, and the tree equals the brute force split
+    inner = n - 1
+    par = b.parent
+    assert par[0] == -1 and np.all(par[1:] >= 0)
+    kids = np.concatenate([b.left, b.right])
+    assert sorted(kids.tolist()) == list(range(1, 2 * n - 1))
+
+    def span(node):
+        if node >= inner:
+            return node - inner, node - inner
+        return span(b.left[node])[0], span(b.right[node])[1]
+
+    got = sorted((span(i)[0], span(i)[1], span(b.left[i])[1]) for i in range(inner))
+    assert got == _karras_bruteforce(b.mcodes_sorted)
+    # refit + emit: node boxes are the exact union of their leaves' boxes
+    sa = a[b.aabb_ids]
+    for i in range(inner):
+        for side, child in ((0, b.left[i]), (6, b.right[i])):
+            s0, s1 = span(child)
+            box = b.inner[i, side:side + 6]
+            assert np.array_equal(box[:3], sa[s0:s1 + 1, 0::2].min(axis=0))
+            assert np.array_equal(box[3:], sa[s0:s1 + 1, 1::2].max(axis=0))
+        ci = b.inner[i, 12:14].view(np.int32)
+        for c, ch in zip(ci, (b.left[i], b.right[i])):
+            assert c == (-(ch - inner + 1) if ch >= inner else 4 * ch)
+
+
+def test_bvh_zero_and_one_box(orc):
+    b0 = orc.bvh_build(np.zeros((0, 6), np.float32), np.zeros(0, np.int32))
+    assert b0.inner.shape == (1, 16) and list(b0.leaf_nodes) == [0, 0]
+    assert np.all(np.isinf(b0.bounds)) and b0.bounds[0] > 0        # empty bounds stay (inf, -inf)
+    a = np.array([[0.1, 0.2, 0.3, 0.4, 0.5, 0.6]], np.float32)
+    b1 = orc.bvh_build(a, np.array([7], np.int32))
+    assert list(b1.leaf_nodes) == [7, 7]
+    assert np.array_equal(b1.bounds, np.array([0.0, 0.2, 0.0, 0.4, 0.0, 0.6], np.float32))  # padded with the zero box
+
+
+# ---------------------------------------------------------------- element split (a2)
+@pytest.mark.parametrize("p,amp", [(1, 0.0), (2, 0.05), (3, 0.1), (4, 0.25)])
+def test_split_partition_and_containment(orc, p, amp):
+    mesh = syn.hex_mesh(2, p, amp, 0.02, 9)
+    aabbs, prims, subrefs = orc.split_elements(mesh)
+    assert aabbs.shape == (8 * 33, 6) and np.array_equal(prims, np.repeat(np.arange(8), 33))
+    rng = np.random.default_rng(p)
+    for e in range(8):
+        sr = subrefs[33 * e:33 * (e + 1)].astype(np.float64)
+        vol = np.prod(sr[:, 3:] - sr[:, :3], axis=1)
+        assert abs(vol.sum() - 1.0) < 1e-12 and np.all(vol > 0)             # the 33 subrefs tile [0,1]^3
+        dofs = mesh.values[mesh.elem_dofs(e)].astype(np.float64)
+        for k in range(33):
+            box = aabbs[33 * e + k]
+            for _ in range(6):
+                r = sr[k, :3] + rng.random(3) * (sr[k, 3:] - sr[k, :3])
+                x = orc.eval_d(dofs, p, r)[0]
+                assert np.all(x >= box[0::2] - 1e-6) and np.all(x <= box[1::2] + 1e-6)
+
+
+def test_split_affine_identity_element(orc):
+    """A lattice control net is the identity map: every sub-box is its subref (x1.000001 about the centre)."""
+    mesh = syn.hex_mesh(1, 3, 0.0, 0.0, 0)
+    aabbs, _, subrefs = orc.split_elements(mesh)
+    lo, hi = subrefs[:, :3].astype(np.float64), subrefs[:, 3:].astype(np.float64)
+    c, half = (lo + hi) / 2, (hi - lo) / 2 * 1.000001
+    assert np.allclose(aabbs[:, 0::2], c - half, atol=1e-7) and np.allclose(aabbs[:, 1::2], c + half, atol=1e-7)
+    # first split is along u (ties -> lowest axis), lower half stays in slot 0
+    assert np.array_equal(subrefs[1], [0.5, 0, 0, 1, 1, 1]) or subrefs[1][0] == 0.5
+
+
+# ---------------------------------------------------------------- locate (a10-a12, a17)
+def test_locate_affine_closed_form(orc):
+    n = 4
+    mesh = syn.hex_mesh(n, 2, 0.0, 0.0, 0)
+    bvh = orc.build_mesh_bvh(mesh)
+    rng = np.random.default_rng(3)
+    pts = (0.002 + 0.996 * rng.random((3000, 3))).astype(np.float32)
+    for dt in (np.float32, np.float64):
+        cells, refs = orc.locate(mesh, bvh, pts, dtype=dt)
+        q = pts.astype(np.float64) * n
+        idx = np.floor(q).astype(int)
+        frac = q - idx
+        interior = np.all((frac > 2e-4) & (frac < 1 - 2e-4), axis=1)
+        exp_cell = idx[:, 0] + n * (idx[:, 1] + n * idx[:, 2])
+        assert np.array_equal(cells[interior], exp_cell[interior])
+        assert np.abs(refs[interior] - frac[interior]).max() < (1e-5 if dt == np.float32 else 1e-9)
+    out = np.array([[1.5, 0.5, 0.5], [-0.1, 0.2, 0.2]], np.float32)
+    c, r = orc.locate(mesh, bvh, out)
+    assert np.all(c == -1) and np.all(r == -1)
+
+
+@pytest.mark.parametrize("make", [lambda: syn.hex_mesh(3, 2, 0.05, 0.02, 4),
+                                  lambda: syn.hex_mesh(2, 4, 0.25, 0.01, 2),
+                                  lambda: syn.mixed_hex_mesh(3, 0.15, 0.05, 8)])
+def test_locate_bvh_loses_no_hits(orc, make):
+    """Brute force over every element (no BVH) finds a containing element exactly when the BVH
+    traversal does, and both reference points map back to the query point."""
+    mesh = make()
+    bvh = orc.build_mesh_bvh(mesh)
+    pts = syn.query_points(800, bvh.bounds, seed=5)
+    cells, refs = orc.locate(mesh, bvh, pts, dtype=np.float64)
+    bc, br = orc.locate_brute_f64(mesh, pts)
+    assert np.array_equal(cells >= 0, bc >= 0)
+    for i in np.nonzero(cells >= 0)[0]:
+        e = int(cells[i])
+        dofs = mesh.values[mesh.elem_dofs(e)].astype(np.float64)
+        p = mesh.order if mesh.orders is None else int(mesh.orders[e])
+        x = orc.eval_d(dofs, p, refs[i])[0]
+        assert np.abs(x - pts[i]).max() < 1e-6
+
+
+def test_refine_is_exact_inverse(orc):
+    mesh = syn.hex_mesh(3, 3, 0.1, 0.01, 2)
+    bvh = orc.build_mesh_bvh(mesh)
+    pts = syn.query_points(500, bvh.bounds, seed=1)
+    c, r = orc.locate(mesh, bvh, pts)
+    ex = orc.refine_f64(mesh, pts, c, r.astype(np.float64))
+    for i in np.nonzero(c >= 0)[0][:100]:
+        dofs = mesh.values[mesh.elem_dofs(int(c[i]))].astype(np.float64)
+        assert np.abs(orc.eval_d(dofs, 3, ex[i])[0] - pts[i]).max() < 1e-13
+
+
+# ---------------------------------------------------------------- colour table, blend, volume (a18-a25)
+def test_color_table_closed_form(orc):
+    rgb = syn.grey_to_red_table()
+    a = syn.table_alpha_for_corrected(0.02, 200)
+    lut = orc.color_table_sample(rgb, [[0.0, a]], smooth=True, factor=10.0 / 200)
+    s = np.arange(1024, dtype=np.float64) / 1023
+    assert np.allclose(lut[:, 0], 0.5 + 0.5 * s, atol=1e-6)
+    assert np.allclose(lut[:, 1], 0.5 - 0.5 * s, atol=1e-6) and np.allclose(lut[:, 2], lut[:, 1])
+    assert np.allclose(lut[:, 3], 0.02, atol=1e-6)
+    step = orc.color_table_sample(rgb, [[0.0, 0.5]], smooth=False)   # dray default: nearest of two
+    assert np.array_equal(step[:512, 0], np.full(512, 0.5, np.float32)) and np.all(step[512:, 0] == 1.0)
+
+
+def _cube_identity(n):
+    return syn.hex_mesh(n, 1, 0.0, 0.0, 0)
+
+
+def test_volume_constant_field_closed_form(orc):
+    """Constant field: every sample has the same colour c, so alpha follows a_{k+1} = a_k + c_a (1 - a_k)
+    in float32; early termination happens on the first sample whose blend reaches alpha >= 0.95."""
+    mesh = _cube_identity(2)
+    mesh.field[:] = 0.25
+    bvh = orc.build_mesh_bvh(mesh)
+    samples = 200
+    a = syn.table_alpha_for_corrected(0.02, samples)
+    lut = orc.color_table_sample(syn.grey_to_red_table(), [[0.0, a]], True, 10.0 / samples)
+    rays = syn.orthographic_rays(4, 4)
+    act = orc.remove_missed_rays(rays, bvh.bounds)
+    parts, located, _ = orc.integrate(mesh, bvh, act, orc.sample_dist(bvh.bounds, samples), lut, 0.0, 1.0)
+    parts = orc.compact_partials(parts)
+    c = lut[int(np.float32(0.25) * np.float32(1023))]
+    f = np.zeros(4, np.float32)
+    for k in range(int(located[0])):
+        alpha = np.float32(c[3] * np.float32(1 - f[3]))
+        f[:3] = f[:3] + c[:3] * alpha
+        f[3] = alpha + f[3]
+    assert len(parts) == 16 and np.all(located == located[0])
+    assert np.array_equal(parts["rgba"][0], f)
+    # the cube is sqrt(3)/200 * k deep: the ray crosses 1.0 in depth with the step of the diagonal
+    step = orc.sample_dist(bvh.bounds, samples)
+    assert abs(int(located[0]) - 1.0 / step) <= 2
+
+
+def test_volume_early_termination(orc):
+    mesh = _cube_identity(2)
+    mesh.field[:] = 0.5
+    bvh = orc.build_mesh_bvh(mesh)
+    samples = 2000          # small step: the ray would cross ~1155 samples without the alpha cut-off
+    a = 0.95                 # table alpha, corrected to 1-(0.05)^(1/200) ~ 0.0149 per sample
+    lut = orc.color_table_sample(syn.grey_to_red_table(), [[0.0, a]], True, 10.0 / samples)
+    rays = syn.orthographic_rays(2, 2)
+    act = orc.remove_missed_rays(rays, bvh.bounds)
+    parts, located, _ = orc.integrate(mesh, bvh, act, orc.sample_dist(bvh.bounds, samples), lut, 0.0, 1.0)
+    parts = orc.compact_partials(parts)
+    ca = lut[511, 3]
+    k = 0
+    al = np.float32(0)
+    while al < np.float32(0.95):
+        al = np.float32(ca * np.float32(1 - al)) + al
+        k += 1
+    assert len(parts) == 4 and np.all(parts["rgba"][:, 3] >= 0.95)
+    assert np.all(located == k + 1)          # one more locate happens after the last blend (volume.cpp:199-204)
+
+
+def test_composite_known_answer(orc):
+    p = np.zeros(3, dtype=syn.PARTIAL_DTYPE)
+    p["pixel_id"] = [0, 0, 2]
+    p["rgba"] = [[0.2, 0.1, 0.0, 0.5], [0.4, 0.4, 0.4, 0.5], [0.1, 0.1, 0.1, 0.25]]
+    img = orc.composite(p, 2, 2)
+    # pixel 0: front (0.2,0.1,0,.5) + back*(1-.5) -> (0.4,0.3,0.2,0.75), then white under it
+    assert np.allclose(img[0], [0.4 + 0.25, 0.3 + 0.25, 0.2 + 0.25, 1.0])
+    assert np.allclose(img[1], [1, 1, 1, 1]) and np.allclose(img[3], [1, 1, 1, 1])
+    assert np.allclose(img[2], [0.1 + 0.75, 0.1 + 0.75, 0.1 + 0.75, 1.0])
+
+
+def test_camera_rays(orc):
+    r = orc.camera_perspective(64, 64, 45.0, (0.5, 0.5, 3.0), (0.5, 0.5, 0.5))
+    d = r["dir"].astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-6)
+    assert np.array_equal(r["pixel_id"], np.arange(64 * 64))
+    c = d[32 * 64 + 32]
+    assert c[2] < -0.9999 and abs(c[0]) < 1e-6 and abs(c[1]) < 1e-6     # centre pixel looks down -z
+    assert d[0, 0] < 0 and d[0, 1] < 0 and d[63, 0] > 0                # i -> +x (right), j -> +y
+    half = math.degrees(math.atan2(-d[32 * 64, 0], -d[32 * 64, 2]))
+    assert abs(half - 22.5) < 1.0
+
+
+def test_remove_missed_rays(orc):
+    rays = syn.orthographic_rays(4, 1)
+    rays["orig"][:, 0] = [-0.5, 0.25, 0.75, 1.5]
+    act = orc.remove_missed_rays(rays, [0, 1, 0, 1, 0, 1])
+    assert list(act["pixel_id"]) == [1, 2]
+    assert np.allclose(act["near"], 1.0 - 0.0005, atol=1e-6) and np.allclose(act["far"], 2.0 + 0.0005, atol=1e-6)
