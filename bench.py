@@ -54,6 +54,8 @@ def main() -> int:
     ap.add_argument("--polish", action="store_true", help="FP32 Newton + FP64 correction where cond(J) > 1e4")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one device copy of 1 GiB first (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -66,6 +68,12 @@ def main() -> int:
     from devil_ray_amd import api, synthetic as syn
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     n, p, res, samples = args.n, args.order, args.res, args.samples
     t0 = time.time()
     mesh = syn.hex_mesh(n, p, 0.1, 0.01, 1234)

@@ -392,12 +392,14 @@ __device__ __forceinline__ int locate_point (const LocateCfg &cfg, TravStack &to
 {
   int cur = 0, sp = 0;
   todo.set (0, kBarrier);
-  while (cur != kBarrier)
+  for (;;)
   {
-    if (cur > -1)
+    // Walk inner nodes until this lane holds a leaf (or the stack is exhausted).  Lanes that reach a
+    // leaf early wait here, so the Newton solve below runs with the wave converged.
+    while (cur > -1)
     {
       const float4 *nd = cfg.nodes + cur;
-      const float4 a = nd[0], b = nd[1], c = nd[2];
+      const float4 a = nd[0], b = nd[1], c = nd[2], ch = nd[3];
       bool in_l = true, in_r = true;
       if (pt[0] < a.x) in_l = false;
       if (pt[1] < a.y) in_l = false;
@@ -418,7 +420,6 @@ __device__ __forceinline__ int locate_point (const LocateCfg &cfg, TravStack &to
       }
       else
       {
-        const float4 ch = nd[3];
         const int lc = __float_as_int (ch.x), rc = __float_as_int (ch.y);
         cur = in_l ? lc : rc;
         if (in_l && in_r)
@@ -432,29 +433,26 @@ __device__ __forceinline__ int locate_point (const LocateCfg &cfg, TravStack &to
         }
       }
     }
+    if (cur == kBarrier) return -1;
+    const int leaf = -cur - 1;
+    const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
+    const int el = __float_as_int (l0.x);
+    float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
+    bool found;
+    if constexpr (P == 0)
+      found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
     else
+      found = try_element_at<P, POLISH> (cfg, cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, pt, x, st);
+    if (found)
     {
-      const int leaf = -cur - 1;
-      const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
-      const int el = __float_as_int (l0.x);
-      float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
-      bool found;
-      if constexpr (P == 0)
-        found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
-      else
-        found = try_element_at<P, POLISH> (cfg, cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, pt, x, st);
-      if (found)
-      {
-        ref[0] = x[0];
-        ref[1] = x[1];
-        ref[2] = x[2];
-        return el;
-      }
-      cur = todo.get (sp);
-      --sp;
+      ref[0] = x[0];
+      ref[1] = x[1];
+      ref[2] = x[2];
+      return el;
     }
+    cur = todo.get (sp);
+    --sp;
   }
-  return -1;
 }
 
 } // namespace drx

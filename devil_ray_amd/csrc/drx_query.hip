@@ -357,51 +357,60 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (Integrate
       p0.rgba[0] = p0.rgba[1] = p0.rgba[2] = p0.rgba[3] = 0.f;
       out5[s] = p0;
     }
+    // integrate_partials' per-ray loop as a state machine with one locate() per iteration:
+    //   search: locate at `distance`; on a miss step on; on a hit open a segment and shade the sample;
+    //   inside: locate at the advanced distance; keep shading while inside, before `far` and alpha < 0.95,
+    //           else close the segment (<= 5 per ray); the next search re-locates at the same distance.
     float distance = ray.near_ + a.sample_dist;
     int segment = 0;
+    bool inseg = false;
+    float depth = 0.f;
     float rgba[4] = { 0.f, 0.f, 0.f, 0.f };
-    for (int s = 0; s < 5; ++s)
+    for (;;)
     {
-      bool found = false;
-      int cell = -1;
+      if (!inseg && !(distance < ray.far_)) break;
+      const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
+                            ray_point (ray.orig[2], distance, ray.dir[2]) };
       float ref[3];
-      while (distance < ray.far_ && !found)
+      const int cell = locate_point<P, POLISH> (a.cfg, todo, pt, ref, st, overflow);
+      if (overflow) break;
+      const bool found = cell != -1;
+      if (found) ++located;
+      bool shade = false;
+      if (!inseg)
       {
-        const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
-                              ray_point (ray.orig[2], distance, ray.dir[2]) };
-        cell = locate_point<P, POLISH> (a.cfg, todo, pt, ref, st, overflow);
-        if (cell != -1)
+        if (found)
         {
-          found = true;
-          ++located;
+          inseg = true;
+          depth = distance;
+          rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
+          shade = true;
         }
         else
           distance += a.sample_dist;
       }
-      if (distance >= ray.far_ || overflow) break;
-      const float depth = distance;
-      rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
-      do
+      else if (distance < ray.far_ && found && rgba[3] < 0.95f)
+        shade = true;
+      else
+      {
+        Partial pp;
+        pp.pixel_id = ray.pixel_id;
+        pp.depth = depth;
+        pp.rgba[0] = rgba[0];
+        pp.rgba[1] = rgba[1];
+        pp.rgba[2] = rgba[2];
+        pp.rgba[3] = rgba[3];
+        out5[segment] = pp;
+        ++segment;
+        if (distance >= ray.far_ || rgba[3] > 0.95f || segment == 5) break;
+        inseg = false;
+      }
+      if (shade)
       {
         const float scalar = field_at<P> (a, cell, ref);
         blend_nc (rgba, color_lookup (a.lut, a.lut_size, a.fmin, a.fmax, a.inv_range, scalar));
         distance += a.sample_dist;
-        const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
-                              ray_point (ray.orig[2], distance, ray.dir[2]) };
-        cell = locate_point<P, POLISH> (a.cfg, todo, pt, ref, st, overflow);
-        found = cell != -1;
-        if (found) ++located;
-      } while (distance < ray.far_ && found && rgba[3] < 0.95f);
-      Partial pp;
-      pp.pixel_id = ray.pixel_id;
-      pp.depth = depth;
-      pp.rgba[0] = rgba[0];
-      pp.rgba[1] = rgba[1];
-      pp.rgba[2] = rgba[2];
-      pp.rgba[3] = rgba[3];
-      out5[segment] = pp;
-      ++segment;
-      if (distance >= ray.far_ || rgba[3] > 0.95f || overflow) break;
+      }
     }
   }
   if (overflow) atomicOr (a.err, 1);
