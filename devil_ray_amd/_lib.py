@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdrx.so")
+# DRX_LIB overrides the library path (e.g. to compare builds made with different compile-time options)
+LIB_PATH = os.environ.get("DRX_LIB", os.path.join(_HERE, "libdrx.so"))
 
 P = C.c_void_p
 I32 = C.c_int32
@@ -27,12 +28,12 @@ class DrxMesh(C.Structure):
 
 class DrxBVH(C.Structure):
     _fields_ = [("inner_nodes", P), ("leaf_nodes", P), ("aabb_ids", P), ("subrefs", P), ("leaf_pack", P),
-                ("n_leaves", I32), ("bounds", F32 * 6)]
+                ("wide_nodes", P), ("n_leaves", I32), ("bounds", F32 * 6)]
 
 
 class DrxLocateOpts(C.Structure):
     _fields_ = [("init_mode", I32), ("fp64_polish", I32), ("cond_threshold", F32), ("newton_steps", P),
-                ("candidates", P)]
+                ("candidates", P), ("traversal", I32)]
 
 
 class DrxIntegrateStats(C.Structure):
@@ -56,6 +57,7 @@ _SIGS = {
     "drx_bvh_split_elements": (I32, [P, P, P, P, I32, I32, P, P, P, P]),
     "drx_bvh_build": (I32, [P, P, I32, P, P, P, P, P, P]),
     "drx_bvh_pack_leaves": (I32, [C.POINTER(DrxBVH), P, P]),
+    "drx_bvh_widen": (I32, [C.POINTER(DrxBVH), P, I64, C.POINTER(I64), P]),
     "drx_locate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I64, P, C.POINTER(DrxLocateOpts), P]),
     "drx_field_eval": (I32, [C.POINTER(DrxMesh), P, I64, P, P]),
     "drx_camera_rays": (I32, [C.POINTER(DrxCamera), P, P]),

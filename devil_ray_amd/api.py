@@ -65,6 +65,8 @@ class BVH:
         self.subrefs = subrefs
         self.mcodes = mcodes
         self.leaf_pack = None
+        self.wide_nodes = None
+        self.use_wide = True
 
     @property
     def n_leaves(self) -> int:
@@ -77,6 +79,7 @@ class BVH:
         v.aabb_ids = self.aabb_ids.data_ptr()
         v.subrefs = self.subrefs.data_ptr() if self.subrefs is not None else None
         v.leaf_pack = self.leaf_pack.data_ptr() if self.leaf_pack is not None else None
+        v.wide_nodes = self.wide_nodes.data_ptr() if (self.wide_nodes is not None and self.use_wide) else None
         v.n_leaves = self.n_leaves
         for q in range(6):
             v.bounds[q] = float(self.bounds[q])
@@ -87,6 +90,15 @@ class BVH:
         self.leaf_pack = torch.empty(self.n_leaves * 8, dtype=torch.float32, device=dev)
         v = self.view()
         L.check(L.lib().drx_bvh_pack_leaves(C.byref(v), _ptr(self.leaf_pack), _stream()), "drx_bvh_pack_leaves")
+
+    def widen(self) -> None:
+        """Build the 8-wide traversal copy (drx_bvh_widen): count, allocate, fill."""
+        v = self.view()
+        cnt = C.c_int64(0)
+        L.check(L.lib().drx_bvh_widen(C.byref(v), None, 0, C.byref(cnt), _stream()), "drx_bvh_widen(count)")
+        self.wide_nodes = torch.empty(int(cnt.value) * 64, dtype=torch.float32, device=_dev())
+        L.check(L.lib().drx_bvh_widen(C.byref(v), _ptr(self.wide_nodes), int(cnt.value), C.byref(cnt), _stream()),
+                "drx_bvh_widen")
 
 
 class LinearBVHBuilder:
@@ -157,6 +169,7 @@ class DeviceGridFunction:
 class LocateOptions:
     fp64_polish: bool = False
     cond_threshold: float = 1e4
+    traversal: int = 0          # 0 auto, 1 per lane, 2 wave-coherent
 
     def c(self, steps=None, cands=None) -> L.DrxLocateOpts:
         o = L.DrxLocateOpts()
@@ -165,6 +178,7 @@ class LocateOptions:
         o.cond_threshold = float(self.cond_threshold)
         o.newton_steps = steps.data_ptr() if steps is not None else None
         o.candidates = cands.data_ptr() if cands is not None else None
+        o.traversal = int(self.traversal)
         return o
 
 
@@ -199,6 +213,7 @@ class UnstructuredMesh:
             bvh = LinearBVHBuilder().construct(aabbs, prims)
             bvh.subrefs = subrefs
             bvh.pack_leaves()
+            bvh.widen()
             self._bvh = bvh
         return self._bvh
 

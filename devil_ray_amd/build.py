@@ -21,27 +21,29 @@ SOURCES = {
     "drx_runtime.hip": [],
     "drx_lbvh.hip": ["-ffp-contract=off"],
     "drx_query.hip": [],
+    "drx_wide.hip": [],
 }
 
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose: bool = False, defines=(), out: str = LIB, obj_dir: str = OBJ) -> str:
+    """Compile and link; `defines` are extra -D flags (compile-time tuning knobs), `out` the .so path."""
+    os.makedirs(obj_dir, exist_ok=True)
     procs = []
     objs = []
     for src, extra in SOURCES.items():
-        obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        obj = os.path.join(obj_dir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + COMMON + extra + [f"-D{d}" for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
     failed = []
     for src, cmd, p in procs:
-        out, _ = p.communicate()
-        text = out.decode(errors="replace")
+        outb, _ = p.communicate()
+        text = outb.decode(errors="replace")
         if p.returncode != 0:
             failed.append((src, text))
         elif verbose and text.strip():
@@ -49,11 +51,11 @@ def build(verbose: bool = False) -> str:
     if failed:
         msg = "\n".join(f"--- {s} ---\n{t}" for s, t in failed)
         raise RuntimeError(f"hipcc failed:\n{msg}")
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.check_call(cmd)
-    shutil.move(tmp, LIB)
-    return LIB
+    shutil.move(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

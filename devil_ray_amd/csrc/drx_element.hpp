@@ -8,6 +8,13 @@
 namespace drx
 {
 
+#ifndef DRX_REG_ELEMENT_MAX_ORDER
+#define DRX_REG_ELEMENT_MAX_ORDER 0   // elements up to this order are held in registers during Newton
+#endif
+#ifndef DRX_EVAL_UNROLL_K
+#define DRX_EVAL_UNROLL_K 1           // w-slabs of control points in flight per step of eval_geom
+#endif
+
 template <int P> struct Ord
 {
   static constexpr int N1 = P + 1;
@@ -69,7 +76,7 @@ __device__ __forceinline__ void eval_geom (const float *__restrict__ g, const T 
   bernstein<P, T> (x[2], Bw, dBw);
   T val[3] = { 0, 0, 0 }, du[3] = { 0, 0, 0 }, dv[3] = { 0, 0, 0 }, dw[3] = { 0, 0, 0 };
   const float *X = g + 4;
-#pragma unroll 1
+#pragma unroll DRX_EVAL_UNROLL_K
   for (int k = 0; k < N1; ++k)
   {
     const T bw = pick<N1, T> (Bw, k), dbw = pick<N1, T> (dBw, k);
@@ -90,6 +97,55 @@ __device__ __forceinline__ void eval_geom (const float *__restrict__ g, const T 
         }
       }
       const T w1 = Bv[j] * bw, w2 = dBv[j] * bw, w3 = Bv[j] * dbw;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+      {
+        val[c] += w1 * a[c];
+        du[c] += w1 * b[c];
+        dv[c] += w2 * a[c];
+        dw[c] += w3 * a[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    y[c] = val[c];
+    J[c][0] = du[c];
+    J[c][1] = dv[c];
+    J[c][2] = dw[c];
+  }
+}
+
+// Same evaluation from a register copy of the element (geometry relative to the block origin).
+template <int P, typename T>
+__device__ __forceinline__ void eval_geom_regs (const float (&C)[3][Ord<P>::NPE], const T x[3], T y[3], T J[3][3])
+{
+  constexpr int N1 = Ord<P>::N1;
+  T Bu[N1], dBu[N1], Bv[N1], dBv[N1], Bw[N1], dBw[N1];
+  bernstein<P, T> (x[0], Bu, dBu);
+  bernstein<P, T> (x[1], Bv, dBv);
+  bernstein<P, T> (x[2], Bw, dBw);
+  T val[3] = { 0, 0, 0 }, du[3] = { 0, 0, 0 }, dv[3] = { 0, 0, 0 }, dw[3] = { 0, 0, 0 };
+#pragma unroll
+  for (int k = 0; k < N1; ++k)
+  {
+#pragma unroll
+    for (int j = 0; j < N1; ++j)
+    {
+      T a[3] = { 0, 0, 0 }, b[3] = { 0, 0, 0 };
+#pragma unroll
+      for (int i = 0; i < N1; ++i)
+      {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+          const T cp = (T)C[c][N1 * (j + N1 * k) + i];
+          a[c] += Bu[i] * cp;
+          b[c] += dBu[i] * cp;
+        }
+      }
+      const T w1 = Bv[j] * Bw[k], w2 = dBv[j] * Bw[k], w3 = Bv[j] * dBw[k];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
       {
@@ -236,10 +292,19 @@ struct NewtonOut
 // steps; each clamps x into [0,1], evaluates phi and J, solves J dx = target - phi by LUP, x += dx;
 // converged when |x - prev|inf < 1e-4; accepted when converged and x in [-1e-4, 1+1e-4]^3.
 // `t` is the target relative to the block's origin.
-template <int P>
+template <int P, bool REGS>
 __device__ __forceinline__ bool newton_f32 (const float *__restrict__ g, const float t[3], float x[3], NewtonOut &no,
                                             bool want_cond)
 {
+  // REGS: the element's control points are loaded once into registers and reused by every iteration
+  float C[REGS ? 3 : 1][REGS ? Ord<P>::NPE : 1];
+  if constexpr (REGS)
+  {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < Ord<P>::NPE; ++q) C[c][q] = g[4 + c * Ord<P>::NP + q];
+  }
   const float tol = 1e-4f;
   float prev[3] = { x[0], x[1], x[2] };
   int steps = 0;
@@ -250,7 +315,10 @@ __device__ __forceinline__ bool newton_f32 (const float *__restrict__ g, const f
 #pragma unroll
     for (int i = 0; i < 3; ++i) x[i] = fminf (1.f, fmaxf (x[i], 0.f));
     float y[3], J[3][3];
-    eval_geom<P, float> (g, x, y, J);
+    if constexpr (REGS)
+      eval_geom_regs<P, float> (C, x, y, J);
+    else
+      eval_geom<P, float> (g, x, y, J);
     float dy[3], dx[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) dy[i] = t[i] - y[i];
@@ -308,6 +376,7 @@ __device__ __forceinline__ void polish_f64 (const float *__restrict__ g, const f
 struct LocateCfg
 {
   const float4 *nodes;      // inner nodes, 4 float4 each
+  const float4 *wide;       // 8-wide nodes, 16 float4 each (nullptr: binary traversal)
   const float4 *leaves;     // packed leaves, 2 float4 each
   const float *mesh;        // packed geometry blocks
   const int64_t *offsets;   // mixed order: per-element block offset (nullptr for uniform)
@@ -330,7 +399,7 @@ __device__ __forceinline__ bool try_element_at (const LocateCfg &cfg, const floa
 {
   const float t[3] = { pt[0] - g[0], pt[1] - g[1], pt[2] - g[2] };
   NewtonOut no;
-  const bool found = newton_f32<P> (g, t, x, no, POLISH);
+  const bool found = newton_f32<P, (P <= DRX_REG_ELEMENT_MAX_ORDER)> (g, t, x, no, POLISH);
   st.newton += no.steps;
   st.candidates += 1;
   if constexpr (POLISH)
@@ -453,6 +522,243 @@ __device__ __forceinline__ int locate_point (const LocateCfg &cfg, TravStack &to
     cur = todo.get (sp);
     --sp;
   }
+}
+
+// Same candidate sequence over the 8-wide collapse: a stack entry is (wide node << 8 | remaining hit
+// mask); hit children are visited in slot (= binary left-to-right) order.
+constexpr int32_t kWideEnd = (int32_t)0x80000000;
+
+__device__ __forceinline__ int32_t wide_child (const float4 *__restrict__ wide, int node, int slot)
+{
+  return __float_as_int (((const float *)(wide + 16 * (int64_t)node))[48 + slot]);
+}
+
+template <int P, bool POLISH>
+__device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStack &todo, const float pt[3],
+                                                  float ref[3], LocateStats &st, bool &overflow)
+{
+  int32_t cur = 0;
+  int sp = 0;   // number of stack entries
+  auto pop = [&] () -> int32_t {
+    if (sp == 0) return kWideEnd;
+    const int32_t e = todo.get (sp - 1);
+    const int node = e >> 8;
+    uint32_t m = (uint32_t)e & 0xffu;
+    const int slot = __builtin_ctz (m);
+    m &= m - 1;
+    if (m)
+      todo.set (sp - 1, (node << 8) | (int32_t)m);
+    else
+      --sp;
+    return wide_child (cfg.wide, node, slot);
+  };
+  for (;;)
+  {
+    while (cur >= 0)
+    {
+      const float4 *nd = cfg.wide + 16 * (int64_t)cur;
+      const float4 lx0 = nd[0], lx1 = nd[1], ly0 = nd[2], ly1 = nd[3], lz0 = nd[4], lz1 = nd[5];
+      const float4 hx0 = nd[6], hx1 = nd[7], hy0 = nd[8], hy1 = nd[9], hz0 = nd[10], hz1 = nd[11];
+      const float lx[8] = { lx0.x, lx0.y, lx0.z, lx0.w, lx1.x, lx1.y, lx1.z, lx1.w };
+      const float ly[8] = { ly0.x, ly0.y, ly0.z, ly0.w, ly1.x, ly1.y, ly1.z, ly1.w };
+      const float lz[8] = { lz0.x, lz0.y, lz0.z, lz0.w, lz1.x, lz1.y, lz1.z, lz1.w };
+      const float hx[8] = { hx0.x, hx0.y, hx0.z, hx0.w, hx1.x, hx1.y, hx1.z, hx1.w };
+      const float hy[8] = { hy0.x, hy0.y, hy0.z, hy0.w, hy1.x, hy1.y, hy1.z, hy1.w };
+      const float hz[8] = { hz0.x, hz0.y, hz0.z, hz0.w, hz1.x, hz1.y, hz1.z, hz1.w };
+      uint32_t mask = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+      {
+        // dray's inclusive point-in-box test (device_mesh.hpp:272-288)
+        const bool in = !(pt[0] < lx[k] || pt[1] < ly[k] || pt[2] < lz[k] || pt[0] > hx[k] || pt[1] > hy[k] ||
+                          pt[2] > hz[k]);
+        mask |= (in ? 1u : 0u) << k;
+      }
+      if (mask == 0)
+      {
+        cur = pop ();
+        continue;
+      }
+      const int slot = __builtin_ctz (mask);
+      mask &= mask - 1;
+      if (mask)
+      {
+        if (sp >= kStackDepth)
+        {
+          overflow = true;
+          return -1;
+        }
+        todo.set (sp++, (cur << 8) | (int32_t)mask);
+      }
+      cur = wide_child (cfg.wide, cur, slot);
+    }
+    if (cur == kWideEnd) return -1;
+    const int leaf = -cur - 1;
+    const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
+    const int el = __float_as_int (l0.x);
+    float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
+    bool found;
+    if constexpr (P == 0)
+      found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
+    else
+      found = try_element_at<P, POLISH> (cfg, cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, pt, x, st);
+    if (found)
+    {
+      ref[0] = x[0];
+      ref[1] = x[1];
+      ref[2] = x[2];
+      return el;
+    }
+    cur = pop ();
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Wave-coherent locate over the 8-wide BVH.  All active lanes of a wavefront call it together.  The
+// wave walks the union of its lanes' depth-first paths with wave-uniform node reads; at every wide node
+// each lane tests its own point and a per-slot ballot keeps, for each child, exactly the lanes whose
+// own traversal would enter it.  Reaching a leaf appends it to the queue (LDS) of every lane in its mask
+// that is still searching, so each lane's queue holds its own depth-first candidate sequence.  Queues are
+// drained (per-lane Newton, first accept wins) when every searching lane holds a candidate or a queue is
+// full; lanes that accepted drop out of all pending masks and the walk resumes.
+// ---------------------------------------------------------------------------------------------
+constexpr int kWaveStack = 96;   // pending (child, lane mask) entries per wave
+// Wave-uniform node reads go through the constant address space so they become scalar loads.
+typedef const float __attribute__ ((address_space (4))) cfloat;
+constexpr int kCandQ = 4;        // queued candidate leaves per lane
+
+struct WaveScratch
+{
+  int32_t *st_code;    // kWaveStack entries of this wave
+  uint64_t *st_mask;   // kWaveStack entries of this wave
+  int32_t *cand;       // kCandQ x 64, this wave's queue: cand[q * 64 + lane]
+};
+
+template <int P, bool POLISH>
+__device__ __forceinline__ int locate_wave (const LocateCfg &cfg, const WaveScratch &ws, const float pt[3],
+                                            float ref[3], LocateStats &st, bool &overflow)
+{
+  const int lane = __lane_id ();
+  const uint64_t me = 1ull << lane;
+  uint64_t searching = __ballot (1);
+  int result = -1;
+  int ncand = 0;
+  int sp = 0;
+  int32_t cur = 0;
+  uint64_t curmask = searching;
+
+  // per-lane drain of the candidate queues; updates `searching`
+  auto drain = [&] () {
+    for (int q = 0; q < ncand; ++q)
+    {
+      if (result >= 0) break;
+      const int leaf = ws.cand[q * 64 + lane];
+      const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
+      const int el = __float_as_int (l0.x);
+      float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
+      bool found;
+      if constexpr (P == 0)
+        found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
+      else
+        found = try_element_at<P, POLISH> (cfg, cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, pt, x, st);
+      if (found)
+      {
+        ref[0] = x[0];
+        ref[1] = x[1];
+        ref[2] = x[2];
+        result = el;
+      }
+    }
+    ncand = 0;
+    searching &= __ballot (result < 0);
+  };
+
+  auto pop = [&] () {
+    for (;;)
+    {
+      if (sp == 0)
+      {
+        cur = kWideEnd;
+        return;
+      }
+      --sp;
+      cur = __builtin_amdgcn_readfirstlane (ws.st_code[sp]);
+      const uint64_t m = ws.st_mask[sp];
+      // readfirstlane returns a signed int: widen through uint32_t so bit 31 does not sign-extend
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane ((int)(uint32_t)m);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane ((int)(uint32_t)(m >> 32));
+      curmask = ((uint64_t)hi << 32) | (uint64_t)lo;
+      curmask &= searching;
+      if (curmask) return;
+    }
+  };
+
+  while (cur != kWideEnd && searching)
+  {
+    if (cur >= 0)
+    {
+      cur = __builtin_amdgcn_readfirstlane (cur);
+      cfloat *nd = (cfloat *)((const float *)cfg.wide + 64 * (int64_t)cur);
+      uint64_t b[8];
+      const bool part = (curmask & me) != 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+      {
+        const bool in = part && !(pt[0] < nd[k] || pt[1] < nd[8 + k] || pt[2] < nd[16 + k] || pt[0] > nd[24 + k] ||
+                                  pt[1] > nd[32 + k] || pt[2] > nd[40 + k]);
+        b[k] = __ballot (in);
+      }
+      int first = -1;
+#pragma unroll
+      for (int k = 7; k >= 0; --k)
+        if (b[k]) first = k;
+      if (first < 0)
+      {
+        pop ();
+        continue;
+      }
+#pragma unroll
+      for (int k = 7; k >= 0; --k)
+      {
+        if (k > first && b[k])
+        {
+          if (sp >= kWaveStack)
+          {
+            overflow = true;
+            return -1;
+          }
+          // every active lane writes the same entry, so each lane later reads back its own store
+          ws.st_code[sp] = __float_as_int (nd[48 + k]);
+          ws.st_mask[sp] = b[k];
+          ++sp;
+        }
+      }
+      cur = __float_as_int (nd[48 + first]);
+      curmask = b[first];
+    }
+    else
+    {
+      // leaf: queue it for the lanes that reach it
+      const uint64_t take = curmask & searching;
+      if (take & me) ws.cand[ncand++ * 64 + lane] = -cur - 1;
+      const bool full = __ballot (ncand == kCandQ) != 0;
+      const bool all_have = (__ballot (ncand > 0) & searching) == searching;
+      if (full || all_have) drain ();
+      pop ();
+    }
+  }
+  if (searching) drain ();
+  return result;
+}
+
+template <int P, bool POLISH, bool WIDE>
+__device__ __forceinline__ int locate_any (const LocateCfg &cfg, TravStack &todo, const float pt[3], float ref[3],
+                                           LocateStats &st, bool &overflow)
+{
+  if constexpr (WIDE)
+    return locate_point_wide<P, POLISH> (cfg, todo, pt, ref, st, overflow);
+  else
+    return locate_point<P, POLISH> (cfg, todo, pt, ref, st, overflow);
 }
 
 } // namespace drx

@@ -55,12 +55,15 @@ static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) /
 // ---------------------------------------------------------------------------------------------
 // Batched locate (UnstructuredMesh::locate, unstructured_mesh.cpp:123-148): one lane per point.
 // ---------------------------------------------------------------------------------------------
-template <int P, bool POLISH>
+template <int P, bool POLISH, int MODE>
 __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate (LocateCfg cfg, const float *__restrict__ pts, int64_t n,
                                                   Location *__restrict__ out, int32_t *__restrict__ steps_out,
                                                   int32_t *__restrict__ cand_out, int32_t *__restrict__ err)
 {
-  __shared__ int32_t stack_lds[kStackLds * 256];
+  __shared__ int32_t stack_lds[MODE == 2 ? 1 : kStackLds * 256];
+  __shared__ int32_t w_code[MODE == 2 ? 4 * kWaveStack : 1];
+  __shared__ uint64_t w_mask[MODE == 2 ? 4 * kWaveStack : 1];
+  __shared__ int32_t w_cand[MODE == 2 ? 4 * kCandQ * 64 : 1];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   TravStack todo;
@@ -71,7 +74,15 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate (LocateCfg cf
   float ref[3] = { -1.f, -1.f, -1.f };
   LocateStats st { 0, 0, 0 };
   bool overflow = false;
-  const int cell = locate_point<P, POLISH> (cfg, todo, pt, ref, st, overflow);
+  int cell;
+  if constexpr (MODE == 2)
+  {
+    const int wv = threadIdx.x >> 6;
+    WaveScratch ws { w_code + wv * kWaveStack, w_mask + wv * kWaveStack, w_cand + wv * kCandQ * 64 };
+    cell = locate_wave<P, POLISH> (cfg, ws, pt, ref, st, overflow);
+  }
+  else
+    cell = locate_any<P, POLISH, MODE == 1> (cfg, todo, pt, ref, st, overflow);
   if (overflow) atomicOr (err, 1);
   Location loc;
   loc.cell = cell;
@@ -331,10 +342,22 @@ __device__ __forceinline__ float field_at (const IntegrateArgs &a, int cell, con
     return eval_scalar<P> (a.field + (int64_t)cell * Ord<P>::FSTRIDE, ref);
 }
 
-template <int P, bool POLISH>
+// MODE: 0 binary per-lane traversal, 1 8-wide per-lane, 2 8-wide wave-coherent (identical results).
+template <int P, bool POLISH, int MODE>
 __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (IntegrateArgs a)
 {
-  __shared__ int32_t stack_lds[kStackLds * 256];
+  constexpr bool WAVE = MODE == 2;
+  __shared__ int32_t stack_lds[WAVE ? 1 : kStackLds * 256];
+  __shared__ int32_t w_code[WAVE ? 4 * kWaveStack : 1];
+  __shared__ uint64_t w_mask[WAVE ? 4 * kWaveStack : 1];
+  __shared__ int32_t w_cand[WAVE ? 4 * kCandQ * 64 : 1];
+  WaveScratch ws;
+  {
+    const int wv = threadIdx.x >> 6;
+    ws.st_code = w_code + (WAVE ? wv * kWaveStack : 0);
+    ws.st_mask = w_mask + (WAVE ? wv * kWaveStack : 0);
+    ws.cand = w_cand + (WAVE ? wv * kCandQ * 64 : 0);
+  }
   TravStack todo;
   todo.lds = stack_lds;
   todo.bdim = blockDim.x;
@@ -372,7 +395,11 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (Integrate
       const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
                             ray_point (ray.orig[2], distance, ray.dir[2]) };
       float ref[3];
-      const int cell = locate_point<P, POLISH> (a.cfg, todo, pt, ref, st, overflow);
+      int cell;
+      if constexpr (WAVE)
+        cell = locate_wave<P, POLISH> (a.cfg, ws, pt, ref, st, overflow);
+      else
+        cell = locate_any<P, POLISH, MODE == 1> (a.cfg, todo, pt, ref, st, overflow);
       if (overflow) break;
       const bool found = cell != -1;
       if (found) ++located;
@@ -513,6 +540,7 @@ static LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx_locate
 {
   LocateCfg c;
   c.nodes = (const float4 *)b->inner_nodes;
+  c.wide = (const float4 *)b->wide_nodes;
   c.leaves = (const float4 *)b->leaf_pack;
   c.mesh = m->data;
   c.offsets = m->elem_offset;
@@ -550,7 +578,20 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   const int P = mesh->elem_order ? 0 : mesh->order;
   const dim3 g (nblk (n, 256)), b (256);
   Location *out = (Location *)out_locations;
-#define DRX_LOC(PP, PO) hipLaunchKernelGGL ((k_locate<PP, PO>), g, b, 0, st, cfg, points, n, out, steps, cands, err)
+  const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
+  if (trav == DRX_TRAVERSE_WAVE && !cfg.wide)
+    return fail (DRX_ERR_ARG, "drx_locate: wave traversal needs bvh->wide_nodes");
+  const int mode = !cfg.wide ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
+#define DRX_LOC(PP, PO)                                                                             \
+  do                                                                                                \
+  {                                                                                                 \
+    if (mode == 2)                                                                                  \
+      hipLaunchKernelGGL ((k_locate<PP, PO, 2>), g, b, 0, st, cfg, points, n, out, steps, cands, err); \
+    else if (mode == 1)                                                                             \
+      hipLaunchKernelGGL ((k_locate<PP, PO, 1>), g, b, 0, st, cfg, points, n, out, steps, cands, err); \
+    else                                                                                            \
+      hipLaunchKernelGGL ((k_locate<PP, PO, 0>), g, b, 0, st, cfg, points, n, out, steps, cands, err); \
+  } while (0)
   if (cfg.polish)
     switch (P)
     {
@@ -739,10 +780,23 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   a.err = err;
   const dim3 g (nblk (n_rays, 256)), b (256);
   const int P = mesh->elem_order ? 0 : mesh->order;
+  const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
+  if (trav == DRX_TRAVERSE_WAVE && !a.cfg.wide)
+    return fail (DRX_ERR_ARG, "drx_volume_integrate: wave traversal needs bvh->wide_nodes");
+  const int mode = !a.cfg.wide ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
   hipEvent_t ev[3];
   for (int q = 0; q < 3; ++q) DRX_HIP (hipEventCreate (&ev[q]));
   DRX_HIP (hipEventRecord (ev[0], st));
-#define DRX_INT(PP, PO) hipLaunchKernelGGL ((k_integrate<PP, PO>), g, b, 0, st, a)
+#define DRX_INT(PP, PO)                                              \
+  do                                                                 \
+  {                                                                  \
+    if (mode == 2)                                                   \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 2>), g, b, 0, st, a); \
+    else if (mode == 1)                                              \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 1>), g, b, 0, st, a); \
+    else                                                             \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 0>), g, b, 0, st, a); \
+  } while (0)
   if (a.cfg.polish)
     switch (P)
     {
@@ -780,7 +834,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   float k_ms = 0.f, t_ms = 0.f;
   DRX_HIP (hipEventElapsedTime (&k_ms, ev[0], ev[1]));
   DRX_HIP (hipEventElapsedTime (&t_ms, ev[0], ev[2]));
-  for (int q = 0; q < 3; ++q) hipEventDestroy (ev[q]);
+  for (int q = 0; q < 3; ++q) (void)hipEventDestroy (ev[q]);
   if (((int32_t *)(hs + 4))[0])
     return fail (DRX_ERR_OVERFLOW, "drx_volume_integrate: BVH traversal stack (%d entries) overflowed", kStackDepth);
   *out_count = total;

@@ -78,6 +78,7 @@ typedef struct drx_bvh
   const int32_t *aabb_ids;    /* original aabb index per leaf (m_aabb_ids) */
   const float *subrefs;       /* 6 floats (lo xyz, hi xyz) per ORIGINAL aabb (mesh m_ref_aabbs) */
   const float *leaf_pack;     /* optional, 8 floats per leaf: {el_id bits, lo xyz, hi xyz, 0}; see drx_bvh_pack_leaves */
+  const float *wide_nodes;    /* optional 8-wide collapse (drx_bvh_widen); used for traversal when set */
   int32_t n_leaves;
   float bounds[6];            /* host copy of m_bounds: xmin, xmax, ymin, ymax, zmin, zmax */
 } drx_bvh;
@@ -99,7 +100,15 @@ typedef struct drx_locate_opts
   float cond_threshold;       /* e.g. 1e4 */
   int32_t *newton_steps;      /* device, nullable: Newton steps summed over candidates, per point */
   int32_t *candidates;        /* device, nullable: candidate leaves tried, per point */
+  int32_t traversal;          /* drx_traversal: how a wavefront walks the BVH (same results either way) */
 } drx_locate_opts;
+
+enum drx_traversal
+{
+  DRX_TRAVERSE_AUTO = 0,      /* library default for the entry point */
+  DRX_TRAVERSE_LANE = 1,      /* every lane walks its own depth-first path */
+  DRX_TRAVERSE_WAVE = 2       /* the wavefront walks the union of its lanes' paths with uniform node reads */
+};
 
 typedef struct drx_integrate_stats
 {
@@ -153,6 +162,11 @@ int drx_bvh_build (const float *aabbs, const int32_t *prims, int32_t n, float *o
                    void *stream);
 /* Fused leaf records for traversal: 8 floats per leaf {el_id bits, subref lo xyz, hi xyz, 0}. */
 int drx_bvh_pack_leaves (const drx_bvh *bvh, float *out_leaf_pack, void *stream);
+/* 8-wide traversal copy of the binary BVH: children in binary left-to-right order, exact float boxes,
+ * so traversal visits the same candidate leaves in the same order as dray's binary depth-first walk.
+ * Node = 64 floats: lo_x[8] lo_y[8] lo_z[8] hi_x[8] hi_y[8] hi_z[8] child[8] (int32 bits) pad[8].
+ * out_wide NULL: only count.  *out_count (host) = number of wide nodes.  Synchronises. */
+int drx_bvh_widen (const drx_bvh *bvh, float *out_wide, int64_t capacity_nodes, int64_t *out_count, void *stream);
 
 /* ---- queries ------------------------------------------------------------------------------- */
 int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,

@@ -188,3 +188,47 @@ def test_volume_orthographic_config0_small(gpu, orc):
     assert np.array_equal(gp["pixel_id"], op["pixel_id"])
     d = np.abs(img - oimg)
     assert d.max() < 5e-3 and d.mean() < 1e-4
+
+
+@pytest.mark.parametrize("name,make", MESHES, ids=[m[0] for m in MESHES])
+def test_wide_bvh_same_candidates_as_binary(gpu, name, make):
+    """The 8-wide traversal must try exactly the binary depth-first candidate sequence: identical
+    locations (bit for bit), Newton step counts and candidate counts per point."""
+    mesh = make()
+    m, bvh = _gpu_bvh(gpu, mesh)
+    assert bvh.wide_nodes is not None
+    pts = gpu.to_device(syn.query_points(30000, bvh.bounds, seed=21).reshape(-1))
+    out = {}
+    for wide in (True, False):
+        bvh.use_wide = wide
+        loc, steps, cands = m.locate(pts, stats=True)
+        out[wide] = (_np(loc), _np(steps), _np(cands))
+    bvh.use_wide = True
+    for a, b in zip(out[True], out[False]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name,make", MESHES, ids=[m[0] for m in MESHES])
+def test_wave_traversal_same_as_lane(gpu, name, make):
+    """Wave-coherent traversal (union of the lanes' paths, per-lane candidate queues) must reproduce the
+    per-lane depth-first result exactly, on coherent (ray-ordered) and incoherent (random) points."""
+    mesh = make()
+    m, bvh = _gpu_bvh(gpu, mesh)
+    rnd = syn.query_points(20000, bvh.bounds, seed=23)
+    b = bvh.bounds
+    t = np.linspace(0, 1, 20000, dtype=np.float32)
+    ray = np.stack([b[0] + (b[1] - b[0]) * t, b[2] + (b[3] - b[2]) * (0.3 + 0.4 * t), b[4] + (b[5] - b[4]) * 0.55 + 0 * t], 1)
+    for pts in (rnd, ray.astype(np.float32)):
+        d = gpu.to_device(np.ascontiguousarray(pts).reshape(-1))
+        res = {}
+        for mode in (1, 2):
+            loc, steps, cands = m.locate(d, gpu.LocateOptions(traversal=mode), stats=True)
+            res[mode] = (_np(loc, LOCATION_DTYPE), _np(steps), _np(cands))
+        (l1, s1, c1), (l2, s2, c2) = res[1], res[2]
+        dc = int((l1["cell"] != l2["cell"]).sum())
+        dr = float(np.abs(l1["ref"] - l2["ref"]).max())
+        print(f"  {name}: cells differ {dc}, max ref diff {dr:.3g}, steps differ {int((s1 != s2).sum())}, "
+              f"cands differ {int((c1 != c2).sum())}", flush=True)
+        assert dc == 0 and np.array_equal(c1, c2)
+        # the two entry points inline separately compiled Newton code: references may differ in the last bits
+        assert dr <= 1e-6 and (s1 != s2).sum() <= 1e-4 * len(s1)
