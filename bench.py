@@ -56,7 +56,20 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one device copy of 1 GiB first (reference kernel for HBM counter scaling)")
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 4],
+                    help="0: 8^3 p2 ortho 256^2 200 spp; 1 (default): 64^3 p3 1024^2 512 spp; "
+                         "2: 2^26 batched queries on a 48^3 mixed-order mesh; 4: 128^3 p4 2048^2 1000 spp + FP64 polish")
+    ap.add_argument("--traversal", type=int, default=0, help="0 auto, 1 per lane, 2 wave-coherent")
+    ap.add_argument("--error-sample", type=int, default=0,
+                    help="report ref-coordinate / RGBA error vs the FP64 oracle on this many points / rays/10")
     args = ap.parse_args()
+    if args.config == 0:
+        args.n, args.order, args.res, args.samples = 8, 2, 256, 200
+    elif args.config == 4:
+        args.n, args.order, args.res, args.samples, args.polish = 128, 4, 2048, 1000, True
+        args.error_sample = args.error_sample or 20000
+    elif args.config == 2:
+        return bench_locate(args)
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); rays shard across GPUs "
@@ -76,7 +89,12 @@ def main() -> int:
         del src, dst
     n, p, res, samples = args.n, args.order, args.res, args.samples
     t0 = time.time()
-    mesh = syn.hex_mesh(n, p, 0.1, 0.01, 1234)
+    if args.config == 0:
+        mesh = syn.hex_mesh(n, p, 0.05, 0.005, 1234, jitter_scaled=False)
+    elif args.config == 4:
+        mesh = syn.hex_mesh(n, p, 0.25, 0.01, 1234)
+    else:
+        mesh = syn.hex_mesh(n, p, 0.1, 0.01, 1234)
     t_gen = time.time() - t0
     m = api.UnstructuredMesh(mesh)
     f = api.UnstructuredField(mesh)
@@ -102,13 +120,15 @@ def main() -> int:
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, syn.table_alpha_for_corrected(0.02, samples))
     vol.table.set_smooth(True)
-    vol.opts = api.LocateOptions(fp64_polish=args.polish, cond_threshold=1e4)
+    vol.opts = api.LocateOptions(fp64_polish=args.polish, cond_threshold=1e4, traversal=args.traversal)
     c = bvh.bounds
     centre = (0.5 * (c[0] + c[1]), 0.5 * (c[2] + c[3]), 0.5 * (c[4] + c[5]))
     cam = api.Camera(res, res, 45.0, pos=(centre[0], centre[1], centre[2] + 2.5), look_at=centre)
+    if args.config == 0:
+        ortho = api.as_bytes(syn.orthographic_rays(res, res))
 
     def frame():
-        rays = cam.create_rays()
+        rays = ortho if args.config == 0 else cam.create_rays()
         parts, cnt = vol.integrate(rays, tile_width=res, tile_height=res)
         img = api.composite(parts, cnt, res, res)
         return img, cnt
@@ -147,7 +167,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32" + ("+f64polish" if args.polish else ""),
         "data": "synthetic (seeded warped Bernstein hex mesh; no dataset)",
-        "config": {"workload": f"volume render, order-{p} hex mesh {n}^3 elements, {res}x{res} perspective, "
+        "config": {"workload": f"volume render, order-{p} hex mesh {n}^3 elements, {res}x{res} "
+                               f"{'orthographic' if args.config == 0 else 'perspective'}, "
                                f"{samples} samples/ray, unlit, grey->red TF opacity 0.02",
                    "elements": n ** 3, "order": p, "bvh_leaves": bvh.n_leaves, "width": res, "height": res,
                    "samples": samples, "fp64_polish": bool(args.polish)},
@@ -159,11 +180,15 @@ def main() -> int:
         "partials_per_frame": int(cnt),
         "integrate_kernel_ms": k_ms,
         "bvh_build_ms": {"cold": bvh_ms, "warm": bvh_ms_warm},
+        "bvh_wide_nodes": int(bvh.wide_nodes.numel() // 64) if bvh.wide_nodes is not None else 0,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
                      "kernel": f"k_integrate<{p},{'true' if args.polish else 'false'}>",
                      "bytes_per_unit": bytes_per_sample, "unit_name": "located sample"},
     }
+
+    if args.error_sample:
+        out["error_vs_fp64"] = error_report(args, api, syn, mesh, m, f, vol, bvh, centre, res, samples)
 
     if not args.no_cpu:
         from oracle import orc
@@ -200,6 +225,116 @@ def main() -> int:
                                "mrays_per_s_equiv": cpu_rate / max(1e-9, located_per_frame / (res * res)) / 1e6}
         out["gpu_over_cpu"] = value / cpu_rate
     out["mesh_generation_s"] = t_gen
+    print(json.dumps(out))
+    return 0
+
+
+def _oracle_bvh(bvh):
+    from oracle import orc
+    ob = orc.OracleBVH()
+    ob.inner = bvh.inner_nodes.cpu().numpy().reshape(-1, 16)
+    ob.leaf_nodes = bvh.leaf_nodes.cpu().numpy()
+    ob.aabb_ids = bvh.aabb_ids.cpu().numpy()
+    ob.subrefs = bvh.subrefs.cpu().numpy().reshape(-1, 6)
+    ob.bounds = np.asarray(bvh.bounds, np.float32)
+    return ob
+
+
+def error_report(args, api, syn, mesh, m, f, vol, bvh, centre, res, samples):
+    """Reference-coordinate error of GPU locate (same options as the render) against the FP64 oracle's exact
+    inverse, and per-ray RGBA error of GPU integrate against an FP64 oracle integrate, on sampled inputs."""
+    import torch
+    from oracle import orc
+    from devil_ray_amd.synthetic import LOCATION_DTYPE, PARTIAL_DTYPE
+    ob = _oracle_bvh(bvh)
+    pts = syn.query_points(args.error_sample, bvh.bounds, seed=99)
+    res_ = {}
+    for polish in (False, True):
+        loc = m.locate(api.to_device(pts.reshape(-1)), api.LocateOptions(fp64_polish=polish, cond_threshold=1e4))
+        l = loc.cpu().numpy().view(LOCATION_DTYPE)
+        ok = l["cell"] >= 0
+        ex = orc.refine_f64(mesh, pts[ok], l["cell"][ok], l["ref"][ok].astype(np.float64))
+        err = np.abs(l["ref"][ok].astype(np.float64) - ex).max(axis=1)
+        res_["polish" if polish else "fp32"] = {"max": float(err.max()), "mean": float(err.mean()),
+                                                 "n_over_1e-5": int((err > 1e-5).sum()), "located": int(ok.sum())}
+    oc, _, steps, _ = orc.locate(mesh, ob, pts, dtype=np.float64, want_steps=True)
+    l = m.locate(api.to_device(pts.reshape(-1)), api.LocateOptions(fp64_polish=True)).cpu().numpy().view(LOCATION_DTYPE)
+    res_["located_id_mismatch_vs_fp64_oracle"] = int((l["cell"] != oc).sum())
+    res_["newton_steps_hist_fp64_oracle"] = np.bincount(steps, minlength=21)[:21].tolist()
+    # RGBA on a spread subset of rays
+    rays = orc.camera_perspective(res, res, 45.0, (centre[0], centre[1], centre[2] + 2.5), centre)
+    if args.config == 0:
+        rays = syn.orthographic_rays(res, res)
+    act = orc.remove_missed_rays(rays, ob.bounds)
+    sub = act[:: max(1, len(act) // max(1, args.error_sample // 10))].copy()
+    lut = vol.table.sample(1024, opacity_factor=10.0 / samples)
+    lo, hi = f.range()
+    op, _, _ = orc.integrate(mesh, ob, sub, orc.sample_dist(ob.bounds, samples), lut, lo, hi, dtype=np.float64)
+    op = orc.compact_partials(op)
+    parts, cnt = vol.integrate(api.as_bytes(sub), active=(api.as_bytes(sub), len(sub)))
+    gp = parts.cpu().numpy()[: cnt * 24].view(PARTIAL_DTYPE)
+    oimg = orc.composite(op, res, res)
+    gimg = orc.composite(gp, res, res)
+    pix = sub["pixel_id"]
+    d = np.abs(gimg[pix] - oimg[pix])
+    res_["rgba_vs_fp64"] = {"rays": int(len(sub)), "max": float(d.max()), "mean": float(d.mean())}
+    return res_
+
+
+def bench_locate(args):
+    """Config 2: 2^26 query points uniform in the bounding box of a 48^3 mixed-order (1..4) hex mesh,
+    warp 0.15; UnstructuredMesh::locate per point; value = located queries/s (all queries, hits and misses)."""
+    import torch
+    from devil_ray_amd import api, synthetic as syn
+    from devil_ray_amd.synthetic import LOCATION_DTYPE
+    torch.cuda.set_device(0)
+    n_pts = 1 << 26
+    mesh = syn.mixed_hex_mesh(48, 0.15, 0.01, 1234)
+    m = api.UnstructuredMesh(mesh)
+    bvh = m.get_bvh()
+    torch.cuda.synchronize()
+    pts_h = syn.query_points(n_pts, bvh.bounds, seed=0)
+    pts = api.to_device(pts_h.reshape(-1))
+    variants = {"lane": api.LocateOptions(traversal=1), "morton+lane": api.LocateOptions(traversal=1, coherent_order=True),
+                "morton+wave": api.LocateOptions(traversal=2, coherent_order=True)}
+    timings = {}
+    locs = {}
+    for name, opt in variants.items():
+        for _ in range(args.warmup):
+            m.locate(pts, opt)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            locs[name] = m.locate(pts, opt)
+        torch.cuda.synchronize()
+        timings[name] = (time.perf_counter() - t) / args.steps
+    best = min(timings, key=timings.get)
+    ref = locs["lane"].cpu().numpy().view(LOCATION_DTYPE)
+    same = {k: bool(np.array_equal(locs[k].cpu().numpy(), locs["lane"].cpu().numpy())) for k in locs}
+    value = n_pts / timings[best]
+    out = {
+        "metric": "located queries/s (batched point location)", "value": value, "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000 * timings[best], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded mixed-order warped hex mesh, counter-hash uniform points)",
+        "config": {"workload": "batched locate, 2^26 points, 48^3 mixed-order (1..4) hex mesh, warp 0.15",
+                   "elements": 48 ** 3, "points": n_pts, "bvh_leaves": bvh.n_leaves, "best_variant": best},
+        "ms_per_variant": {k: 1000 * v for k, v in timings.items()}, "variants_identical": same,
+        "hit_fraction": float((ref["cell"] >= 0).mean()),
+        "roofline": {"bound": "hbm", "achieved": value * (28 + 12 * 56) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": value * (28 + 12 * 56) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_unit": 28 + 12 * 56, "unit_name": "query (E[(p+1)^3] = 56 over p ~ U{1..4})"},
+    }
+    if not args.no_cpu:
+        from oracle import orc
+        ob = _oracle_bvh(bvh)
+        sub = pts_h[:: 64].copy()
+        t = time.perf_counter()
+        oc, _ = orc.locate(mesh, ob, sub)
+        tc = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": len(sub) / tc, "unit": "queries/s", "cores": orc.max_threads(), "kind": "port",
+                               "sample": f"every 64th query ({len(sub)} points, {tc:.1f} s)"}
+        out["located_id_mismatch_vs_oracle_sample"] = int((ref["cell"][::64] != oc).sum())
     print(json.dumps(out))
     return 0
 

@@ -33,7 +33,7 @@ class DrxBVH(C.Structure):
 
 class DrxLocateOpts(C.Structure):
     _fields_ = [("init_mode", I32), ("fp64_polish", I32), ("cond_threshold", F32), ("newton_steps", P),
-                ("candidates", P), ("traversal", I32)]
+                ("candidates", P), ("traversal", I32), ("coherent_order", I32)]
 
 
 class DrxIntegrateStats(C.Structure):

@@ -170,6 +170,7 @@ class LocateOptions:
     fp64_polish: bool = False
     cond_threshold: float = 1e4
     traversal: int = 0          # 0 auto, 1 per lane, 2 wave-coherent
+    coherent_order: bool = False  # locate: process queries in Morton order
 
     def c(self, steps=None, cands=None) -> L.DrxLocateOpts:
         o = L.DrxLocateOpts()
@@ -179,6 +180,7 @@ class LocateOptions:
         o.newton_steps = steps.data_ptr() if steps is not None else None
         o.candidates = cands.data_ptr() if cands is not None else None
         o.traversal = int(self.traversal)
+        o.coherent_order = 1 if self.coherent_order else 0
         return o
 
 

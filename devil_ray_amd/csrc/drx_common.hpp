@@ -89,4 +89,12 @@ inline int npe_pad_of (int p) { return (npe_of (p) + 3) & ~3; }
 // Exclusive scan of int32 flags (device), n elements; writes scan[n] and *total (host).
 int exclusive_scan_i32 (const int32_t *in, int32_t *out, int32_t n, int32_t *total_host, hipStream_t st);
 
+// Stable LSD radix sort of (key, value) pairs over the low `bits` bits (8-bit digits); ping-pongs between
+// A and B; *keys_out / *vals_out point at whichever holds the result.
+int radix_sort_pairs (uint32_t *kA, int32_t *vA, uint32_t *kB, int32_t *vB, int32_t *hist, int n, int bits,
+                      hipStream_t st, uint32_t **keys_out, int32_t **vals_out);
+int radix_tiles (int n);
+// Permutation of n query points in Morton order over bounds6 (host); lives in the arena's sort slots.
+int morton_order_points (const float *pts, int n, const float *bounds6, hipStream_t st, const int32_t **order_out);
+
 } // namespace drx

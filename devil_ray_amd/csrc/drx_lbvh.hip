@@ -597,6 +597,78 @@ __global__ void k_pack_leaves (const int32_t *__restrict__ leaf_nodes, const int
 
 static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+int radix_tiles (int n) { return (n + kSortTile - 1) / kSortTile; }
+
+int radix_sort_pairs (uint32_t *kA, int32_t *vA, uint32_t *kB, int32_t *vB, int32_t *hist, int n, int bits,
+                      hipStream_t st, uint32_t **keys_out, int32_t **vals_out)
+{
+  const int n_tiles = radix_tiles (n);
+  uint32_t *ks = kA, *kd = kB;
+  int32_t *vs = vA, *vd = vB;
+  int32_t *offs = hist + 256 * (size_t)n_tiles;
+  for (int shift = 0; shift < bits; shift += 8)
+  {
+    hipLaunchKernelGGL (k_radix_hist, dim3 (n_tiles), dim3 (kSortBlock), 0, st, ks, n, shift, n_tiles, hist);
+    DRX_LAUNCH_CHECK ();
+    int rc = exclusive_scan_i32 (hist, offs, 256 * n_tiles, nullptr, st);
+    if (rc != DRX_OK) return rc;
+    hipLaunchKernelGGL (k_radix_scatter, dim3 (n_tiles), dim3 (kSortBlock), 0, st, ks, vs, n, shift, n_tiles, offs, kd,
+                        vd);
+    DRX_LAUNCH_CHECK ();
+    std::swap (ks, kd);
+    std::swap (vs, vd);
+  }
+  *keys_out = ks;
+  *vals_out = vs;
+  return DRX_OK;
+}
+
+// Morton codes of query points in the normalised frame of `bounds` (same quantisation as the LBVH).
+__global__ void k_point_morton (const float *__restrict__ pts, int n, float minx, float miny, float minz, float ix,
+                                float iy, float iz, uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float mins[3] = { minx, miny, minz }, inv[3] = { ix, iy, iz };
+  uint32_t q[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    float x = (pts[3 * (int64_t)i + d] - mins[d]) * inv[d];
+    x = fminf (fmaxf (x * 1024.0f, 0.0f), 1023.0f);
+    q[d] = (uint32_t)x;
+  }
+  keys[i] = (expand_bits32 (q[2]) << 2) | (expand_bits32 (q[1]) << 1) | expand_bits32 (q[0]);
+  vals[i] = i;
+}
+
+int morton_order_points (const float *pts, int n, const float *bounds6, hipStream_t st, const int32_t **order_out)
+{
+  const int n_tiles = radix_tiles (n);
+  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n);
+  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n);
+  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n);
+  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n);
+  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8));
+  if (!kA || !kB || !vA || !vB || !hist) return DRX_ERR_HIP;
+  float minc[3], inv[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    minc[d] = bounds6[2 * d];
+    const float ext = bounds6[2 * d + 1] - bounds6[2 * d];
+    inv[d] = (ext > 0.f) ? 1.f / ext : 0.f;
+  }
+  hipLaunchKernelGGL (k_point_morton, dim3 (nblk (n, 256)), dim3 (256), 0, st, pts, n, minc[0], minc[1], minc[2], inv[0],
+                      inv[1], inv[2], kA, vA);
+  DRX_LAUNCH_CHECK ();
+  uint32_t *ks;
+  int32_t *vs;
+  int rc = radix_sort_pairs (kA, vA, kB, vB, hist, n, 30, st, &ks, &vs);
+  if (rc) return rc;
+  *order_out = vs;
+  return DRX_OK;
+}
+
 } // namespace drx
 
 using namespace drx;
@@ -725,20 +797,11 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
                       inv[1], inv[2], kA, vA);
   DRX_LAUNCH_CHECK ();
   // stable radix sort, 8-bit digits over 30-bit keys
-  uint32_t *ks = kA, *kd = kB;
-  int32_t *vs = vA, *vd = vB;
-  int32_t *offs = hist + 256 * (size_t)n_tiles;
-  for (int shift = 0; shift < 30; shift += 8)
+  uint32_t *ks = nullptr;
+  int32_t *vs = nullptr;
   {
-    hipLaunchKernelGGL (k_radix_hist, dim3 (n_tiles), dim3 (kSortBlock), 0, st, ks, n, shift, n_tiles, hist);
-    DRX_LAUNCH_CHECK ();
-    int rc = exclusive_scan_i32 (hist, offs, 256 * n_tiles, nullptr, st);
+    int rc = radix_sort_pairs (kA, vA, kB, vB, hist, n, 30, st, &ks, &vs);
     if (rc != DRX_OK) return rc;
-    hipLaunchKernelGGL (k_radix_scatter, dim3 (n_tiles), dim3 (kSortBlock), 0, st, ks, vs, n, shift, n_tiles, offs, kd,
-                        vd);
-    DRX_LAUNCH_CHECK ();
-    std::swap (ks, kd);
-    std::swap (vs, vd);
   }
   // ks = sorted codes, vs = ids
   DRX_HIP (hipMemcpyAsync (out_aabb_ids, vs, sizeof (int32_t) * n, hipMemcpyDeviceToDevice, st));

@@ -57,15 +57,17 @@ static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) /
 // ---------------------------------------------------------------------------------------------
 template <int P, bool POLISH, int MODE>
 __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate (LocateCfg cfg, const float *__restrict__ pts, int64_t n,
-                                                  Location *__restrict__ out, int32_t *__restrict__ steps_out,
-                                                  int32_t *__restrict__ cand_out, int32_t *__restrict__ err)
+                                                  const int32_t *__restrict__ order, Location *__restrict__ out,
+                                                  int32_t *__restrict__ steps_out, int32_t *__restrict__ cand_out,
+                                                  int32_t *__restrict__ err)
 {
   __shared__ int32_t stack_lds[MODE == 2 ? 1 : kStackLds * 256];
   __shared__ int32_t w_code[MODE == 2 ? 4 * kWaveStack : 1];
   __shared__ uint64_t w_mask[MODE == 2 ? 4 * kWaveStack : 1];
   __shared__ int32_t w_cand[MODE == 2 ? 4 * kCandQ * 64 : 1];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t i = order ? (int64_t)order[t] : t;
   TravStack todo;
   todo.lds = stack_lds;
   todo.bdim = blockDim.x;
@@ -582,15 +584,22 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   if (trav == DRX_TRAVERSE_WAVE && !cfg.wide)
     return fail (DRX_ERR_ARG, "drx_locate: wave traversal needs bvh->wide_nodes");
   const int mode = !cfg.wide ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
+  const int32_t *order = nullptr;
+  if (opts && opts->coherent_order)
+  {
+    if (n > (1ll << 31) - 1) return fail (DRX_ERR_ARG, "drx_locate: coherent_order supports < 2^31 points");
+    int rc2 = morton_order_points (points, (int)n, bvh->bounds, st, &order);
+    if (rc2) return rc2;
+  }
 #define DRX_LOC(PP, PO)                                                                             \
   do                                                                                                \
   {                                                                                                 \
     if (mode == 2)                                                                                  \
-      hipLaunchKernelGGL ((k_locate<PP, PO, 2>), g, b, 0, st, cfg, points, n, out, steps, cands, err); \
+      hipLaunchKernelGGL ((k_locate<PP, PO, 2>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
     else if (mode == 1)                                                                             \
-      hipLaunchKernelGGL ((k_locate<PP, PO, 1>), g, b, 0, st, cfg, points, n, out, steps, cands, err); \
+      hipLaunchKernelGGL ((k_locate<PP, PO, 1>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
     else                                                                                            \
-      hipLaunchKernelGGL ((k_locate<PP, PO, 0>), g, b, 0, st, cfg, points, n, out, steps, cands, err); \
+      hipLaunchKernelGGL ((k_locate<PP, PO, 0>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
   } while (0)
   if (cfg.polish)
     switch (P)

@@ -15,9 +15,8 @@ faces.  Jitter is uniform in ``[-j, j]`` per component from a counter-based hash
 that face, so the mesh keeps tiling the unit cube.
 
 Mixed-order meshes (one order per element, drawn from ``orders``) keep faces conforming by making every
-element face the bilinear patch of its (jittered, shared) corner vertices: edge and face control points are
-the trilinear interpolation of the corners, and only strictly interior control points receive the warp and
-their own jitter.
+element face the bilinear patch of its (warped, jittered, shared) corner vertices; strictly interior control
+points add the warp's curvature (its deviation from the trilinear interpolant) and their own jitter.
 """
 from __future__ import annotations
 
@@ -126,17 +125,26 @@ def hex_mesh(n: int, p: int, amp: float, jitter: float, seed: int, jitter_scaled
 
 
 def mixed_hex_mesh(n: int, amp: float, jitter: float, seed: int, orders=(1, 2, 3, 4)) -> HexMesh:
-    """Mixed-order mesh: per-element order drawn uniformly from ``orders`` (seeded), conforming faces."""
+    """Mixed-order mesh: per-element order drawn uniformly from ``orders`` (seeded), conforming faces.
+
+    Corner vertices are shared and carry the global warp plus jitter (``jitter`` x spacing 1/n, zero normal
+    component on the cube faces).  Every edge and face control point is the trilinear interpolation of the
+    element's corners, so each face is the bilinear patch of its four corners in every order and faces
+    between elements of different orders coincide.  Strictly interior control points add the warp's
+    deviation from its own trilinear interpolant (the curvature the corners cannot carry) and a jitter of
+    ``jitter`` x spacing / order.
+    """
     n_el = n ** 3
     ords = np.asarray(orders, dtype=np.int32)
     pick = (uniform_pm1(seed, 7, np.arange(n_el, dtype=np.uint64)) + 1.0) * 0.5
     el_order = ords[np.minimum((pick * len(ords)).astype(np.int64), len(ords) - 1)]
-    # shared corner vertices
     m = n + 1
     g = np.arange(m, dtype=np.float64) / n
     vi = np.arange(m ** 3, dtype=np.int64)
     VI, VJ, VK = vi % m, (vi // m) % m, vi // (m * m)
-    corners = np.stack([g[VI], g[VJ], g[VK]], axis=1)
+    lattice = np.stack([g[VI], g[VJ], g[VK]], axis=1)
+    corner_warp = warp_displacement(lattice, amp)
+    corners = lattice + corner_warp
     jit = jitter * (1.0 / n)
     if jit != 0.0:
         cnt = vi.astype(np.uint64) * np.uint64(3)
@@ -158,21 +166,24 @@ def mixed_hex_mesh(n: int, amp: float, jitter: float, seed: int, orders=(1, 2, 3
         li, lj, lk = loc % (p + 1), (loc // (p + 1)) % (p + 1), loc // ((p + 1) ** 2)
         u, v, w = li / p, lj / p, lk / p
         c = np.zeros((len(sel), len(loc), 3))
+        cw = np.zeros((len(sel), len(loc), 3))
         for dz in (0, 1):
             for dy in (0, 1):
                 for dx in (0, 1):
                     vid = (ex[sel] + dx) + m * ((ey[sel] + dy) + m * (ez[sel] + dz))
                     wt = (u if dx else 1 - u) * (v if dy else 1 - v) * (w if dz else 1 - w)
                     c += wt[None, :, None] * corners[vid][:, None, :]
+                    cw += wt[None, :, None] * corner_warp[vid][:, None, :]
         interior = (li > 0) & (li < p) & (lj > 0) & (lj < p) & (lk > 0) & (lk < p)
         if interior.any():
-            ip = c[:, interior, :].reshape(-1, 3)
-            disp = warp_displacement(ip, amp)
+            x0 = np.stack([(ex[sel][:, None] + u[None, interior]) / n, (ey[sel][:, None] + v[None, interior]) / n,
+                           (ez[sel][:, None] + w[None, interior]) / n], axis=2).reshape(-1, 3)
+            disp = warp_displacement(x0, amp) - cw[:, interior, :].reshape(-1, 3)
             if jit != 0.0:
                 gid = (offsets[sel][:, None] + loc[interior][None, :]).reshape(-1).astype(np.uint64) * np.uint64(3)
                 for cc in range(3):
                     disp[:, cc] += uniform_pm1(seed, 2, gid + np.uint64(cc)) * (jit / p)
-            c[:, interior, :] = (ip + disp).reshape(len(sel), -1, 3)
+            c[:, interior, :] += disp.reshape(len(sel), -1, 3)
         idx = (offsets[sel][:, None] + loc[None, :]).reshape(-1)
         values[idx] = c.reshape(-1, 3)
     values32 = values.astype(np.float32)

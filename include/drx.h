@@ -101,6 +101,7 @@ typedef struct drx_locate_opts
   int32_t *newton_steps;      /* device, nullable: Newton steps summed over candidates, per point */
   int32_t *candidates;        /* device, nullable: candidate leaves tried, per point */
   int32_t traversal;          /* drx_traversal: how a wavefront walks the BVH (same results either way) */
+  int32_t coherent_order;     /* drx_locate: 1 = process queries in Morton order (results stay in input order) */
 } drx_locate_opts;
 
 enum drx_traversal

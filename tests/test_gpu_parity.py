@@ -224,6 +224,10 @@ def test_wave_traversal_same_as_lane(gpu, name, make):
         for mode in (1, 2):
             loc, steps, cands = m.locate(d, gpu.LocateOptions(traversal=mode), stats=True)
             res[mode] = (_np(loc, LOCATION_DTYPE), _np(steps), _np(cands))
+        for mode in (1, 2):   # Morton-ordered processing returns results in input order, unchanged
+            loc, steps, cands = m.locate(d, gpu.LocateOptions(traversal=mode, coherent_order=True), stats=True)
+            assert np.array_equal(_np(loc), res[mode][0].view(np.uint8).reshape(-1))
+            assert np.array_equal(_np(steps), res[mode][1]) and np.array_equal(_np(cands), res[mode][2])
         (l1, s1, c1), (l2, s2, c2) = res[1], res[2]
         dc = int((l1["cell"] != l2["cell"]).sum())
         dr = float(np.abs(l1["ref"] - l2["ref"]).max())
