@@ -33,7 +33,13 @@ class DrxBVH(C.Structure):
 
 class DrxLocateOpts(C.Structure):
     _fields_ = [("init_mode", I32), ("fp64_polish", I32), ("cond_threshold", F32), ("newton_steps", P),
-                ("candidates", P), ("traversal", I32), ("coherent_order", I32)]
+                ("candidates", P), ("traversal", I32), ("coherent_order", I32), ("mlp", P),
+                ("mlp_newton_steps", I32)]
+
+
+class DrxMlpTrainOpts(C.Structure):
+    _fields_ = [("batch", I32), ("steps", I32), ("step0", I32), ("total_steps", I32), ("sample0", I64),
+                ("lr", F32), ("lr_final", F32), ("beta1", F32), ("beta2", F32), ("eps", F32)]
 
 
 class DrxIntegrateStats(C.Structure):
@@ -68,6 +74,14 @@ _SIGS = {
                                    C.POINTER(DrxIntegrateStats), P]),
     "drx_composite": (I32, [P, I32, I32, I32, P, P]),
     "drx_color_table_sample": (I32, [P, I32, P, I32, I32, F32, I32, F32, P]),
+    "drx_mlp_param_count": (I64, []),
+    "drx_mlp_state_bytes": (I64, []),
+    "drx_mlp_packed_bytes": (I64, []),
+    "drx_mlp_init": (I32, [P, C.c_uint64, P]),
+    "drx_mlp_features": (I32, [C.POINTER(DrxMesh), P, P, I64, P, P, P, P]),
+    "drx_mlp_train": (I32, [P, P, P, I64, C.POINTER(DrxMlpTrainOpts), P, P]),
+    "drx_mlp_pack": (I32, [P, P, P]),
+    "drx_mlp_infer": (I32, [P, P, P, I64, P, P]),
 }
 
 EXPORTED = sorted(_SIGS)

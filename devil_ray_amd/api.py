@@ -169,8 +169,10 @@ class DeviceGridFunction:
 class LocateOptions:
     fp64_polish: bool = False
     cond_threshold: float = 1e4
-    traversal: int = 0          # 0 auto, 1 per lane, 2 wave-coherent
+    traversal: int = 0          # 0 auto, 1 per lane (8-wide), 2 wave-coherent, 3 per lane (binary)
     coherent_order: bool = False  # locate: process queries in Morton order
+    mlp: Optional[torch.Tensor] = None   # packed learned initializer (LearnedInitializer.pack())
+    mlp_newton_steps: int = 4            # Newton budget from the learned guess before the plain fallback
 
     def c(self, steps=None, cands=None) -> L.DrxLocateOpts:
         o = L.DrxLocateOpts()
@@ -181,6 +183,10 @@ class LocateOptions:
         o.candidates = cands.data_ptr() if cands is not None else None
         o.traversal = int(self.traversal)
         o.coherent_order = 1 if self.coherent_order else 0
+        if self.mlp is not None:
+            o.init_mode = 1
+            o.mlp = self.mlp.data_ptr()
+            o.mlp_newton_steps = int(self.mlp_newton_steps)
         return o
 
 
@@ -387,3 +393,65 @@ def composite(partials: torch.Tensor, n: int, width: int, height: int) -> torch.
     img = torch.empty(width * height * 4, dtype=torch.float32, device=_dev())
     L.check(L.lib().drx_composite(_ptr(partials), n, width, height, _ptr(img), _stream()), "drx_composite")
     return img
+
+
+# ---------------------------------------------------------------------------------------------
+class LearnedInitializer:
+    """Learned Newton initial guess: MLP 27 -> 64 x4 (GELU) -> 3, BF16 on MFMA (drx_mlp_* in libdrx).
+
+    Not part of dray: dray starts Newton from LocateHack::eval_inverse's guess
+    (data_model/device_mesh.hpp:131-165); plain SubRef-centre Newton stays the fallback.
+    """
+
+    def __init__(self, seed: int = 0):
+        dev = _dev()
+        lib = L.lib()
+        self.state = torch.empty(int(lib.drx_mlp_state_bytes()), dtype=torch.uint8, device=dev)
+        self.n_params = int(lib.drx_mlp_param_count())
+        self.steps_taken = 0
+        L.check(lib.drx_mlp_init(_ptr(self.state), int(seed), _stream()), "drx_mlp_init")
+
+    def params(self) -> torch.Tensor:
+        """fp32 master parameters (W1 [64,32], b1, W2..W4 [64,64], b2..b4, W5 [16,64], b5 [16]) - a view."""
+        return self.state[: 4 * self.n_params].view(torch.float32)
+
+    @staticmethod
+    def features(mesh: "UnstructuredMesh", points: torch.Tensor, locations: torch.Tensor):
+        """(x bf16 [n,32], target float32 [n,3] = located ref - base, base float32 [n,3])."""
+        n = int(points.numel() // 3)
+        dev = _dev()
+        x = torch.empty((max(n, 1), 32), dtype=torch.bfloat16, device=dev)
+        t = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+        b = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+        mv = mesh.gf.view()
+        L.check(L.lib().drx_mlp_features(C.byref(mv), _ptr(points), _ptr(locations), n, _ptr(x), _ptr(t), _ptr(b),
+                                         _stream()), "drx_mlp_features")
+        return x[:n], t[:n], b[:n]
+
+    def train(self, x: torch.Tensor, target: torch.Tensor, steps: int, batch: int = 16384, lr: float = 2e-3,
+              lr_final: float = 2e-5, total_steps: int = 0, beta1: float = 0.9, beta2: float = 0.999,
+              eps: float = 1e-8, record_loss: bool = True):
+        """Run `steps` Adam steps (continuing the schedule of earlier calls); returns per-step MSE or None."""
+        n = int(x.shape[0])
+        o = L.DrxMlpTrainOpts()
+        o.batch, o.steps, o.step0 = int(batch), int(steps), int(self.steps_taken)
+        o.total_steps = int(total_steps) if total_steps else int(self.steps_taken + steps)
+        o.sample0 = (self.steps_taken * int(batch)) % max(n, 1)
+        o.lr, o.lr_final, o.beta1, o.beta2, o.eps = float(lr), float(lr_final), float(beta1), float(beta2), float(eps)
+        loss = torch.zeros(max(steps, 1), dtype=torch.float32, device=_dev()) if record_loss else None
+        L.check(L.lib().drx_mlp_train(_ptr(self.state), _ptr(x), _ptr(target), n, C.byref(o), _ptr(loss), _stream()),
+                "drx_mlp_train")
+        self.steps_taken += int(steps)
+        return loss[:steps] if record_loss else None
+
+    def pack(self) -> torch.Tensor:
+        out = torch.empty(int(L.lib().drx_mlp_packed_bytes()), dtype=torch.uint8, device=_dev())
+        L.check(L.lib().drx_mlp_pack(_ptr(self.state), _ptr(out), _stream()), "drx_mlp_pack")
+        return out
+
+    @staticmethod
+    def infer(packed: torch.Tensor, x: torch.Tensor, base: Optional[torch.Tensor] = None) -> torch.Tensor:
+        n = int(x.shape[0])
+        out = torch.empty((max(n, 1), 3), dtype=torch.float32, device=_dev())
+        L.check(L.lib().drx_mlp_infer(_ptr(packed), _ptr(x), _ptr(base), n, _ptr(out), _stream()), "drx_mlp_infer")
+        return out[:n]

@@ -22,7 +22,12 @@ SOURCES = {
     "drx_lbvh.hip": ["-ffp-contract=off"],
     "drx_query.hip": [],
     "drx_wide.hip": [],
+    "drx_mlp.hip": [],
+    "dray_host.hip": [],
 }
+ROOT = os.path.dirname(HERE)
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "dray_api_main.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "dray_api_test")
 
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -58,5 +63,18 @@ def build(verbose: bool = False, defines=(), out: str = LIB, obj_dir: str = OBJ)
     return out
 
 
+def build_cpp_test(lib: str = LIB, out: str = CPP_TEST_BIN) -> str:
+    """The C++ program that drives the path through the dray-named classes (linked against libdrx.so)."""
+    tmp = out + ".tmp"
+    rpath = os.path.relpath(os.path.dirname(lib), os.path.dirname(out))
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC, "-o", tmp,
+           "-L", os.path.dirname(lib), "-l:" + os.path.basename(lib), f"-Wl,-rpath,$ORIGIN/{rpath}",
+           "-Wl,-rpath-link,/opt/rocm/lib"]
+    subprocess.check_call(cmd)
+    shutil.move(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_test())

@@ -1,0 +1,380 @@
+// Host C++ layer with dray's names over the libdrx C ABI (declared in include/dray/drx_dray.hpp).
+#include "../../include/dray/drx_dray.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace dray
+{
+
+void drx_check (int status, const char *what)
+{
+  if (status != DRX_OK) throw DRayError (std::string (what) + ": " + drx_last_error ());
+}
+
+static void hip_check (hipError_t e, const char *what)
+{
+  if (e != hipSuccess) throw DRayError (std::string (what) + ": " + hipGetErrorString (e));
+}
+
+namespace detail
+{
+Buffer::~Buffer ()
+{
+  if (device) (void)hipFree (device);
+  if (host) std::free (host);
+}
+
+void Buffer::resize (size_t b)
+{
+  if (b == bytes) return;
+  if (device) (void)hipFree (device);
+  if (host) std::free (host);
+  device = nullptr;
+  host = nullptr;
+  bytes = b;
+  host_dirty = device_dirty = false;
+}
+
+void *Buffer::get_host ()
+{
+  if (bytes == 0) return nullptr;
+  if (!host)
+  {
+    host = std::calloc (1, bytes);
+    if (!host) throw DRayError ("Array: host allocation failed");
+  }
+  if (host_dirty && device)
+    hip_check (hipMemcpy (host, device, bytes, hipMemcpyDeviceToHost), "Array: device->host copy");
+  host_dirty = false;
+  return host;
+}
+
+void *Buffer::get_device ()
+{
+  if (bytes == 0) return nullptr;
+  if (!device)
+  {
+    hip_check (hipMalloc (&device, bytes), "Array: hipMalloc");
+    if (!host) hip_check (hipMemset (device, 0, bytes), "Array: hipMemset");
+    else device_dirty = true;
+  }
+  if (device_dirty && host)
+    hip_check (hipMemcpy (device, host, bytes, hipMemcpyHostToDevice), "Array: host->device copy");
+  device_dirty = false;
+  return device;
+}
+} // namespace detail
+
+// ---- LinearBVHBuilder ------------------------------------------------------------------------------------
+BVH LinearBVHBuilder::construct (Array<AABB<>> aabbs)
+{
+  Array<int32> ids (aabbs.size ());
+  int32 *p = ids.get_host_ptr ();
+  for (size_t i = 0; i < aabbs.size (); ++i) p[i] = (int32)i;
+  return construct (aabbs, ids);
+}
+
+BVH LinearBVHBuilder::construct (Array<AABB<>> aabbs, Array<int32> prim_ids)
+{
+  const int32 n = (int32)aabbs.size ();
+  const int32 m = n < 2 ? 2 : n;
+  BVH bvh;
+  bvh.m_inner_nodes.resize (4 * (size_t)(m - 1));
+  bvh.m_leaf_nodes.resize (m);
+  bvh.m_aabb_ids.resize (m);
+  float bounds[6];
+  drx_check (drx_bvh_build (n ? (const float *)aabbs.get_device_ptr_const () : nullptr,
+                            n ? prim_ids.get_device_ptr_const () : nullptr, n,
+                            (float *)bvh.m_inner_nodes.get_device_ptr (), bvh.m_leaf_nodes.get_device_ptr (),
+                            bvh.m_aabb_ids.get_device_ptr (), bounds, nullptr, nullptr),
+             "LinearBVHBuilder::construct");
+  for (int d = 0; d < 3; ++d) bvh.m_bounds.m_ranges[d] = { bounds[2 * d], bounds[2 * d + 1] };
+  return bvh;
+}
+
+// ---- UnstructuredMesh ----------------------------------------------------------------------------------
+UnstructuredMesh::UnstructuredMesh (const GridFunction<3> &dof_data, int32 poly_order)
+: m_dof_data (dof_data), m_poly_order (poly_order)
+{
+  if (poly_order < 1 || poly_order > DRX_MAX_ORDER)
+    throw DRayError ("UnstructuredMesh: order " + std::to_string (poly_order) + " not supported on the GPU path");
+  const int32 npe = (poly_order + 1) * (poly_order + 1) * (poly_order + 1);
+  if (m_dof_data.m_el_dofs == 0) m_dof_data.m_el_dofs = npe;
+  if (m_dof_data.m_el_dofs != npe) throw DRayError ("UnstructuredMesh: element dof count does not match the order");
+  if (m_dof_data.m_size_el == 0) m_dof_data.m_size_el = (int32)(m_dof_data.m_ctrl_idx.size () / npe);
+}
+
+drx_mesh UnstructuredMesh::device_view ()
+{
+  const int32 stride = drx_mesh_stride (m_poly_order, 3);
+  if (m_packed.size () == 0)
+  {
+    m_packed.resize ((size_t)stride * cells ());
+    drx_check (drx_mesh_pack (m_dof_data.m_ctrl_idx.get_device_ptr_const (),
+                              (const float *)m_dof_data.m_values.get_device_ptr_const (), 3, nullptr, nullptr,
+                              nullptr, m_poly_order, cells (), m_packed.get_device_ptr (), nullptr),
+               "UnstructuredMesh: drx_mesh_pack");
+  }
+  drx_mesh v;
+  v.data = m_packed.get_device_ptr_const ();
+  v.elem_offset = nullptr;
+  v.elem_order = nullptr;
+  v.n_el = cells ();
+  v.order = m_poly_order;
+  v.ncomp = 3;
+  v.stride = stride;
+  return v;
+}
+
+BVH &UnstructuredMesh::get_bvh ()
+{
+  if (!m_is_constructed)
+  {
+    // construct_bvh (data_model/mesh_utils.cpp:514-677)
+    const int32 n = cells () * DRX_SPLITS_PER_ELEMENT;
+    Array<AABB<>> aabbs (n);
+    Array<int32> prims (n);
+    m_ref_aabbs.resize (n);
+    drx_check (drx_bvh_split_elements (m_dof_data.m_ctrl_idx.get_device_ptr_const (),
+                                       (const float *)m_dof_data.m_values.get_device_ptr_const (), nullptr, nullptr,
+                                       m_poly_order, cells (), (float *)aabbs.get_device_ptr (),
+                                       prims.get_device_ptr (), (float *)m_ref_aabbs.get_device_ptr (), nullptr),
+               "construct_bvh: drx_bvh_split_elements");
+    m_bvh = LinearBVHBuilder ().construct (aabbs, prims);
+    drx_bvh v = bvh_view ();
+    m_bvh.m_leaf_pack.resize (8 * (size_t)m_bvh.m_leaf_nodes.size ());
+    drx_check (drx_bvh_pack_leaves (&v, m_bvh.m_leaf_pack.get_device_ptr (), nullptr), "drx_bvh_pack_leaves");
+    int64_t cnt = 0;
+    v = bvh_view ();
+    drx_check (drx_bvh_widen (&v, nullptr, 0, &cnt, nullptr), "drx_bvh_widen");
+    m_bvh.m_wide_nodes.resize (64 * (size_t)cnt);
+    drx_check (drx_bvh_widen (&v, m_bvh.m_wide_nodes.get_device_ptr (), cnt, &cnt, nullptr), "drx_bvh_widen");
+    m_is_constructed = true;
+  }
+  return m_bvh;
+}
+
+drx_bvh UnstructuredMesh::bvh_view ()
+{
+  drx_bvh v;
+  v.inner_nodes = (const float *)m_bvh.m_inner_nodes.get_device_ptr_const ();
+  v.leaf_nodes = m_bvh.m_leaf_nodes.get_device_ptr_const ();
+  v.aabb_ids = m_bvh.m_aabb_ids.get_device_ptr_const ();
+  v.subrefs = (const float *)m_ref_aabbs.get_device_ptr_const ();
+  v.leaf_pack = m_bvh.m_leaf_pack.size () ? m_bvh.m_leaf_pack.get_device_ptr_const () : nullptr;
+  v.wide_nodes = m_bvh.m_wide_nodes.size () ? m_bvh.m_wide_nodes.get_device_ptr_const () : nullptr;
+  v.n_leaves = (int32)m_bvh.m_leaf_nodes.size ();
+  for (int d = 0; d < 3; ++d)
+  {
+    v.bounds[2 * d] = m_bvh.m_bounds.m_ranges[d].m_min;
+    v.bounds[2 * d + 1] = m_bvh.m_bounds.m_ranges[d].m_max;
+  }
+  return v;
+}
+
+AABB<> UnstructuredMesh::bounds () { return get_bvh ().m_bounds; }
+
+Array<Location> UnstructuredMesh::locate (Array<Vec<Float, 3>> &wpoints)
+{
+  get_bvh ();
+  drx_mesh mv = device_view ();
+  drx_bvh bv = bvh_view ();
+  Array<Location> locs (wpoints.size ());
+  drx_locate_opts o {};
+  o.init_mode = DRX_INIT_SUBREF_CENTRE;
+  o.cond_threshold = 1e4f;
+  o.traversal = DRX_TRAVERSE_AUTO;
+  if (wpoints.size ())
+    drx_check (drx_locate (&mv, &bv, (const float *)wpoints.get_device_ptr_const (), (int64_t)wpoints.size (),
+                           locs.get_device_ptr (), &o, nullptr),
+               "UnstructuredMesh::locate");
+  return locs;
+}
+
+// ---- UnstructuredField -----------------------------------------------------------------------------------
+UnstructuredField::UnstructuredField (const GridFunction<1> &dof_data, int32 poly_order, const std::string &name)
+: m_dof_data (dof_data), m_poly_order (poly_order), m_name (name)
+{
+  const int32 npe = (poly_order + 1) * (poly_order + 1) * (poly_order + 1);
+  if (m_dof_data.m_el_dofs == 0) m_dof_data.m_el_dofs = npe;
+  if (m_dof_data.m_size_el == 0) m_dof_data.m_size_el = (int32)(m_dof_data.m_ctrl_idx.size () / npe);
+}
+
+Range UnstructuredField::range () const
+{
+  const Vec<Float, 1> *v = m_dof_data.m_values.get_host_ptr_const ();
+  Range r = { INFINITY, -INFINITY };
+  for (size_t i = 0; i < m_dof_data.m_values.size (); ++i)
+  {
+    r.m_min = std::min (r.m_min, v[i][0]);
+    r.m_max = std::max (r.m_max, v[i][0]);
+  }
+  return r;
+}
+
+drx_mesh UnstructuredField::device_view ()
+{
+  const int32 stride = drx_mesh_stride (m_poly_order, 1);
+  if (m_packed.size () == 0)
+  {
+    m_packed.resize ((size_t)stride * m_dof_data.m_size_el);
+    drx_check (drx_mesh_pack (m_dof_data.m_ctrl_idx.get_device_ptr_const (),
+                              (const float *)m_dof_data.m_values.get_device_ptr_const (), 1, nullptr, nullptr,
+                              nullptr, m_poly_order, m_dof_data.m_size_el, m_packed.get_device_ptr (), nullptr),
+               "UnstructuredField: drx_mesh_pack");
+  }
+  drx_mesh v;
+  v.data = m_packed.get_device_ptr_const ();
+  v.elem_offset = nullptr;
+  v.elem_order = nullptr;
+  v.n_el = m_dof_data.m_size_el;
+  v.order = m_poly_order;
+  v.ncomp = 1;
+  v.stride = stride;
+  return v;
+}
+
+void UnstructuredField::eval (const Array<Location> locs, Array<Float> &values)
+{
+  if (values.size () != locs.size ()) values.resize (locs.size ());
+  drx_mesh fv = device_view ();
+  if (locs.size ())
+    drx_check (drx_field_eval (&fv, locs.get_device_ptr_const (), (int64_t)locs.size (), values.get_device_ptr (), nullptr),
+               "Field::eval");
+}
+
+// ---- ColorTable ------------------------------------------------------------------------------------------
+void ColorTable::add_point (float32 position, const Vec3f &c)
+{
+  m_rgb.insert (m_rgb.end (), { position, c[0], c[1], c[2], 1.f });
+}
+void ColorTable::add_point (float32 position, const Vec4f &c)
+{
+  m_rgb.insert (m_rgb.end (), { position, c[0], c[1], c[2], c[3] });
+}
+void ColorTable::add_alpha (float32 position, float32 alpha) { m_alpha.insert (m_alpha.end (), { position, alpha }); }
+
+ColorTable ColorTable::correct_opacity (const float32 &factor) const
+{
+  ColorTable c (*this);
+  for (size_t i = 1; i < c.m_alpha.size (); i += 2) c.m_alpha[i] = 1.f - std::pow ((1.f - c.m_alpha[i]), factor);
+  return c;
+}
+
+void ColorTable::sample (int num_samples, Array<Vec<float32, 4>> &colors, float32 alpha_scale) const
+{
+  colors.resize (num_samples);
+  drx_check (drx_color_table_sample (m_rgb.data (), (int32)(m_rgb.size () / 5), m_alpha.data (),
+                                     (int32)(m_alpha.size () / 2), m_smooth ? 1 : 0, -1.f, num_samples, alpha_scale,
+                                     (float *)colors.get_host_ptr ()),
+             "ColorTable::sample");
+}
+
+// ---- Camera ----------------------------------------------------------------------------------------------
+void Camera::set_width (const int32 &w)
+{
+  if (w <= 0) throw DRayError ("Camera width must be greater than zero.");
+  m_width = w;
+}
+void Camera::set_height (const int32 &h)
+{
+  if (h <= 0) throw DRayError ("Camera height must be greater than zero.");
+  m_height = h;
+}
+void Camera::set_fov (const float32 &degrees)
+{
+  if (degrees <= 0) throw DRayError ("Camera feild of view must be greater than zero.");
+  if (degrees > 180) throw DRayError ("Camera feild of view must be less than 180.");
+  m_fov_y = degrees;
+}
+void Camera::create_rays (Array<Ray> &rays)
+{
+  rays.resize ((size_t)m_width * m_height);
+  drx_camera c;
+  c.width = m_width;
+  c.height = m_height;
+  c.fov_y = m_fov_y;
+  c.zoom = m_zoom;
+  for (int i = 0; i < 3; ++i)
+  {
+    c.pos[i] = m_pos[i];
+    c.look_at[i] = m_look_at[i];
+    c.up[i] = m_up[i];
+  }
+  drx_check (drx_camera_rays (&c, rays.get_device_ptr (), nullptr), "Camera::create_rays");
+}
+
+// ---- Volume ----------------------------------------------------------------------------------------------
+// |bounds diagonal| / samples in float32, unfused like dray's (volume.cpp:89-91)
+static float sample_distance (const float *bounds6, int32 samples)
+{
+#pragma clang fp contract(off)
+  float s = 0.f;
+  for (int d = 0; d < 3; ++d)
+  {
+    const float e = bounds6[2 * d + 1] - bounds6[2 * d];
+    s += e * e;
+  }
+  return std::sqrt (s) / float (samples);
+}
+
+Volume::Volume (UnstructuredMesh &mesh, UnstructuredField &field) : m_mesh (mesh), m_field (field) {}
+
+Array<VolumePartial> Volume::integrate (Array<Ray> &rays)
+{
+  if (m_use_lighting) throw DRayError ("Volume::integrate: lit shading is not available; use_lighting(false)");
+  if (!m_range_set) m_range = m_field.range ();
+  AABB<> b = m_mesh.bounds ();
+  const float bounds6[6] = { b.m_ranges[0].m_min, b.m_ranges[0].m_max, b.m_ranges[1].m_min,
+                             b.m_ranges[1].m_max, b.m_ranges[2].m_min, b.m_ranges[2].m_max };
+  // opacity-corrected table sampled to 1024 entries (volume.cpp:81-87, color_map.cpp:13)
+  Array<Vec<float32, 4>> lut;
+  m_table.correct_opacity (10.f / m_samples).sample (1024, lut);
+  const float sample_dist = sample_distance (bounds6, m_samples);
+  Array<Ray> active (std::max<size_t> (rays.size (), 1));
+  int32 n_active = 0;
+  if (rays.size ())
+    drx_check (drx_remove_missed_rays (rays.get_device_ptr_const (), (int32)rays.size (), bounds6,
+                                       active.get_device_ptr (), &n_active, nullptr),
+               "remove_missed_rays");
+  Array<int32> order;
+  if (m_tile_w > 0 && m_tile_h > 0 && n_active > 0)
+  {
+    order.resize ((size_t)n_active);
+    drx_check (drx_ray_tile_order (active.get_device_ptr_const (), n_active, m_tile_w, m_tile_h,
+                                   order.get_device_ptr (), nullptr),
+               "drx_ray_tile_order");
+  }
+  Array<VolumePartial> partials ((size_t)std::max (1, 5 * n_active));
+  int32 count = 0;
+  drx_mesh mv = m_mesh.device_view ();
+  drx_mesh fv = m_field.device_view ();
+  drx_bvh bv = m_mesh.bvh_view ();
+  drx_locate_opts o {};
+  o.cond_threshold = 1e4f;
+  drx_check (drx_volume_integrate (&mv, &fv, &bv, active.get_device_ptr_const (), n_active,
+                                   (const float *)lut.get_device_ptr_const (), 1024, m_range.m_min, m_range.m_max,
+                                   sample_dist, order.size () ? order.get_device_ptr_const () : nullptr, &o, partials.get_device_ptr (), &count, &m_stats, nullptr),
+             "Volume::integrate");
+  // keep the first `count` partials (compacted, ray order)
+  Array<VolumePartial> out ((size_t)count);
+  if (count)
+    hip_check (hipMemcpy (out.get_device_ptr (), partials.get_device_ptr_const (), sizeof (VolumePartial) * (size_t)count,
+                          hipMemcpyDeviceToDevice),
+               "Volume::integrate: partials copy");
+  return out;
+}
+
+Array<Vec<float32, 4>> Volume::composite (Array<VolumePartial> &partials, int32 width, int32 height)
+{
+  Array<Vec<float32, 4>> img ((size_t)width * height);
+  drx_check (drx_composite (partials.size () ? (const void *)partials.get_device_ptr_const () : nullptr,
+                            (int32)partials.size (), width, height, (float *)img.get_device_ptr (), nullptr),
+             "Volume::save");
+  return img;
+}
+
+} // namespace dray

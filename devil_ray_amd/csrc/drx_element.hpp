@@ -286,6 +286,7 @@ struct NewtonOut
 {
   int steps;
   float cond;
+  bool conv;
 };
 
 // eval_inverse_local + IterativeMethod::solve (element.hpp:435-486, newton_solver.hpp:50-73): up to 20
@@ -294,7 +295,7 @@ struct NewtonOut
 // `t` is the target relative to the block's origin.
 template <int P, bool REGS>
 __device__ __forceinline__ bool newton_f32 (const float *__restrict__ g, const float t[3], float x[3], NewtonOut &no,
-                                            bool want_cond)
+                                            bool want_cond, int max_steps = 20)
 {
   // REGS: the element's control points are loaded once into registers and reused by every iteration
   float C[REGS ? 3 : 1][REGS ? Ord<P>::NPE : 1];
@@ -310,7 +311,7 @@ __device__ __forceinline__ bool newton_f32 (const float *__restrict__ g, const f
   int steps = 0;
   bool conv = false;
   float Jl[3][3];
-  while (steps < 20 && !conv)
+  while (steps < max_steps && !conv)
   {
 #pragma unroll
     for (int i = 0; i < 3; ++i) x[i] = fminf (1.f, fmaxf (x[i], 0.f));
@@ -339,6 +340,7 @@ __device__ __forceinline__ bool newton_f32 (const float *__restrict__ g, const f
     for (int i = 0; i < 3; ++i) prev[i] = x[i];
   }
   no.steps = steps;
+  no.conv = conv;
   float mn = 2.f, mx = -1.f;
 #pragma unroll
   for (int d = 0; d < 3; ++d)
@@ -391,6 +393,7 @@ struct LocateStats
   int newton;
   int candidates;
   int polished;
+  int fallbacks;   // learned-start Newton runs that did not converge and were redone from the SubRef centre
 };
 
 template <int P, bool POLISH>
@@ -524,14 +527,9 @@ __device__ __forceinline__ int locate_point (const LocateCfg &cfg, TravStack &to
   }
 }
 
-// Same candidate sequence over the 8-wide collapse: a stack entry is (wide node << 8 | remaining hit
-// mask); hit children are visited in slot (= binary left-to-right) order.
+// Same candidate sequence over the 8-wide collapse: the hit children of a node beyond the first are
+// pushed as child codes, highest slot first, so they pop in slot (= binary left-to-right) order.
 constexpr int32_t kWideEnd = (int32_t)0x80000000;
-
-__device__ __forceinline__ int32_t wide_child (const float4 *__restrict__ wide, int node, int slot)
-{
-  return __float_as_int (((const float *)(wide + 16 * (int64_t)node))[48 + slot]);
-}
 
 template <int P, bool POLISH>
 __device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStack &todo, const float pt[3],
@@ -539,19 +537,6 @@ __device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStac
 {
   int32_t cur = 0;
   int sp = 0;   // number of stack entries
-  auto pop = [&] () -> int32_t {
-    if (sp == 0) return kWideEnd;
-    const int32_t e = todo.get (sp - 1);
-    const int node = e >> 8;
-    uint32_t m = (uint32_t)e & 0xffu;
-    const int slot = __builtin_ctz (m);
-    m &= m - 1;
-    if (m)
-      todo.set (sp - 1, (node << 8) | (int32_t)m);
-    else
-      --sp;
-    return wide_child (cfg.wide, node, slot);
-  };
   for (;;)
   {
     while (cur >= 0)
@@ -559,12 +544,15 @@ __device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStac
       const float4 *nd = cfg.wide + 16 * (int64_t)cur;
       const float4 lx0 = nd[0], lx1 = nd[1], ly0 = nd[2], ly1 = nd[3], lz0 = nd[4], lz1 = nd[5];
       const float4 hx0 = nd[6], hx1 = nd[7], hy0 = nd[8], hy1 = nd[9], hz0 = nd[10], hz1 = nd[11];
+      const float4 c0 = nd[12], c1 = nd[13];
       const float lx[8] = { lx0.x, lx0.y, lx0.z, lx0.w, lx1.x, lx1.y, lx1.z, lx1.w };
       const float ly[8] = { ly0.x, ly0.y, ly0.z, ly0.w, ly1.x, ly1.y, ly1.z, ly1.w };
       const float lz[8] = { lz0.x, lz0.y, lz0.z, lz0.w, lz1.x, lz1.y, lz1.z, lz1.w };
       const float hx[8] = { hx0.x, hx0.y, hx0.z, hx0.w, hx1.x, hx1.y, hx1.z, hx1.w };
       const float hy[8] = { hy0.x, hy0.y, hy0.z, hy0.w, hy1.x, hy1.y, hy1.z, hy1.w };
       const float hz[8] = { hz0.x, hz0.y, hz0.z, hz0.w, hz1.x, hz1.y, hz1.z, hz1.w };
+      const int32_t ch[8] = { __float_as_int (c0.x), __float_as_int (c0.y), __float_as_int (c0.z), __float_as_int (c0.w),
+                              __float_as_int (c1.x), __float_as_int (c1.y), __float_as_int (c1.z), __float_as_int (c1.w) };
       uint32_t mask = 0;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
@@ -576,21 +564,26 @@ __device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStac
       }
       if (mask == 0)
       {
-        cur = pop ();
+        cur = (sp == 0) ? kWideEnd : todo.get (--sp);
         continue;
       }
-      const int slot = __builtin_ctz (mask);
-      mask &= mask - 1;
-      if (mask)
+      const int first = __builtin_ctz (mask);
+      int32_t next = ch[0];
+#pragma unroll
+      for (int k = 7; k >= 0; --k)
       {
-        if (sp >= kStackDepth)
+        if (k == first) next = ch[k];
+        if (k > first && ((mask >> k) & 1u))
         {
-          overflow = true;
-          return -1;
+          if (sp >= kStackDepth)
+          {
+            overflow = true;
+            return -1;
+          }
+          todo.set (sp++, ch[k]);
         }
-        todo.set (sp++, (cur << 8) | (int32_t)mask);
       }
-      cur = wide_child (cfg.wide, cur, slot);
+      cur = next;
     }
     if (cur == kWideEnd) return -1;
     const int leaf = -cur - 1;
@@ -609,8 +602,18 @@ __device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStac
       ref[2] = x[2];
       return el;
     }
-    cur = pop ();
+    cur = (sp == 0) ? kWideEnd : todo.get (--sp);
   }
+}
+
+template <int P, bool POLISH, bool WIDE>
+__device__ __forceinline__ int locate_any (const LocateCfg &cfg, TravStack &todo, const float pt[3], float ref[3],
+                                           LocateStats &st, bool &overflow)
+{
+  if constexpr (WIDE)
+    return locate_point_wide<P, POLISH> (cfg, todo, pt, ref, st, overflow);
+  else
+    return locate_point<P, POLISH> (cfg, todo, pt, ref, st, overflow);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -749,16 +752,6 @@ __device__ __forceinline__ int locate_wave (const LocateCfg &cfg, const WaveScra
   }
   if (searching) drain ();
   return result;
-}
-
-template <int P, bool POLISH, bool WIDE>
-__device__ __forceinline__ int locate_any (const LocateCfg &cfg, TravStack &todo, const float pt[3], float ref[3],
-                                           LocateStats &st, bool &overflow)
-{
-  if constexpr (WIDE)
-    return locate_point_wide<P, POLISH> (cfg, todo, pt, ref, st, overflow);
-  else
-    return locate_point<P, POLISH> (cfg, todo, pt, ref, st, overflow);
 }
 
 } // namespace drx

@@ -1,0 +1,176 @@
+// Point location with the learned Newton initializer.  Same traversal and candidate order as
+// locate_point (DeviceMesh::locate, device_mesh.hpp:247-354); at each candidate the wavefront runs the
+// MLP for all its lanes at once (MFMA needs the whole wave), then each lane runs Newton from its
+// learned guess with a small step budget and, if that run has not converged, the plain Newton from the
+// SubRef centre, which also decides acceptance in that case.
+#pragma once
+
+#include "drx_element.hpp"
+#include "drx_mlp.hpp"
+
+namespace drx
+{
+
+template <int P, bool POLISH>
+__device__ __forceinline__ bool try_element_learned (const LocateCfg &cfg, const float *g, const float pt[3],
+                                                     float x[3], const float xc[3], int budget, LocateStats &st)
+{
+  const float t[3] = { pt[0] - g[0], pt[1] - g[1], pt[2] - g[2] };
+  NewtonOut no;
+  bool found = false;
+  st.candidates += 1;
+  // pass 0: from the learned guess within `budget` steps; pass 1 (only if pass 0 did not converge):
+  // dray's Newton from the SubRef centre with the full 20-step budget
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass)
+  {
+    if (pass == 1)
+    {
+      x[0] = xc[0];
+      x[1] = xc[1];
+      x[2] = xc[2];
+      st.fallbacks += 1;
+    }
+    found = newton_f32<P, false> (g, t, x, no, POLISH, pass == 0 ? budget : 20);
+    st.newton += no.steps;
+    if (no.conv) break;
+  }
+  if constexpr (POLISH)
+  {
+    if (found && !(no.cond <= cfg.cond_threshold))
+    {
+      polish_f64<P> (g, pt, x);
+      st.polished += 1;
+    }
+  }
+  return found;
+}
+
+template <bool POLISH>
+__device__ __forceinline__ bool try_element_learned_mixed (const LocateCfg &cfg, int order, const float *g,
+                                                           const float pt[3], float x[3], const float xc[3], int budget,
+                                                           LocateStats &st)
+{
+  switch (order)
+  {
+  case 1: return try_element_learned<1, POLISH> (cfg, g, pt, x, xc, budget, st);
+  case 2: return try_element_learned<2, POLISH> (cfg, g, pt, x, xc, budget, st);
+  case 3: return try_element_learned<3, POLISH> (cfg, g, pt, x, xc, budget, st);
+  default: return try_element_learned<4, POLISH> (cfg, g, pt, x, xc, budget, st);
+  }
+}
+
+// Lanes with active == false take part in the wavefront's MLP rounds only.  lx / ly: this wave's LDS
+// scratch for mlp_wave_infer.
+template <int P, bool POLISH>
+__device__ __forceinline__ int locate_point_mlp (const LocateCfg &cfg, TravStack &todo, const float pt[3], float ref[3],
+                                                 LocateStats &st, bool &overflow, bool active, const MlpNet &net,
+                                                 int budget, __bf16 *lx, float *ly)
+{
+  int cur = 0, sp = 0;
+  if (active) todo.set (0, kBarrier);
+  bool done = !active;
+  int result = -1;
+  for (;;)
+  {
+    if (!done)
+    {
+      while (cur > -1)
+      {
+        const float4 *nd = cfg.nodes + cur;
+        const float4 a = nd[0], b = nd[1], c = nd[2], ch = nd[3];
+        bool in_l = true, in_r = true;
+        if (pt[0] < a.x) in_l = false;
+        if (pt[1] < a.y) in_l = false;
+        if (pt[2] < a.z) in_l = false;
+        if (pt[0] > a.w) in_l = false;
+        if (pt[1] > b.x) in_l = false;
+        if (pt[2] > b.y) in_l = false;
+        if (pt[0] < b.z) in_r = false;
+        if (pt[1] < b.w) in_r = false;
+        if (pt[2] < c.x) in_r = false;
+        if (pt[0] > c.y) in_r = false;
+        if (pt[1] > c.z) in_r = false;
+        if (pt[2] > c.w) in_r = false;
+        if (!in_l && !in_r)
+        {
+          cur = todo.get (sp);
+          --sp;
+        }
+        else
+        {
+          const int lc = __float_as_int (ch.x), rc = __float_as_int (ch.y);
+          cur = in_l ? lc : rc;
+          if (in_l && in_r)
+          {
+            if (sp + 1 >= kStackDepth)
+            {
+              overflow = true;
+              cur = kBarrier;
+              break;
+            }
+            todo.set (++sp, rc);
+          }
+        }
+      }
+      if (cur == kBarrier) done = true;
+    }
+    const bool has = !done;
+    if (__ballot (has) == 0) break;
+    float f[mlp::IN_PAD], base[3] = { 0.5f, 0.5f, 0.5f }, xc[3] = { 0.5f, 0.5f, 0.5f };
+    const float *g = cfg.mesh;
+    int el = -1, order = P;
+    if (has)
+    {
+      const int leaf = -cur - 1;
+      const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
+      el = __float_as_int (l0.x);
+      xc[0] = (l0.y + l1.x) * 0.5f;   // subref_center
+      xc[1] = (l0.z + l1.y) * 0.5f;
+      xc[2] = (l0.w + l1.z) * 0.5f;
+      if constexpr (P == 0)
+      {
+        g = cfg.mesh + cfg.offsets[el];
+        order = cfg.orders[el];
+        mlp_features_mixed (order, g, pt, f, base);
+      }
+      else
+      {
+        g = cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE;
+        mlp_features<P> (g, pt, f, base);
+      }
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < mlp::IN_PAD; ++k) f[k] = 0.f;
+    }
+    float corr[3];
+    mlp_wave_infer (net, f, corr, lx, ly);
+    if (has)
+    {
+      float x[3] = { base[0] + corr[0], base[1] + corr[1], base[2] + corr[2] };
+      bool found;
+      if constexpr (P == 0)
+        found = try_element_learned_mixed<POLISH> (cfg, order, g, pt, x, xc, budget, st);
+      else
+        found = try_element_learned<P, POLISH> (cfg, g, pt, x, xc, budget, st);
+      if (found)
+      {
+        ref[0] = x[0];
+        ref[1] = x[1];
+        ref[2] = x[2];
+        result = el;
+        done = true;
+      }
+      else
+      {
+        cur = todo.get (sp);
+        --sp;
+      }
+    }
+  }
+  return result;
+}
+
+} // namespace drx

@@ -2,6 +2,7 @@
 // compositing for gfx950.
 #include "drx_common.hpp"
 #include "drx_element.hpp"
+#include "drx_locate_mlp.hpp"
 #include "drx_wave.hpp"
 
 #include <cmath>
@@ -74,7 +75,7 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate (LocateCfg cf
   todo.tid = threadIdx.x;
   const float pt[3] = { pts[3 * i], pts[3 * i + 1], pts[3 * i + 2] };
   float ref[3] = { -1.f, -1.f, -1.f };
-  LocateStats st { 0, 0, 0 };
+  LocateStats st { 0, 0, 0, 0 };
   bool overflow = false;
   int cell;
   if constexpr (MODE == 2)
@@ -85,6 +86,53 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate (LocateCfg cf
   }
   else
     cell = locate_any<P, POLISH, MODE == 1> (cfg, todo, pt, ref, st, overflow);
+  if (overflow) atomicOr (err, 1);
+  Location loc;
+  loc.cell = cell;
+  loc.ref[0] = cell >= 0 ? ref[0] : -1.f;
+  loc.ref[1] = cell >= 0 ? ref[1] : -1.f;
+  loc.ref[2] = cell >= 0 ? ref[2] : -1.f;
+  out[i] = loc;
+  if (steps_out) steps_out[i] = st.newton;
+  if (cand_out) cand_out[i] = st.candidates;
+}
+
+// Mesh::locate with the learned initial guess (binary per-lane traversal).  Lanes past n stay in the
+// kernel as idle participants of their wavefront's MLP rounds.
+template <int P, bool POLISH>
+__global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate_mlp (LocateCfg cfg, MlpNet net, int budget,
+                                                                       const float *__restrict__ pts, int64_t n,
+                                                                       const int32_t *__restrict__ order,
+                                                                       Location *__restrict__ out,
+                                                                       int32_t *__restrict__ steps_out,
+                                                                       int32_t *__restrict__ cand_out,
+                                                                       int32_t *__restrict__ err)
+{
+  __shared__ int32_t stack_lds[kStackLds * 256];
+  __shared__ __attribute__ ((aligned (16))) __bf16 lx[4 * 64 * mlp::IN_PAD];
+  __shared__ __attribute__ ((aligned (16))) float ly[4 * 64 * 4];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = t < n;
+  if (__ballot (active) == 0) return;
+  const int64_t i = active ? (order ? (int64_t)order[t] : t) : 0;
+  TravStack todo;
+  todo.lds = stack_lds;
+  todo.bdim = blockDim.x;
+  todo.tid = threadIdx.x;
+  float pt[3] = { 0.f, 0.f, 0.f };
+  if (active)
+  {
+    pt[0] = pts[3 * i];
+    pt[1] = pts[3 * i + 1];
+    pt[2] = pts[3 * i + 2];
+  }
+  float ref[3] = { -1.f, -1.f, -1.f };
+  LocateStats st { 0, 0, 0, 0 };
+  bool overflow = false;
+  const int wv = threadIdx.x >> 6;
+  const int cell = locate_point_mlp<P, POLISH> (cfg, todo, pt, ref, st, overflow, active, net, budget,
+                                                lx + wv * 64 * mlp::IN_PAD, ly + wv * 64 * 4);
+  if (!active) return;
   if (overflow) atomicOr (err, 1);
   Location loc;
   loc.cell = cell;
@@ -324,6 +372,8 @@ struct IntegrateArgs
   Partial *slots;
   unsigned long long *stats;   // located, newton, candidates, polished
   int32_t *err;
+  MlpNet net;                  // learned initial guess (MODE 3)
+  int budget;                  // Newton steps from the learned guess before the plain fallback
 };
 
 template <int P>
@@ -349,13 +399,16 @@ template <int P, bool POLISH, int MODE>
 __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (IntegrateArgs a)
 {
   constexpr bool WAVE = MODE == 2;
+  constexpr bool LEARNED = MODE == 3;   // learned Newton start; the whole wavefront stays in the loop
   __shared__ int32_t stack_lds[WAVE ? 1 : kStackLds * 256];
   __shared__ int32_t w_code[WAVE ? 4 * kWaveStack : 1];
   __shared__ uint64_t w_mask[WAVE ? 4 * kWaveStack : 1];
   __shared__ int32_t w_cand[WAVE ? 4 * kCandQ * 64 : 1];
+  __shared__ __attribute__ ((aligned (16))) __bf16 mlp_x[LEARNED ? 4 * 64 * mlp::IN_PAD : 8];
+  __shared__ __attribute__ ((aligned (16))) float mlp_y[LEARNED ? 4 * 64 * 4 : 4];
   WaveScratch ws;
+  const int wv = threadIdx.x >> 6;
   {
-    const int wv = threadIdx.x >> 6;
     ws.st_code = w_code + (WAVE ? wv * kWaveStack : 0);
     ws.st_mask = w_mask + (WAVE ? wv * kWaveStack : 0);
     ws.cand = w_cand + (WAVE ? wv * kCandQ * 64 : 0);
@@ -365,23 +418,34 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (Integrate
   todo.bdim = blockDim.x;
   todo.tid = threadIdx.x;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool has_ray = t < a.n_rays;
   unsigned long long located = 0;
-  LocateStats st { 0, 0, 0 };
+  LocateStats st { 0, 0, 0, 0 };
   bool overflow = false;
-  if (t < a.n_rays)
+  if (LEARNED || has_ray)
   {
-    const int ri = a.order ? a.order[t] : t;
-    const Ray ray = a.rays[ri];
-    Partial *out5 = a.slots + 5 * (int64_t)ri;
-#pragma unroll
-    for (int s = 0; s < 5; ++s)
+    const int ri = has_ray ? (a.order ? a.order[t] : t) : 0;
+    Ray ray;
+    if (has_ray)
+      ray = a.rays[ri];
+    else
     {
-      Partial p0;
-      p0.pixel_id = -1;
-      p0.depth = INFINITY;
-      p0.rgba[0] = p0.rgba[1] = p0.rgba[2] = p0.rgba[3] = 0.f;
-      out5[s] = p0;
+      ray.dir[0] = ray.dir[1] = ray.dir[2] = 0.f;
+      ray.orig[0] = ray.orig[1] = ray.orig[2] = 0.f;
+      ray.near_ = ray.far_ = 0.f;
+      ray.pixel_id = -1;
     }
+    Partial *out5 = a.slots + 5 * (int64_t)ri;
+    if (has_ray)
+#pragma unroll
+      for (int s = 0; s < 5; ++s)
+      {
+        Partial p0;
+        p0.pixel_id = -1;
+        p0.depth = INFINITY;
+        p0.rgba[0] = p0.rgba[1] = p0.rgba[2] = p0.rgba[3] = 0.f;
+        out5[s] = p0;
+      }
     // integrate_partials' per-ray loop as a state machine with one locate() per iteration:
     //   search: locate at `distance`; on a miss step on; on a hit open a segment and shade the sample;
     //   inside: locate at the advanced distance; keep shading while inside, before `far` and alpha < 0.95,
@@ -391,21 +455,39 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (Integrate
     bool inseg = false;
     float depth = 0.f;
     float rgba[4] = { 0.f, 0.f, 0.f, 0.f };
+    bool alive = has_ray;
     for (;;)
     {
-      if (!inseg && !(distance < ray.far_)) break;
+      if (alive && !inseg && !(distance < ray.far_)) alive = false;
+      if constexpr (LEARNED)
+      {
+        if (__ballot (alive) == 0) break;
+      }
+      else if (!alive)
+        break;
       const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
                             ray_point (ray.orig[2], distance, ray.dir[2]) };
       float ref[3];
       int cell;
-      if constexpr (WAVE)
+      if constexpr (LEARNED)
+      {
+        cell = locate_point_mlp<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, alive, a.net, a.budget,
+                                            mlp_x + wv * 64 * mlp::IN_PAD, mlp_y + wv * 64 * 4);
+        if (!alive) continue;
+      }
+      else if constexpr (WAVE)
         cell = locate_wave<P, POLISH> (a.cfg, ws, pt, ref, st, overflow);
       else
         cell = locate_any<P, POLISH, MODE == 1> (a.cfg, todo, pt, ref, st, overflow);
-      if (overflow) break;
+      if (overflow)
+      {
+        alive = false;
+        continue;
+      }
       const bool found = cell != -1;
       if (found) ++located;
       bool shade = false;
+      bool ray_done = false;
       if (!inseg)
       {
         if (found)
@@ -431,8 +513,15 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (Integrate
         pp.rgba[3] = rgba[3];
         out5[segment] = pp;
         ++segment;
-        if (distance >= ray.far_ || rgba[3] > 0.95f || segment == 5) break;
-        inseg = false;
+        if (distance >= ray.far_ || rgba[3] > 0.95f || segment == 5)
+          ray_done = true;
+        else
+          inseg = false;
+      }
+      if (ray_done)
+      {
+        alive = false;
+        continue;
       }
       if (shade)
       {
@@ -568,8 +657,10 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   if (n < 0) return fail (DRX_ERR_ARG, "drx_locate: n < 0");
   if (n == 0) return DRX_OK;
   if (!points || !out_locations) return fail (DRX_ERR_ARG, "drx_locate: null points/output");
-  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE)
-    return fail (DRX_ERR_UNSUPPORTED, "drx_locate: init_mode %d not available", opts->init_mode);
+  const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
+  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
+    return fail (DRX_ERR_ARG, "drx_locate: unknown init_mode %d", opts->init_mode);
+  if (learned && !opts->mlp) return fail (DRX_ERR_ARG, "drx_locate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
   hipStream_t st = as_stream (stream);
   int32_t *err = (int32_t *)arena (SLOT_FLAGS, 64);
   if (!err) return DRX_ERR_HIP;
@@ -583,7 +674,7 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !cfg.wide)
     return fail (DRX_ERR_ARG, "drx_locate: wave traversal needs bvh->wide_nodes");
-  const int mode = !cfg.wide ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
+  const int mode = (!cfg.wide || trav == DRX_TRAVERSE_BINARY) ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
   const int32_t *order = nullptr;
   if (opts && opts->coherent_order)
   {
@@ -591,10 +682,15 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
     int rc2 = morton_order_points (points, (int)n, bvh->bounds, st, &order);
     if (rc2) return rc2;
   }
+  const MlpNet net = mlp_net (learned ? opts->mlp : nullptr);
+  const int budget = (learned && opts->mlp_newton_steps > 0) ? opts->mlp_newton_steps : 4;
 #define DRX_LOC(PP, PO)                                                                             \
   do                                                                                                \
   {                                                                                                 \
-    if (mode == 2)                                                                                  \
+    if (learned)                                                                                    \
+      hipLaunchKernelGGL ((k_locate_mlp<PP, PO>), g, b, 0, st, cfg, net, budget, points, n, order, out, steps, \
+                          cands, err);                                                              \
+    else if (mode == 2)                                                                             \
       hipLaunchKernelGGL ((k_locate<PP, PO, 2>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
     else if (mode == 1)                                                                             \
       hipLaunchKernelGGL ((k_locate<PP, PO, 1>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
@@ -751,8 +847,11 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   if (!out_count) return fail (DRX_ERR_ARG, "drx_volume_integrate: null out_count");
   if (n_rays < 0 || lut_size < 1 || !lut) return fail (DRX_ERR_ARG, "drx_volume_integrate: bad rays/lut");
   if (!(field_min <= field_max)) return fail (DRX_ERR_ARG, "ColorMap scalar range never set");
-  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE)
-    return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: init_mode %d not available", opts->init_mode);
+  const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
+  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
+    return fail (DRX_ERR_ARG, "drx_volume_integrate: unknown init_mode %d", opts->init_mode);
+  if (learned && !opts->mlp)
+    return fail (DRX_ERR_ARG, "drx_volume_integrate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
   *out_count = 0;
   if (stats) *stats = drx_integrate_stats { 0, 0, 0, 0, 0.f, 0.f };
   if (n_rays == 0) return DRX_OK;
@@ -787,19 +886,23 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   a.slots = slots;
   a.stats = dstats;
   a.err = err;
+  a.net = mlp_net (learned ? opts->mlp : nullptr);
+  a.budget = (learned && opts->mlp_newton_steps > 0) ? opts->mlp_newton_steps : 4;
   const dim3 g (nblk (n_rays, 256)), b (256);
   const int P = mesh->elem_order ? 0 : mesh->order;
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !a.cfg.wide)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: wave traversal needs bvh->wide_nodes");
-  const int mode = !a.cfg.wide ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
+  const int mode = learned ? 3 : (!a.cfg.wide || trav == DRX_TRAVERSE_BINARY) ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
   hipEvent_t ev[3];
   for (int q = 0; q < 3; ++q) DRX_HIP (hipEventCreate (&ev[q]));
   DRX_HIP (hipEventRecord (ev[0], st));
 #define DRX_INT(PP, PO)                                              \
   do                                                                 \
   {                                                                  \
-    if (mode == 2)                                                   \
+    if (mode == 3)                                                   \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 3>), g, b, 0, st, a); \
+    else if (mode == 2)                                              \
       hipLaunchKernelGGL ((k_integrate<PP, PO, 2>), g, b, 0, st, a); \
     else if (mode == 1)                                              \
       hipLaunchKernelGGL ((k_integrate<PP, PO, 1>), g, b, 0, st, a); \

@@ -16,6 +16,13 @@ __device__ __forceinline__ int32_t wave_reduce_add (int32_t v)
   return v;
 }
 
+__device__ __forceinline__ float wave_reduce_add (float v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor (v, o, 64);
+  return v;
+}
+
 __device__ __forceinline__ float wave_reduce_min (float v)
 {
 #pragma unroll

@@ -165,8 +165,8 @@ int drx_bvh_widen (const drx_bvh *bvh, float *out_wide, int64_t capacity_nodes, 
     int32_t n_next = 0;
     int rc = exclusive_scan_i32 (n_inner, base, n_f, &n_next, st);
     if (rc) return rc;
-    if (total + n_next > (1ll << 24))
-      return fail (DRX_ERR_UNSUPPORTED, "drx_bvh_widen: more than 2^24 wide nodes");
+    if (total + n_next > (1ll << 31) - 1)
+      return fail (DRX_ERR_UNSUPPORTED, "drx_bvh_widen: more than 2^31 wide nodes");
     if (out_wide && total + n_next > capacity_nodes)
       return fail (DRX_ERR_ARG, "drx_bvh_widen: capacity %lld nodes is too small", (long long)capacity_nodes);
     int32_t *nf = nullptr;

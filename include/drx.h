@@ -26,6 +26,8 @@
  *   drx_color_table_sample  ColorTable::correct_opacity + sample      color_table.cpp:152-324
  *   drx_mesh_pack           (no counterpart: GridFunction -> element-local SoA conversion done once at
  *                           upload; replaces DeviceMesh::get_elem's indirect reads, device_mesh.hpp:79-96)
+ *   drx_mlp_*               (no counterpart: learned Newton initial guess; dray starts Newton from
+ *                           LocateHack::eval_inverse's guess, device_mesh.hpp:131-165,327-334)
  */
 #ifndef DRX_H
 #define DRX_H
@@ -90,7 +92,7 @@ typedef struct drx_bvh
 enum drx_init_mode
 {
   DRX_INIT_SUBREF_CENTRE = 0, /* Newton starts at the candidate leaf's SubRef centre */
-  DRX_INIT_MLP = 1            /* learned initial guess (not available in this build: DRX_ERR_UNSUPPORTED) */
+  DRX_INIT_MLP = 1            /* learned initial guess (opts->mlp), plain SubRef-centre Newton as fallback */
 };
 
 typedef struct drx_locate_opts
@@ -102,13 +104,30 @@ typedef struct drx_locate_opts
   int32_t *candidates;        /* device, nullable: candidate leaves tried, per point */
   int32_t traversal;          /* drx_traversal: how a wavefront walks the BVH (same results either way) */
   int32_t coherent_order;     /* drx_locate: 1 = process queries in Morton order (results stay in input order) */
+  const void *mlp;            /* DRX_INIT_MLP: packed initializer (drx_mlp_pack), device */
+  int32_t mlp_newton_steps;   /* DRX_INIT_MLP: Newton budget from the learned guess before the plain
+                                 fallback (<= 0: 4) */
 } drx_locate_opts;
+
+/* Training schedule of the learned initializer (drx_mlp_train): Adam, cosine decay lr -> lr_final over
+ * total_steps.  Step k reads rows [sample0 + k*batch, +batch) of the data set, wrapping around. */
+typedef struct drx_mlp_train_opts
+{
+  int32_t batch;              /* rows per step, a multiple of 64 */
+  int32_t steps;              /* steps this call runs */
+  int32_t step0;              /* Adam steps already taken (bias correction, schedule position) */
+  int32_t total_steps;        /* length of the schedule (<= 0: step0 + steps) */
+  int64_t sample0;            /* first row of this call */
+  float lr, lr_final;         /* lr_final < 0: constant lr */
+  float beta1, beta2, eps;    /* <= 0: 0.9, 0.999, 1e-8 */
+} drx_mlp_train_opts;
 
 enum drx_traversal
 {
   DRX_TRAVERSE_AUTO = 0,      /* library default for the entry point */
   DRX_TRAVERSE_LANE = 1,      /* every lane walks its own depth-first path */
-  DRX_TRAVERSE_WAVE = 2       /* the wavefront walks the union of its lanes' paths with uniform node reads */
+  DRX_TRAVERSE_WAVE = 2,      /* the wavefront walks the union of its lanes' paths with uniform node reads */
+  DRX_TRAVERSE_BINARY = 3     /* every lane walks dray's binary tree (ignores bvh->wide_nodes) */
 };
 
 typedef struct drx_integrate_stats
@@ -196,6 +215,25 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
 /* Partials (grouped per pixel, as integrate emits them) -> RGBA image over a white background. */
 int drx_composite (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba,
                    void *stream);
+
+/* ---- learned Newton initializer ----------------------------------------------------------- */
+/* MLP 27 -> 64 -> 64 -> 64 -> 64 -> 3, GELU (sigmoid form), BF16 weights/activations, FP32 accumulate.
+ * Features (bf16, 32 per row, 27 used): the query and the element's 8 corner control points in the frame
+ * of the corner box; the network predicts a correction to base = (q' + 1) / 2.                     */
+int64_t drx_mlp_param_count (void);   /* fp32 master parameters */
+int64_t drx_mlp_state_bytes (void);   /* training state: master weights, Adam moments, bf16 copies */
+int64_t drx_mlp_packed_bytes (void);  /* inference blob (drx_mlp_pack) */
+int drx_mlp_init (void *state, uint64_t seed, void *stream);
+/* Features of (points[i], locations[i].cell) rows: out_x 32 bf16 per row; out_target (nullable) 3 floats
+ * per row = located ref - base (training target); out_base (nullable) 3 floats.  Unlocated rows: zeros. */
+int drx_mlp_features (const drx_mesh *mesh, const float *points, const void *locations, int64_t n, void *out_x,
+                      float *out_target, float *out_base, void *stream);
+/* opts->steps Adam steps on mean squared error; out_loss (device, nullable) gets each step's MSE. */
+int drx_mlp_train (void *state, const void *x, const float *target, int64_t n, const drx_mlp_train_opts *opts,
+                   float *out_loss, void *stream);
+int drx_mlp_pack (const void *state, void *out_packed, void *stream);
+/* out[3 i + d] = base[3 i + d] (or 0 when base is NULL) + MLP(x_i)[d]. */
+int drx_mlp_infer (const void *packed, const void *x, const float *base, int64_t n, float *out, void *stream);
 
 /* ---- host-side colour table ---------------------------------------------------------------- */
 /* ColorTable with rgb points (pos, r, g, b, a) and alpha points (pos, alpha), host arrays; opacity

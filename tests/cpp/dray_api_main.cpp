@@ -1,0 +1,161 @@
+// Drives the whole volume-render path through dray's class names (include/dray/drx_dray.hpp), the way a
+// dray application does (dray's tests/t_dray_volume_render.cpp: mesh -> Camera -> Volume::integrate ->
+// composite), reading a mesh from raw files and writing every intermediate result back as raw files for
+// the Python test to compare.
+//
+//   dray_api_test <in_dir> <out_dir>
+// in_dir:  params.txt  "order n_el n_pts width height samples alpha0 fov px py pz lx ly lz"
+//          ctrl.bin (int32), coords.bin (float32 xyz), field.bin (float32), points.bin (float32 xyz)
+// out_dir: inner.bin leaf.bin ids.bin bounds.bin loc.bin vals.bin rays.bin partials.bin image.bin
+//          stats.txt, and "errors ok" on stdout once the error paths have thrown as expected.
+#include "dray/drx_dray.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+#include <vector>
+
+using namespace dray;
+
+template <typename T> static std::vector<T> read_raw (const std::string &path)
+{
+  std::ifstream f (path, std::ios::binary | std::ios::ate);
+  if (!f) throw DRayError ("cannot open " + path);
+  const size_t bytes = (size_t)f.tellg ();
+  std::vector<T> v (bytes / sizeof (T));
+  f.seekg (0);
+  f.read ((char *)v.data (), (std::streamsize)bytes);
+  return v;
+}
+
+template <typename T> static void write_raw (const std::string &path, const T *p, size_t n)
+{
+  std::ofstream f (path, std::ios::binary);
+  f.write ((const char *)p, (std::streamsize)(n * sizeof (T)));
+  if (!f) throw DRayError ("cannot write " + path);
+}
+
+template <typename T> static void write_array (const std::string &path, const Array<T> &a)
+{
+  write_raw (path, a.get_host_ptr_const (), a.size ());
+}
+
+template <typename T> static Array<T> to_array (const std::vector<T> &v)
+{
+  Array<T> a (v.size ());
+  T *p = a.get_host_ptr ();
+  for (size_t i = 0; i < v.size (); ++i) p[i] = v[i];
+  return a;
+}
+
+template <typename F> static bool throws (F f)
+{
+  try
+  {
+    f ();
+  }
+  catch (const DRayError &)
+  {
+    return true;
+  }
+  return false;
+}
+
+int main (int argc, char **argv)
+{
+  if (argc != 3)
+  {
+    std::fprintf (stderr, "usage: %s <in_dir> <out_dir>\n", argv[0]);
+    return 2;
+  }
+  const std::string in = argv[1], out = argv[2];
+  try
+  {
+    drx_check (drx_init (0), "drx_init");
+    int order, n_el, n_pts, width, height, samples;
+    float alpha0, fov, pos[3], look[3];
+    {
+      std::ifstream pf (in + "/params.txt");
+      pf >> order >> n_el >> n_pts >> width >> height >> samples >> alpha0 >> fov >> pos[0] >> pos[1] >> pos[2] >>
+      look[0] >> look[1] >> look[2];
+      if (!pf) throw DRayError ("bad params.txt");
+    }
+    const std::vector<int32> ctrl = read_raw<int32> (in + "/ctrl.bin");
+    const std::vector<Vec<Float, 3>> coords = read_raw<Vec<Float, 3>> (in + "/coords.bin");
+    const std::vector<Vec<Float, 1>> field = read_raw<Vec<Float, 1>> (in + "/field.bin");
+    const std::vector<Vec<Float, 3>> points = read_raw<Vec<Float, 3>> (in + "/points.bin");
+
+    GridFunction<3> gf;
+    gf.m_ctrl_idx = to_array (ctrl);
+    gf.m_values = to_array (coords);
+    gf.m_size_el = n_el;
+    GridFunction<1> ff;
+    ff.m_ctrl_idx = gf.m_ctrl_idx;   // shallow: same connectivity as dray's field on the mesh's dofs
+    ff.m_values = to_array (field);
+    ff.m_size_el = n_el;
+
+    UnstructuredMesh mesh (gf, order);
+    UnstructuredField fld (ff, order, "f");
+
+    // BVH (LinearBVHBuilder through construct_bvh)
+    BVH &bvh = mesh.get_bvh ();
+    write_array (out + "/inner.bin", bvh.m_inner_nodes);
+    write_array (out + "/leaf.bin", bvh.m_leaf_nodes);
+    write_array (out + "/ids.bin", bvh.m_aabb_ids);
+    AABB<> b = mesh.bounds ();
+    write_raw (out + "/bounds.bin", &b.m_ranges[0].m_min, 6);
+
+    // Mesh::locate + Field::eval
+    Array<Vec<Float, 3>> wp = to_array (points);
+    Array<Location> locs = mesh.locate (wp);
+    Array<Float> vals;
+    fld.eval (locs, vals);
+    write_array (out + "/loc.bin", locs);
+    write_array (out + "/vals.bin", vals);
+
+    // Camera -> Volume::integrate -> composite
+    Camera cam;
+    cam.set_width (width);
+    cam.set_height (height);
+    cam.set_fov (fov);
+    cam.set_pos ({ { pos[0], pos[1], pos[2] } });
+    cam.set_look_at ({ { look[0], look[1], look[2] } });
+    Array<Ray> rays;
+    cam.create_rays (rays);
+    write_array (out + "/rays.bin", rays);
+
+    Volume vol (mesh, fld);
+    vol.use_lighting (false);
+    vol.samples (samples);
+    vol.tile_order (width, height);
+    ColorTable &tab = vol.color_table ();
+    tab.add_point (0.f, Vec3f { { 0.5f, 0.5f, 0.5f } });
+    tab.add_point (1.f, Vec3f { { 1.f, 0.f, 0.f } });
+    tab.add_alpha (0.f, alpha0);
+    tab.set_smooth (true);
+    Array<VolumePartial> parts = vol.integrate (rays);
+    Array<Vec<float32, 4>> img = vol.composite (parts, width, height);
+    write_array (out + "/partials.bin", parts);
+    write_array (out + "/image.bin", img);
+    const drx_integrate_stats st = vol.last_stats ();
+    {
+      std::ofstream sf (out + "/stats.txt");
+      sf << st.located_samples << " " << st.newton_steps << " " << st.candidates << " " << parts.size () << "\n";
+    }
+
+    // error behaviour mirrors dray's DRAY_ERROR checks
+    bool ok = true;
+    ok &= throws ([] { Camera c; c.set_width (0); });
+    ok &= throws ([] { Camera c; c.set_fov (181.f); });
+    ok &= throws ([&] { UnstructuredMesh m (gf, 5); });
+    ok &= throws ([&] { Volume v (mesh, fld); Array<Ray> r = rays; v.integrate (r); });   // lit shading
+    std::printf ("%s\n", ok ? "errors ok" : "errors FAILED");
+    return ok ? 0 : 1;
+  }
+  catch (const std::exception &e)
+  {
+    std::fprintf (stderr, "dray_api_test: %s\n", e.what ());
+    return 1;
+  }
+}

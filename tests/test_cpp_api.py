@@ -1,0 +1,99 @@
+"""The C++ host layer (include/dray/drx_dray.hpp): dray's class names over the C ABI.
+
+tests/cpp/dray_api_test drives mesh -> BVH -> locate -> field eval -> camera -> Volume::integrate ->
+composite through those classes in a process of its own; every output must be bit-identical to the same
+calls made through the C ABI from Python, whose parity with the oracle the other GPU tests establish, and
+the BVH bit-identical to the oracle's.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from devil_ray_amd import build as B
+from devil_ray_amd import synthetic as syn
+from devil_ray_amd.synthetic import LOCATION_DTYPE, PARTIAL_DTYPE, RAY_DTYPE
+
+BIN = B.CPP_TEST_BIN
+
+
+def _need_bin():
+    if not os.path.exists(BIN):
+        pytest.fail(f"{BIN} is not built (run devil_ray_amd/build.py)")
+
+
+def test_cpp_program_links_and_reports_usage():
+    _need_bin()
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_library_exports_dray_classes():
+    out = subprocess.run(["nm", "-DC", "--defined-only", B.LIB], capture_output=True, text=True, check=True).stdout
+    for sym in ("dray::UnstructuredMesh::get_bvh()", "dray::UnstructuredMesh::locate(", "dray::Volume::integrate(",
+                "dray::Volume::composite(", "dray::LinearBVHBuilder::construct(", "dray::Camera::create_rays(",
+                "dray::UnstructuredField::eval(", "dray::ColorTable::sample("):
+        assert sym in out, f"libdrx.so does not export {sym}"
+
+
+@pytest.mark.gpu
+def test_cpp_api_matches_c_abi(gpu, orc, tmp_path):
+    import torch
+    _need_bin()
+    mesh = syn.hex_mesh(5, 3, 0.1, 0.01, 2)
+    w, h, samples, opacity = 40, 36, 96, 0.02
+    alpha0 = syn.table_alpha_for_corrected(opacity, samples)
+    pos, look = (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)
+    ind, outd = tmp_path / "in", tmp_path / "out"
+    ind.mkdir()
+    outd.mkdir()
+    ob = orc.build_mesh_bvh(mesh)
+    pts = syn.query_points(20000, ob.bounds, seed=5)
+    mesh.ctrl_idx.astype(np.int32).tofile(ind / "ctrl.bin")
+    mesh.values.astype(np.float32).tofile(ind / "coords.bin")
+    mesh.field.astype(np.float32).tofile(ind / "field.bin")
+    pts.astype(np.float32).tofile(ind / "points.bin")
+    (ind / "params.txt").write_text(" ".join(str(x) for x in (
+        mesh.order, mesh.n_el, len(pts), w, h, samples, repr(float(alpha0)), 45.0, *pos, *look)))
+    print(f"[cpp api] p3 n5, {len(pts)} points, {w}x{h} @ {samples} spp", flush=True)
+    r = subprocess.run([BIN, str(ind), str(outd)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "errors ok" in r.stdout
+
+    # BVH: bit-exact with the oracle
+    inner = np.fromfile(outd / "inner.bin", np.float32).reshape(-1, 16)
+    assert np.array_equal(inner[:, :14].view(np.uint32), ob.inner[:, :14].view(np.uint32))
+    assert np.array_equal(np.fromfile(outd / "leaf.bin", np.int32), ob.leaf_nodes)
+    assert np.array_equal(np.fromfile(outd / "ids.bin", np.int32), ob.aabb_ids)
+    assert np.array_equal(np.fromfile(outd / "bounds.bin", np.float32).view(np.uint32), ob.bounds.view(np.uint32))
+
+    # the same calls from Python
+    m = gpu.UnstructuredMesh(mesh)
+    f = gpu.UnstructuredField(mesh)
+    loc = m.locate(gpu.to_device(pts.reshape(-1)))
+    vals = f.eval(loc)
+    torch.cuda.synchronize()
+    assert np.array_equal(np.fromfile(outd / "loc.bin", np.uint8), loc.cpu().numpy())
+    assert np.array_equal(np.fromfile(outd / "vals.bin", np.float32).view(np.uint32),
+                          vals.cpu().numpy().view(np.uint32))
+    rays = gpu.Camera(w, h, 45.0, pos=pos, look_at=look).create_rays()
+    assert np.array_equal(np.fromfile(outd / "rays.bin", np.uint8), rays.cpu().numpy())
+    vol = gpu.Volume(m, f)
+    vol.use_lighting = False
+    vol.samples(samples)
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, alpha0)
+    vol.table.set_smooth(True)
+    parts, n = vol.integrate(rays, tile_width=w, tile_height=h)
+    img = gpu.composite(parts, n, w, h)
+    torch.cuda.synchronize()
+    cp = np.fromfile(outd / "partials.bin", np.uint8)
+    assert len(cp) == n * PARTIAL_DTYPE.itemsize
+    assert np.array_equal(cp, parts.cpu().numpy()[: n * PARTIAL_DTYPE.itemsize])
+    assert np.array_equal(np.fromfile(outd / "image.bin", np.float32).view(np.uint32),
+                          img.cpu().numpy().view(np.uint32))
+    st = (outd / "stats.txt").read_text().split()
+    assert int(st[0]) == vol.last_stats.located_samples and int(st[3]) == n
+    assert LOCATION_DTYPE.itemsize == 16 and RAY_DTYPE.itemsize == 36
