@@ -52,6 +52,17 @@ class DrxCamera(C.Structure):
                 ("up", F32 * 3), ("zoom", F32)]
 
 
+class DrxPointLight(C.Structure):
+    _fields_ = [("pos", F32 * 3), ("amb", F32 * 3), ("diff", F32 * 3), ("spec", F32 * 3), ("spec_pow", F32)]
+
+
+class DrxShading(C.Structure):
+    _fields_ = [("use_lighting", I32), ("n_lights", I32), ("lights", C.POINTER(DrxPointLight))]
+
+
+MAX_LIGHTS = 8
+
+
 _SIGS = {
     "drx_init": (I32, [I32]),
     "drx_last_error": (C.c_char_p, []),
@@ -70,8 +81,9 @@ _SIGS = {
     "drx_remove_missed_rays": (I32, [P, I32, P, P, C.POINTER(I32), P]),
     "drx_ray_tile_order": (I32, [P, I32, I32, I32, P, P]),
     "drx_volume_integrate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I32, P, I32, F32,
-                                   F32, F32, P, C.POINTER(DrxLocateOpts), P, C.POINTER(I32),
-                                   C.POINTER(DrxIntegrateStats), P]),
+                                   F32, F32, P, C.POINTER(DrxLocateOpts), C.POINTER(DrxShading), P,
+                                   C.POINTER(I32), C.POINTER(DrxIntegrateStats), P]),
+    "drx_default_light": (I32, [C.POINTER(DrxCamera), C.POINTER(DrxPointLight)]),
     "drx_composite": (I32, [P, I32, I32, I32, P, P]),
     "drx_color_table_sample": (I32, [P, I32, P, I32, I32, F32, I32, F32, P]),
     "drx_mlp_param_count": (I64, []),

@@ -351,7 +351,7 @@ class Volume:
         self.mesh = mesh
         self.field_ = field
         self.m_samples = 100
-        self.use_lighting = True   # dray default; only unlit shading is implemented (raise otherwise)
+        self.use_lighting = True   # dray default (VolumeShader::shaded_color with the given lights)
         self.table = ColorTable()
         self.range = None
         self.opts = LocateOptions()
@@ -363,10 +363,21 @@ class Volume:
     def color_map_range(self, lo: float, hi: float):
         self.range = (float(lo), float(hi))
 
-    def integrate(self, rays: torch.Tensor, tile_width: int = 0, tile_height: int = 0, active=None):
-        """Volume::integrate (volume.cpp:323-351) -> (compacted partials uint8 tensor, count)."""
-        if self.use_lighting:
-            raise L.DrxError("Volume::integrate: lit shading is not implemented; call use_lighting=False")
+    def integrate(self, rays: torch.Tensor, tile_width: int = 0, tile_height: int = 0, active=None, lights=None):
+        """Volume::integrate(rays, lights) (volume.cpp:323-351) -> (compacted partials uint8 tensor, count).
+        lights: PointLights (13 floats each: pos, amb, diff, spec, spec_pow), used when use_lighting; with
+        no lights a lit sample contributes nothing, as in dray (the Renderer passes default_light)."""
+        lt = np.zeros((0, 13), np.float32) if lights is None else np.asarray(lights, np.float32).reshape(-1, 13)
+        if self.use_lighting and lt.shape[0] > L.MAX_LIGHTS:
+            raise L.DrxError(f"Volume::integrate: at most {L.MAX_LIGHTS} lights")
+        larr = (L.DrxPointLight * max(1, lt.shape[0]))()
+        for i, row in enumerate(lt):
+            larr[i].pos[:] = row[0:3].tolist()
+            larr[i].amb[:] = row[3:6].tolist()
+            larr[i].diff[:] = row[6:9].tolist()
+            larr[i].spec[:] = row[9:12].tolist()
+            larr[i].spec_pow = float(row[12])
+        shading = L.DrxShading(1 if self.use_lighting else 0, int(lt.shape[0]), larr)
         bounds = self.mesh.bounds()
         if active is None:
             active, n_act = remove_missed_rays(rays, bounds)
@@ -382,10 +393,18 @@ class Volume:
         o = self.opts.c()
         mv, fv, bv = self.mesh.gf.view(), self.field_.gf.view(), self.mesh.get_bvh().view()
         L.check(L.lib().drx_volume_integrate(C.byref(mv), C.byref(fv), C.byref(bv), _ptr(active), n_act, _ptr(lut),
-                                             1024, lo, hi, sd, _ptr(order), C.byref(o), _ptr(out), C.byref(cnt),
-                                             C.byref(st), _stream()), "drx_volume_integrate")
+                                             1024, lo, hi, sd, _ptr(order), C.byref(o), C.byref(shading), _ptr(out),
+                                             C.byref(cnt), C.byref(st), _stream()), "drx_volume_integrate")
         self.last_stats = st
         return out, int(cnt.value)
+
+
+def default_light(camera: "Camera") -> np.ndarray:
+    """detail::default_light (rendering/renderer.cpp:141-159): 13 floats (pos, amb, diff, spec, spec_pow)."""
+    cc = camera.c()
+    out = L.DrxPointLight()
+    L.check(L.lib().drx_default_light(C.byref(cc), C.byref(out)), "drx_default_light")
+    return np.array(list(out.pos) + list(out.amb) + list(out.diff) + list(out.spec) + [out.spec_pow], np.float32)
 
 
 def composite(partials: torch.Tensor, n: int, width: int, height: int) -> torch.Tensor:

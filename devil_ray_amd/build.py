@@ -21,6 +21,7 @@ SOURCES = {
     "drx_runtime.hip": [],
     "drx_lbvh.hip": ["-ffp-contract=off"],
     "drx_query.hip": [],
+    "drx_integrate.hip": [],
     "drx_wide.hip": [],
     "drx_mlp.hip": [],
     "dray_host.hip": [],

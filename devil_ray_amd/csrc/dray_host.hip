@@ -290,6 +290,44 @@ void Camera::set_fov (const float32 &degrees)
   if (degrees > 180) throw DRayError ("Camera feild of view must be less than 180.");
   m_fov_y = degrees;
 }
+drx_camera Camera::view () const
+{
+  drx_camera c;
+  c.width = m_width;
+  c.height = m_height;
+  c.fov_y = m_fov_y;
+  c.zoom = m_zoom;
+  for (int i = 0; i < 3; ++i)
+  {
+    c.pos[i] = m_pos[i];
+    c.look_at[i] = m_look_at[i];
+    c.up[i] = m_up[i];
+  }
+  return c;
+}
+
+PointLight make_point_light (const Vec3f &pos)
+{
+  PointLight l;
+  for (int i = 0; i < 3; ++i)
+  {
+    l.pos[i] = pos[i];
+    l.amb[i] = 0.4f;
+    l.diff[i] = 0.75f;
+    l.spec[i] = 0.8f;
+  }
+  l.spec_pow = 15.f;
+  return l;
+}
+
+PointLight default_light (Camera &camera)
+{
+  const drx_camera c = camera.view ();
+  PointLight l;
+  drx_check (drx_default_light (&c, &l), "default_light");
+  return l;
+}
+
 void Camera::create_rays (Array<Ray> &rays)
 {
   rays.resize ((size_t)m_width * m_height);
@@ -325,7 +363,18 @@ Volume::Volume (UnstructuredMesh &mesh, UnstructuredField &field) : m_mesh (mesh
 
 Array<VolumePartial> Volume::integrate (Array<Ray> &rays)
 {
-  if (m_use_lighting) throw DRayError ("Volume::integrate: lit shading is not available; use_lighting(false)");
+  Array<PointLight> none;
+  return integrate (rays, none);
+}
+
+Array<VolumePartial> Volume::integrate (Array<Ray> &rays, Array<PointLight> &lights)
+{
+  if (m_use_lighting && lights.size () > DRX_MAX_LIGHTS)
+    throw DRayError ("Volume::integrate: at most " + std::to_string (DRX_MAX_LIGHTS) + " lights");
+  drx_shading shading;
+  shading.use_lighting = m_use_lighting ? 1 : 0;
+  shading.n_lights = (int32)lights.size ();
+  shading.lights = lights.size () ? lights.get_host_ptr_const () : nullptr;
   if (!m_range_set) m_range = m_field.range ();
   AABB<> b = m_mesh.bounds ();
   const float bounds6[6] = { b.m_ranges[0].m_min, b.m_ranges[0].m_max, b.m_ranges[1].m_min,
@@ -357,7 +406,8 @@ Array<VolumePartial> Volume::integrate (Array<Ray> &rays)
   o.cond_threshold = 1e4f;
   drx_check (drx_volume_integrate (&mv, &fv, &bv, active.get_device_ptr_const (), n_active,
                                    (const float *)lut.get_device_ptr_const (), 1024, m_range.m_min, m_range.m_max,
-                                   sample_dist, order.size () ? order.get_device_ptr_const () : nullptr, &o, partials.get_device_ptr (), &count, &m_stats, nullptr),
+                                   sample_dist, order.size () ? order.get_device_ptr_const () : nullptr, &o,
+                                   &shading, partials.get_device_ptr (), &count, &m_stats, nullptr),
              "Volume::integrate");
   // keep the first `count` partials (compacted, ray order)
   Array<VolumePartial> out ((size_t)count);

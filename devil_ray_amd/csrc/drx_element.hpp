@@ -193,6 +193,42 @@ __device__ __forceinline__ float eval_scalar (const float *__restrict__ f, const
   return val;
 }
 
+// Scalar value and reference-space gradient (for lit shading).
+template <int P>
+__device__ __forceinline__ float eval_scalar_d (const float *__restrict__ f, const float x[3], float d[3])
+{
+  constexpr int N1 = Ord<P>::N1;
+  float Bu[N1], dBu[N1], Bv[N1], dBv[N1], Bw[N1], dBw[N1];
+  bernstein<P, float> (x[0], Bu, dBu);
+  bernstein<P, float> (x[1], Bv, dBv);
+  bernstein<P, float> (x[2], Bw, dBw);
+  float val = 0.f, du = 0.f, dv = 0.f, dw = 0.f;
+#pragma unroll
+  for (int k = 0; k < N1; ++k)
+  {
+#pragma unroll
+    for (int j = 0; j < N1; ++j)
+    {
+      float a = 0.f, b = 0.f;
+      const int row = N1 * (j + N1 * k);
+#pragma unroll
+      for (int i = 0; i < N1; ++i)
+      {
+        a += Bu[i] * f[row + i];
+        b += dBu[i] * f[row + i];
+      }
+      val += Bv[j] * Bw[k] * a;
+      du += Bv[j] * Bw[k] * b;
+      dv += dBv[j] * Bw[k] * a;
+      dw += Bv[j] * dBw[k] * a;
+    }
+  }
+  d[0] = du;
+  d[1] = dv;
+  d[2] = dw;
+  return val;
+}
+
 // Crout LUP with partial pivoting, as matrix_mult_inv (matrix.hpp:221-373,461-477).  A is consumed.
 template <typename T>
 __device__ __forceinline__ void lup_solve3 (T A[3][3], const T b[3], T x[3])

@@ -1,0 +1,508 @@
+// Volume::integrate on gfx950: the per-ray march (locate + field / lit shading + LUT + blend, <= 5
+// partials per ray) and partial compaction.
+#include "drx_common.hpp"
+#include "drx_element.hpp"
+#include "drx_locate_mlp.hpp"
+#include "drx_query_common.hpp"
+#include "drx_wave.hpp"
+
+#include <cmath>
+
+namespace drx
+{
+
+// ---------------------------------------------------------------------------------------------
+// Volume integration: integrate_partials' per-ray loop (rendering/volume.cpp:134-217).
+// ---------------------------------------------------------------------------------------------
+struct IntegrateArgs
+{
+  LocateCfg cfg;
+  const float *field;
+  const int64_t *field_offsets;
+  const Ray *rays;
+  const int32_t *order;
+  int n_rays;
+  const float4 *lut;
+  int lut_size;
+  float fmin, fmax, inv_range;
+  float sample_dist;
+  Partial *slots;
+  unsigned long long *stats;   // located, newton, candidates, polished
+  int32_t *err;
+  MlpNet net;                  // learned initial guess (MODE 3)
+  int budget;                  // Newton steps from the learned guess before the plain fallback
+  int n_lights;                // lit shading (LIT kernels)
+  drx_point_light lights[DRX_MAX_LIGHTS];
+};
+
+template <int P>
+__device__ __forceinline__ float field_at (const IntegrateArgs &a, int cell, const float ref[3])
+{
+  if constexpr (P == 0)
+  {
+    const float *f = a.field + a.field_offsets[cell];
+    switch (a.cfg.orders[cell])
+    {
+    case 1: return eval_scalar<1> (f, ref);
+    case 2: return eval_scalar<2> (f, ref);
+    case 3: return eval_scalar<3> (f, ref);
+    default: return eval_scalar<4> (f, ref);
+    }
+  }
+  else
+    return eval_scalar<P> (a.field + (int64_t)cell * Ord<P>::FSTRIDE, ref);
+}
+
+// MODE: 0 binary per-lane traversal, 1 8-wide per-lane, 2 8-wide wave-coherent (identical results).
+__device__ __forceinline__ float dot3 (const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ __forceinline__ void normalize3 (float v[3])
+{
+  const float m = sqrtf (dot3 (v, v));
+  v[0] /= m;
+  v[1] /= m;
+  v[2] /= m;
+}
+__device__ __forceinline__ float clamp01f (float v) { return fminf (1.f, fmaxf (0.f, v)); }
+
+// VolumeShader::scalar_gradient + shaded_color (rendering/volume_shader.hpp:39-158) for order P:
+// physical gradient = ref gradient x J^-1 (three LUP solves, like matrix_inverse), then per light
+// ambient + diffuse + Blinn-Phong specular, gradient flipped toward the light and the half vector
+// (flips persist across lights), channels clamped after each light; alpha <= 0.01 gives zero.
+template <int P>
+__device__ float4 lit_color_at (const IntegrateArgs &a, const float *g, const float *f, const float ref[3],
+                                const Ray &ray)
+{
+  float y[3], J[3][3];
+  eval_geom<P, float> (g, ref, y, J);
+  float dref[3];
+  const float scalar = eval_scalar_d<P> (f, ref, dref);
+  float inv[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    float A[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) A[r][q] = J[r][q];
+    const float e[3] = { c == 0 ? 1.f : 0.f, c == 1 ? 1.f : 0.f, c == 2 ? 1.f : 0.f };
+    float x[3];
+    lup_solve3<float> (A, e, x);
+    inv[0][c] = x[0];
+    inv[1][c] = x[1];
+    inv[2][c] = x[2];
+  }
+  float grad[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) grad[c] = dref[0] * inv[0][c] + dref[1] * inv[1][c] + dref[2] * inv[2][c];
+  const float4 col = color_lookup (a.lut, a.lut_size, a.fmin, a.fmax, a.inv_range, scalar);
+  float4 acc = make_float4 (0.f, 0.f, 0.f, 0.f);
+  if ((double)col.w > 0.01)
+  {
+    normalize3 (grad);
+    const float wp[3] = { g[0] + y[0], g[1] + y[1], g[2] + y[2] };
+    const float view[3] = { -ray.dir[0], -ray.dir[1], -ray.dir[2] };
+    for (int l = 0; l < a.n_lights; ++l)
+    {
+      const drx_point_light &L = a.lights[l];
+      float ld[3] = { L.pos[0] - wp[0], L.pos[1] - wp[1], L.pos[2] - wp[2] };
+      normalize3 (ld);
+      if (dot3 (ld, grad) < 0.f)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) grad[i] = -grad[i];
+      const float diffuse = clamp01f (dot3 (ld, grad));
+      float sc[3] = { L.amb[0] * col.x, L.amb[1] * col.y, L.amb[2] * col.z };
+      sc[0] += diffuse * L.diff[0] * col.x;
+      sc[1] += diffuse * L.diff[1] * col.y;
+      sc[2] += diffuse * L.diff[2] * col.z;
+      float hv[3] = { 0.5f * (view[0] + ld[0]), 0.5f * (view[1] + ld[1]), 0.5f * (view[2] + ld[2]) };
+      normalize3 (hv);
+      if (dot3 (grad, hv) < 0.f)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) grad[i] = -grad[i];
+      const float doth = clamp01f (dot3 (grad, hv));
+      const float intensity = powf (doth, L.spec_pow);
+      sc[0] += intensity * L.spec[0] * col.x;
+      sc[1] += intensity * L.spec[1] * col.y;
+      sc[2] += intensity * L.spec[2] * col.z;
+      acc.x = clamp01f (acc.x + sc[0]);
+      acc.y = clamp01f (acc.y + sc[1]);
+      acc.z = clamp01f (acc.z + sc[2]);
+      acc.w += col.w;
+    }
+  }
+  return acc;
+}
+
+template <int P>
+__device__ __forceinline__ float4 lit_color (const IntegrateArgs &a, int cell, const float ref[3], const Ray &ray)
+{
+  if constexpr (P == 0)
+  {
+    const float *g = a.cfg.mesh + a.cfg.offsets[cell];
+    const float *f = a.field + a.field_offsets[cell];
+    switch (a.cfg.orders[cell])
+    {
+    case 1: return lit_color_at<1> (a, g, f, ref, ray);
+    case 2: return lit_color_at<2> (a, g, f, ref, ray);
+    case 3: return lit_color_at<3> (a, g, f, ref, ray);
+    default: return lit_color_at<4> (a, g, f, ref, ray);
+    }
+  }
+  else
+    return lit_color_at<P> (a, a.cfg.mesh + (int64_t)cell * Ord<P>::GSTRIDE, a.field + (int64_t)cell * Ord<P>::FSTRIDE,
+                            ref, ray);
+}
+
+template <int P, bool POLISH, int MODE, bool LIT>
+__global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (IntegrateArgs a)
+{
+  constexpr bool WAVE = MODE == 2;
+  constexpr bool LEARNED = MODE == 3;   // learned Newton start; the whole wavefront stays in the loop
+  __shared__ int32_t stack_lds[WAVE ? 1 : kStackLds * 256];
+  __shared__ int32_t w_code[WAVE ? 4 * kWaveStack : 1];
+  __shared__ uint64_t w_mask[WAVE ? 4 * kWaveStack : 1];
+  __shared__ int32_t w_cand[WAVE ? 4 * kCandQ * 64 : 1];
+  __shared__ __attribute__ ((aligned (16))) __bf16 mlp_x[LEARNED ? 4 * 64 * mlp::IN_PAD : 8];
+  __shared__ __attribute__ ((aligned (16))) float mlp_y[LEARNED ? 4 * 64 * 4 : 4];
+  WaveScratch ws;
+  const int wv = threadIdx.x >> 6;
+  {
+    ws.st_code = w_code + (WAVE ? wv * kWaveStack : 0);
+    ws.st_mask = w_mask + (WAVE ? wv * kWaveStack : 0);
+    ws.cand = w_cand + (WAVE ? wv * kCandQ * 64 : 0);
+  }
+  TravStack todo;
+  todo.lds = stack_lds;
+  todo.bdim = blockDim.x;
+  todo.tid = threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool has_ray = t < a.n_rays;
+  unsigned long long located = 0;
+  LocateStats st { 0, 0, 0, 0 };
+  bool overflow = false;
+  if (LEARNED || has_ray)
+  {
+    const int ri = has_ray ? (a.order ? a.order[t] : t) : 0;
+    Ray ray;
+    if (has_ray)
+      ray = a.rays[ri];
+    else
+    {
+      ray.dir[0] = ray.dir[1] = ray.dir[2] = 0.f;
+      ray.orig[0] = ray.orig[1] = ray.orig[2] = 0.f;
+      ray.near_ = ray.far_ = 0.f;
+      ray.pixel_id = -1;
+    }
+    Partial *out5 = a.slots + 5 * (int64_t)ri;
+    if (has_ray)
+#pragma unroll
+      for (int s = 0; s < 5; ++s)
+      {
+        Partial p0;
+        p0.pixel_id = -1;
+        p0.depth = INFINITY;
+        p0.rgba[0] = p0.rgba[1] = p0.rgba[2] = p0.rgba[3] = 0.f;
+        out5[s] = p0;
+      }
+    // integrate_partials' per-ray loop as a state machine with one locate() per iteration:
+    //   search: locate at `distance`; on a miss step on; on a hit open a segment and shade the sample;
+    //   inside: locate at the advanced distance; keep shading while inside, before `far` and alpha < 0.95,
+    //           else close the segment (<= 5 per ray); the next search re-locates at the same distance.
+    float distance = ray.near_ + a.sample_dist;
+    int segment = 0;
+    bool inseg = false;
+    float depth = 0.f;
+    float rgba[4] = { 0.f, 0.f, 0.f, 0.f };
+    bool alive = has_ray;
+    for (;;)
+    {
+      if (alive && !inseg && !(distance < ray.far_)) alive = false;
+      if constexpr (LEARNED)
+      {
+        if (__ballot (alive) == 0) break;
+      }
+      else if (!alive)
+        break;
+      const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
+                            ray_point (ray.orig[2], distance, ray.dir[2]) };
+      float ref[3];
+      int cell;
+      if constexpr (LEARNED)
+      {
+        cell = locate_point_mlp<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, alive, a.net, a.budget,
+                                            mlp_x + wv * 64 * mlp::IN_PAD, mlp_y + wv * 64 * 4);
+        if (!alive) continue;
+      }
+      else if constexpr (WAVE)
+        cell = locate_wave<P, POLISH> (a.cfg, ws, pt, ref, st, overflow);
+      else
+        cell = locate_any<P, POLISH, MODE == 1> (a.cfg, todo, pt, ref, st, overflow);
+      if (overflow)
+      {
+        alive = false;
+        continue;
+      }
+      const bool found = cell != -1;
+      if (found) ++located;
+      bool shade = false;
+      bool ray_done = false;
+      if (!inseg)
+      {
+        if (found)
+        {
+          inseg = true;
+          depth = distance;
+          rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
+          shade = true;
+        }
+        else
+          distance += a.sample_dist;
+      }
+      else if (distance < ray.far_ && found && rgba[3] < 0.95f)
+        shade = true;
+      else
+      {
+        Partial pp;
+        pp.pixel_id = ray.pixel_id;
+        pp.depth = depth;
+        pp.rgba[0] = rgba[0];
+        pp.rgba[1] = rgba[1];
+        pp.rgba[2] = rgba[2];
+        pp.rgba[3] = rgba[3];
+        out5[segment] = pp;
+        ++segment;
+        if (distance >= ray.far_ || rgba[3] > 0.95f || segment == 5)
+          ray_done = true;
+        else
+          inseg = false;
+      }
+      if (ray_done)
+      {
+        alive = false;
+        continue;
+      }
+      if (shade)
+      {
+        if constexpr (LIT)
+          blend_nc (rgba, lit_color<P> (a, cell, ref, ray));
+        else
+          blend_nc (rgba, color_lookup (a.lut, a.lut_size, a.fmin, a.fmax, a.inv_range, field_at<P> (a, cell, ref)));
+        distance += a.sample_dist;
+      }
+    }
+  }
+  if (overflow) atomicOr (a.err, 1);
+  // wave-aggregated statistics
+  unsigned long long v[4] = { located, (unsigned long long)st.newton, (unsigned long long)st.candidates,
+                              (unsigned long long)st.polished };
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+  {
+    unsigned long long x = v[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor (x, o, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd (&a.stats[q], x);
+  }
+}
+
+__global__ void k_flag_partials (const Partial *__restrict__ p, int64_t n, int32_t *__restrict__ flags)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = (p[i].depth == INFINITY) ? 0 : 1;
+}
+
+__global__ void k_compact_partials (const Partial *__restrict__ p, int64_t n, const int32_t *__restrict__ flags,
+                                    const int32_t *__restrict__ pos, Partial *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flags[i]) out[pos[i]] = p[i];
+}
+
+} // namespace drx
+
+using namespace drx;
+
+extern "C" {
+
+int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx_bvh *bvh, const void *active_rays,
+                          int32_t n_rays, const float *lut, int32_t lut_size, float field_min, float field_max,
+                          float sample_dist, const int32_t *ray_order, const drx_locate_opts *opts,
+                          const drx_shading *shading, void *out_partials, int32_t *out_count,
+                          drx_integrate_stats *stats, void *stream)
+{
+  int rc = check_mesh (mesh, 3, "drx_volume_integrate(mesh)");
+  if (rc) return rc;
+  if ((rc = check_mesh (field, 1, "drx_volume_integrate(field)"))) return rc;
+  if ((rc = check_bvh (bvh, "drx_volume_integrate"))) return rc;
+  if ((mesh->elem_order == nullptr) != (field->elem_order == nullptr) ||
+      (!mesh->elem_order && mesh->order != field->order))
+    return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: mesh and field orders must match");
+  if (!out_count) return fail (DRX_ERR_ARG, "drx_volume_integrate: null out_count");
+  if (n_rays < 0 || lut_size < 1 || !lut) return fail (DRX_ERR_ARG, "drx_volume_integrate: bad rays/lut");
+  if (!(field_min <= field_max)) return fail (DRX_ERR_ARG, "ColorMap scalar range never set");
+  const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
+  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
+    return fail (DRX_ERR_ARG, "drx_volume_integrate: unknown init_mode %d", opts->init_mode);
+  if (learned && !opts->mlp)
+    return fail (DRX_ERR_ARG, "drx_volume_integrate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
+  const bool lit = shading && shading->use_lighting;
+  if (lit && (shading->n_lights < 0 || shading->n_lights > DRX_MAX_LIGHTS || (shading->n_lights > 0 && !shading->lights)))
+    return fail (DRX_ERR_ARG, "drx_volume_integrate: %d lights (at most %d, host array)", shading->n_lights,
+                 DRX_MAX_LIGHTS);
+  *out_count = 0;
+  if (stats) *stats = drx_integrate_stats { 0, 0, 0, 0, 0.f, 0.f };
+  if (n_rays == 0) return DRX_OK;
+  if (!active_rays || !out_partials) return fail (DRX_ERR_ARG, "drx_volume_integrate: null rays/output");
+  if ((int64_t)n_rays * 5 > (1ll << 31) - 1) return fail (DRX_ERR_ARG, "drx_volume_integrate: too many rays");
+  hipStream_t st = as_stream (stream);
+  const int64_t n_slots = 5 * (int64_t)n_rays;
+  Partial *slots = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n_slots);
+  int32_t *flags = (int32_t *)arena (SLOT_FLAGS, sizeof (int32_t) * 2 * (size_t)n_slots + 64);
+  unsigned long long *dstats = (unsigned long long *)arena (SLOT_STATS, 8 * sizeof (unsigned long long));
+  if (!slots || !flags || !dstats) return DRX_ERR_HIP;
+  int32_t *pos = flags + n_slots;
+  int32_t *err = (int32_t *)(dstats + 4);
+  hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, (int32_t *)dstats, (int64_t)16, 0);
+  IntegrateArgs a;
+  a.cfg = make_cfg (mesh, bvh, opts);
+  a.field = field->data;
+  a.field_offsets = field->elem_offset;
+  a.rays = (const Ray *)active_rays;
+  a.order = ray_order;
+  a.n_rays = n_rays;
+  a.lut = (const float4 *)lut;
+  a.lut_size = lut_size;
+  a.fmin = field_min;
+  a.fmax = field_max;
+  {
+    // rcp_safe(range length) on the host, float32 (device_color_map.hpp:45, math.hpp:213-217)
+    const float len = field_max - field_min;
+    a.inv_range = 1.0f / ((std::fabs (len) < 1e-8f) ? (std::signbit (len) ? -1e-8f : 1e-8f) : len);
+  }
+  a.sample_dist = sample_dist;
+  a.slots = slots;
+  a.stats = dstats;
+  a.err = err;
+  a.net = mlp_net (learned ? opts->mlp : nullptr);
+  a.budget = (learned && opts->mlp_newton_steps > 0) ? opts->mlp_newton_steps : 4;
+  a.n_lights = lit ? shading->n_lights : 0;
+  for (int l = 0; l < a.n_lights; ++l) a.lights[l] = shading->lights[l];
+  const dim3 g (nblk (n_rays, 256)), b (256);
+  const int P = mesh->elem_order ? 0 : mesh->order;
+  const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
+  if (trav == DRX_TRAVERSE_WAVE && !a.cfg.wide)
+    return fail (DRX_ERR_ARG, "drx_volume_integrate: wave traversal needs bvh->wide_nodes");
+  const int mode = learned ? 3 : (!a.cfg.wide || trav == DRX_TRAVERSE_BINARY) ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
+  hipEvent_t ev[3];
+  for (int q = 0; q < 3; ++q) DRX_HIP (hipEventCreate (&ev[q]));
+  DRX_HIP (hipEventRecord (ev[0], st));
+#define DRX_INT2(PP, PO, LT)                                             \
+  do                                                                     \
+  {                                                                      \
+    if (mode == 3)                                                       \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 3, LT>), g, b, 0, st, a); \
+    else if (mode == 2)                                                  \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 2, LT>), g, b, 0, st, a); \
+    else if (mode == 1)                                                  \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 1, LT>), g, b, 0, st, a); \
+    else                                                                 \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 0, LT>), g, b, 0, st, a); \
+  } while (0)
+#define DRX_INT(PP, PO)          \
+  do                             \
+  {                              \
+    if (lit)                     \
+      DRX_INT2 (PP, PO, true);   \
+    else                         \
+      DRX_INT2 (PP, PO, false);  \
+  } while (0)
+  if (a.cfg.polish)
+    switch (P)
+    {
+    case 0: DRX_INT (0, true); break;
+    case 1: DRX_INT (1, true); break;
+    case 2: DRX_INT (2, true); break;
+    case 3: DRX_INT (3, true); break;
+    default: DRX_INT (4, true); break;
+    }
+  else
+    switch (P)
+    {
+    case 0: DRX_INT (0, false); break;
+    case 1: DRX_INT (1, false); break;
+    case 2: DRX_INT (2, false); break;
+    case 3: DRX_INT (3, false); break;
+    default: DRX_INT (4, false); break;
+    }
+#undef DRX_INT
+#undef DRX_INT2
+  DRX_LAUNCH_CHECK ();
+  DRX_HIP (hipEventRecord (ev[1], st));
+  DRX_LAUNCH_CHECK ();
+  hipLaunchKernelGGL (k_flag_partials, dim3 (nblk (n_slots, 256)), dim3 (256), 0, st, slots, n_slots, flags);
+  DRX_LAUNCH_CHECK ();
+  int32_t total = 0;
+  rc = exclusive_scan_i32 (flags, pos, (int32_t)n_slots, &total, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL (k_compact_partials, dim3 (nblk (n_slots, 256)), dim3 (256), 0, st, slots, n_slots, flags, pos,
+                      (Partial *)out_partials);
+  DRX_LAUNCH_CHECK ();
+  DRX_HIP (hipEventRecord (ev[2], st));
+  unsigned long long hs[8];
+  DRX_HIP (hipMemcpyAsync (hs, dstats, sizeof hs, hipMemcpyDeviceToHost, st));
+  DRX_HIP (hipStreamSynchronize (st));
+  float k_ms = 0.f, t_ms = 0.f;
+  DRX_HIP (hipEventElapsedTime (&k_ms, ev[0], ev[1]));
+  DRX_HIP (hipEventElapsedTime (&t_ms, ev[0], ev[2]));
+  for (int q = 0; q < 3; ++q) (void)hipEventDestroy (ev[q]);
+  if (((int32_t *)(hs + 4))[0])
+    return fail (DRX_ERR_OVERFLOW, "drx_volume_integrate: BVH traversal stack (%d entries) overflowed", kStackDepth);
+  *out_count = total;
+  if (stats)
+  {
+    stats->located_samples = (int64_t)hs[0];
+    stats->newton_steps = (int64_t)hs[1];
+    stats->candidates = (int64_t)hs[2];
+    stats->polished = (int64_t)hs[3];
+    stats->kernel_ms = k_ms;
+    stats->total_ms = t_ms;
+  }
+  return DRX_OK;
+}
+
+// detail::default_light (rendering/renderer.cpp:141-159): 0.1 x |look_at - pos| above the camera along its
+// orthonormalised up vector; the other members keep PointLight's defaults.
+int drx_default_light (const drx_camera *cam, drx_point_light *out)
+{
+  if (!cam || !out) return fail (DRX_ERR_ARG, "drx_default_light: null pointer");
+  auto normalize = [] (float v[3]) {
+    float s = 0.f;
+    for (int i = 0; i < 3; ++i) s += v[i] * v[i];
+    const float m = sqrtf (s);
+    for (int i = 0; i < 3; ++i) v[i] /= m;
+  };
+  float up[3] = { cam->up[0], cam->up[1], cam->up[2] };
+  normalize (up);
+  float look[3];
+  for (int i = 0; i < 3; ++i) look[i] = cam->look_at[i] - cam->pos[i];
+  float m2 = 0.f;
+  for (int i = 0; i < 3; ++i) m2 += look[i] * look[i];
+  const float mag = sqrtf (m2);
+  float right[3] = { look[1] * up[2] - look[2] * up[1], look[2] * up[0] - look[0] * up[2], look[0] * up[1] - look[1] * up[0] };
+  normalize (right);
+  float mu[3] = { right[1] * look[2] - right[2] * look[1], right[2] * look[0] - right[0] * look[2],
+                  right[0] * look[1] - right[1] * look[0] };
+  normalize (mu);
+  for (int i = 0; i < 3; ++i)
+  {
+    out->pos[i] = cam->pos[i] + .1f * mag * mu[i];
+    out->amb[i] = 0.4f;
+    out->diff[i] = 0.75f;
+    out->spec[i] = 0.8f;
+  }
+  out->spec_pow = 15.f;
+  return DRX_OK;
+}
+
+} // extern "C"

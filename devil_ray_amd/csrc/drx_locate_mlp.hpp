@@ -146,7 +146,7 @@ __device__ __forceinline__ int locate_point_mlp (const LocateCfg &cfg, TravStack
       for (int k = 0; k < mlp::IN_PAD; ++k) f[k] = 0.f;
     }
     float corr[3];
-    mlp_wave_infer (net, f, corr, lx, ly);
+    mlp_wave_infer (net, f, has, corr, lx, ly);
     if (has)
     {
       float x[3] = { base[0] + corr[0], base[1] + corr[1], base[2] + corr[2] };

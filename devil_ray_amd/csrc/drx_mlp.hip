@@ -221,9 +221,8 @@ __global__ void __launch_bounds__ (64) k_mlp_train (const bf16x8 *__restrict__ x
         for (int r = 0; r < 4; ++r)
         {
           const float z = acc[r];
-          const float sg = gelu_sigmoid (z);
-          const float h = z * sg;
-          const float dg = sg + 1.702f * z * sg * (1.f - sg);
+          const float h = gelu (z);
+          const float dg = gelu_grad (z);
           hv[r] = (__bf16)h;
           hf[l - 1][(16 * ob + 4 * g + r) * 64 + 16 * sb + j] = hv[r];
           gp[l - 1][(16 * sb + 4 * ob + r) * 64 + lane] = (__bf16)dg;

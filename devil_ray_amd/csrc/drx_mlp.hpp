@@ -59,12 +59,26 @@ __host__ __device__ constexpr int t_off (int l) { return l == 2 ? T2 : l == 3 ? 
 __host__ __device__ constexpr int t_pad (int l) { return l == 5 ? 32 : H; }
 } // namespace mlp
 
-// GELU in its sigmoid form x * sigmoid(1.702 x) (Hendrycks & Gimpel, 2016), hardware exp2 and rcp.
-__device__ __forceinline__ float gelu_sigmoid (float x)
+// GELU x * Phi(x) with the normal CDF Phi replaced by a cubic smoothstep over [-2, 2]:
+//   Phi~(x) = 1/2 + u (3/4 - u^2/4),  u = clamp(x / 2, -1, 1)   (|Phi~ - Phi| < 0.025, C1 at +-2)
+// Six full-rate VALU operations and no transcendental; training uses the same function.
+__device__ __forceinline__ float gelu_cdf (float x, float &u)
 {
-  return __builtin_amdgcn_rcpf (1.f + __builtin_amdgcn_exp2f (-2.45546696f * x));   // 1.702 * log2(e)
+  u = __builtin_amdgcn_fmed3f (0.5f * x, -1.f, 1.f);
+  return fmaf (u, fmaf (-0.25f * u, u, 0.75f), 0.5f);
 }
-__device__ __forceinline__ float gelu (float x) { return x * gelu_sigmoid (x); }
+__device__ __forceinline__ float gelu (float x)
+{
+  float u;
+  return x * gelu_cdf (x, u);
+}
+// d/dx [x Phi~(x)] = Phi~(x) + x Phi~'(x), Phi~'(x) = 3/8 (1 - u^2) inside [-2, 2], 0 outside
+__device__ __forceinline__ float gelu_grad (float x)
+{
+  float u;
+  const float h = gelu_cdf (x, u);
+  return fabsf (x) < 2.f ? fmaf (x * 0.375f, fmaf (-u, u, 1.f), h) : h;
+}
 
 __device__ __forceinline__ floatx4 mfma_bf16 (bf16x8 a, bf16x8 b, floatx4 c)
 {
@@ -227,12 +241,14 @@ __device__ __forceinline__ void wave_lds_sync ()
   __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Wavefront-wide inference for 64 lanes, each with its own feature row f (lanes without work pass
-// zeros).  lx: 64 x 32 bf16 and ly: 64 x 4 floats of this wave's LDS.  Returns the correction in out.
-__device__ __forceinline__ void mlp_wave_infer (const MlpNet &net, const float f[mlp::IN_PAD], float out[3],
+// Wavefront-wide inference for 64 lanes, each with its own feature row f; lanes without work (has ==
+// false) get no output, and 16-lane column blocks without any work are skipped.  lx: 64 x 32 bf16 and
+// ly: 64 x 4 floats of this wave's LDS.  Returns the correction in out.
+__device__ __forceinline__ void mlp_wave_infer (const MlpNet &net, const float f[mlp::IN_PAD], bool has, float out[3],
                                                 __bf16 *lx, float *ly)
 {
   const int lane = __lane_id (), j = lane & 15, g = lane >> 4;
+  const uint64_t work = __ballot (has);
   bf16x8 *row = (bf16x8 *)(lx + lane * mlp::IN_PAD);
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -247,6 +263,7 @@ __device__ __forceinline__ void mlp_wave_infer (const MlpNet &net, const float f
 #pragma unroll 1
   for (int sb0 = 0; sb0 < 4; sb0 += SB)
   {
+    if (((work >> (16 * sb0)) & ((SB == 1) ? 0xFFFFull : ~0ull >> (64 - 16 * SB))) == 0) continue;
     bf16x8 bin[SB];
     floatx4 y[SB];
 #pragma unroll

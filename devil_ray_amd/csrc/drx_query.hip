@@ -1,8 +1,8 @@
-// Point location, field evaluation, camera rays, ray set-up, volume integration, partial compaction and
-// compositing for gfx950.
+// Point location, field evaluation, camera rays, ray set-up and compositing for gfx950.
 #include "drx_common.hpp"
 #include "drx_element.hpp"
 #include "drx_locate_mlp.hpp"
+#include "drx_query_common.hpp"
 #include "drx_wave.hpp"
 
 #include <cmath>
@@ -10,48 +10,6 @@
 
 namespace drx
 {
-
-// ---- arithmetic that must round like dray's float32 CPU path (no FMA contraction) --------------
-__device__ __forceinline__ float ray_point (float o, float t, float d)
-{
-#pragma clang fp contract(off)
-  return o + t * d;
-}
-
-__device__ __forceinline__ float mul_sub_nc (float a, float b, float c)
-{
-#pragma clang fp contract(off)
-  return a * b - c;
-}
-
-__device__ __forceinline__ void blend_nc (float f[4], const float4 b)   // rendering/colors.hpp:15-23
-{
-#pragma clang fp contract(off)
-  const float a = b.w * (1.f - f[3]);
-  f[0] = f[0] + b.x * a;
-  f[1] = f[1] + b.y * a;
-  f[2] = f[2] + b.z * a;
-  f[3] = a + f[3];
-}
-
-// DeviceColorMap::color (device_color_map.hpp:61-77)
-__device__ __forceinline__ float4 color_lookup (const float4 *__restrict__ lut, int size, float mn, float mx, float inv,
-                                                float scalar)
-{
-#pragma clang fp contract(off)
-  const float s = fminf (mx, fmaxf (mn, scalar));
-  const float s2 = (scalar != scalar) ? mn : s;   // std::max(min, NaN) keeps min
-  const float normalized = (s2 - mn) * inv;
-  int idx = (int)(normalized * (float)(size - 1));
-  idx = min (max (idx, 0), size - 1);
-  return lut[idx];
-}
-
-static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
-
-#ifndef DRX_WAVES_PER_EU
-#define DRX_WAVES_PER_EU 4
-#endif
 
 // ---------------------------------------------------------------------------------------------
 // Batched locate (UnstructuredMesh::locate, unstructured_mesh.cpp:123-148): one lane per point.
@@ -327,11 +285,6 @@ __device__ __forceinline__ int64_t tile_rank (int pid, int w, int h)
   return ((int64_t)(ty * tiles_x + tx) << 6) + ((j & 7) << 3) + (i & 7);
 }
 
-__global__ void k_fill_i32q (int32_t *p, int64_t n, int32_t v)
-{
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
 
 __global__ void k_tile_scatter (const Ray *__restrict__ rays, int n, int w, int h, int32_t *__restrict__ slots)
 {
@@ -352,210 +305,6 @@ __global__ void k_compact_i32 (const int32_t *__restrict__ v, int64_t n, const i
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && v[i] >= 0) out[pos[i]] = v[i];
-}
-
-// ---------------------------------------------------------------------------------------------
-// Volume integration: integrate_partials' per-ray loop (rendering/volume.cpp:134-217).
-// ---------------------------------------------------------------------------------------------
-struct IntegrateArgs
-{
-  LocateCfg cfg;
-  const float *field;
-  const int64_t *field_offsets;
-  const Ray *rays;
-  const int32_t *order;
-  int n_rays;
-  const float4 *lut;
-  int lut_size;
-  float fmin, fmax, inv_range;
-  float sample_dist;
-  Partial *slots;
-  unsigned long long *stats;   // located, newton, candidates, polished
-  int32_t *err;
-  MlpNet net;                  // learned initial guess (MODE 3)
-  int budget;                  // Newton steps from the learned guess before the plain fallback
-};
-
-template <int P>
-__device__ __forceinline__ float field_at (const IntegrateArgs &a, int cell, const float ref[3])
-{
-  if constexpr (P == 0)
-  {
-    const float *f = a.field + a.field_offsets[cell];
-    switch (a.cfg.orders[cell])
-    {
-    case 1: return eval_scalar<1> (f, ref);
-    case 2: return eval_scalar<2> (f, ref);
-    case 3: return eval_scalar<3> (f, ref);
-    default: return eval_scalar<4> (f, ref);
-    }
-  }
-  else
-    return eval_scalar<P> (a.field + (int64_t)cell * Ord<P>::FSTRIDE, ref);
-}
-
-// MODE: 0 binary per-lane traversal, 1 8-wide per-lane, 2 8-wide wave-coherent (identical results).
-template <int P, bool POLISH, int MODE>
-__global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (IntegrateArgs a)
-{
-  constexpr bool WAVE = MODE == 2;
-  constexpr bool LEARNED = MODE == 3;   // learned Newton start; the whole wavefront stays in the loop
-  __shared__ int32_t stack_lds[WAVE ? 1 : kStackLds * 256];
-  __shared__ int32_t w_code[WAVE ? 4 * kWaveStack : 1];
-  __shared__ uint64_t w_mask[WAVE ? 4 * kWaveStack : 1];
-  __shared__ int32_t w_cand[WAVE ? 4 * kCandQ * 64 : 1];
-  __shared__ __attribute__ ((aligned (16))) __bf16 mlp_x[LEARNED ? 4 * 64 * mlp::IN_PAD : 8];
-  __shared__ __attribute__ ((aligned (16))) float mlp_y[LEARNED ? 4 * 64 * 4 : 4];
-  WaveScratch ws;
-  const int wv = threadIdx.x >> 6;
-  {
-    ws.st_code = w_code + (WAVE ? wv * kWaveStack : 0);
-    ws.st_mask = w_mask + (WAVE ? wv * kWaveStack : 0);
-    ws.cand = w_cand + (WAVE ? wv * kCandQ * 64 : 0);
-  }
-  TravStack todo;
-  todo.lds = stack_lds;
-  todo.bdim = blockDim.x;
-  todo.tid = threadIdx.x;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool has_ray = t < a.n_rays;
-  unsigned long long located = 0;
-  LocateStats st { 0, 0, 0, 0 };
-  bool overflow = false;
-  if (LEARNED || has_ray)
-  {
-    const int ri = has_ray ? (a.order ? a.order[t] : t) : 0;
-    Ray ray;
-    if (has_ray)
-      ray = a.rays[ri];
-    else
-    {
-      ray.dir[0] = ray.dir[1] = ray.dir[2] = 0.f;
-      ray.orig[0] = ray.orig[1] = ray.orig[2] = 0.f;
-      ray.near_ = ray.far_ = 0.f;
-      ray.pixel_id = -1;
-    }
-    Partial *out5 = a.slots + 5 * (int64_t)ri;
-    if (has_ray)
-#pragma unroll
-      for (int s = 0; s < 5; ++s)
-      {
-        Partial p0;
-        p0.pixel_id = -1;
-        p0.depth = INFINITY;
-        p0.rgba[0] = p0.rgba[1] = p0.rgba[2] = p0.rgba[3] = 0.f;
-        out5[s] = p0;
-      }
-    // integrate_partials' per-ray loop as a state machine with one locate() per iteration:
-    //   search: locate at `distance`; on a miss step on; on a hit open a segment and shade the sample;
-    //   inside: locate at the advanced distance; keep shading while inside, before `far` and alpha < 0.95,
-    //           else close the segment (<= 5 per ray); the next search re-locates at the same distance.
-    float distance = ray.near_ + a.sample_dist;
-    int segment = 0;
-    bool inseg = false;
-    float depth = 0.f;
-    float rgba[4] = { 0.f, 0.f, 0.f, 0.f };
-    bool alive = has_ray;
-    for (;;)
-    {
-      if (alive && !inseg && !(distance < ray.far_)) alive = false;
-      if constexpr (LEARNED)
-      {
-        if (__ballot (alive) == 0) break;
-      }
-      else if (!alive)
-        break;
-      const float pt[3] = { ray_point (ray.orig[0], distance, ray.dir[0]), ray_point (ray.orig[1], distance, ray.dir[1]),
-                            ray_point (ray.orig[2], distance, ray.dir[2]) };
-      float ref[3];
-      int cell;
-      if constexpr (LEARNED)
-      {
-        cell = locate_point_mlp<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, alive, a.net, a.budget,
-                                            mlp_x + wv * 64 * mlp::IN_PAD, mlp_y + wv * 64 * 4);
-        if (!alive) continue;
-      }
-      else if constexpr (WAVE)
-        cell = locate_wave<P, POLISH> (a.cfg, ws, pt, ref, st, overflow);
-      else
-        cell = locate_any<P, POLISH, MODE == 1> (a.cfg, todo, pt, ref, st, overflow);
-      if (overflow)
-      {
-        alive = false;
-        continue;
-      }
-      const bool found = cell != -1;
-      if (found) ++located;
-      bool shade = false;
-      bool ray_done = false;
-      if (!inseg)
-      {
-        if (found)
-        {
-          inseg = true;
-          depth = distance;
-          rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
-          shade = true;
-        }
-        else
-          distance += a.sample_dist;
-      }
-      else if (distance < ray.far_ && found && rgba[3] < 0.95f)
-        shade = true;
-      else
-      {
-        Partial pp;
-        pp.pixel_id = ray.pixel_id;
-        pp.depth = depth;
-        pp.rgba[0] = rgba[0];
-        pp.rgba[1] = rgba[1];
-        pp.rgba[2] = rgba[2];
-        pp.rgba[3] = rgba[3];
-        out5[segment] = pp;
-        ++segment;
-        if (distance >= ray.far_ || rgba[3] > 0.95f || segment == 5)
-          ray_done = true;
-        else
-          inseg = false;
-      }
-      if (ray_done)
-      {
-        alive = false;
-        continue;
-      }
-      if (shade)
-      {
-        const float scalar = field_at<P> (a, cell, ref);
-        blend_nc (rgba, color_lookup (a.lut, a.lut_size, a.fmin, a.fmax, a.inv_range, scalar));
-        distance += a.sample_dist;
-      }
-    }
-  }
-  if (overflow) atomicOr (a.err, 1);
-  // wave-aggregated statistics
-  unsigned long long v[4] = { located, (unsigned long long)st.newton, (unsigned long long)st.candidates,
-                              (unsigned long long)st.polished };
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-  {
-    unsigned long long x = v[q];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor (x, o, 64);
-    if ((threadIdx.x & 63) == 0 && x) atomicAdd (&a.stats[q], x);
-  }
-}
-
-__global__ void k_flag_partials (const Partial *__restrict__ p, int64_t n, int32_t *__restrict__ flags)
-{
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) flags[i] = (p[i].depth == INFINITY) ? 0 : 1;
-}
-
-__global__ void k_compact_partials (const Partial *__restrict__ p, int64_t n, const int32_t *__restrict__ flags,
-                                    const int32_t *__restrict__ pos, Partial *__restrict__ out)
-{
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && flags[i]) out[pos[i]] = p[i];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -602,44 +351,6 @@ __global__ void k_background (float4 *__restrict__ img, int npix)
     c.w = c.w + 1.f * a;
     img[i] = c;
   }
-}
-
-static int check_mesh (const drx_mesh *m, int ncomp, const char *what)
-{
-  if (!m || !m->data) return fail (DRX_ERR_ARG, "%s: null mesh", what);
-  if (m->ncomp != ncomp) return fail (DRX_ERR_ARG, "%s: expected ncomp %d, got %d", what, ncomp, m->ncomp);
-  if ((m->elem_order == nullptr) != (m->elem_offset == nullptr))
-    return fail (DRX_ERR_ARG, "%s: mixed-order meshes need elem_order and elem_offset", what);
-  if (!m->elem_order)
-  {
-    if (m->order < 1 || m->order > DRX_MAX_ORDER)
-      return fail (DRX_ERR_ARG, "%s: order %d outside 1..%d", what, m->order, DRX_MAX_ORDER);
-    if (m->stride != drx_mesh_stride (m->order, ncomp))
-      return fail (DRX_ERR_ARG, "%s: stride %d does not match order %d", what, m->stride, m->order);
-  }
-  return DRX_OK;
-}
-
-static int check_bvh (const drx_bvh *b, const char *what)
-{
-  if (!b || !b->inner_nodes || !b->leaf_pack) return fail (DRX_ERR_ARG, "%s: bvh needs inner_nodes and leaf_pack", what);
-  if (b->n_leaves < 2) return fail (DRX_ERR_ARG, "%s: bvh has %d leaves (dray pads to >= 2)", what, b->n_leaves);
-  return DRX_OK;
-}
-
-static LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx_locate_opts *o)
-{
-  LocateCfg c;
-  c.nodes = (const float4 *)b->inner_nodes;
-  c.wide = (const float4 *)b->wide_nodes;
-  c.leaves = (const float4 *)b->leaf_pack;
-  c.mesh = m->data;
-  c.offsets = m->elem_offset;
-  c.orders = m->elem_order;
-  c.stride = m->stride;
-  c.polish = o ? (o->fp64_polish != 0) : false;
-  c.cond_threshold = o ? o->cond_threshold : 1e4f;
-  return c;
 }
 
 } // namespace drx
@@ -829,136 +540,6 @@ int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t heig
     return fail (DRX_ERR_ARG, "drx_ray_tile_order: rays are not distinct pixels of a %dx%d image", width, height);
   hipLaunchKernelGGL (k_compact_i32, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots, slots_n, pos, out_order);
   DRX_LAUNCH_CHECK ();
-  return DRX_OK;
-}
-
-int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx_bvh *bvh, const void *active_rays,
-                          int32_t n_rays, const float *lut, int32_t lut_size, float field_min, float field_max,
-                          float sample_dist, const int32_t *ray_order, const drx_locate_opts *opts, void *out_partials,
-                          int32_t *out_count, drx_integrate_stats *stats, void *stream)
-{
-  int rc = check_mesh (mesh, 3, "drx_volume_integrate(mesh)");
-  if (rc) return rc;
-  if ((rc = check_mesh (field, 1, "drx_volume_integrate(field)"))) return rc;
-  if ((rc = check_bvh (bvh, "drx_volume_integrate"))) return rc;
-  if ((mesh->elem_order == nullptr) != (field->elem_order == nullptr) ||
-      (!mesh->elem_order && mesh->order != field->order))
-    return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: mesh and field orders must match");
-  if (!out_count) return fail (DRX_ERR_ARG, "drx_volume_integrate: null out_count");
-  if (n_rays < 0 || lut_size < 1 || !lut) return fail (DRX_ERR_ARG, "drx_volume_integrate: bad rays/lut");
-  if (!(field_min <= field_max)) return fail (DRX_ERR_ARG, "ColorMap scalar range never set");
-  const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
-  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
-    return fail (DRX_ERR_ARG, "drx_volume_integrate: unknown init_mode %d", opts->init_mode);
-  if (learned && !opts->mlp)
-    return fail (DRX_ERR_ARG, "drx_volume_integrate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
-  *out_count = 0;
-  if (stats) *stats = drx_integrate_stats { 0, 0, 0, 0, 0.f, 0.f };
-  if (n_rays == 0) return DRX_OK;
-  if (!active_rays || !out_partials) return fail (DRX_ERR_ARG, "drx_volume_integrate: null rays/output");
-  if ((int64_t)n_rays * 5 > (1ll << 31) - 1) return fail (DRX_ERR_ARG, "drx_volume_integrate: too many rays");
-  hipStream_t st = as_stream (stream);
-  const int64_t n_slots = 5 * (int64_t)n_rays;
-  Partial *slots = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n_slots);
-  int32_t *flags = (int32_t *)arena (SLOT_FLAGS, sizeof (int32_t) * 2 * (size_t)n_slots + 64);
-  unsigned long long *dstats = (unsigned long long *)arena (SLOT_STATS, 8 * sizeof (unsigned long long));
-  if (!slots || !flags || !dstats) return DRX_ERR_HIP;
-  int32_t *pos = flags + n_slots;
-  int32_t *err = (int32_t *)(dstats + 4);
-  hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, (int32_t *)dstats, (int64_t)16, 0);
-  IntegrateArgs a;
-  a.cfg = make_cfg (mesh, bvh, opts);
-  a.field = field->data;
-  a.field_offsets = field->elem_offset;
-  a.rays = (const Ray *)active_rays;
-  a.order = ray_order;
-  a.n_rays = n_rays;
-  a.lut = (const float4 *)lut;
-  a.lut_size = lut_size;
-  a.fmin = field_min;
-  a.fmax = field_max;
-  {
-    // rcp_safe(range length) on the host, float32 (device_color_map.hpp:45, math.hpp:213-217)
-    const float len = field_max - field_min;
-    a.inv_range = 1.0f / ((std::fabs (len) < 1e-8f) ? (std::signbit (len) ? -1e-8f : 1e-8f) : len);
-  }
-  a.sample_dist = sample_dist;
-  a.slots = slots;
-  a.stats = dstats;
-  a.err = err;
-  a.net = mlp_net (learned ? opts->mlp : nullptr);
-  a.budget = (learned && opts->mlp_newton_steps > 0) ? opts->mlp_newton_steps : 4;
-  const dim3 g (nblk (n_rays, 256)), b (256);
-  const int P = mesh->elem_order ? 0 : mesh->order;
-  const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
-  if (trav == DRX_TRAVERSE_WAVE && !a.cfg.wide)
-    return fail (DRX_ERR_ARG, "drx_volume_integrate: wave traversal needs bvh->wide_nodes");
-  const int mode = learned ? 3 : (!a.cfg.wide || trav == DRX_TRAVERSE_BINARY) ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
-  hipEvent_t ev[3];
-  for (int q = 0; q < 3; ++q) DRX_HIP (hipEventCreate (&ev[q]));
-  DRX_HIP (hipEventRecord (ev[0], st));
-#define DRX_INT(PP, PO)                                              \
-  do                                                                 \
-  {                                                                  \
-    if (mode == 3)                                                   \
-      hipLaunchKernelGGL ((k_integrate<PP, PO, 3>), g, b, 0, st, a); \
-    else if (mode == 2)                                              \
-      hipLaunchKernelGGL ((k_integrate<PP, PO, 2>), g, b, 0, st, a); \
-    else if (mode == 1)                                              \
-      hipLaunchKernelGGL ((k_integrate<PP, PO, 1>), g, b, 0, st, a); \
-    else                                                             \
-      hipLaunchKernelGGL ((k_integrate<PP, PO, 0>), g, b, 0, st, a); \
-  } while (0)
-  if (a.cfg.polish)
-    switch (P)
-    {
-    case 0: DRX_INT (0, true); break;
-    case 1: DRX_INT (1, true); break;
-    case 2: DRX_INT (2, true); break;
-    case 3: DRX_INT (3, true); break;
-    default: DRX_INT (4, true); break;
-    }
-  else
-    switch (P)
-    {
-    case 0: DRX_INT (0, false); break;
-    case 1: DRX_INT (1, false); break;
-    case 2: DRX_INT (2, false); break;
-    case 3: DRX_INT (3, false); break;
-    default: DRX_INT (4, false); break;
-    }
-#undef DRX_INT
-  DRX_LAUNCH_CHECK ();
-  DRX_HIP (hipEventRecord (ev[1], st));
-  DRX_LAUNCH_CHECK ();
-  hipLaunchKernelGGL (k_flag_partials, dim3 (nblk (n_slots, 256)), dim3 (256), 0, st, slots, n_slots, flags);
-  DRX_LAUNCH_CHECK ();
-  int32_t total = 0;
-  rc = exclusive_scan_i32 (flags, pos, (int32_t)n_slots, &total, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL (k_compact_partials, dim3 (nblk (n_slots, 256)), dim3 (256), 0, st, slots, n_slots, flags, pos,
-                      (Partial *)out_partials);
-  DRX_LAUNCH_CHECK ();
-  DRX_HIP (hipEventRecord (ev[2], st));
-  unsigned long long hs[8];
-  DRX_HIP (hipMemcpyAsync (hs, dstats, sizeof hs, hipMemcpyDeviceToHost, st));
-  DRX_HIP (hipStreamSynchronize (st));
-  float k_ms = 0.f, t_ms = 0.f;
-  DRX_HIP (hipEventElapsedTime (&k_ms, ev[0], ev[1]));
-  DRX_HIP (hipEventElapsedTime (&t_ms, ev[0], ev[2]));
-  for (int q = 0; q < 3; ++q) (void)hipEventDestroy (ev[q]);
-  if (((int32_t *)(hs + 4))[0])
-    return fail (DRX_ERR_OVERFLOW, "drx_volume_integrate: BVH traversal stack (%d entries) overflowed", kStackDepth);
-  *out_count = total;
-  if (stats)
-  {
-    stats->located_samples = (int64_t)hs[0];
-    stats->newton_steps = (int64_t)hs[1];
-    stats->candidates = (int64_t)hs[2];
-    stats->polished = (int64_t)hs[3];
-    stats->kernel_ms = k_ms;
-    stats->total_ms = t_ms;
-  }
   return DRX_OK;
 }
 

@@ -1,0 +1,96 @@
+// Helpers shared by the query (drx_query.hip) and integration (drx_integrate.hip) translation units.
+#pragma once
+
+#include "drx_common.hpp"
+#include "drx_element.hpp"
+
+namespace drx
+{
+
+// ---- arithmetic that must round like dray's float32 CPU path (no FMA contraction) --------------
+__device__ __forceinline__ float ray_point (float o, float t, float d)
+{
+#pragma clang fp contract(off)
+  return o + t * d;
+}
+
+__device__ __forceinline__ float mul_sub_nc (float a, float b, float c)
+{
+#pragma clang fp contract(off)
+  return a * b - c;
+}
+
+__device__ __forceinline__ void blend_nc (float f[4], const float4 b)   // rendering/colors.hpp:15-23
+{
+#pragma clang fp contract(off)
+  const float a = b.w * (1.f - f[3]);
+  f[0] = f[0] + b.x * a;
+  f[1] = f[1] + b.y * a;
+  f[2] = f[2] + b.z * a;
+  f[3] = a + f[3];
+}
+
+// DeviceColorMap::color (device_color_map.hpp:61-77)
+__device__ __forceinline__ float4 color_lookup (const float4 *__restrict__ lut, int size, float mn, float mx, float inv,
+                                                float scalar)
+{
+#pragma clang fp contract(off)
+  const float s = fminf (mx, fmaxf (mn, scalar));
+  const float s2 = (scalar != scalar) ? mn : s;   // std::max(min, NaN) keeps min
+  const float normalized = (s2 - mn) * inv;
+  int idx = (int)(normalized * (float)(size - 1));
+  idx = min (max (idx, 0), size - 1);
+  return lut[idx];
+}
+
+static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#ifndef DRX_WAVES_PER_EU
+#define DRX_WAVES_PER_EU 4
+#endif
+
+static __global__ void k_fill_i32q (int32_t *p, int64_t n, int32_t v)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+static inline int check_mesh (const drx_mesh *m, int ncomp, const char *what)
+{
+  if (!m || !m->data) return fail (DRX_ERR_ARG, "%s: null mesh", what);
+  if (m->ncomp != ncomp) return fail (DRX_ERR_ARG, "%s: expected ncomp %d, got %d", what, ncomp, m->ncomp);
+  if ((m->elem_order == nullptr) != (m->elem_offset == nullptr))
+    return fail (DRX_ERR_ARG, "%s: mixed-order meshes need elem_order and elem_offset", what);
+  if (!m->elem_order)
+  {
+    if (m->order < 1 || m->order > DRX_MAX_ORDER)
+      return fail (DRX_ERR_ARG, "%s: order %d outside 1..%d", what, m->order, DRX_MAX_ORDER);
+    if (m->stride != drx_mesh_stride (m->order, ncomp))
+      return fail (DRX_ERR_ARG, "%s: stride %d does not match order %d", what, m->stride, m->order);
+  }
+  return DRX_OK;
+}
+
+static inline int check_bvh (const drx_bvh *b, const char *what)
+{
+  if (!b || !b->inner_nodes || !b->leaf_pack) return fail (DRX_ERR_ARG, "%s: bvh needs inner_nodes and leaf_pack", what);
+  if (b->n_leaves < 2) return fail (DRX_ERR_ARG, "%s: bvh has %d leaves (dray pads to >= 2)", what, b->n_leaves);
+  return DRX_OK;
+}
+
+static inline LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx_locate_opts *o)
+{
+  LocateCfg c;
+  c.nodes = (const float4 *)b->inner_nodes;
+  c.wide = (const float4 *)b->wide_nodes;
+  c.leaves = (const float4 *)b->leaf_pack;
+  c.mesh = m->data;
+  c.offsets = m->elem_offset;
+  c.orders = m->elem_order;
+  c.stride = m->stride;
+  c.polish = o ? (o->fp64_polish != 0) : false;
+  c.cond_threshold = o ? o->cond_threshold : 1e4f;
+  return c;
+}
+
+} // namespace drx

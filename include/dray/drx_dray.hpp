@@ -212,6 +212,10 @@ private:
   bool m_smooth = false;          // dray's default (color_table.cpp:67-71)
 };
 
+// ---- lights (rendering/point_light.hpp:15-33) ---------------------------------------------------------------
+typedef drx_point_light PointLight;   // {pos, amb, diff, spec, spec_pow}
+PointLight make_point_light (const Vec3f &pos);   // dray's member defaults (amb 0.4, diff 0.75, spec 0.8, pow 15)
+
 // ---- camera ----------------------------------------------------------------------------------------------
 class Camera
 {
@@ -225,7 +229,11 @@ public:
   void set_zoom (float32 z) { m_zoom = z; }
   int32 get_width () const { return m_width; }
   int32 get_height () const { return m_height; }
+  Vec3f get_pos () const { return m_pos; }
+  Vec3f get_look_at () const { return m_look_at; }
+  Vec3f get_up () const { return m_up; }
   void create_rays (Array<Ray> &rays);
+  drx_camera view () const;
 
 private:
   int32 m_width = 500, m_height = 500;
@@ -244,8 +252,11 @@ public:
   void field_range (float32 lo, float32 hi) { m_range = { lo, hi }; m_range_set = true; }
   // libdrx extra: rays came from a width x height camera; process them in 8x8 pixel tiles (same results)
   void tile_order (int32 width, int32 height) { m_tile_w = width; m_tile_h = height; }
-  // Volume::integrate (volume.cpp:323-351): returns compacted partials (<= 5 per active ray)
-  Array<VolumePartial> integrate (Array<Ray> &rays);
+  // Volume::integrate (volume.cpp:323-351): returns compacted partials (<= 5 per active ray).  With
+  // use_lighting (dray's default) every light shades each sample (VolumeShader::shaded_color); without
+  // lights a lit sample contributes nothing, as in dray.
+  Array<VolumePartial> integrate (Array<Ray> &rays, Array<PointLight> &lights);
+  Array<VolumePartial> integrate (Array<Ray> &rays);   // no lights
   // Volume::save's compositing + white background (volume.cpp:369-416), RGBA per pixel (no PNG)
   Array<Vec<float32, 4>> composite (Array<VolumePartial> &partials, int32 width, int32 height);
   drx_integrate_stats last_stats () const { return m_stats; }
@@ -261,5 +272,8 @@ private:
   int32 m_tile_w = 0, m_tile_h = 0;
   drx_integrate_stats m_stats {};
 };
+
+// detail::default_light (rendering/renderer.cpp:141-159): the light the Renderer adds when none is given.
+PointLight default_light (Camera &camera);
 
 } // namespace dray

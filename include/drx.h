@@ -20,7 +20,9 @@
  *   drx_camera_rays         Camera::create_rays / gen_perspective     rendering/camera.cpp:208-243,402-483
  *   drx_remove_missed_rays  remove_missed_rays                        ray.cpp:102-241
  *   drx_volume_integrate    Volume::integrate -> integrate_partials   rendering/volume.hpp:40,
- *                                                                     rendering/volume.cpp:69-228,323-351
+ *                           (+ VolumeShader::color / shaded_color)    rendering/volume.cpp:69-228,323-351,
+ *                                                                     rendering/volume_shader.hpp:39-167
+ *   drx_default_light       detail::default_light                     rendering/renderer.cpp:141-159
  *   drx_composite           Volume::save compositing + Framebuffer::  rendering/volume.cpp:369-416,
  *                           composite_background                      rendering/framebuffer.cpp:146-162
  *   drx_color_table_sample  ColorTable::correct_opacity + sample      color_table.cpp:152-324
@@ -140,6 +142,27 @@ typedef struct drx_integrate_stats
   float total_ms;             /* the whole call: integrate + partial compaction (HIP events) */
 } drx_integrate_stats;
 
+/* PointLight (rendering/point_light.hpp:15-33); dray's defaults: amb 0.4, diff 0.75, spec 0.8, spec_pow 15. */
+typedef struct drx_point_light
+{
+  float pos[3];
+  float amb[3];
+  float diff[3];
+  float spec[3];
+  float spec_pow;
+} drx_point_light;
+
+#define DRX_MAX_LIGHTS 8
+
+/* Volume shading (rendering/volume.cpp:162-171): use_lighting 0 = VolumeShader::color (colour map only),
+ * 1 = VolumeShader::shaded_color with lights[0..n_lights) (host array, n_lights <= DRX_MAX_LIGHTS). */
+typedef struct drx_shading
+{
+  int32_t use_lighting;
+  int32_t n_lights;
+  const drx_point_light *lights;
+} drx_shading;
+
 typedef struct drx_camera
 {
   int32_t width, height;
@@ -203,21 +226,24 @@ int drx_remove_missed_rays (const void *rays, int32_t n, const float *bounds6_ho
 int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t height, int32_t *out_order,
                         void *stream);
 /* Volume::integrate on active rays: samples = |bounds diagonal|/sample_dist steps, field colour from
- * the (already opacity-corrected) LUT of lut_size RGBA entries over [field_min, field_max], front-to-back
- * blend, <= 5 partials per ray.  out_partials must hold 5*n_rays partials; on return it holds the
- * compacted partials (depth != inf, ray order) and *out_count (host) their number.  ray_order may be
- * NULL.  stats (host, nullable).  Synchronises. */
+ * the (already opacity-corrected) LUT of lut_size RGBA entries over [field_min, field_max], lit per
+ * `shading` (NULL = unlit), front-to-back blend, <= 5 partials per ray.  out_partials must hold
+ * 5*n_rays partials; on return it holds the compacted partials (depth != inf, ray order) and *out_count
+ * (host) their number.  ray_order may be NULL.  stats (host, nullable).  Synchronises. */
 int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx_bvh *bvh,
                           const void *active_rays, int32_t n_rays, const float *lut, int32_t lut_size,
                           float field_min, float field_max, float sample_dist, const int32_t *ray_order,
-                          const drx_locate_opts *opts, void *out_partials, int32_t *out_count,
-                          drx_integrate_stats *stats, void *stream);
+                          const drx_locate_opts *opts, const drx_shading *shading, void *out_partials,
+                          int32_t *out_count, drx_integrate_stats *stats, void *stream);
+/* Renderer's default light for a camera (rendering/renderer.cpp:141-159), host. */
+int drx_default_light (const drx_camera *cam, drx_point_light *out_light);
 /* Partials (grouped per pixel, as integrate emits them) -> RGBA image over a white background. */
 int drx_composite (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba,
                    void *stream);
 
 /* ---- learned Newton initializer ----------------------------------------------------------- */
-/* MLP 27 -> 64 -> 64 -> 64 -> 64 -> 3, GELU (sigmoid form), BF16 weights/activations, FP32 accumulate.
+/* MLP 27 -> 64 -> 64 -> 64 -> 64 -> 3, GELU x*Phi(x) with Phi a cubic smoothstep over [-2, 2], BF16
+ * weights/activations, FP32 accumulate.
  * Features (bf16, 32 per row, 27 used): the query and the element's 8 corner control points in the frame
  * of the corner box; the network predicts a correction to base = (q' + 1) / 2.                     */
 int64_t drx_mlp_param_count (void);   /* fp32 master parameters */

@@ -745,10 +745,110 @@ static inline void blend (float f[4], const float b[4])
 
 struct RayStats { i64 located = 0; i64 newton = 0; i64 candidates = 0; };
 
+// Vec::normalize (vec.hpp:211-221): magnitude = sqrtf(sum of squares), then component-wise division.
+static void vnormalize (float v[3])
+{
+  float s = 0.f;
+  for (int i = 0; i < 3; ++i) s += v[i] * v[i];
+  const float m = sqrtf (s);
+  for (int i = 0; i < 3; ++i) v[i] /= m;
+}
+
+// PointLight (rendering/point_light.hpp:15-33); dray's defaults: amb 0.4, diff 0.75, spec 0.8, spec_pow 15.
+struct Light { float pos[3], amb[3], diff[3], spec[3], spec_pow; };
+static_assert (sizeof (Light) == 52, "PointLight is 13 floats");
+
+static inline float dot3 (const float a[3], const float b[3])
+{
+  float s = a[0] * b[0];
+  s += a[1] * b[1];
+  s += a[2] * b[2];
+  return s;
+}
+static inline float clampf (float v, float lo, float hi) { return std::min (hi, std::max (lo, v)); }   // math.hpp:235-238
+
+// VolumeShader::scalar_gradient + shaded_color (rendering/volume_shader.hpp:39-158): physical field
+// gradient = ref-space gradient x J^-1 (matrix_inverse, matrix.hpp:431-457: LUP, one solve per unit
+// column), then per light ambient + diffuse + Blinn-Phong specular on the colour-map sample, with the
+// gradient flipped toward the light / half vector (the flips persist across lights) and the colour
+// channels clamped to [0, 1] after each light.  Samples with alpha <= 0.01 contribute nothing.
+template <typename T>
+static void shaded_color (const MeshView<T> &mesh, const MeshView<T> &field, int el, const T ref[3], const Ray &ray,
+                          const ColorMapT<T> &cmap, const Light *lights, int n_lights, std::vector<T> &scratch,
+                          float out[4])
+{
+  const int p = mesh.order_of (el);
+  const int npe = npe_of (p);
+  scratch.resize ((size_t)4 * npe);
+  T *gd = scratch.data (), *fd = gd + 3 * npe;
+  mesh.template gather<3> (el, gd);
+  field.template gather<1> (el, fd);
+  T wp[3], jac[3][3], sv[1], fdr[3][1];
+  eval_d<T, 3> (gd, p, ref, wp, jac);
+  eval_d<T, 1> (fd, p, ref, sv, fdr);
+  T J[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) J[r][c] = jac[c][r];   // set_col(rdim, d phi / d ref_rdim)
+  T inv[3][3];
+  for (int c = 0; c < 3; ++c)
+  {
+    T e[3] = { T (0), T (0), T (0) };
+    e[c] = T (1);
+    T x[3];
+    bool valid;
+    lup_solve<T> (J, e, x, valid);
+    for (int r = 0; r < 3; ++r) inv[r][c] = x[r];
+  }
+  T grad[3];
+  for (int c = 0; c < 3; ++c)
+  {
+    T sum = fdr[0][0] * inv[0][c];
+    sum = sum + fdr[1][0] * inv[1][c];
+    sum = sum + fdr[2][0] * inv[2][c];
+    grad[c] = sum;
+  }
+  float col[4];
+  cmap.color (sv[0], col);
+  float acc[4] = { 0.f, 0.f, 0.f, 0.f };
+  if (col[3] > 0.01)
+  {
+    T m2 = grad[0] * grad[0];
+    m2 += grad[1] * grad[1];
+    m2 += grad[2] * grad[2];
+    const T mag = std::sqrt (m2);
+    float fg[3];
+    for (int i = 0; i < 3; ++i) fg[i] = float (grad[i] / mag);
+    const float view[3] = { -ray.dir[0], -ray.dir[1], -ray.dir[2] };
+    for (int l = 0; l < n_lights; ++l)
+    {
+      const Light &L = lights[l];
+      float ld[3];
+      for (int i = 0; i < 3; ++i) ld[i] = L.pos[i] - float (wp[i]);
+      vnormalize (ld);
+      if (dot3 (ld, fg) < 0.f)
+        for (int i = 0; i < 3; ++i) fg[i] = -fg[i];
+      const float diffuse = clampf (dot3 (ld, fg), 0.f, 1.f);
+      float sc[4] = { L.amb[0] * col[0], L.amb[1] * col[1], L.amb[2] * col[2], col[3] };
+      for (int c = 0; c < 3; ++c) sc[c] += diffuse * L.diff[c] * col[c];
+      float hv[3];
+      for (int i = 0; i < 3; ++i) hv[i] = 0.5f * (view[i] + ld[i]);
+      vnormalize (hv);
+      if (dot3 (fg, hv) < 0.f)
+        for (int i = 0; i < 3; ++i) fg[i] = -fg[i];
+      const float doth = clampf (dot3 (fg, hv), 0.f, 1.f);
+      const float intensity = std::pow (doth, L.spec_pow);
+      for (int c = 0; c < 3; ++c) sc[c] += intensity * L.spec[c] * col[c];
+      for (int c = 0; c < 4; ++c) acc[c] += sc[c];
+      for (int c = 0; c < 3; ++c) acc[c] = clampf (acc[c], 0.f, 1.f);
+    }
+  }
+  for (int c = 0; c < 4; ++c) out[c] = acc[c];
+}
+
 template <typename T>
 static void integrate_ray (const MeshView<T> &mesh, const MeshView<T> &field, const BVHView &bvh,
-                           const Ray &ray, T sample_dist, const ColorMapT<T> &cmap, Partial *out5,
-                           std::vector<T> &scratch, RayStats &rs)
+                           const Ray &ray, T sample_dist, const ColorMapT<T> &cmap, const Light *lights,
+                           int n_lights, Partial *out5, std::vector<T> &scratch, RayStats &rs)
 {
   for (int s = 0; s < 5; ++s) { out5[s].pixel_id = -1; out5[s].depth = inf32 (); for (int c = 0; c < 4; ++c) out5[s].rgba[c] = 0.f; }
   T distance = T (ray.near_) + sample_dist;
@@ -775,9 +875,11 @@ static void integrate_ray (const MeshView<T> &mesh, const MeshView<T> &field, co
     for (int c = 0; c < 4; ++c) partial.rgba[c] = 0.f;
     do
     {
-      const T scalar = field_eval<T> (field, cell, ref, scratch);
       float sc[4];
-      cmap.color (scalar, sc);
+      if (lights)   // use_lighting: VolumeShader::shaded_color, else VolumeShader::color (volume.cpp:162-171)
+        shaded_color<T> (mesh, field, cell, ref, ray, cmap, lights, n_lights, scratch, sc);
+      else
+        cmap.color (field_eval<T> (field, cell, ref, scratch), sc);
       blend (partial.rgba, sc);
       distance += sample_dist;
       T pt[3];
@@ -797,14 +899,6 @@ static void integrate_ray (const MeshView<T> &mesh, const MeshView<T> &field, co
 // Camera::gen_perspective per-pixel part (rendering/camera.cpp:457-481) and the host set-up
 // (:402-440, set_fov :132-163).
 struct CamSetup { float nlook[3], dx[3], dy[3], pos[3]; int w, h; };
-
-static void vnormalize (float v[3])
-{
-  float s = 0.f;
-  for (int i = 0; i < 3; ++i) s += v[i] * v[i];
-  const float m = sqrtf (s);
-  for (int i = 0; i < 3; ++i) v[i] /= m;
-}
 
 static CamSetup camera_setup (int w, int h, float fov_y, const float pos[3], const float look_at[3], const float up_in[3], float zoom)
 {
@@ -1140,7 +1234,8 @@ float orc_sample_dist (const float *bounds6, int samples)
     for (i64 i = 0; i < n_rays; ++i)                                                                  \
     {                                                                                                 \
       RayStats rs;                                                                                    \
-      integrate_ray<T> (m, f, bv, rays[i], T (sample_dist), cm, parts + 5 * i, scratch, rs);          \
+      integrate_ray<T> (m, f, bv, rays[i], T (sample_dist), cm, (const Light *)lights, n_lights,       \
+                        parts + 5 * i, scratch, rs);                                                  \
       if (ray_located) ray_located[i] = (i32)rs.located;                                              \
       tl += rs.located; tn += rs.newton; tc += rs.candidates;                                         \
     }                                                                                                 \
@@ -1154,15 +1249,72 @@ int orc_integrate_f32 (const i32 *ctrl_idx, const float *values, const i32 *f_ct
                        const i32 *orders, const i64 *offsets, const i64 *f_offsets, int order, int n_el,
                        const float *inner, const i32 *leaf_nodes, const i32 *aabb_ids, const float *subrefs,
                        const void *rays_v, i64 n_rays, float sample_dist, const float *lut, int lut_size,
-                       double fmin_, double fmax_, void *out_partials, i32 *ray_located, i64 *stats3, int nthreads)
+                       double fmin_, double fmax_, void *out_partials, i32 *ray_located, i64 *stats3, int nthreads,
+                       const float *lights, int n_lights)
 { ORC_INTEGRATE_BODY (float) }
 
 int orc_integrate_f64 (const i32 *ctrl_idx, const double *values, const i32 *f_ctrl_idx, const double *f_values,
                        const i32 *orders, const i64 *offsets, const i64 *f_offsets, int order, int n_el,
                        const float *inner, const i32 *leaf_nodes, const i32 *aabb_ids, const float *subrefs,
                        const void *rays_v, i64 n_rays, float sample_dist, const float *lut, int lut_size,
-                       double fmin_, double fmax_, void *out_partials, i32 *ray_located, i64 *stats3, int nthreads)
+                       double fmin_, double fmax_, void *out_partials, i32 *ray_located, i64 *stats3, int nthreads,
+                       const float *lights, int n_lights)
 { ORC_INTEGRATE_BODY (double) }
+
+// One shaded_color sample (for testing the lit path on its own): element `cell` at reference point ref,
+// ray direction dir; out = RGBA.
+#define ORC_SHADE_BODY(T)                                                                              \
+  MeshView<T> m { ctrl_idx, values, orders, offsets, order, n_el };                                   \
+  MeshView<T> f { f_ctrl_idx, f_values, orders, f_offsets, order, n_el };                             \
+  ColorMapT<T> cm;                                                                                    \
+  cm.lut = lut; cm.size = lut_size; cm.mn = T (fmin_); cm.mx = T (fmax_);                             \
+  { const T len = cm.mx - cm.mn;                                                                     \
+    cm.inv_range = T (1) / ((std::fabs (len) < T (1e-8)) ? (std::signbit (len) ? T (-1e-8) : T (1e-8)) : len); } \
+  Ray ray {};                                                                                         \
+  for (int d = 0; d < 3; ++d) ray.dir[d] = dir[d];                                                    \
+  std::vector<T> scratch;                                                                             \
+  const T r[3] = { T (ref[0]), T (ref[1]), T (ref[2]) };                                              \
+  shaded_color<T> (m, f, cell, r, ray, cm, (const Light *)lights, n_lights, scratch, out4);
+
+void orc_shade_f32 (const i32 *ctrl_idx, const float *values, const i32 *f_ctrl_idx, const float *f_values,
+                    const i32 *orders, const i64 *offsets, const i64 *f_offsets, int order, int n_el, int cell,
+                    const double *ref, const float *dir, const float *lut, int lut_size, double fmin_, double fmax_,
+                    const float *lights, int n_lights, float *out4)
+{ ORC_SHADE_BODY (float) }
+
+void orc_shade_f64 (const i32 *ctrl_idx, const double *values, const i32 *f_ctrl_idx, const double *f_values,
+                    const i32 *orders, const i64 *offsets, const i64 *f_offsets, int order, int n_el, int cell,
+                    const double *ref, const float *dir, const float *lut, int lut_size, double fmin_, double fmax_,
+                    const float *lights, int n_lights, float *out4)
+{ ORC_SHADE_BODY (double) }
+
+// Renderer's default light (rendering/renderer.cpp:141-159): 0.1 x |look - pos| above the camera along
+// the camera's orthonormalised up; out = 13 floats (pos, amb, diff, spec, spec_pow) with dray's defaults.
+void orc_default_light (const float *pos, const float *look_at, const float *up_in, float *out13)
+{
+  float up[3] = { up_in[0], up_in[1], up_in[2] };
+  vnormalize (up);
+  float look[3];
+  for (int i = 0; i < 3; ++i) look[i] = look_at[i] - pos[i];
+  float m2 = 0.f;
+  for (int i = 0; i < 3; ++i) m2 += look[i] * look[i];
+  const float mag = sqrtf (m2);
+  float right[3] = { look[1] * up[2] - look[2] * up[1], look[2] * up[0] - look[0] * up[2], look[0] * up[1] - look[1] * up[0] };
+  vnormalize (right);
+  float mu[3] = { right[1] * look[2] - right[2] * look[1], right[2] * look[0] - right[0] * look[2],
+                  right[0] * look[1] - right[1] * look[0] };
+  vnormalize (mu);
+  Light L;
+  for (int i = 0; i < 3; ++i)
+  {
+    L.pos[i] = pos[i] + .1f * mag * mu[i];
+    L.amb[i] = 0.4f;
+    L.diff[i] = 0.75f;
+    L.spec[i] = 0.8f;
+  }
+  L.spec_pow = 15.f;
+  std::memcpy (out13, &L, sizeof L);
+}
 
 // compact_partials (volume.cpp:42-67): keep depth != +inf, order preserved.  Returns count.
 int orc_compact_partials (const void *in, i64 n, void *out)

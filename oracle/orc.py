@@ -41,8 +41,13 @@ _SIGS = {
     "orc_camera_perspective": (None, [I32, I32, F32, P, P, P, F32, P]),
     "orc_remove_missed_rays": (I32, [P, I32, P, P]),
     "orc_sample_dist": (F32, [P, I32]),
-    "orc_integrate_f32": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P, P, I32]),
-    "orc_integrate_f64": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P, P, I32]),
+    "orc_integrate_f32": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P, P, I32,
+                                P, I32]),
+    "orc_integrate_f64": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P, P, I32,
+                                P, I32]),
+    "orc_default_light": (None, [P, P, P, P]),
+    "orc_shade_f32": (None, [P, P, P, P, P, P, P, I32, I32, I32, P, P, P, I32, F64, F64, P, I32, P]),
+    "orc_shade_f64": (None, [P, P, P, P, P, P, P, I32, I32, I32, P, P, P, I32, F64, F64, P, I32, P]),
     "orc_compact_partials": (I32, [P, I64, P]),
     "orc_composite": (None, [P, I32, I32, I32, P]),
     "orc_color_table_sample": (None, [P, I32, P, I32, I32, F32, I32, F32, P]),
@@ -227,8 +232,35 @@ def color_table_sample(rgb_pts, alpha_pts, smooth=True, factor=-1.0, num_samples
     return out
 
 
-def integrate(mesh, bvh, active_rays, sample_dist_, lut, fmin, fmax, dtype=np.float32, nthreads=0):
-    """Returns (partials[5*n] uncompacted, per-ray located counts, stats[located, newton, candidates])."""
+def default_light(pos, look_at, up=(0, 1, 0)) -> np.ndarray:
+    """Renderer's default PointLight for a camera: 13 floats (pos, amb, diff, spec, spec_pow)."""
+    out = np.zeros(13, np.float32)
+    a = [np.ascontiguousarray(np.asarray(v, np.float32)) for v in (pos, look_at, up)]
+    lib().orc_default_light(ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(out))
+    return out
+
+
+def shade(mesh, cell, ref, ray_dir, lut, fmin, fmax, lights, dtype=np.float64, field=None):
+    """VolumeShader::shaded_color for one sample -> RGBA float32[4].  field: per-control-point scalars
+    (default mesh.field)."""
+    ctrl, vals, orders, offs = _mesh_args(mesh, dtype)
+    fctrl, fvals, _, foffs = _mesh_args(mesh, dtype, field=True)
+    if field is not None:
+        fvals = np.ascontiguousarray(np.asarray(field, dtype))
+    out = np.zeros(4, np.float32)
+    r = np.ascontiguousarray(np.asarray(ref, np.float64))
+    d = np.ascontiguousarray(np.asarray(ray_dir, np.float32))
+    lut_ = np.ascontiguousarray(lut.astype(np.float32))
+    lt = np.ascontiguousarray(np.asarray(lights, np.float32).reshape(-1, 13))
+    fn = lib().orc_shade_f32 if dtype == np.float32 else lib().orc_shade_f64
+    fn(ptr(ctrl), ptr(vals), ptr(fctrl), ptr(fvals), ptr(orders), ptr(offs), ptr(foffs), mesh.order, mesh.n_el,
+       int(cell), ptr(r), ptr(d), ptr(lut_), lut_.shape[0], float(fmin), float(fmax), ptr(lt), lt.shape[0], ptr(out))
+    return out
+
+
+def integrate(mesh, bvh, active_rays, sample_dist_, lut, fmin, fmax, dtype=np.float32, nthreads=0, lights=None):
+    """Returns (partials[5*n] uncompacted, per-ray located counts, stats[located, newton, candidates]).
+    lights: None = unlit (VolumeShader::color); else float32 [k, 13] PointLights (shaded_color)."""
     from devil_ray_amd.synthetic import PARTIAL_DTYPE
     n = int(active_rays.shape[0])
     ctrl, vals, orders, offs = _mesh_args(mesh, dtype)
@@ -238,10 +270,12 @@ def integrate(mesh, bvh, active_rays, sample_dist_, lut, fmin, fmax, dtype=np.fl
     stats = np.zeros(3, np.int64)
     lut_ = np.ascontiguousarray(lut.astype(np.float32))
     rays = np.ascontiguousarray(active_rays)
+    lt = None if lights is None else np.ascontiguousarray(np.asarray(lights, np.float32).reshape(-1, 13))
     fn = lib().orc_integrate_f32 if dtype == np.float32 else lib().orc_integrate_f64
     fn(ptr(ctrl), ptr(vals), ptr(fctrl), ptr(fvals), ptr(orders), ptr(offs), ptr(foffs), mesh.order, mesh.n_el,
        ptr(bvh.inner), ptr(bvh.leaf_nodes), ptr(bvh.aabb_ids), ptr(bvh.subrefs), ptr(rays), n, float(sample_dist_),
-       ptr(lut_), lut_.shape[0], float(fmin), float(fmax), ptr(parts), ptr(located), ptr(stats), nthreads)
+       ptr(lut_), lut_.shape[0], float(fmin), float(fmax), ptr(parts), ptr(located), ptr(stats), nthreads,
+       ptr(lt), 0 if lt is None else lt.shape[0])
     return parts, located, stats
 
 

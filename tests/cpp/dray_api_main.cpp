@@ -138,6 +138,19 @@ int main (int argc, char **argv)
     Array<Vec<float32, 4>> img = vol.composite (parts, width, height);
     write_array (out + "/partials.bin", parts);
     write_array (out + "/image.bin", img);
+    // lit render with the Renderer's default light (dray's default shading)
+    {
+      Volume lit (mesh, fld);
+      lit.samples (samples);
+      lit.tile_order (width, height);
+      lit.color_table () = tab;
+      Array<PointLight> lights (1);
+      lights.get_host_ptr ()[0] = default_light (cam);
+      Array<VolumePartial> lp = lit.integrate (rays, lights);
+      Array<Vec<float32, 4>> limg = lit.composite (lp, width, height);
+      write_array (out + "/lit_image.bin", limg);
+      write_raw (out + "/light.bin", &lights.get_host_ptr_const ()[0].pos[0], 13);
+    }
     const drx_integrate_stats st = vol.last_stats ();
     {
       std::ofstream sf (out + "/stats.txt");
@@ -149,7 +162,11 @@ int main (int argc, char **argv)
     ok &= throws ([] { Camera c; c.set_width (0); });
     ok &= throws ([] { Camera c; c.set_fov (181.f); });
     ok &= throws ([&] { UnstructuredMesh m (gf, 5); });
-    ok &= throws ([&] { Volume v (mesh, fld); Array<Ray> r = rays; v.integrate (r); });   // lit shading
+    ok &= throws ([&] {   // more lights than the library takes
+      Volume v (mesh, fld);
+      Array<PointLight> many (DRX_MAX_LIGHTS + 1);
+      v.integrate (rays, many);
+    });
     std::printf ("%s\n", ok ? "errors ok" : "errors FAILED");
     return ok ? 0 : 1;
   }

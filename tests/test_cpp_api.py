@@ -94,6 +94,17 @@ def test_cpp_api_matches_c_abi(gpu, orc, tmp_path):
     assert np.array_equal(cp, parts.cpu().numpy()[: n * PARTIAL_DTYPE.itemsize])
     assert np.array_equal(np.fromfile(outd / "image.bin", np.float32).view(np.uint32),
                           img.cpu().numpy().view(np.uint32))
+    # lit render with the default light: the same as the Python call
+    light = np.fromfile(outd / "light.bin", np.float32)
+    assert np.allclose(light, gpu.default_light(gpu.Camera(w, h, 45.0, pos=pos, look_at=look)))
+    vol2 = gpu.Volume(m, f)
+    vol2.samples(samples)
+    vol2.table = vol.table
+    parts2, n2 = vol2.integrate(rays, tile_width=w, tile_height=h, lights=light[None, :])
+    img2 = gpu.composite(parts2, n2, w, h)
+    torch.cuda.synchronize()
+    assert np.array_equal(np.fromfile(outd / "lit_image.bin", np.float32).view(np.uint32),
+                          img2.cpu().numpy().view(np.uint32))
     st = (outd / "stats.txt").read_text().split()
     assert int(st[0]) == vol.last_stats.located_samples and int(st[3]) == n
     assert LOCATION_DTYPE.itemsize == 16 and RAY_DTYPE.itemsize == 36

@@ -139,17 +139,18 @@ def test_camera_and_ray_setup_bit_exact(gpu, orc):
     assert np.array_equal(_np(act)[: n * 36], oa.view(np.uint8))
 
 
-def _render_both(gpu, orc, mesh, rays_np, w, h, samples, opacity=0.02, tile=True):
+def _render_both(gpu, orc, mesh, rays_np, w, h, samples, opacity=0.02, tile=True, lights=None):
     m, bvh = _gpu_bvh(gpu, mesh)
     f = gpu.UnstructuredField(mesh)
     vol = gpu.Volume(m, f)
-    vol.use_lighting = False
+    vol.use_lighting = lights is not None
     vol.samples(samples)
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, syn.table_alpha_for_corrected(opacity, samples))
     vol.table.set_smooth(True)
-    parts, n = vol.integrate(gpu.as_bytes(rays_np), tile_width=w if tile else 0, tile_height=h if tile else 0)
+    parts, n = vol.integrate(gpu.as_bytes(rays_np), tile_width=w if tile else 0, tile_height=h if tile else 0,
+                             lights=lights)
     img = _np(gpu.composite(parts, n, w, h)).reshape(-1, 4)
     gp = _np(parts, np.uint8)[: n * 24].view(PARTIAL_DTYPE)
     # oracle
@@ -157,7 +158,7 @@ def _render_both(gpu, orc, mesh, rays_np, w, h, samples, opacity=0.02, tile=True
     lut = vol.table.sample(1024, opacity_factor=10.0 / samples)
     lo, hi = f.range()
     act = orc.remove_missed_rays(rays_np, ob.bounds)
-    op, _, stats = orc.integrate(mesh, ob, act, orc.sample_dist(ob.bounds, samples), lut, lo, hi)
+    op, _, stats = orc.integrate(mesh, ob, act, orc.sample_dist(ob.bounds, samples), lut, lo, hi, lights=lights)
     op = orc.compact_partials(op)
     oimg = orc.composite(op, w, h)
     return gp, img, op, oimg, vol.last_stats, stats
@@ -177,6 +178,29 @@ def test_volume_integrate_parity(gpu, orc, name, make):
     print(f"  rgba max diff {d.max():.3g} mean {d.mean():.3g}", flush=True)
     assert np.mean(d.max(axis=1) > RGBA_TOL) < 0.01
     assert d.mean() < 1e-4
+
+
+@pytest.mark.parametrize("name,make", [MESHES[1], MESHES[2], MESHES[4]], ids=["p2_n4", "p3_n5", "mixed_n4"])
+def test_volume_lit_parity(gpu, orc, name, make):
+    """use_lighting (dray's default): VolumeShader::shaded_color with the Renderer's default light, and a
+    second light, against the oracle's lit render."""
+    mesh = make()
+    w = h = 48
+    pos, look = (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)
+    rays = orc.camera_perspective(w, h, 45.0, pos, look)
+    L0 = orc.default_light(pos, look)
+    assert np.allclose(gpu.default_light(gpu.Camera(w, h, 45.0, pos=pos, look_at=look)), L0)
+    L1 = L0.copy()
+    L1[:3] = (-1.0, 2.0, 0.5)
+    for lights in (L0[None, :], np.stack([L0, L1])):
+        gp, img, op, oimg, gst, ost = _render_both(gpu, orc, mesh, rays, w, h, samples=128, lights=lights)
+        d = np.abs(img - oimg)
+        print(f"  {name} lights={len(lights)} partials {len(gp)}/{len(op)} rgba max {d.max():.3g} mean {d.mean():.3g}",
+              flush=True)
+        assert abs(len(gp) - len(op)) <= max(2, len(op) // 500)
+        assert np.mean(d.max(axis=1) > RGBA_TOL) < 0.01
+        assert d.mean() < 2e-4
+        assert img[:, :3].max() > 0.05   # the lit image is not black
 
 
 def test_volume_orthographic_config0_small(gpu, orc):

@@ -367,3 +367,122 @@ def test_remove_missed_rays(orc):
     act = orc.remove_missed_rays(rays, [0, 1, 0, 1, 0, 1])
     assert list(act["pixel_id"]) == [1, 2]
     assert np.allclose(act["near"], 1.0 - 0.0005, atol=1e-6) and np.allclose(act["far"], 2.0 + 0.0005, atol=1e-6)
+
+
+# ---- lit shading (VolumeShader::shaded_color) ------------------------------------------------
+def test_default_light_closed_form(orc):
+    # camera on +z looking at the centre, up y: the light sits 0.1 x distance above the camera
+    assert np.allclose(orc.default_light((0.5, 0.5, 2.5), (0.5, 0.5, 0.5))[:3], (0.5, 0.7, 2.5), atol=1e-6)
+    # camera on +x looking at the origin, up z: right = look x up = +y, up' = right x look = +z
+    L = orc.default_light((3.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    assert np.allclose(L[:3], (3.0, 0.0, 0.3), atol=1e-6)
+    assert np.allclose(L[3:], [0.4] * 3 + [0.75] * 3 + [0.8] * 3 + [15.0])
+
+
+def _sheared_linear_mesh(a, c):
+    """Order-2 mesh of 2^3 elements under a non-symmetric affine map x = A r + b; field f = a.x + c
+    (exact in the Bernstein basis, so the physical gradient is exactly a)."""
+    mesh = syn.hex_mesh(2, 2, 0.0, 0.0, 0)
+    A = np.array([[1.0, 0.3, -0.2], [0.1, 0.8, 0.25], [-0.15, 0.05, 1.2]])
+    mesh.values = (mesh.values.astype(np.float64) @ A.T + np.array([0.2, -0.1, 0.3])).astype(np.float32)
+    mesh.field = (mesh.values.astype(np.float64) @ np.asarray(a) + c).astype(np.float32)
+    return mesh
+
+
+def _lit_setup(orc, a, c, cell, ref):
+    mesh = _sheared_linear_mesh(a, c)
+    lut = np.zeros((1024, 4), np.float32)
+    lut[:, 0] = np.linspace(0.2, 0.9, 1024)
+    lut[:, 1] = 0.5
+    lut[:, 2] = np.linspace(0.9, 0.1, 1024)
+    lut[:, 3] = 0.3
+    dofs = mesh.values[mesh.elem_dofs(cell)].astype(np.float64)
+    wp, _ = orc.eval_d(dofs, 2, ref)
+    scalar = float(np.dot(a, wp) + c)
+    lo, hi = float(mesh.field.min()), float(mesh.field.max())
+    idx = int(np.float32((scalar - lo) / (hi - lo)) * np.float32(1023))
+    return mesh, lut, wp, lut[idx], lo, hi
+
+
+def _light(pos, amb=0.4, diff=0.75, spec=0.8, pw=15.0):
+    return np.array(list(pos) + [amb] * 3 + [diff] * 3 + [spec] * 3 + [pw], np.float32)
+
+
+def test_shade_headlight_closed_form(orc):
+    """Light on the gradient line and the viewer looking down it: diffuse = specular = 1, so
+    rgb = clamp((amb + diff + spec) * colour), alpha = colour alpha."""
+    a = np.array([0.3, -0.2, 0.9])
+    ah = a / np.linalg.norm(a)
+    mesh, lut, wp, col, lo, hi = _lit_setup(orc, a, 0.1, 5, (0.3, 0.6, 0.2))
+    out = orc.shade(mesh, 5, (0.3, 0.6, 0.2), -ah, lut, lo, hi, _light(wp + 4.0 * ah))
+    exp = np.minimum(1.0, (0.4 + 0.75 + 0.8) * col[:3])
+    assert np.allclose(out[:3], exp, atol=2e-5) and out[3] == col[3]
+
+
+def test_shade_back_light_flips_gradient(orc):
+    """Light behind the surface (ld . g = -1): the gradient flips, diffuse = 1; viewer perpendicular to
+    the gradient: half vector at 45 deg, specular = (1/sqrt 2)^15."""
+    a = np.array([0.3, -0.2, 0.9])
+    ah = a / np.linalg.norm(a)
+    perp = np.cross(ah, [1.0, 0.0, 0.0])
+    perp /= np.linalg.norm(perp)
+    mesh, lut, wp, col, lo, hi = _lit_setup(orc, a, 0.1, 2, (0.7, 0.2, 0.5))
+    out = orc.shade(mesh, 2, (0.7, 0.2, 0.5), -perp, lut, lo, hi, _light(wp - 3.0 * ah))
+    spec = (1.0 / np.sqrt(2.0)) ** 15
+    exp = np.minimum(1.0, (0.4 + 0.75 + 0.8 * spec) * col[:3])
+    assert np.allclose(out[:3], exp, atol=2e-5)
+
+
+def test_shade_grazing_light_ambient_only(orc):
+    """Light in the tangent plane and viewer along the light: diffuse = 0 and specular = 0^15."""
+    a = np.array([0.3, -0.2, 0.9])
+    ah = a / np.linalg.norm(a)
+    t = np.cross(ah, [0.0, 1.0, 0.0])
+    t /= np.linalg.norm(t)
+    mesh, lut, wp, col, lo, hi = _lit_setup(orc, a, 0.1, 6, (0.4, 0.4, 0.6))
+    out = orc.shade(mesh, 6, (0.4, 0.4, 0.6), -t, lut, lo, hi, _light(wp + 2.0 * t))
+    assert np.allclose(out[:3], 0.4 * col[:3], atol=1e-4)
+
+
+def test_shade_two_lights_and_transparent_samples(orc):
+    a = np.array([0.3, -0.2, 0.9])
+    ah = a / np.linalg.norm(a)
+    mesh, lut, wp, col, lo, hi = _lit_setup(orc, a, 0.1, 5, (0.3, 0.6, 0.2))
+    L = _light(wp + 4.0 * ah)
+    two = orc.shade(mesh, 5, (0.3, 0.6, 0.2), -ah, lut, lo, hi, np.stack([L, L]))
+    assert np.allclose(two[:3], np.minimum(1.0, 2 * (0.4 + 0.75 + 0.8) * col[:3]), atol=2e-5)
+    assert np.isclose(two[3], 2 * col[3])          # each light adds the sample alpha (dray's accumulation)
+    lut2 = lut.copy()
+    lut2[:, 3] = 0.01                              # alpha <= 0.01: the sample contributes nothing
+    assert np.all(orc.shade(mesh, 5, (0.3, 0.6, 0.2), -ah, lut2, lo, hi, L) == 0)
+
+
+def test_volume_lit_matches_per_sample_shading(orc):
+    """A lit render of one ray equals blending shade() at the ray's located samples."""
+    a = np.array([0.3, -0.2, 0.9])
+    mesh = _sheared_linear_mesh(a, 0.1)
+    ob = orc.build_mesh_bvh(mesh)
+    lut = np.zeros((1024, 4), np.float32)
+    lut[:, 0] = np.linspace(0.2, 0.9, 1024)
+    lut[:, 1] = 0.5
+    lut[:, 3] = 0.05
+    lo, hi = float(mesh.field.min()), float(mesh.field.max())
+    rays = orc.camera_perspective(8, 8, 40.0, (0.7, 0.6, 3.0), (0.6, 0.4, 0.9))
+    act = orc.remove_missed_rays(rays, ob.bounds)
+    L = orc.default_light((0.7, 0.6, 3.0), (0.6, 0.4, 0.9))
+    sd = orc.sample_dist(ob.bounds, 64)
+    parts, _, _ = orc.integrate(mesh, ob, act[:1], sd, lut, lo, hi, dtype=np.float64, lights=L)
+    r = act[0]
+    d = np.float64(r["near"]) + sd
+    f = np.zeros(4, np.float32)
+    while d < r["far"] and f[3] < 0.95:
+        pt = r["orig"].astype(np.float64) + d * r["dir"].astype(np.float64)
+        c, rf = orc.locate(mesh, ob, pt[None, :], dtype=np.float64)
+        if c[0] < 0:
+            break
+        s = orc.shade(mesh, int(c[0]), rf[0], r["dir"], lut, lo, hi, L)
+        al = s[3] * (np.float32(1) - f[3])
+        f[:3] = f[:3] + s[:3] * al
+        f[3] = al + f[3]
+        d += sd
+    assert np.allclose(parts[0]["rgba"], f, atol=1e-5)
