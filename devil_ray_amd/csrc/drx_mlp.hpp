@@ -60,7 +60,7 @@ __host__ __device__ constexpr int t_pad (int l) { return l == 5 ? 32 : H; }
 } // namespace mlp
 
 // GELU x * Phi(x) with the normal CDF Phi replaced by a cubic smoothstep over [-2, 2]:
-//   Phi~(x) = 1/2 + u (3/4 - u^2/4),  u = clamp(x / 2, -1, 1)   (|Phi~ - Phi| < 0.025, C1 at +-2)
+//   Phi~(x) = 1/2 + u (3/4 - u^2/4),  u = clamp(x / 2, -1, 1)   (|Phi~ - Phi| < 0.029, C1 at +-2)
 // Six full-rate VALU operations and no transcendental; training uses the same function.
 __device__ __forceinline__ float gelu_cdf (float x, float &u)
 {
