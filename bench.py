@@ -301,7 +301,8 @@ def bench_locate(args):
     torch.cuda.synchronize()
     pts_h = syn.query_points(n_pts, bvh.bounds, seed=0)
     pts = api.to_device(pts_h.reshape(-1))
-    variants = {"lane": api.LocateOptions(traversal=1), "morton+lane": api.LocateOptions(traversal=1, coherent_order=True),
+    variants = {"binary": api.LocateOptions(traversal=3), "morton+binary": api.LocateOptions(traversal=3, coherent_order=True),
+                "morton+wide": api.LocateOptions(traversal=1, coherent_order=True),
                 "morton+wave": api.LocateOptions(traversal=2, coherent_order=True)}
     timings = {}
     locs = {}
@@ -315,8 +316,8 @@ def bench_locate(args):
         torch.cuda.synchronize()
         timings[name] = (time.perf_counter() - t) / args.steps
     best = min(timings, key=timings.get)
-    ref = locs["lane"].cpu().numpy().view(LOCATION_DTYPE)
-    same = {k: bool(np.array_equal(locs[k].cpu().numpy(), locs["lane"].cpu().numpy())) for k in locs}
+    ref = locs["binary"].cpu().numpy().view(LOCATION_DTYPE)
+    same = {k: bool(np.array_equal(locs[k].cpu().numpy(), locs["binary"].cpu().numpy())) for k in locs}
     value = n_pts / timings[best]
     out = {
         "metric": "located queries/s (batched point location)", "value": value, "unit": "queries/s", "n_gpus": 1,
@@ -434,15 +435,15 @@ def bench_mlp(args):
     # ---- held-out evaluation ----
     hist_len = 41
     agg = {k: {"ms": 0.0, "steps": 0, "hist": np.zeros(hist_len, np.int64), "err": 0.0}
-           for k in ("plain", "plain_binary", "learned")}
+           for k in ("plain", "plain_wide", "learned")}
     mism_total, n_total, pred_err = 0, 0, []
     for s in range(32, 40):
         mesh = syn.hex_mesh(64, 3, _amp_of_seed(s), 0.01, 1000 + s)
         m = api.UnstructuredMesh(mesh)
         pts_h = syn.query_points(args.mlp_eval_points, m.bounds(), seed=s)
         pts = api.to_device(pts_h.reshape(-1))
-        # the learned start walks the binary tree; plain is timed with the library default and with that walk
-        opts = {"plain": api.LocateOptions(), "plain_binary": api.LocateOptions(traversal=3),
+        # plain = the library default (binary walk, as the learned start uses); plain_wide = the 8-wide walk
+        opts = {"plain": api.LocateOptions(), "plain_wide": api.LocateOptions(traversal=1),
                 "learned": api.LocateOptions(mlp=packed)}
         res = {}
         for k, o in opts.items():
@@ -487,7 +488,7 @@ def bench_mlp(args):
                    "training_pairs": n_pairs, "epochs": args.mlp_epochs, "batch": batch, "train_steps": steps,
                    "eval_points_per_mesh": args.mlp_eval_points, "newton_budget_from_guess": 4},
         "plain_queries_per_s": plain_rate, "speedup_vs_plain": value / plain_rate,
-        "plain_binary_queries_per_s": n_total / (agg["plain_binary"]["ms"] / 1e3),
+        "plain_wide_queries_per_s": n_total / (agg["plain_wide"]["ms"] / 1e3),
         "newton_steps_per_query": {k: agg[k]["steps"] / n_total for k in agg},
         "newton_steps_hist": {k: agg[k]["hist"].tolist() for k in agg},
         "max_ref_err_vs_fp64": {k: agg[k]["err"] for k in agg},

@@ -169,7 +169,7 @@ class DeviceGridFunction:
 class LocateOptions:
     fp64_polish: bool = False
     cond_threshold: float = 1e4
-    traversal: int = 0          # 0 auto, 1 per lane (8-wide), 2 wave-coherent, 3 per lane (binary)
+    traversal: int = 0          # 0 auto (= 3), 1 per lane (8-wide), 2 wave-coherent (8-wide), 3 per lane (binary)
     coherent_order: bool = False  # locate: process queries in Morton order
     mlp: Optional[torch.Tensor] = None   # packed learned initializer (LearnedInitializer.pack())
     mlp_newton_steps: int = 4            # Newton budget from the learned guess before the plain fallback

@@ -393,7 +393,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !a.cfg.wide)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: wave traversal needs bvh->wide_nodes");
-  const int mode = learned ? 3 : (!a.cfg.wide || trav == DRX_TRAVERSE_BINARY) ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
+  const int mode = learned ? 3 : trav == DRX_TRAVERSE_WAVE ? 2 : (trav == DRX_TRAVERSE_LANE && a.cfg.wide) ? 1 : 0;
   hipEvent_t ev[3];
   for (int q = 0; q < 3; ++q) DRX_HIP (hipEventCreate (&ev[q]));
   DRX_HIP (hipEventRecord (ev[0], st));

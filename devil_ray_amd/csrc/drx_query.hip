@@ -385,7 +385,7 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !cfg.wide)
     return fail (DRX_ERR_ARG, "drx_locate: wave traversal needs bvh->wide_nodes");
-  const int mode = (!cfg.wide || trav == DRX_TRAVERSE_BINARY) ? 0 : (trav == DRX_TRAVERSE_WAVE ? 2 : 1);
+  const int mode = trav == DRX_TRAVERSE_WAVE ? 2 : (trav == DRX_TRAVERSE_LANE && cfg.wide) ? 1 : 0;
   const int32_t *order = nullptr;
   if (opts && opts->coherent_order)
   {

@@ -126,10 +126,11 @@ typedef struct drx_mlp_train_opts
 
 enum drx_traversal
 {
-  DRX_TRAVERSE_AUTO = 0,      /* library default for the entry point */
-  DRX_TRAVERSE_LANE = 1,      /* every lane walks its own depth-first path */
-  DRX_TRAVERSE_WAVE = 2,      /* the wavefront walks the union of its lanes' paths with uniform node reads */
-  DRX_TRAVERSE_BINARY = 3     /* every lane walks dray's binary tree (ignores bvh->wide_nodes) */
+  DRX_TRAVERSE_AUTO = 0,      /* library default: DRX_TRAVERSE_BINARY (fastest measured on MI355X) */
+  DRX_TRAVERSE_LANE = 1,      /* every lane walks the 8-wide collapse (bvh->wide_nodes) depth-first */
+  DRX_TRAVERSE_WAVE = 2,      /* the wavefront walks the union of its lanes' paths over the 8-wide
+                                 collapse with uniform node reads */
+  DRX_TRAVERSE_BINARY = 3     /* every lane walks dray's binary tree depth-first */
 };
 
 typedef struct drx_integrate_stats

@@ -223,12 +223,10 @@ def test_wide_bvh_same_candidates_as_binary(gpu, name, make):
     assert bvh.wide_nodes is not None
     pts = gpu.to_device(syn.query_points(30000, bvh.bounds, seed=21).reshape(-1))
     out = {}
-    for wide in (True, False):
-        bvh.use_wide = wide
-        loc, steps, cands = m.locate(pts, stats=True)
-        out[wide] = (_np(loc), _np(steps), _np(cands))
-    bvh.use_wide = True
-    for a, b in zip(out[True], out[False]):
+    for trav in (1, 3):   # 8-wide per-lane walk, binary per-lane walk
+        loc, steps, cands = m.locate(pts, gpu.LocateOptions(traversal=trav), stats=True)
+        out[trav] = (_np(loc), _np(steps), _np(cands))
+    for a, b in zip(out[1], out[3]):
         assert np.array_equal(a, b)
 
 
