@@ -467,8 +467,11 @@ __device__ __forceinline__ bool try_element_mixed (const LocateCfg &cfg, int el,
   }
 }
 
+#ifndef DRX_STACK_LDS
+#define DRX_STACK_LDS 16   // entries of each lane's traversal stack kept in LDS
+#endif
 constexpr int kStackDepth = 64;
-constexpr int kStackLds = 16;   // hot part of each lane's traversal stack lives in LDS
+constexpr int kStackLds = DRX_STACK_LDS;   // hot part of each lane's traversal stack lives in LDS
 constexpr int kBarrier = -2000000000;
 
 // Per-lane traversal stack: entries [0, kStackLds) in LDS (entry-major, so the 64 lanes of a wave hit
