@@ -29,6 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE)
+# scaled by what the counters read for a 1 GiB device copy (x2.0, x1.0); the summary with its commands is
+# profiles/k_integrate_p3_64cubed_1024sq_512spp.md.  Keyed by (config, kernel) so that other workloads or
+# kernel variants report null rather than a stale figure.
+PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,false>"): 6.89e9 + 2.68e9}
 
 
 def _cpu_model() -> str:
@@ -159,6 +164,8 @@ def main() -> int:
     k_ms = float(np.mean(kernel_ms))
     bytes_per_sample = 16 * (p + 1) ** 3
     achieved_gbs = located_per_frame * bytes_per_sample / (k_ms * 1e-3) / 1e9
+    walk = {0: 0, 3: 0, 1: 1, 2: 2}[args.traversal]
+    kernel_name = f"k_integrate<{p},{'true' if args.polish else 'false'},{walk},false>"
 
     out = {
         "metric": "located samples/s (volume render; also Mrays/s)",
@@ -188,8 +195,8 @@ def main() -> int:
         "bvh_build_ms": {"cold": bvh_ms, "warm": bvh_ms_warm},
         "bvh_wide_nodes": int(bvh.wide_nodes.numel() // 64) if bvh.wide_nodes is not None else 0,
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_integrate<{p},{'true' if args.polish else 'false'}>",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": PMC_TRAFFIC_BYTES.get((args.config, kernel_name)),
+                     "kernel": kernel_name,
                      "bytes_per_unit": bytes_per_sample, "unit_name": "located sample"},
     }
 

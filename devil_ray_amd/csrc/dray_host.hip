@@ -359,34 +359,116 @@ static float sample_distance (const float *bounds6, int32 samples)
   return std::sqrt (s) / float (samples);
 }
 
-Volume::Volume (UnstructuredMesh &mesh, UnstructuredField &field) : m_mesh (mesh), m_field (field) {}
+// ---- DataSet / Collection ------------------------------------------------------------------------------------
+void DataSet::add_field (std::shared_ptr<UnstructuredField> field) { m_fields.push_back (std::move (field)); }
 
-Array<VolumePartial> Volume::integrate (Array<Ray> &rays)
+bool DataSet::has_field (const std::string &name) const
 {
-  Array<PointLight> none;
-  return integrate (rays, none);
+  for (const auto &f : m_fields)
+    if (f->name () == name) return true;
+  return false;
+}
+
+UnstructuredField *DataSet::field (const std::string &name)
+{
+  for (auto &f : m_fields)
+    if (f->name () == name) return f.get ();
+  throw DRayError ("No field named '" + name + "'");
+}
+
+std::vector<std::string> DataSet::fields () const
+{
+  std::vector<std::string> out;
+  for (const auto &f : m_fields) out.push_back (f->name ());
+  return out;
+}
+
+DataSet Collection::domain (int32 index)
+{
+  if (index < 0 || index >= size ()) throw DRayError ("Collection: bad domain index " + std::to_string (index));
+  return m_domains[index];
+}
+
+bool Collection::has_field (const std::string &name)
+{
+  bool res = true;
+  for (auto &d : m_domains) res = res && d.has_field (name);
+  return res;
+}
+
+Range Collection::range (const std::string &name)
+{
+  Range r;
+  for (auto &d : m_domains) r.include (d.field (name)->range ());
+  return r;
+}
+
+AABB<3> Collection::bounds ()
+{
+  AABB<3> b;
+  for (auto &d : m_domains) b.include (d.mesh ()->bounds ());
+  return b;
+}
+
+std::string Collection::field_list ()
+{
+  std::string s = "[";
+  if (!m_domains.empty ())
+    for (const auto &n : m_domains[0].fields ()) s += " " + n;
+  return s + " ]";
+}
+
+// ---- Volume --------------------------------------------------------------------------------------------------
+Volume::Volume (Collection &collection) : m_collection (collection)
+{
+  // dray's constructor adds a default alpha ramp to the colour map's table (volume.cpp:274-285)
+  m_color_map.color_table ().add_alpha (0.1000f, .0f);
+  m_color_map.color_table ().add_alpha (1.0000f, .7f);
+  m_bounds = m_collection.bounds ();
+}
+
+void Volume::input (Collection &collection)
+{
+  m_collection = collection;
+  m_active_domain = 0;
+  m_bounds = m_collection.bounds ();
+}
+
+void Volume::field (const std::string &name)
+{
+  m_field = name;
+  m_field_range = m_collection.range (name);
 }
 
 Array<VolumePartial> Volume::integrate (Array<Ray> &rays, Array<PointLight> &lights)
 {
+  if (m_field.empty ()) throw DRayError ("Field never set");
+  if (!m_color_map.range_set ()) m_color_map.scalar_range (m_field_range);
   if (m_use_lighting && lights.size () > DRX_MAX_LIGHTS)
     throw DRayError ("Volume::integrate: at most " + std::to_string (DRX_MAX_LIGHTS) + " lights");
+  DataSet ds = m_collection.domain (m_active_domain);
+  UnstructuredMesh &mesh = *ds.mesh ();
+  UnstructuredField &fld = *ds.field (m_field);
   drx_shading shading;
   shading.use_lighting = m_use_lighting ? 1 : 0;
   shading.n_lights = (int32)lights.size ();
   shading.lights = lights.size () ? lights.get_host_ptr_const () : nullptr;
-  if (!m_range_set) m_range = m_field.range ();
-  AABB<> b = m_mesh.bounds ();
-  const float bounds6[6] = { b.m_ranges[0].m_min, b.m_ranges[0].m_max, b.m_ranges[1].m_min,
-                             b.m_ranges[1].m_max, b.m_ranges[2].m_min, b.m_ranges[2].m_max };
+  const Range range = m_color_map.scalar_range ();
+  const AABB<> mb = mesh.bounds ();
+  const float mesh6[6] = { mb.m_ranges[0].m_min, mb.m_ranges[0].m_max, mb.m_ranges[1].m_min,
+                           mb.m_ranges[1].m_max, mb.m_ranges[2].m_min, mb.m_ranges[2].m_max };
+  const float bounds6[6] = { m_bounds.m_ranges[0].m_min, m_bounds.m_ranges[0].m_max, m_bounds.m_ranges[1].m_min,
+                             m_bounds.m_ranges[1].m_max, m_bounds.m_ranges[2].m_min, m_bounds.m_ranges[2].m_max };
   // opacity-corrected table sampled to 1024 entries (volume.cpp:81-87, color_map.cpp:13)
   Array<Vec<float32, 4>> lut;
-  m_table.correct_opacity (10.f / m_samples).sample (1024, lut);
+  m_color_map.color_table ().correct_opacity (10.f / m_samples).sample (1024, lut);
+  // sample distance from the collection's bounds (volume.cpp:89-91); rays culled against the domain's
+  // mesh bounds (volume.cpp:106)
   const float sample_dist = sample_distance (bounds6, m_samples);
   Array<Ray> active (std::max<size_t> (rays.size (), 1));
   int32 n_active = 0;
   if (rays.size ())
-    drx_check (drx_remove_missed_rays (rays.get_device_ptr_const (), (int32)rays.size (), bounds6,
+    drx_check (drx_remove_missed_rays (rays.get_device_ptr_const (), (int32)rays.size (), mesh6,
                                        active.get_device_ptr (), &n_active, nullptr),
                "remove_missed_rays");
   Array<int32> order;
@@ -399,13 +481,13 @@ Array<VolumePartial> Volume::integrate (Array<Ray> &rays, Array<PointLight> &lig
   }
   Array<VolumePartial> partials ((size_t)std::max (1, 5 * n_active));
   int32 count = 0;
-  drx_mesh mv = m_mesh.device_view ();
-  drx_mesh fv = m_field.device_view ();
-  drx_bvh bv = m_mesh.bvh_view ();
+  drx_mesh mv = mesh.device_view ();
+  drx_mesh fv = fld.device_view ();
+  drx_bvh bv = mesh.bvh_view ();
   drx_locate_opts o {};
   o.cond_threshold = 1e4f;
   drx_check (drx_volume_integrate (&mv, &fv, &bv, active.get_device_ptr_const (), n_active,
-                                   (const float *)lut.get_device_ptr_const (), 1024, m_range.m_min, m_range.m_max,
+                                   (const float *)lut.get_device_ptr_const (), 1024, range.m_min, range.m_max,
                                    sample_dist, order.size () ? order.get_device_ptr_const () : nullptr, &o,
                                    &shading, partials.get_device_ptr (), &count, &m_stats, nullptr),
              "Volume::integrate");
@@ -416,6 +498,50 @@ Array<VolumePartial> Volume::integrate (Array<Ray> &rays, Array<PointLight> &lig
                           hipMemcpyDeviceToDevice),
                "Volume::integrate: partials copy");
   return out;
+}
+
+// ---- PointLocation -------------------------------------------------------------------------------------------
+static __global__ void k_fill_value (Float *p, int64_t n, Float v)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+PointLocation::Result PointLocation::execute (Collection &collection, Array<Vec<Float, 3>> &points)
+{
+  if (m_vars.empty ()) throw DRayError ("PointLocation: must specify at least 1 variables.");
+  if (points.size () == 0) throw DRayError ("PointLocation: must specify at least 1 point.");
+  std::vector<std::string> valid;
+  for (const auto &v : m_vars)
+    if (collection.has_field (v)) valid.push_back (v);
+  if (valid.empty ())
+    throw DRayError ("PointLocation: must specify at least one valid field. Known fields " + collection.field_list ());
+  const int64_t n = (int64_t)points.size ();
+  std::vector<Array<Float>> values (valid.size ());
+  for (auto &v : values)
+  {
+    v.resize ((size_t)n);
+    hipLaunchKernelGGL (k_fill_value, dim3 ((unsigned)((n + 255) / 256)), dim3 (256), 0, 0, v.get_device_ptr (), n,
+                        m_empty_val);
+    hip_check (hipGetLastError (), "PointLocation: fill");
+  }
+  for (int32 d = 0; d < collection.local_size (); ++d)
+  {
+    DataSet ds = collection.domain (d);
+    Array<Location> locs = ds.mesh ()->locate (points);
+    // has_data: any point located in this domain
+    const Location *lh = locs.get_host_ptr_const ();
+    bool has = false;
+    for (int64_t i = 0; i < n && !has; ++i) has = lh[i].m_cell_id != -1;
+    if (!has) continue;
+    for (size_t f = 0; f < valid.size (); ++f) ds.field (valid[f])->eval (locs, values[f]);
+  }
+  Result r;
+  r.m_points = points;
+  r.m_values = values;
+  r.m_vars = m_vars;
+  r.m_empty_val = m_empty_val;
+  return r;
 }
 
 Array<Vec<float32, 4>> Volume::composite (Array<VolumePartial> &partials, int32 width, int32 height)

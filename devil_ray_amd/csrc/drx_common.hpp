@@ -50,6 +50,7 @@ enum ArenaSlot
   SLOT_PARTIALS,
   SLOT_FLAGS,
   SLOT_STATS,
+  SLOT_PYRAMID,
   SLOT_COUNT
 };
 void *arena (int slot, size_t bytes);   // nullptr + error set on failure

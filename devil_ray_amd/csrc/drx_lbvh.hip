@@ -49,9 +49,6 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
   constexpr int NPE = (PMAX + 1) * (PMAX + 1) * (PMAX + 1);
   constexpr int DPL = (NPE + 63) / 64;   // dofs per lane
   __shared__ float nets[kSplitBoxes * 3 * NPE];
-  __shared__ float boxes[kSplitBoxes * 6];
-  __shared__ float refs[kSplitBoxes * 6];
-  __shared__ float measure[kSplitBoxes];
 
   const int e = blockIdx.x;
   if (e >= n_el) return;
@@ -79,40 +76,46 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
       }
     }
   }
-  float b0[6];
+  // Per-box state lives in registers: lane b (< 33) holds box b's hull, SubRef and measure.
+  float my_box[6], my_ref[6], my_meas = -INFINITY;
+  {
+    float b0[6];
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
-  {
-    b0[2 * c] = wave_reduce_min (mn[c]);
-    b0[2 * c + 1] = wave_reduce_max (mx[c]);
-  }
-  if (lane == 0)
-  {
-    for (int q = 0; q < 6; ++q) boxes[q] = b0[q];
-    measure[0] = box_measure (b0);
+    for (int c = 0; c < 3; ++c)
+    {
+      b0[2 * c] = wave_reduce_min (mn[c]);
+      b0[2 * c + 1] = wave_reduce_max (mx[c]);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) my_box[q] = b0[q];
+#pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-      refs[d] = 0.f;
-      refs[3 + d] = 1.f;
+      my_ref[d] = 0.f;
+      my_ref[3 + d] = 1.f;
     }
+    if (lane == 0) my_meas = box_measure (b0);
   }
   __syncthreads ();
 
   for (int count = 1; count <= kSplitBoxes - 1; ++count)
   {
-    // pick the box (uniform across the wave)
+    // pick the box: dray's scan keeps the first strictly larger measure (ties -> lowest index; a NaN
+    // never compares larger, and a NaN first measure is never replaced)
     int max_id = 0;
-    float max_m = measure[0];
-    for (int b = 1; b < count; ++b)
     {
-      const float m = measure[b];
-      if (m > max_m)
+      const float m0 = __int_as_float (__builtin_amdgcn_readfirstlane (__float_as_int (my_meas)));
+      if (m0 == m0)
       {
-        max_id = b;
-        max_m = m;
+        const float cand = (lane < count) ? my_meas : -INFINITY;
+        const float M = wave_reduce_max (cand);
+        const uint64_t hit = __ballot (lane < count && cand == M);
+        max_id = __ffsll ((unsigned long long)hit) - 1;
       }
     }
-    const float *sr = refs + 6 * max_id;
+    float sr[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sr[q] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (my_ref[q]), max_id));
     int axis = 0;
     float maxlen = sr[3] - sr[0];
     for (int d = 1; d < 3; ++d)
@@ -125,12 +128,16 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
       }
     }
     const float split_pt = sr[axis] * (1.0f - 0.5f) + sr[3 + axis] * 0.5f;
-    __syncthreads ();
-    if (lane < 6)
+    if (lane == count)   // complement: lo[axis] = split
     {
-      const float v = refs[6 * max_id + lane];
-      refs[6 * count + lane] = (lane == axis) ? split_pt : v;        // complement: lo[axis] = split
-      if (lane == 3 + axis) refs[6 * max_id + lane] = split_pt;      // in place: hi[axis] = split
+#pragma unroll
+      for (int q = 0; q < 6; ++q) my_ref[q] = (q == axis) ? split_pt : sr[q];
+    }
+    if (lane == max_id)   // in place: hi[axis] = split
+    {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (d == axis) my_ref[3 + d] = split_pt;
     }
     // copy mother -> daughter
     float *mom = nets + max_id * 3 * NPE;
@@ -209,15 +216,17 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
       bb[2 * c] = wave_reduce_min (bmn[c]);
       bb[2 * c + 1] = wave_reduce_max (bmx[c]);
     }
-    if (lane == 0)
+    if (lane == max_id)
     {
-      for (int q = 0; q < 6; ++q)
-      {
-        boxes[6 * max_id + q] = ba[q];
-        boxes[6 * count + q] = bb[q];
-      }
-      measure[max_id] = box_measure (ba);
-      measure[count] = box_measure (bb);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) my_box[q] = ba[q];
+      my_meas = box_measure (ba);
+    }
+    if (lane == count)
+    {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) my_box[q] = bb[q];
+      my_meas = box_measure (bb);
     }
     __syncthreads ();
   }
@@ -225,7 +234,7 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
   if (lane < kSplitBoxes)
   {
     float b[6];
-    for (int q = 0; q < 6; ++q) b[q] = boxes[6 * lane + q];
+    for (int q = 0; q < 6; ++q) b[q] = my_box[q];
     const float s = (float)1.000001;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -248,7 +257,7 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
     for (int q = 0; q < 6; ++q)
     {
       out_aabbs[6 * k + q] = b[q];
-      out_subrefs[6 * k + q] = refs[6 * lane + q];
+      out_subrefs[6 * k + q] = my_ref[q];
     }
     out_prims[k] = e;
   }
@@ -441,7 +450,8 @@ __device__ __forceinline__ int karras_delta (int a, int b, int inner_size, const
 
 // Karras 2012 (linear_bvh_builder.cpp:168-251), including the float ceil of the split search.
 __global__ void k_build_tree (const uint32_t *__restrict__ mc, int inner, int32_t *__restrict__ left,
-                              int32_t *__restrict__ right, int32_t *__restrict__ parent)
+                              int32_t *__restrict__ right, int32_t *__restrict__ parent, int32_t *__restrict__ range_lo,
+                              int32_t *__restrict__ range_hi)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= inner) return;
@@ -462,6 +472,8 @@ __global__ void k_build_tree (const uint32_t *__restrict__ mc, int inner, int32_
     if (t == 1) break;
   }
   const int split = i + s * d + min (d, 0);
+  range_lo[i] = min (i, j);   // node i covers sorted leaves [min(i, j), max(i, j)]
+  range_hi[i] = max (i, j);
   if (min (i, j) == split)
   {
     parent[split + inner] = i;
@@ -517,36 +529,55 @@ __device__ __forceinline__ void box_union (float *o, const float *a, const float
   }
 }
 
-// propagate_aabbs (linear_bvh_builder.cpp:253-317): the first thread to reach a node stops, the second
-// merges both children.  The counter update is acq_rel at agent scope so that the second thread sees the
-// first one's child box on any XCD.
-__global__ void k_refit (const int32_t *__restrict__ left, const int32_t *__restrict__ right,
-                         const int32_t *__restrict__ parent, const float *__restrict__ leaf_aabbs, float *inner_aabbs,
-                         int32_t *counters, int n_leaves)
+// propagate_aabbs (linear_bvh_builder.cpp:253-317) without atomics.  A node's box is the exact min/max
+// union of the boxes of its contiguous range of sorted leaves (the value propagate_aabbs computes, since
+// min/max do not depend on order).  A
+// pyramid of pairwise unions over the sorted leaf boxes (level l entry i covers leaves [i 2^l, (i+1) 2^l))
+// answers each node's range with at most 2 log2(n) reads, so every node is computed independently.
+__device__ __forceinline__ void box_include (float *acc, const float *x)
+{
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (!(x[2 * d] > x[2 * d + 1]))
+    {
+      acc[2 * d] = fminf (acc[2 * d], x[2 * d]);
+      acc[2 * d + 1] = fmaxf (acc[2 * d + 1], x[2 * d]);
+      acc[2 * d] = fminf (acc[2 * d], x[2 * d + 1]);
+      acc[2 * d + 1] = fmaxf (acc[2 * d + 1], x[2 * d + 1]);
+    }
+}
+
+__global__ void k_pyramid_level (const float *__restrict__ src, int n_src, float *__restrict__ dst)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_leaves) return;
-  const int inner = n_leaves - 1;
-  int cur = parent[inner + i];
-  while (cur != -1)
+  if (2 * i >= n_src) return;
+  float acc[6] = { INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY };
+  box_include (acc, src + 12 * (int64_t)i);
+  if (2 * i + 1 < n_src) box_include (acc, src + 12 * (int64_t)i + 6);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) dst[6 * (int64_t)i + q] = acc[q];
+}
+
+constexpr int kMaxLevels = 32;
+struct Pyramid
+{
+  const float *level[kMaxLevels];
+};
+
+__global__ void k_refit_ranges (Pyramid pyr, const int32_t *__restrict__ range_lo, const int32_t *__restrict__ range_hi,
+                                int inner, float *__restrict__ inner_aabbs)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= inner) return;
+  float acc[6] = { INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY };
+  int a = range_lo[i], b = range_hi[i] + 1;
+  for (int l = 0; a < b; ++l, a >>= 1, b >>= 1)
   {
-    const int old = __hip_atomic_fetch_add (&counters[cur], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    const int lc = left[cur], rc = right[cur];
-    float a[6], b[6], o[6];
-    const float *pa = (lc >= inner) ? leaf_aabbs + 6 * (int64_t)(lc - inner) : inner_aabbs + 6 * (int64_t)lc;
-    const float *pb = (rc >= inner) ? leaf_aabbs + 6 * (int64_t)(rc - inner) : inner_aabbs + 6 * (int64_t)rc;
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-    {
-      a[q] = pa[q];
-      b[q] = pb[q];
-    }
-    box_union (o, a, b);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) inner_aabbs[6 * (int64_t)cur + q] = o[q];
-    cur = parent[cur];
+    if (a & 1) box_include (acc, pyr.level[l] + 6 * (int64_t)(a++));
+    if (b & 1) box_include (acc, pyr.level[l] + 6 * (int64_t)(--b));
   }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) inner_aabbs[6 * (int64_t)i + q] = acc[q];
 }
 
 // emit (linear_bvh_builder.cpp:319-397)
@@ -742,18 +773,17 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
   int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * n);
   int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * n);
   int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8));
-  // tree slot: sorted aabbs (6n) | inner aabbs (6 inner) | left | right | parent (2n-1) | counters | partial bounds
+  // tree slot: sorted aabbs (6n) | inner aabbs (6 inner) | left | right | parent (2n-1) | sorted prims | partial bounds
   const int red_blocks = std::min (1024, (int)nblk (n, kRedBlock));
   const size_t tree_floats = 6 * (size_t)n + 6 * (size_t)inner;
-  const size_t tree_ints = (size_t)inner * 3 + (2 * (size_t)n - 1) + (size_t)n;
+  const size_t tree_ints = (size_t)inner * 2 + (2 * (size_t)n - 1) + (size_t)n;
   char *tree = (char *)arena (SLOT_TREE, tree_floats * 4 + tree_ints * 4 + 6 * 4 * (size_t)red_blocks + 64);
   if (!kA || !kB || !vA || !vB || !hist || !tree) return DRX_ERR_HIP;
   float *s_aabbs = (float *)tree;
   float *i_aabbs = s_aabbs + 6 * (size_t)n;
   int32_t *left = (int32_t *)(i_aabbs + 6 * (size_t)inner);
   int32_t *right = left + inner;
-  int32_t *counters = right + inner;
-  int32_t *parent = counters + inner;
+  int32_t *parent = right + inner;
   int32_t *s_prims = parent + (2 * (size_t)n - 1);
   float *partial = (float *)(s_prims + n);
 
@@ -809,9 +839,36 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
   hipLaunchKernelGGL (k_gather_boxes, dim3 (nblk (n, 256)), dim3 (256), 0, st, vs, aabbs, prims, n, s_aabbs, s_prims);
   DRX_LAUNCH_CHECK ();
   DRX_HIP (hipMemcpyAsync (out_leaf_nodes, s_prims, sizeof (int32_t) * n, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL (k_build_tree, dim3 (nblk (inner, 256)), dim3 (256), 0, st, ks, inner, left, right, parent);
-  hipLaunchKernelGGL (k_fill_i32, dim3 (nblk (inner, 256)), dim3 (256), 0, st, counters, (int64_t)inner, 0);
-  hipLaunchKernelGGL (k_refit, dim3 (nblk (n, 256)), dim3 (256), 0, st, left, right, parent, s_aabbs, i_aabbs, counters, n);
+  // pyramid levels 1.. over the sorted leaf boxes (level 0), plus the node ranges
+  size_t pyr_floats = 0;
+  int n_levels = 1;
+  for (int64_t m = n; m > 1; m = (m + 1) / 2)
+  {
+    pyr_floats += 6 * (size_t)((m + 1) / 2);
+    ++n_levels;
+  }
+  if (n_levels > kMaxLevels) return fail (DRX_ERR_ARG, "drx_bvh_build: too many boxes for the refit pyramid");
+  char *pyr_mem = (char *)arena (SLOT_PYRAMID, pyr_floats * sizeof (float) + 2 * sizeof (int32_t) * (size_t)inner + 64);
+  if (!pyr_mem) return DRX_ERR_HIP;
+  int32_t *range_lo = (int32_t *)pyr_mem;
+  int32_t *range_hi = range_lo + inner;
+  float *levels = (float *)(range_hi + inner);
+  hipLaunchKernelGGL (k_build_tree, dim3 (nblk (inner, 256)), dim3 (256), 0, st, ks, inner, left, right, parent,
+                      range_lo, range_hi);
+  Pyramid pyr;
+  pyr.level[0] = s_aabbs;
+  {
+    float *dst = levels;
+    int l = 1;
+    for (int64_t m = n; m > 1; m = (m + 1) / 2, ++l)
+    {
+      hipLaunchKernelGGL (k_pyramid_level, dim3 (nblk ((m + 1) / 2, 256)), dim3 (256), 0, st, pyr.level[l - 1], (int)m, dst);
+      pyr.level[l] = dst;
+      dst += 6 * (size_t)((m + 1) / 2);
+    }
+    for (; l < kMaxLevels; ++l) pyr.level[l] = nullptr;
+  }
+  hipLaunchKernelGGL (k_refit_ranges, dim3 (nblk (inner, 256)), dim3 (256), 0, st, pyr, range_lo, range_hi, inner, i_aabbs);
   hipLaunchKernelGGL (k_emit, dim3 (nblk (inner, 256)), dim3 (256), 0, st, left, right, s_aabbs, i_aabbs, inner,
                       (float4 *)out_inner);
   DRX_LAUNCH_CHECK ();

@@ -23,18 +23,35 @@ __device__ __forceinline__ float wave_reduce_add (float v)
   return v;
 }
 
+// DPP move of v within the wavefront; rows not in ROW_MASK keep `old`.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_move (float old, float v)
+{
+  return __int_as_float (__builtin_amdgcn_update_dpp (__float_as_int (old), __float_as_int (v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Min over the 64 lanes with DPP steps (quad swaps, half-row and row mirrors, row broadcasts): the total
+// ends in lane 63 and is read back to every lane.  min/max are exact, so the order does not matter.
 __device__ __forceinline__ float wave_reduce_min (float v)
 {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fminf (v, __shfl_xor (v, o, 64));
-  return v;
+  v = fminf (v, dpp_move<0xB1> (v, v));        // quad_perm [1,0,3,2]
+  v = fminf (v, dpp_move<0x4E> (v, v));        // quad_perm [2,3,0,1]
+  v = fminf (v, dpp_move<0x141> (v, v));       // row_half_mirror
+  v = fminf (v, dpp_move<0x140> (v, v));       // row_mirror
+  v = fminf (v, dpp_move<0x142, 0xA> (v, v));  // row_bcast15 into rows 1, 3
+  v = fminf (v, dpp_move<0x143, 0xC> (v, v));  // row_bcast31 into rows 2, 3
+  return __int_as_float (__builtin_amdgcn_readlane (__float_as_int (v), 63));
 }
 
 __device__ __forceinline__ float wave_reduce_max (float v)
 {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf (v, __shfl_xor (v, o, 64));
-  return v;
+  v = fmaxf (v, dpp_move<0xB1> (v, v));
+  v = fmaxf (v, dpp_move<0x4E> (v, v));
+  v = fmaxf (v, dpp_move<0x141> (v, v));
+  v = fmaxf (v, dpp_move<0x140> (v, v));
+  v = fmaxf (v, dpp_move<0x142, 0xA> (v, v));
+  v = fmaxf (v, dpp_move<0x143, 0xC> (v, v));
+  return __int_as_float (__builtin_amdgcn_readlane (__float_as_int (v), 63));
 }
 
 // Inclusive prefix sum across the 64 lanes.

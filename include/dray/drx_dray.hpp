@@ -7,7 +7,11 @@
 //   UnstructuredField        data_model/unstructured_field.hpp (eval, range)
 //   ColorTable               color_table.hpp (add_point, add_alpha, set_smooth, correct_opacity, sample)
 //   Camera                   rendering/camera.hpp (set_width/height/fov/pos/look_at/up, create_rays)
-//   Volume                   rendering/volume.hpp:25-60 (field, samples, use_lighting, color_map, integrate, save)
+//   DataSet, Collection      data_model/data_set.hpp:20-59, data_model/collection.hpp:16-55
+//   ColorMap                 color_map.hpp (color_table, scalar_range)
+//   Volume                   rendering/volume.hpp:25-60 (Volume(Collection&), field, samples, use_lighting,
+//                            color_map, active_domain, integrate)
+//   PointLocation            queries/point_location.hpp:14-33 (add_var, empty_val, execute)
 // Errors throw dray::DRayError (error.hpp) carrying drx_last_error().  All compute runs in libdrx's
 // gfx950 kernels; there is no host fallback.
 #pragma once
@@ -96,13 +100,30 @@ typedef Vec<float32, 4> Vec4f;
 
 struct Range
 {
-  float32 m_min, m_max;
+  float32 m_min = 3.402823466e+38f, m_max = -3.402823466e+38f;   // empty (range.hpp)
   float32 min () const { return m_min; }
   float32 max () const { return m_max; }
+  bool is_empty () const { return m_min > m_max; }
+  float32 length () const { return m_max - m_min; }
+  void include (float32 v)
+  {
+    m_min = m_min < v ? m_min : v;
+    m_max = m_max > v ? m_max : v;
+  }
+  void include (const Range &r)
+  {
+    if (r.is_empty ()) return;
+    include (r.m_min);
+    include (r.m_max);
+  }
 };
 template <int dim = 3> struct AABB
 {
   Range m_ranges[dim];   // [xmin, xmax, ymin, ymax, zmin, zmax]
+  void include (const AABB &o)
+  {
+    for (int d = 0; d < dim; ++d) m_ranges[d].include (o.m_ranges[d]);
+  }
 };
 
 struct Location
@@ -241,36 +262,121 @@ private:
   Vec3f m_pos = { { 0.f, 0.f, 0.f } }, m_look_at = { { 0.f, 0.f, -1.f } }, m_up = { { 0.f, 1.f, 0.f } };
 };
 
+// ---- data sets and collections (data_model/data_set.hpp, data_model/collection.hpp) ----------------------
+class DataSet
+{
+public:
+  DataSet () {}
+  explicit DataSet (std::shared_ptr<UnstructuredMesh> mesh) : m_mesh (std::move (mesh)) {}
+  void add_field (std::shared_ptr<UnstructuredField> field);
+  bool has_field (const std::string &name) const;
+  UnstructuredMesh *mesh () { return m_mesh.get (); }
+  UnstructuredField *field (const std::string &name);   // throws DRayError when missing
+  int32 number_of_fields () const { return (int32)m_fields.size (); }
+  std::vector<std::string> fields () const;
+
+private:
+  std::shared_ptr<UnstructuredMesh> m_mesh;
+  std::vector<std::shared_ptr<UnstructuredField>> m_fields;
+};
+
+class Collection
+{
+public:
+  void add_domain (const DataSet &domain) { m_domains.push_back (domain); }
+  DataSet domain (int32 index);
+  int32 size () const { return (int32)m_domains.size (); }         // one rank: all domains are local
+  int32 local_size () const { return (int32)m_domains.size (); }
+  bool has_field (const std::string &name);                         // every domain has it
+  Range range (const std::string &name);                            // union of the domains' field ranges
+  AABB<3> bounds ();                                                // union of the domains' mesh bounds
+  std::string field_list ();
+
+private:
+  std::vector<DataSet> m_domains;
+};
+
+// ---- colour map (color_map.hpp) ----------------------------------------------------------------------------
+class ColorMap
+{
+public:
+  ColorTable &color_table () { return m_table; }
+  void color_table (const ColorTable &t) { m_table = t; }
+  void scalar_range (const Range &r)
+  {
+    m_range = r;
+    m_range_set = true;
+  }
+  Range scalar_range () const { return m_range; }
+  bool range_set () const { return m_range_set; }
+
+private:
+  ColorTable m_table;
+  Range m_range;
+  bool m_range_set = false;
+};
+
 // ---- volume ----------------------------------------------------------------------------------------------
 class Volume
 {
 public:
-  Volume (UnstructuredMesh &mesh, UnstructuredField &field);
+  explicit Volume (Collection &collection);   // adds dray's default alpha ramp (0.1, 0) - (1.0, 0.7)
+  void input (Collection &collection);
+  void field (const std::string &name);       // also takes the collection's range of the field
+  std::string field () const { return m_field; }
   void samples (int32 num_samples) { m_samples = num_samples; }
   void use_lighting (bool do_it) { m_use_lighting = do_it; }
-  ColorTable &color_table () { return m_table; }
-  void field_range (float32 lo, float32 hi) { m_range = { lo, hi }; m_range_set = true; }
+  ColorMap &color_map () { return m_color_map; }
+  void active_domain (int32 index) { m_active_domain = index; }
+  int32 active_domain () const { return m_active_domain; }
+  int32 num_domains () { return m_collection.size (); }
   // libdrx extra: rays came from a width x height camera; process them in 8x8 pixel tiles (same results)
-  void tile_order (int32 width, int32 height) { m_tile_w = width; m_tile_h = height; }
-  // Volume::integrate (volume.cpp:323-351): returns compacted partials (<= 5 per active ray).  With
-  // use_lighting (dray's default) every light shades each sample (VolumeShader::shaded_color); without
-  // lights a lit sample contributes nothing, as in dray.
+  void tile_order (int32 width, int32 height)
+  {
+    m_tile_w = width;
+    m_tile_h = height;
+  }
+  // Volume::integrate (volume.cpp:323-351) on the active domain: compacted partials (<= 5 per active
+  // ray).  With use_lighting (dray's default) every light shades each sample
+  // (VolumeShader::shaded_color); without lights a lit sample contributes nothing, as in dray.
   Array<VolumePartial> integrate (Array<Ray> &rays, Array<PointLight> &lights);
-  Array<VolumePartial> integrate (Array<Ray> &rays);   // no lights
   // Volume::save's compositing + white background (volume.cpp:369-416), RGBA per pixel (no PNG)
   Array<Vec<float32, 4>> composite (Array<VolumePartial> &partials, int32 width, int32 height);
   drx_integrate_stats last_stats () const { return m_stats; }
 
 private:
-  UnstructuredMesh &m_mesh;
-  UnstructuredField &m_field;
   int32 m_samples = 100;          // volume.cpp:275
+  ColorMap m_color_map;
+  Collection m_collection;        // shallow copy (its domains share meshes and fields)
+  std::string m_field;
+  AABB<3> m_bounds;
   bool m_use_lighting = true;     // volume.cpp:277
-  ColorTable m_table;
-  Range m_range = { 0.f, 0.f };
-  bool m_range_set = false;
+  int32 m_active_domain = 0;
+  Range m_field_range;
   int32 m_tile_w = 0, m_tile_h = 0;
   drx_integrate_stats m_stats {};
+};
+
+// ---- point location query (queries/point_location.hpp) -----------------------------------------------------
+class PointLocation
+{
+public:
+  struct Result
+  {
+    Array<Vec<Float, 3>> m_points;
+    std::vector<Array<Float>> m_values;
+    std::vector<std::string> m_vars;
+    Float m_empty_val;
+  };
+  void empty_val (const Float val) { m_empty_val = val; }
+  void add_var (const std::string &var) { m_vars.push_back (var); }
+  // every domain locates every point; a located point takes that domain's field values (later domains
+  // overwrite earlier ones), others keep empty_val (point_location.cpp:176-287, one rank)
+  Result execute (Collection &collection, Array<Vec<Float, 3>> &points);
+
+private:
+  Float m_empty_val = 0;
+  std::vector<std::string> m_vars;
 };
 
 // detail::default_light (rendering/renderer.cpp:141-159): the light the Renderer adds when none is given.

@@ -5,14 +5,17 @@
 //
 //   dray_api_test <in_dir> <out_dir>
 // in_dir:  params.txt  "order n_el n_pts width height samples alpha0 fov px py pz lx ly lz"
-//          ctrl.bin (int32), coords.bin (float32 xyz), field.bin (float32), points.bin (float32 xyz)
+//          ctrl.bin (int32), coords.bin (float32 xyz), field.bin (float32), points.bin (float32 xyz),
+//          pl_points.bin (float32 xyz, PointLocation queries)
 // out_dir: inner.bin leaf.bin ids.bin bounds.bin loc.bin vals.bin rays.bin partials.bin image.bin
-//          stats.txt, and "errors ok" on stdout once the error paths have thrown as expected.
+//          lit_image.bin light.bin bounds2.bin range2.bin pl_values.bin stats.txt, and "errors ok" on
+//          stdout once the error paths have thrown as expected.
 #include "dray/drx_dray.hpp"
 
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -95,8 +98,14 @@ int main (int argc, char **argv)
     ff.m_values = to_array (field);
     ff.m_size_el = n_el;
 
-    UnstructuredMesh mesh (gf, order);
-    UnstructuredField fld (ff, order, "f");
+    auto mesh_p = std::make_shared<UnstructuredMesh> (gf, order);
+    auto fld_p = std::make_shared<UnstructuredField> (ff, order, "f");
+    UnstructuredMesh &mesh = *mesh_p;
+    UnstructuredField &fld = *fld_p;
+    DataSet ds (mesh_p);
+    ds.add_field (fld_p);
+    Collection col;
+    col.add_domain (ds);
 
     // BVH (LinearBVHBuilder through construct_bvh)
     BVH &bvh = mesh.get_bvh ();
@@ -125,25 +134,29 @@ int main (int argc, char **argv)
     cam.create_rays (rays);
     write_array (out + "/rays.bin", rays);
 
-    Volume vol (mesh, fld);
+    Volume vol (col);
+    vol.field ("f");
     vol.use_lighting (false);
     vol.samples (samples);
     vol.tile_order (width, height);
-    ColorTable &tab = vol.color_table ();
+    ColorTable tab;
     tab.add_point (0.f, Vec3f { { 0.5f, 0.5f, 0.5f } });
     tab.add_point (1.f, Vec3f { { 1.f, 0.f, 0.f } });
     tab.add_alpha (0.f, alpha0);
     tab.set_smooth (true);
-    Array<VolumePartial> parts = vol.integrate (rays);
+    vol.color_map ().color_table (tab);
+    Array<PointLight> no_lights;
+    Array<VolumePartial> parts = vol.integrate (rays, no_lights);
     Array<Vec<float32, 4>> img = vol.composite (parts, width, height);
     write_array (out + "/partials.bin", parts);
     write_array (out + "/image.bin", img);
     // lit render with the Renderer's default light (dray's default shading)
     {
-      Volume lit (mesh, fld);
+      Volume lit (col);
+      lit.field ("f");
       lit.samples (samples);
       lit.tile_order (width, height);
-      lit.color_table () = tab;
+      lit.color_map ().color_table (tab);
       Array<PointLight> lights (1);
       lights.get_host_ptr ()[0] = default_light (cam);
       Array<VolumePartial> lp = lit.integrate (rays, lights);
@@ -152,6 +165,35 @@ int main (int argc, char **argv)
       write_raw (out + "/light.bin", &lights.get_host_ptr_const ()[0].pos[0], 13);
     }
     const drx_integrate_stats st = vol.last_stats ();
+
+    // PointLocation over two domains: the mesh and a copy shifted by +1 in x whose field is f + 10
+    {
+      std::vector<Vec<Float, 3>> shifted = coords;
+      for (auto &v : shifted) v[0] = v[0] + 1.0f;
+      std::vector<Vec<Float, 1>> f10 = field;
+      for (auto &v : f10) v[0] = v[0] + 10.0f;
+      GridFunction<3> gf2 = gf;
+      gf2.m_values = to_array (shifted);
+      GridFunction<1> ff2 = ff;
+      ff2.m_values = to_array (f10);
+      DataSet ds2 (std::make_shared<UnstructuredMesh> (gf2, order));
+      ds2.add_field (std::make_shared<UnstructuredField> (ff2, order, "f"));
+      Collection two;
+      two.add_domain (ds);
+      two.add_domain (ds2);
+      const AABB<> b2 = two.bounds ();
+      write_raw (out + "/bounds2.bin", &b2.m_ranges[0].m_min, 6);
+      const Range r2 = two.range ("f");
+      write_raw (out + "/range2.bin", &r2.m_min, 2);
+      std::vector<Vec<Float, 3>> qp = read_raw<Vec<Float, 3>> (in + "/pl_points.bin");
+      Array<Vec<Float, 3>> qa = to_array (qp);
+      PointLocation pl;
+      pl.add_var ("f");
+      pl.add_var ("missing");
+      pl.empty_val (-7.f);
+      PointLocation::Result res = pl.execute (two, qa);
+      write_array (out + "/pl_values.bin", res.m_values[0]);
+    }
     {
       std::ofstream sf (out + "/stats.txt");
       sf << st.located_samples << " " << st.newton_steps << " " << st.candidates << " " << parts.size () << "\n";
@@ -163,9 +205,20 @@ int main (int argc, char **argv)
     ok &= throws ([] { Camera c; c.set_fov (181.f); });
     ok &= throws ([&] { UnstructuredMesh m (gf, 5); });
     ok &= throws ([&] {   // more lights than the library takes
-      Volume v (mesh, fld);
+      Volume v (col);
+      v.field ("f");
       Array<PointLight> many (DRX_MAX_LIGHTS + 1);
       v.integrate (rays, many);
+    });
+    ok &= throws ([&] {   // Volume::integrate before field() (volume.cpp:331-334)
+      Volume v (col);
+      v.integrate (rays, no_lights);
+    });
+    ok &= throws ([&] {   // PointLocation without a valid variable
+      PointLocation pl;
+      pl.add_var ("nope");
+      Array<Vec<Float, 3>> one = to_array (std::vector<Vec<Float, 3>> (1));
+      pl.execute (col, one);
     });
     std::printf ("%s\n", ok ? "errors ok" : "errors FAILED");
     return ok ? 0 : 1;

@@ -54,6 +54,10 @@ def test_cpp_api_matches_c_abi(gpu, orc, tmp_path):
     mesh.values.astype(np.float32).tofile(ind / "coords.bin")
     mesh.field.astype(np.float32).tofile(ind / "field.bin")
     pts.astype(np.float32).tofile(ind / "points.bin")
+    rng = np.random.default_rng(3)
+    plp = np.concatenate([rng.random((2000, 3)) * [2.0, 1.0, 1.0], rng.random((50, 3)) + [0.0, 1.2, 0.0]])
+    plp = plp.astype(np.float32)
+    plp.tofile(ind / "pl_points.bin")
     (ind / "params.txt").write_text(" ".join(str(x) for x in (
         mesh.order, mesh.n_el, len(pts), w, h, samples, repr(float(alpha0)), 45.0, *pos, *look)))
     print(f"[cpp api] p3 n5, {len(pts)} points, {w}x{h} @ {samples} spp", flush=True)
@@ -105,6 +109,25 @@ def test_cpp_api_matches_c_abi(gpu, orc, tmp_path):
     torch.cuda.synchronize()
     assert np.array_equal(np.fromfile(outd / "lit_image.bin", np.float32).view(np.uint32),
                           img2.cpu().numpy().view(np.uint32))
+    # PointLocation over two domains (the mesh, and a copy shifted by +1 in x with field + 10):
+    # a point takes the last domain that locates it, else empty_val (-7)
+    m2h = syn.HexMesh(n=mesh.n, order=mesh.order, ctrl_idx=mesh.ctrl_idx,
+                      values=(mesh.values + np.float32([1.0, 0.0, 0.0])).astype(np.float32),
+                      field=(mesh.field + np.float32(10.0)).astype(np.float32))
+    m2, f2 = gpu.UnstructuredMesh(m2h), gpu.UnstructuredField(m2h)
+    dpl = gpu.to_device(plp.reshape(-1))
+    l0, l1 = m.locate(dpl), m2.locate(dpl)
+    v0, v1 = f.eval(l0).cpu().numpy(), f2.eval(l1).cpu().numpy()
+    c0 = l0.cpu().numpy().view(LOCATION_DTYPE)["cell"]
+    c1 = l1.cpu().numpy().view(LOCATION_DTYPE)["cell"]
+    exp = np.where(c1 >= 0, v1, np.where(c0 >= 0, v0, np.float32(-7.0)))
+    got = np.fromfile(outd / "pl_values.bin", np.float32)
+    assert np.array_equal(got.view(np.uint32), exp.astype(np.float32).view(np.uint32))
+    assert (c0 >= 0).sum() > 500 and (c1 >= 0).sum() > 500 and (exp == -7).sum() >= 50
+    b2 = np.fromfile(outd / "bounds2.bin", np.float32)
+    assert b2[0] == ob.bounds[0] and np.isclose(b2[1], ob.bounds[1] + 1.0) and np.array_equal(b2[2:], ob.bounds[2:])
+    r2 = np.fromfile(outd / "range2.bin", np.float32)
+    assert r2[0] == mesh.field.min() and r2[1] == np.float32(mesh.field.max() + np.float32(10.0))
     st = (outd / "stats.txt").read_text().split()
     assert int(st[0]) == vol.last_stats.located_samples and int(st[3]) == n
     assert LOCATION_DTYPE.itemsize == 16 and RAY_DTYPE.itemsize == 36
