@@ -85,6 +85,7 @@ _SIGS = {
                                    C.POINTER(I32), C.POINTER(DrxIntegrateStats), P]),
     "drx_default_light": (I32, [C.POINTER(DrxCamera), C.POINTER(DrxPointLight)]),
     "drx_composite": (I32, [P, I32, I32, I32, P, P]),
+    "drx_composite_domains": (I32, [P, I32, I32, I32, P, P]),
     "drx_color_table_sample": (I32, [P, I32, P, I32, I32, F32, I32, F32, P]),
     "drx_mlp_param_count": (I64, []),
     "drx_mlp_state_bytes": (I64, []),

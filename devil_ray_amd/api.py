@@ -356,6 +356,7 @@ class Volume:
         self.range = None
         self.opts = LocateOptions()
         self.last_stats = None
+        self.sample_bounds = None   # dray's m_bounds (the Collection's bounds) for the sample distance
 
     def samples(self, n: int):
         self.m_samples = int(n)
@@ -385,7 +386,7 @@ class Volume:
             active, n_act = active
         lo, hi = self.range if self.range is not None else self.field_.range()
         lut = to_device(self.table.sample(1024, opacity_factor=10.0 / self.m_samples).reshape(-1))
-        sd = sample_distance(bounds, self.m_samples)
+        sd = sample_distance(bounds if self.sample_bounds is None else self.sample_bounds, self.m_samples)
         order = ray_tile_order(active, n_act, tile_width, tile_height) if (tile_width and n_act) else None
         out = torch.empty(max(5 * n_act, 1) * PARTIAL_DTYPE.itemsize, dtype=torch.uint8, device=_dev())
         cnt = C.c_int32(0)
@@ -405,6 +406,17 @@ def default_light(camera: "Camera") -> np.ndarray:
     out = L.DrxPointLight()
     L.check(L.lib().drx_default_light(C.byref(cc), C.byref(out)), "drx_default_light")
     return np.array(list(out.pos) + list(out.amb) + list(out.diff) + list(out.spec) + [out.spec_pow], np.float32)
+
+
+def composite_domains(domain_partials, width: int, height: int) -> torch.Tensor:
+    """Renderer::render's cross-domain compositing: [(partials uint8 tensor, count), ...] of several
+    domains -> RGBA float32 [w*h*4] (partials sorted by pixel and depth, blended front to back)."""
+    parts = [p[: n * PARTIAL_DTYPE.itemsize] for p, n in domain_partials if n > 0]
+    n = sum(int(q.numel()) for q in parts) // PARTIAL_DTYPE.itemsize
+    cat = torch.cat(parts) if parts else torch.empty(PARTIAL_DTYPE.itemsize, dtype=torch.uint8, device=_dev())
+    img = torch.empty(width * height * 4, dtype=torch.float32, device=_dev())
+    L.check(L.lib().drx_composite_domains(_ptr(cat), n, width, height, _ptr(img), _stream()), "drx_composite_domains")
+    return img
 
 
 def composite(partials: torch.Tensor, n: int, width: int, height: int) -> torch.Tensor:

@@ -500,6 +500,28 @@ Array<VolumePartial> Volume::integrate (Array<Ray> &rays, Array<PointLight> &lig
   return out;
 }
 
+Array<Vec<float32, 4>> composite_domains (std::vector<Array<VolumePartial>> &domain_partials, int32 width,
+                                          int32 height)
+{
+  size_t total = 0;
+  for (auto &p : domain_partials) total += p.size ();
+  Array<VolumePartial> all (std::max<size_t> (total, 1));
+  size_t off = 0;
+  for (auto &p : domain_partials)
+  {
+    if (p.size ())
+      hip_check (hipMemcpy (all.get_device_ptr () + off, p.get_device_ptr_const (), sizeof (VolumePartial) * p.size (),
+                            hipMemcpyDeviceToDevice),
+                 "composite_domains: gather");
+    off += p.size ();
+  }
+  Array<Vec<float32, 4>> img ((size_t)width * height);
+  drx_check (drx_composite_domains (all.get_device_ptr_const (), (int32)total, width, height,
+                                    (float *)img.get_device_ptr (), nullptr),
+             "composite_domains");
+  return img;
+}
+
 // ---- PointLocation -------------------------------------------------------------------------------------------
 static __global__ void k_fill_value (Float *p, int64_t n, Float v)
 {

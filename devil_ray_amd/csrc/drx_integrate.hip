@@ -155,7 +155,7 @@ __device__ __forceinline__ float4 lit_color (const IntegrateArgs &a, int cell, c
 }
 
 template <int P, bool POLISH, int MODE, bool LIT>
-__global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_integrate (IntegrateArgs a)
+__global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_integrate (IntegrateArgs a)
 {
   constexpr bool WAVE = MODE == 2;
   constexpr bool LEARNED = MODE == 3;   // learned Newton start; the whole wavefront stays in the loop

@@ -15,7 +15,7 @@ namespace drx
 // Batched locate (UnstructuredMesh::locate, unstructured_mesh.cpp:123-148): one lane per point.
 // ---------------------------------------------------------------------------------------------
 template <int P, bool POLISH, int MODE>
-__global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate (LocateCfg cfg, const float *__restrict__ pts, int64_t n,
+__global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_locate (LocateCfg cfg, const float *__restrict__ pts, int64_t n,
                                                   const int32_t *__restrict__ order, Location *__restrict__ out,
                                                   int32_t *__restrict__ steps_out, int32_t *__restrict__ cand_out,
                                                   int32_t *__restrict__ err)
@@ -58,7 +58,7 @@ __global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate (LocateCfg cf
 // Mesh::locate with the learned initial guess (binary per-lane traversal).  Lanes past n stay in the
 // kernel as idle participants of their wavefront's MLP rounds.
 template <int P, bool POLISH>
-__global__ void __launch_bounds__ (256, DRX_WAVES_PER_EU) k_locate_mlp (LocateCfg cfg, MlpNet net, int budget,
+__global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, 3)) k_locate_mlp (LocateCfg cfg, MlpNet net, int budget,
                                                                        const float *__restrict__ pts, int64_t n,
                                                                        const int32_t *__restrict__ order,
                                                                        Location *__restrict__ out,
@@ -311,6 +311,33 @@ __global__ void k_compact_i32 (const int32_t *__restrict__ v, int64_t n, const i
 // Volume::save compositing (volume.cpp:369-416) and Framebuffer::composite_background
 // (framebuffer.cpp:146-162).  One lane per run of equal pixel ids.
 // ---------------------------------------------------------------------------------------------
+// Keys for the cross-domain partial sort: depth as an order-preserving unsigned key, then pixel id.
+__global__ void k_depth_keys (const Partial *__restrict__ p, int n, uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = __float_as_uint (p[i].depth);
+  keys[i] = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  vals[i] = i;
+}
+
+__global__ void k_pixel_keys (const Partial *__restrict__ p, const int32_t *__restrict__ perm, int n,
+                              uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  keys[i] = (uint32_t)p[j].pixel_id;
+  vals[i] = j;
+}
+
+__global__ void k_gather_partials (const Partial *__restrict__ p, const int32_t *__restrict__ perm, int n,
+                                   Partial *__restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = p[perm[i]];
+}
+
 __global__ void k_fill_f32 (float *p, int64_t n, float v)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -541,6 +568,43 @@ int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t heig
   hipLaunchKernelGGL (k_compact_i32, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots, slots_n, pos, out_order);
   DRX_LAUNCH_CHECK ();
   return DRX_OK;
+}
+
+int drx_composite_domains (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba,
+                           void *stream)
+{
+  if (n < 0 || width <= 0 || height <= 0 || !out_rgba || (n > 0 && !partials))
+    return fail (DRX_ERR_ARG, "drx_composite_domains: bad arguments");
+  if (n == 0) return drx_composite (partials, n, width, height, out_rgba, stream);
+  hipStream_t st = as_stream (stream);
+  const int n_tiles = radix_tiles (n);
+  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n);
+  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n);
+  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n);
+  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n);
+  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8));
+  Partial *sorted = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n);
+  if (!kA || !kB || !vA || !vB || !hist || !sorted) return DRX_ERR_HIP;
+  const Partial *in = (const Partial *)partials;
+  // stable LSD order: by depth first, then (stably) by pixel -> (pixel, depth) order
+  hipLaunchKernelGGL (k_depth_keys, dim3 (nblk (n, 256)), dim3 (256), 0, st, in, n, kA, vA);
+  DRX_LAUNCH_CHECK ();
+  uint32_t *ks;
+  int32_t *vs;
+  int rc = radix_sort_pairs (kA, vA, kB, vB, hist, n, 32, st, &ks, &vs);
+  if (rc) return rc;
+  uint32_t *k2 = (ks == kA) ? kB : kA;
+  int32_t *v2 = (vs == vA) ? vB : vA;
+  hipLaunchKernelGGL (k_pixel_keys, dim3 (nblk (n, 256)), dim3 (256), 0, st, in, vs, n, k2, v2);
+  DRX_LAUNCH_CHECK ();
+  const int64_t npix = (int64_t)width * height;
+  int pix_bits = 1;
+  while (pix_bits < 32 && (int64_t (1) << pix_bits) < npix) ++pix_bits;
+  rc = radix_sort_pairs (k2, v2, ks, vs, hist, n, pix_bits, st, &ks, &vs);
+  if (rc) return rc;
+  hipLaunchKernelGGL (k_gather_partials, dim3 (nblk (n, 256)), dim3 (256), 0, st, in, vs, n, sorted);
+  DRX_LAUNCH_CHECK ();
+  return drx_composite (sorted, n, width, height, out_rgba, stream);
 }
 
 int drx_composite (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba, void *stream)

@@ -46,8 +46,17 @@ __device__ __forceinline__ float4 color_lookup (const float4 *__restrict__ lut, 
 static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 #ifndef DRX_WAVES_PER_EU
-#define DRX_WAVES_PER_EU 4
+#define DRX_WAVES_PER_EU 0   // 0: per-kernel choice below; otherwise a fixed minimum waves per SIMD
 #endif
+
+// Minimum waves per SIMD for the locate / integrate kernels (sets the VGPR budget: 5 -> 96, 4 -> 128).
+// Measured on MI355X: order <= 3 FP32 walks run fastest at 5 (config 1: 19.97 ms vs 22.37 ms at 4, some
+// spills included); order 4, FP64 polish and the learned start need the 128-register budget (config 4:
+// 297.5 ms at 4 vs 323 ms at 5).
+__host__ __device__ constexpr int waves_per_simd (int P, bool polish, int mode)
+{
+  return DRX_WAVES_PER_EU > 0 ? DRX_WAVES_PER_EU : ((P >= 1 && P <= 3 && !polish && mode != 3) ? 5 : 4);
+}
 
 static __global__ void k_fill_i32q (int32_t *p, int64_t n, int32_t v)
 {

@@ -382,4 +382,9 @@ private:
 // detail::default_light (rendering/renderer.cpp:141-159): the light the Renderer adds when none is given.
 PointLight default_light (Camera &camera);
 
+// Renderer::render's cross-domain volume compositing (rendering/renderer.cpp:307-330): the partials of
+// every domain, sorted by pixel and depth, blended front to back over a white background -> RGBA.
+Array<Vec<float32, 4>> composite_domains (std::vector<Array<VolumePartial>> &domain_partials, int32 width,
+                                          int32 height);
+
 } // namespace dray

@@ -23,6 +23,7 @@
  *                           (+ VolumeShader::color / shaded_color)    rendering/volume.cpp:69-228,323-351,
  *                                                                     rendering/volume_shader.hpp:39-167
  *   drx_default_light       detail::default_light                     rendering/renderer.cpp:141-159
+ *   drx_composite_domains   Renderer::render volume partial compositing  rendering/renderer.cpp:307-330
  *   drx_composite           Volume::save compositing + Framebuffer::  rendering/volume.cpp:369-416,
  *                           composite_background                      rendering/framebuffer.cpp:146-162
  *   drx_color_table_sample  ColorTable::correct_opacity + sample      color_table.cpp:152-324
@@ -236,6 +237,12 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
                           float field_min, float field_max, float sample_dist, const int32_t *ray_order,
                           const drx_locate_opts *opts, const drx_shading *shading, void *out_partials,
                           int32_t *out_count, drx_integrate_stats *stats, void *stream);
+/* Renderer::render's cross-domain volume compositing (rendering/renderer.cpp:307-330, done there by
+ * apcomp's PartialCompositor): partials of any number of domains in any order are sorted by (pixel id,
+ * depth), blended front to back per pixel and put over a white background.  Pixel ids must be in
+ * [0, width*height). */
+int drx_composite_domains (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba,
+                           void *stream);
 /* Renderer's default light for a camera (rendering/renderer.cpp:141-159), host. */
 int drx_default_light (const drx_camera *cam, drx_point_light *out_light);
 /* Partials (grouped per pixel, as integrate emits them) -> RGBA image over a white background. */

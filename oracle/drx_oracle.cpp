@@ -1329,6 +1329,21 @@ int orc_compact_partials (const void *in, i64 n, void *out)
 
 // Volume::save compositing (volume.cpp:369-416) + Framebuffer::composite_background
 // (framebuffer.cpp:146-162; background white).  out_rgba: 4 floats per pixel.
+void orc_composite (const void *parts_v, int n, int w, int h, float *out_rgba);
+
+// Renderer::render's cross-domain compositing (rendering/renderer.cpp:307-330 via apcomp's
+// PartialCompositor, which is not part of the reference sources): partials sorted by (pixel id, depth),
+// ties kept in input order, then composited per pixel front to back as Volume::save does.
+void orc_composite_domains (const void *parts_v, int n, int w, int h, float *out_rgba)
+{
+  const Partial *p = (const Partial *)parts_v;
+  std::vector<Partial> v (p, p + n);
+  std::stable_sort (v.begin (), v.end (), [] (const Partial &a, const Partial &b) {
+    return a.pixel_id != b.pixel_id ? a.pixel_id < b.pixel_id : a.depth < b.depth;
+  });
+  orc_composite (v.data (), n, w, h, out_rgba);
+}
+
 void orc_composite (const void *parts_v, int n, int w, int h, float *out_rgba)
 {
   const Partial *p = (const Partial *)parts_v;

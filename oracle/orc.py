@@ -50,6 +50,7 @@ _SIGS = {
     "orc_shade_f64": (None, [P, P, P, P, P, P, P, I32, I32, I32, P, P, P, I32, F64, F64, P, I32, P]),
     "orc_compact_partials": (I32, [P, I64, P]),
     "orc_composite": (None, [P, I32, I32, I32, P]),
+    "orc_composite_domains": (None, [P, I32, I32, I32, P]),
     "orc_color_table_sample": (None, [P, I32, P, I32, I32, F32, I32, F32, P]),
 }
 
@@ -283,6 +284,14 @@ def compact_partials(parts):
     out = np.zeros_like(parts)
     k = lib().orc_compact_partials(ptr(parts), int(parts.shape[0]), ptr(out))
     return out[:k].copy()
+
+
+def composite_domains(parts, w, h):
+    """Cross-domain compositing: sort by (pixel, depth), then composite like composite()."""
+    out = np.zeros((w * h, 4), np.float32)
+    p = np.ascontiguousarray(parts)
+    lib().orc_composite_domains(ptr(p), int(p.shape[0]), w, h, ptr(out))
+    return out
 
 
 def composite(parts, w, h):

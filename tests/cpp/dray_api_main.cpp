@@ -8,7 +8,7 @@
 //          ctrl.bin (int32), coords.bin (float32 xyz), field.bin (float32), points.bin (float32 xyz),
 //          pl_points.bin (float32 xyz, PointLocation queries)
 // out_dir: inner.bin leaf.bin ids.bin bounds.bin loc.bin vals.bin rays.bin partials.bin image.bin
-//          lit_image.bin light.bin bounds2.bin range2.bin pl_values.bin stats.txt, and "errors ok" on
+//          lit_image.bin light.bin bounds2.bin range2.bin pl_values.bin two_image.bin stats.txt, and "errors ok" on
 //          stdout once the error paths have thrown as expected.
 #include "dray/drx_dray.hpp"
 
@@ -193,6 +193,21 @@ int main (int argc, char **argv)
       pl.empty_val (-7.f);
       PointLocation::Result res = pl.execute (two, qa);
       write_array (out + "/pl_values.bin", res.m_values[0]);
+
+      // Renderer-style volume render of the two-domain collection: every domain, then cross-domain compositing
+      Volume v2 (two);
+      v2.field ("f");
+      v2.use_lighting (false);
+      v2.samples (samples);
+      v2.color_map ().color_table (tab);
+      std::vector<Array<VolumePartial>> dp;
+      for (int d = 0; d < v2.num_domains (); ++d)
+      {
+        v2.active_domain (d);
+        dp.push_back (v2.integrate (rays, no_lights));
+      }
+      Array<Vec<float32, 4>> timg = composite_domains (dp, width, height);
+      write_array (out + "/two_image.bin", timg);
     }
     {
       std::ofstream sf (out + "/stats.txt");

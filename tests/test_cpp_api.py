@@ -128,6 +128,20 @@ def test_cpp_api_matches_c_abi(gpu, orc, tmp_path):
     assert b2[0] == ob.bounds[0] and np.isclose(b2[1], ob.bounds[1] + 1.0) and np.array_equal(b2[2:], ob.bounds[2:])
     r2 = np.fromfile(outd / "range2.bin", np.float32)
     assert r2[0] == mesh.field.min() and r2[1] == np.float32(mesh.field.max() + np.float32(10.0))
+    # the two-domain render: each domain with the collection's bounds and field range, then composite_domains
+    dparts = []
+    for mm, ff in ((m, f), (m2, f2)):
+        vv = gpu.Volume(mm, ff)
+        vv.use_lighting = False
+        vv.samples(samples)
+        vv.sample_bounds = b2
+        vv.color_map_range(float(r2[0]), float(r2[1]))
+        vv.table = vol.table
+        dparts.append(vv.integrate(rays))
+    timg = gpu.composite_domains(dparts, w, h)
+    torch.cuda.synchronize()
+    assert np.array_equal(np.fromfile(outd / "two_image.bin", np.float32).view(np.uint32),
+                          timg.cpu().numpy().view(np.uint32))
     st = (outd / "stats.txt").read_text().split()
     assert int(st[0]) == vol.last_stats.located_samples and int(st[3]) == n
     assert LOCATION_DTYPE.itemsize == 16 and RAY_DTYPE.itemsize == 36

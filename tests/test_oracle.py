@@ -486,3 +486,43 @@ def test_volume_lit_matches_per_sample_shading(orc):
         f[3] = al + f[3]
         d += sd
     assert np.allclose(parts[0]["rgba"], f, atol=1e-5)
+
+
+# ---- cross-domain compositing ------------------------------------------------------------------------------
+def _partials(rows):
+    from devil_ray_amd.synthetic import PARTIAL_DTYPE
+    a = np.zeros(len(rows), PARTIAL_DTYPE)
+    for i, (pid, depth, rgba) in enumerate(rows):
+        a[i]["pixel_id"], a[i]["depth"], a[i]["rgba"] = pid, depth, rgba
+    return a
+
+
+def test_composite_domains_orders_by_depth(orc):
+    """Two domains' partials for one pixel arrive back-to-front and interleaved with another pixel: the
+    result is the front-to-back blend in depth order over white."""
+    front, back = (0.2, 0.1, 0.0, 0.4), (0.0, 0.3, 0.5, 0.5)
+    p = _partials([(1, 2.0, back), (0, 1.0, (0.1, 0.1, 0.1, 0.2)), (1, 1.0, front)])
+    out = orc.composite_domains(p, 2, 1)
+    a = 1 - front[3]
+    blended = np.array([front[c] + back[c] * a for c in range(3)] + [front[3] + back[3] * a], np.float32)
+    exp1 = blended[:3] + (1 - blended[3]) * 1.0          # white background under alpha < 1
+    assert np.allclose(out[1, :3], exp1, atol=1e-6) and np.isclose(out[1, 3], 1.0)
+    assert np.allclose(out[0, :3], np.array([0.1] * 3) + 0.8, atol=1e-6)
+
+
+def test_composite_domains_matches_single_domain_and_is_order_free(orc):
+    mesh = syn.hex_mesh(4, 2, 0.05, 0.01, 3)
+    ob = orc.build_mesh_bvh(mesh)
+    w = h = 24
+    rays = orc.camera_perspective(w, h, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5))
+    act = orc.remove_missed_rays(rays, ob.bounds)
+    lut = np.zeros((1024, 4), np.float32)
+    lut[:, 0] = np.linspace(0, 1, 1024)
+    lut[:, 3] = 0.05
+    parts, _, _ = orc.integrate(mesh, ob, act, orc.sample_dist(ob.bounds, 64), lut, float(mesh.field.min()),
+                                float(mesh.field.max()))
+    parts = orc.compact_partials(parts)
+    ref = orc.composite(parts, w, h)
+    assert np.array_equal(orc.composite_domains(parts, w, h), ref)            # already in (pixel, depth) order
+    shuffled = parts[np.random.default_rng(0).permutation(len(parts))]
+    assert np.array_equal(orc.composite_domains(shuffled, w, h), ref)         # any input order
