@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy (x2.0, x1.0); the summary with its commands is
 # profiles/k_integrate_p3_64cubed_1024sq_512spp.md.  Keyed by (config, kernel) so that other workloads or
 # kernel variants report null rather than a stale figure.
-PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,false>"): 6.89e9 + 2.68e9}
+PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,false>"): 15.85e9 + 8.00e9}
 
 
 def _cpu_model() -> str:
@@ -72,6 +72,11 @@ def main() -> int:
     ap.add_argument("--error-sample", type=int, default=0,
                     help="report ref-coordinate / RGBA error vs the FP64 oracle on this many points / rays/10")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); rays shard across GPUs "
+              "with no exchange, so the multi-GPU figure is N x the one-GPU value by construction and is not measured.",
+              file=sys.stderr)
+        return 2
     if args.config == 0:
         args.n, args.order, args.res, args.samples = 8, 2, 256, 200
     elif args.config == 4:
@@ -82,11 +87,6 @@ def main() -> int:
     elif args.config == 3:
         return bench_mlp(args)
 
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); rays shard across GPUs "
-              "with no exchange, so the multi-GPU figure is N x the one-GPU value by construction and is not measured.",
-              file=sys.stderr)
-        return 2
 
     import torch
     from devil_ray_amd import api, synthetic as syn

@@ -14,6 +14,9 @@ namespace drx
 #ifndef DRX_EVAL_UNROLL_K
 #define DRX_EVAL_UNROLL_K 1           // w-slabs of control points in flight per step of eval_geom
 #endif
+#ifndef DRX_EVAL_ROTATE_MIN_ORDER
+#define DRX_EVAL_ROTATE_MIN_ORDER 4   // orders >= this rotate the w-weight arrays per slab instead of selecting
+#endif                                // by slab index (measured: faster at order 4, slower at order 3)
 
 template <int P> struct Ord
 {
@@ -79,7 +82,25 @@ __device__ __forceinline__ void eval_geom (const float *__restrict__ g, const T 
 #pragma unroll DRX_EVAL_UNROLL_K
   for (int k = 0; k < N1; ++k)
   {
-    const T bw = pick<N1, T> (Bw, k), dbw = pick<N1, T> (dBw, k);
+    T bw, dbw;
+    if constexpr (P >= DRX_EVAL_ROTATE_MIN_ORDER)
+    {
+      // slab k's w-weights are always in slot 0: the arrays rotate by one each slab, so they stay in
+      // registers (the compiler turns a runtime-indexed read of a 5-entry array into scratch memory)
+      bw = Bw[0];
+      dbw = dBw[0];
+#pragma unroll
+      for (int i = 0; i < N1 - 1; ++i)
+      {
+        Bw[i] = Bw[i + 1];
+        dBw[i] = dBw[i + 1];
+      }
+    }
+    else
+    {
+      bw = pick<N1, T> (Bw, k);
+      dbw = pick<N1, T> (dBw, k);
+    }
     const float *slab = X + N1 * N1 * k;
 #pragma unroll
     for (int j = 0; j < N1; ++j)

@@ -366,8 +366,22 @@ __global__ void __launch_bounds__ (64) k_mlp_train (const bf16x8 *__restrict__ x
   if (lane == 0) part[NPARAM] = loss;
 }
 
-// Adam over the summed per-wave gradients (fixed summation order: deterministic), then the bf16
-// copies.  Thread NPARAM writes the step's mean squared error.
+// First level of the gradient sum: group g adds its share of the per-wave rows, in order.
+constexpr int kGradGroups = 16;
+__global__ void __launch_bounds__ (256) k_mlp_grad_groups (const float *__restrict__ partials, int nwaves,
+                                                           float *__restrict__ groups)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > NPARAM) return;
+  const int g = blockIdx.y;
+  const int w0 = (int)((int64_t)nwaves * g / kGradGroups), w1 = (int)((int64_t)nwaves * (g + 1) / kGradGroups);
+  float sum = 0.f;
+  for (int w = w0; w < w1; ++w) sum += partials[(int64_t)w * PSTRIDE + p];
+  groups[(int64_t)g * PSTRIDE + p] = sum;
+}
+
+// Adam over the gradient (the group sums added in a fixed order: deterministic), then the bf16 copies.
+// Thread NPARAM writes the step's mean squared error.
 __global__ void __launch_bounds__ (256) k_mlp_adam (MlpState s, const float *__restrict__ partials, int nwaves, float lr,
                                                     float b1, float b2, float eps, float bc1, float bc2,
                                                     float loss_scale, float *__restrict__ loss_out)
@@ -529,8 +543,9 @@ int drx_mlp_train (void *state, const void *x, const float *target, int64_t n, c
   if (o->batch <= 0 || o->batch % 64 != 0) return fail (DRX_ERR_ARG, "drx_mlp_train: batch %d is not a positive multiple of 64", o->batch);
   if (o->steps < 0) return fail (DRX_ERR_ARG, "drx_mlp_train: negative step count");
   const int nw = o->batch / 64;
-  float *partials = (float *)arena (SLOT_TREE, (size_t)nw * PSTRIDE * sizeof (float));
+  float *partials = (float *)arena (SLOT_TREE, (size_t)(nw + kGradGroups) * PSTRIDE * sizeof (float));
   if (!partials) return DRX_ERR_HIP;
+  float *groups = partials + (size_t)nw * PSTRIDE;
   const MlpState s = state_view (state);
   const hipStream_t st = as_stream (stream);
   const double b1 = o->beta1 > 0 ? o->beta1 : 0.9, b2 = o->beta2 > 0 ? o->beta2 : 0.999;
@@ -547,7 +562,10 @@ int drx_mlp_train (void *state, const void *x, const float *target, int64_t n, c
     // cosine decay from lr to lr_final over total steps
     const double ph = fmin (1.0, (double)(t - 1) / (double)(total > 1 ? total - 1 : 1));
     const float lr = (float)(lr_final + 0.5 * (o->lr - lr_final) * (1.0 + cos (M_PI * ph)));
-    hipLaunchKernelGGL (k_mlp_adam, dim3 ((NPARAM + 1 + 255) / 256), dim3 (256), 0, st, s, partials, nw, lr, (float)b1,
+    hipLaunchKernelGGL (k_mlp_grad_groups, dim3 ((NPARAM + 1 + 255) / 256, kGradGroups), dim3 (256), 0, st, partials, nw,
+                        groups);
+    DRX_LAUNCH_CHECK ();
+    hipLaunchKernelGGL (k_mlp_adam, dim3 ((NPARAM + 1 + 255) / 256), dim3 (256), 0, st, s, groups, kGradGroups, lr, (float)b1,
                         (float)b2, eps, (float)(1.0 - pow (b1, t)), (float)(1.0 - pow (b2, t)),
                         1.f / (float)(o->batch * OUT), out_loss ? out_loss + k : nullptr);
     DRX_LAUNCH_CHECK ();
