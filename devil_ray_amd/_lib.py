@@ -17,6 +17,7 @@ I64 = C.c_int64
 F32 = C.c_float
 
 DRX_OK = 0
+DRX_ERR_ARG = 1
 SPLITS_PER_ELEMENT = 33
 MAX_ORDER = 4
 
@@ -86,6 +87,10 @@ _SIGS = {
     "drx_default_light": (I32, [C.POINTER(DrxCamera), C.POINTER(DrxPointLight)]),
     "drx_composite": (I32, [P, I32, I32, I32, P, P]),
     "drx_composite_domains": (I32, [P, I32, I32, I32, P, P]),
+    "drx_framebuffer_clear": (I32, [P, P, I32, P, P]),
+    "drx_partials_to_framebuffer": (I32, [P, I32, I32, I32, P, P, I32, P]),
+    "drx_framebuffer_composite_background": (I32, [P, I32, P, P]),
+    "drx_save_png": (I32, [P, I32, I32, C.c_char_p]),
     "drx_color_table_sample": (I32, [P, I32, P, I32, I32, F32, I32, F32, P]),
     "drx_mlp_param_count": (I64, []),
     "drx_mlp_state_bytes": (I64, []),

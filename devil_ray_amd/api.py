@@ -426,6 +426,97 @@ def composite(partials: torch.Tensor, n: int, width: int, height: int) -> torch.
     return img
 
 
+class Framebuffer:
+    """dray::Framebuffer (rendering/framebuffer.cpp): RGBA colours (4 floats per pixel) and depths on the
+    device, pixel id = x + y * width with row 0 at the bottom; cleared to transparent black and +inf."""
+
+    def __init__(self, width: int = 1024, height: int = 1024):
+        if width <= 0 or height <= 0:
+            raise L.DrxError("Framebuffer: width and height must be positive")
+        self.width, self.height = int(width), int(height)
+        n = self.width * self.height
+        self.colors = torch.empty(n * 4, dtype=torch.float32, device=_dev())
+        self.depths = torch.empty(n, dtype=torch.float32, device=_dev())
+        self.bg_color = (1.0, 1.0, 1.0, 1.0)
+        self.fg_color = (0.0, 0.0, 0.0, 1.0)
+        self.clear()
+
+    def clear(self, color=(0.0, 0.0, 0.0, 0.0)):
+        c = np.ascontiguousarray(np.asarray(color, np.float32).reshape(4))
+        L.check(L.lib().drx_framebuffer_clear(_ptr(self.colors), _ptr(self.depths), self.width * self.height,
+                                              c.ctypes.data, _stream()), "drx_framebuffer_clear")
+
+    def background_color(self, color):
+        self.bg_color = tuple(float(x) for x in color)
+
+    def composite_background(self):
+        c = np.ascontiguousarray(np.asarray(self.bg_color, np.float32).reshape(4))
+        L.check(L.lib().drx_framebuffer_composite_background(_ptr(self.colors), self.width * self.height,
+                                                             c.ctypes.data, _stream()),
+                "drx_framebuffer_composite_background")
+
+    def save(self, name: str):
+        """Framebuffer::save: writes name + ".png" (8-bit RGBA, row 0 at the bottom of the picture)."""
+        host = np.ascontiguousarray(self.colors.cpu().numpy())
+        L.check(L.lib().drx_save_png(host.ctypes.data, self.width, self.height, (name + ".png").encode()),
+                "drx_save_png")
+
+
+def partials_to_framebuffer(domain_partials, fb: Framebuffer, blend: bool = False):
+    """apcomp PartialCompositor::composite + detail::partials_to_framebuffer (renderer.cpp:97-138,
+    307-330): [(partials uint8 tensor, count), ...] of any number of domains -> fb's colours and depths."""
+    parts = [p[: n * PARTIAL_DTYPE.itemsize] for p, n in domain_partials if n > 0]
+    n = sum(int(q.numel()) for q in parts) // PARTIAL_DTYPE.itemsize
+    if n == 0:
+        return
+    cat = torch.cat(parts) if len(parts) > 1 else parts[0]
+    L.check(L.lib().drx_partials_to_framebuffer(_ptr(cat), n, fb.width, fb.height, _ptr(fb.colors), _ptr(fb.depths),
+                                                1 if blend else 0, _stream()), "drx_partials_to_framebuffer")
+
+
+class Renderer:
+    """dray::Renderer::render's volume branch (rendering/renderer.cpp:229-337): one Volume per domain;
+    every domain integrates the camera's rays with the renderer's lights (default_light(camera) when
+    none were added), and the partials of all domains are composited by (pixel, depth) into a fresh
+    Framebuffer.  With several domains, volumes whose sample bounds or colour-map range are unset take
+    the union over the domains (dray's Volume uses its Collection's bounds and field range)."""
+
+    def __init__(self):
+        self.volumes = []
+        self.lights = []
+
+    def volume(self, volumes):
+        self.volumes = list(volumes) if isinstance(volumes, (list, tuple)) else [volumes]
+
+    def add_light(self, light):
+        self.lights.append(np.asarray(light, np.float32).reshape(13))
+
+    def clear_lights(self):
+        self.lights = []
+
+    def render(self, camera: Camera) -> Framebuffer:
+        fb = Framebuffer(camera.width, camera.height)
+        if not self.volumes:
+            return fb
+        rays = camera.create_rays()
+        lights = self.lights if self.lights else [default_light(camera)]
+        if len(self.volumes) > 1:
+            b = np.stack([v.mesh.bounds() for v in self.volumes]).astype(np.float32)
+            union_b = np.array([b[:, 0].min(), b[:, 1].max(), b[:, 2].min(), b[:, 3].max(), b[:, 4].min(),
+                                b[:, 5].max()], np.float32)
+            rg = [v.field_.range() for v in self.volumes]
+            union_r = (min(r[0] for r in rg), max(r[1] for r in rg))
+            for v in self.volumes:
+                if v.sample_bounds is None:
+                    v.sample_bounds = union_b
+                if v.range is None:
+                    v.range = union_r
+        parts = [v.integrate(rays, tile_width=camera.width, tile_height=camera.height, lights=lights)
+                 for v in self.volumes]
+        partials_to_framebuffer(parts, fb, blend=False)
+        return fb
+
+
 # ---------------------------------------------------------------------------------------------
 class LearnedInitializer:
     """Learned Newton initial guess: MLP 27 -> 64 x4 (GELU) -> 3, BF16 on MFMA (drx_mlp_* in libdrx).

@@ -24,6 +24,7 @@ SOURCES = {
     "drx_integrate.hip": [],
     "drx_wide.hip": [],
     "drx_mlp.hip": [],
+    "drx_image.hip": [],
     "dray_host.hip": [],
 }
 ROOT = os.path.dirname(HERE)

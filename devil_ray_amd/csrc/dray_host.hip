@@ -345,6 +345,108 @@ void Camera::create_rays (Array<Ray> &rays)
   drx_check (drx_camera_rays (&c, rays.get_device_ptr (), nullptr), "Camera::create_rays");
 }
 
+// Orbit helpers restated from dray's Vec / Matrix / rotate (vec.hpp, matrix.hpp, transform_3d.hpp:126-157)
+// with the same float operation order, so a rotated camera gives the same rays as dray's.
+namespace
+{
+Vec3f v_sub (const Vec3f &a, const Vec3f &b) { return { { a[0] - b[0], a[1] - b[1], a[2] - b[2] } }; }
+Vec3f v_add (const Vec3f &a, const Vec3f &b) { return { { a[0] + b[0], a[1] + b[1], a[2] + b[2] } }; }
+Vec3f v_scale (const Vec3f &a, float32 s) { return { { a[0] * s, a[1] * s, a[2] * s } }; }
+float32 v_mag (const Vec3f &a)
+{
+#pragma clang fp contract(off)
+  float32 sum = 0.f;
+  for (int i = 0; i < 3; ++i) sum += a[i] * a[i];
+  return sqrtf (sum);
+}
+void v_normalize (Vec3f &a)
+{
+  const float32 m = v_mag (a);
+  for (int i = 0; i < 3; ++i) a[i] /= m;
+}
+Vec3f v_cross (const Vec3f &a, const Vec3f &b)
+{
+#pragma clang fp contract(off)
+  return { { a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0] } };
+}
+// rotate(degrees, axis) applied to a direction (transform_vector: w = 0)
+Vec3f rotate_vector (float32 degrees, Vec3f axis, const Vec3f &v)
+{
+#pragma clang fp contract(off)
+  const float32 rad = 0.01745329251994329547437168059786927f * degrees;
+  v_normalize (axis);
+  const float32 sn = sinf (rad), cs = cosf (rad);
+  float32 m[3][3];
+  m[0][0] = axis[0] * axis[0] * (1 - cs) + cs;
+  m[0][1] = axis[0] * axis[1] * (1 - cs) - axis[2] * sn;
+  m[0][2] = axis[0] * axis[2] * (1 - cs) + axis[1] * sn;
+  m[1][0] = axis[1] * axis[0] * (1 - cs) + axis[2] * sn;
+  m[1][1] = axis[1] * axis[1] * (1 - cs) + cs;
+  m[1][2] = axis[1] * axis[2] * (1 - cs) - axis[0] * sn;
+  m[2][0] = axis[2] * axis[0] * (1 - cs) - axis[1] * sn;
+  m[2][1] = axis[2] * axis[1] * (1 - cs) + axis[0] * sn;
+  m[2][2] = axis[2] * axis[2] * (1 - cs) + cs;
+  Vec3f r;
+  for (int i = 0; i < 3; ++i)
+  {
+    float32 d = 0.f;
+    for (int j = 0; j < 3; ++j) d += m[i][j] * v[j];
+    d += 0.f * 0.f;   // the homogeneous column (w = 0)
+    r[i] = d;
+  }
+  return r;
+}
+} // namespace
+
+void Camera::reset_to_bounds (const AABB<3> bounds, const float64 xpad, const float64 ypad, const float64 zpad)
+{
+#pragma clang fp contract(off)
+  AABB<3> db;
+  const float64 pads[3] = { xpad, ypad, zpad };
+  for (int d = 0; d < 3; ++d)
+  {
+    const float64 pad = pads[d] * (bounds.m_ranges[d].max () - bounds.m_ranges[d].min ());
+    db.m_ranges[d].include ((float32)(bounds.m_ranges[d].max () + pad));
+    db.m_ranges[d].include ((float32)(bounds.m_ranges[d].min () - pad));
+  }
+  Vec3f proj_dir = v_sub (m_pos, m_look_at);
+  v_normalize (proj_dir);
+  const Vec3f center = { { db.m_ranges[0].center (), db.m_ranges[1].center (), db.m_ranges[2].center () } };
+  m_look_at = center;
+  const Vec3f extent = { { db.m_ranges[0].length (), db.m_ranges[1].length (), db.m_ranges[2].length () } };
+  const float32 diagonal = v_mag (extent);
+  m_pos = v_add (center, v_scale (v_scale (proj_dir, diagonal), 1.0f));
+  set_fov (60.0f);
+}
+
+void Camera::elevate (const float32 degrees)
+{
+#pragma clang fp contract(off)
+  Vec3f look = v_sub (m_look_at, m_pos);
+  const float32 distance = v_mag (look);
+  Vec3f right = v_cross (look, m_up);
+  Vec3f up = v_cross (right, look);
+  v_normalize (look);
+  v_normalize (right);
+  v_normalize (up);
+  look = rotate_vector (-degrees, right, look);
+  m_pos = v_sub (m_look_at, v_scale (look, distance));
+}
+
+void Camera::azimuth (const float32 degrees)
+{
+#pragma clang fp contract(off)
+  Vec3f look = v_sub (m_look_at, m_pos);
+  const float32 distance = v_mag (look);
+  Vec3f right = v_cross (look, m_up);
+  Vec3f up = v_cross (right, look);
+  v_normalize (look);
+  v_normalize (right);
+  v_normalize (up);
+  look = rotate_vector (degrees, up, look);
+  m_pos = v_sub (m_look_at, v_scale (look, distance));
+}
+
 // ---- Volume ----------------------------------------------------------------------------------------------
 // |bounds diagonal| / samples in float32, unfused like dray's (volume.cpp:89-91)
 static float sample_distance (const float *bounds6, int32 samples)
@@ -520,6 +622,87 @@ Array<Vec<float32, 4>> composite_domains (std::vector<Array<VolumePartial>> &dom
                                     (float *)img.get_device_ptr (), nullptr),
              "composite_domains");
   return img;
+}
+
+// ---- Framebuffer / Renderer ------------------------------------------------------------------------------
+Framebuffer::Framebuffer () : Framebuffer (1024, 1024) {}
+
+Framebuffer::Framebuffer (const int32 width, const int32 height) : m_width (width), m_height (height)
+{
+  if (width <= 0 || height <= 0) throw DRayError ("Framebuffer: width and height must be positive");
+  m_colors.resize ((size_t)width * height);
+  m_depths.resize ((size_t)width * height);
+  clear ();
+}
+
+void Framebuffer::clear () { clear ({ { 0.f, 0.f, 0.f, 0.f } }); }
+
+void Framebuffer::clear (const Vec<float32, 4> &color)
+{
+  drx_check (drx_framebuffer_clear ((float *)m_colors.get_device_ptr (), m_depths.get_device_ptr (),
+                                    m_width * m_height, color.m_data, nullptr),
+             "Framebuffer::clear");
+}
+
+void Framebuffer::composite_background ()
+{
+  drx_check (drx_framebuffer_composite_background ((float *)m_colors.get_device_ptr (), m_width * m_height,
+                                                   m_bg_color.m_data, nullptr),
+             "Framebuffer::composite_background");
+}
+
+void Framebuffer::save (const std::string name)
+{
+  const std::string path = name + ".png";
+  drx_check (drx_save_png ((const float *)m_colors.get_host_ptr_const (), m_width, m_height, path.c_str ()),
+             "Framebuffer::save");
+}
+
+Framebuffer Renderer::render (Camera &camera)
+{
+  Array<Ray> rays;
+  camera.create_rays (rays);
+  Framebuffer framebuffer (camera.get_width (), camera.get_height ());
+  if (!m_volume) return framebuffer;
+  Array<PointLight> lights;
+  if (!m_lights.empty ())
+  {
+    lights.resize (m_lights.size ());
+    for (size_t i = 0; i < m_lights.size (); ++i) lights.get_host_ptr ()[i] = m_lights[i];
+  }
+  else
+  {
+    lights.resize (1);
+    lights.get_host_ptr ()[0] = default_light (camera);
+  }
+  m_volume->tile_order (camera.get_width (), camera.get_height ());   // 8x8 tile order, same results
+  std::vector<Array<VolumePartial>> domain_partials;
+  for (int32 d = 0; d < m_volume->num_domains (); ++d)
+  {
+    m_volume->active_domain (d);
+    domain_partials.push_back (m_volume->integrate (rays, lights));
+  }
+  size_t total = 0;
+  for (auto &p : domain_partials) total += p.size ();
+  if (total == 0) return framebuffer;
+  Array<VolumePartial> all (total);
+  size_t off = 0;
+  for (auto &p : domain_partials)
+  {
+    if (p.size ())
+      hip_check (hipMemcpy (all.get_device_ptr () + off, p.get_device_ptr_const (), sizeof (VolumePartial) * p.size (),
+                            hipMemcpyDeviceToDevice),
+                 "Renderer::render: gather partials");
+    off += p.size ();
+  }
+  Array<Vec<float32, 4>> colors = framebuffer.colors ();
+  Array<float32> depths = framebuffer.depths ();
+  // no surface traceables: the volume replaces the cleared pixels (partials_to_framebuffer, blend false)
+  drx_check (drx_partials_to_framebuffer (all.get_device_ptr_const (), (int32)total, camera.get_width (),
+                                          camera.get_height (), (float *)colors.get_device_ptr (),
+                                          depths.get_device_ptr (), 0, nullptr),
+             "Renderer::render");
+  return framebuffer;
 }
 
 // ---- PointLocation -------------------------------------------------------------------------------------------

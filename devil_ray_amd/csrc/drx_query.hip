@@ -344,7 +344,19 @@ __global__ void k_fill_f32 (float *p, int64_t n, float v)
   if (i < n) p[i] = v;
 }
 
-__global__ void k_composite_runs (const Partial *__restrict__ p, int n, int npix, float4 *__restrict__ img)
+__global__ void k_fill_f4 (float4 *p, int64_t n, float4 v)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// One lane per run of equal pixel ids (partials grouped per pixel, front to back).  The run is blended
+// with pre_mult_alpha_blend_host (colors.hpp:45-53) and written as detail::partials_to_framebuffer does
+// (rendering/renderer.cpp:97-138): blend = 0 replaces the pixel; blend = 1 puts the run over what the
+// framebuffer holds (a surface behind the volume): colour = run + (1 - run alpha) * framebuffer.  depth
+// (nullable) takes the depth of the run's first (nearest) partial.
+__global__ void k_composite_runs (const Partial *__restrict__ p, int n, int npix, float4 *__restrict__ img,
+                                  float *__restrict__ depth, int blend)
 {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -360,10 +372,23 @@ __global__ void k_composite_runs (const Partial *__restrict__ p, int n, int npix
     c[2] = c[2] + p[k].rgba[2] * a;
     c[3] = c[3] + p[k].rgba[3] * a;
   }
-  if (pid >= 0 && pid < npix) img[pid] = make_float4 (c[0], c[1], c[2], c[3]);
+  if (pid < 0 || pid >= npix) return;
+  if (blend)
+  {
+    const float4 b = img[pid];
+    const float o = 1.f - c[3];
+    c[0] = c[0] + o * b.x;
+    c[1] = c[1] + o * b.y;
+    c[2] = c[2] + o * b.z;
+    c[3] = c[3] + o * b.w;
+  }
+  img[pid] = make_float4 (c[0], c[1], c[2], c[3]);
+  if (depth) depth[pid] = p[i].depth;
 }
 
-__global__ void k_background (float4 *__restrict__ img, int npix)
+// Framebuffer::composite_background (framebuffer.cpp:146-162) with blend_pre_alpha (colors.hpp:25-33):
+// pixels with alpha < 1 are put over the background colour.
+__global__ void k_background (float4 *__restrict__ img, int npix, float4 bg)
 {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -372,10 +397,10 @@ __global__ void k_background (float4 *__restrict__ img, int npix)
   if (c.w < 1.f)
   {
     const float a = 1.f - c.w;
-    c.x = c.x + 1.f * a;
-    c.y = c.y + 1.f * a;
-    c.z = c.z + 1.f * a;
-    c.w = c.w + 1.f * a;
+    c.x = c.x + bg.x * a;
+    c.y = c.y + bg.y * a;
+    c.z = c.z + bg.z * a;
+    c.w = c.w + bg.w * a;
     img[i] = c;
   }
 }
@@ -570,13 +595,10 @@ int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t heig
   return DRX_OK;
 }
 
-int drx_composite_domains (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba,
-                           void *stream)
+// Stable (pixel id, depth) order of partials from any number of domains, ties kept in input order: an
+// LSD radix sort by depth (order-preserving unsigned key), then stably by pixel id.
+static int sort_partials_pixel_depth (const Partial *in, int32_t n, int64_t npix, hipStream_t st, Partial **out)
 {
-  if (n < 0 || width <= 0 || height <= 0 || !out_rgba || (n > 0 && !partials))
-    return fail (DRX_ERR_ARG, "drx_composite_domains: bad arguments");
-  if (n == 0) return drx_composite (partials, n, width, height, out_rgba, stream);
-  hipStream_t st = as_stream (stream);
   const int n_tiles = radix_tiles (n);
   uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n);
   uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n);
@@ -585,8 +607,6 @@ int drx_composite_domains (const void *partials, int32_t n, int32_t width, int32
   int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8));
   Partial *sorted = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n);
   if (!kA || !kB || !vA || !vB || !hist || !sorted) return DRX_ERR_HIP;
-  const Partial *in = (const Partial *)partials;
-  // stable LSD order: by depth first, then (stably) by pixel -> (pixel, depth) order
   hipLaunchKernelGGL (k_depth_keys, dim3 (nblk (n, 256)), dim3 (256), 0, st, in, n, kA, vA);
   DRX_LAUNCH_CHECK ();
   uint32_t *ks;
@@ -597,27 +617,88 @@ int drx_composite_domains (const void *partials, int32_t n, int32_t width, int32
   int32_t *v2 = (vs == vA) ? vB : vA;
   hipLaunchKernelGGL (k_pixel_keys, dim3 (nblk (n, 256)), dim3 (256), 0, st, in, vs, n, k2, v2);
   DRX_LAUNCH_CHECK ();
-  const int64_t npix = (int64_t)width * height;
   int pix_bits = 1;
   while (pix_bits < 32 && (int64_t (1) << pix_bits) < npix) ++pix_bits;
   rc = radix_sort_pairs (k2, v2, ks, vs, hist, n, pix_bits, st, &ks, &vs);
   if (rc) return rc;
   hipLaunchKernelGGL (k_gather_partials, dim3 (nblk (n, 256)), dim3 (256), 0, st, in, vs, n, sorted);
   DRX_LAUNCH_CHECK ();
-  return drx_composite (sorted, n, width, height, out_rgba, stream);
+  *out = sorted;
+  return DRX_OK;
+}
+
+static bool image_ok (int32_t width, int32_t height)
+{
+  return width > 0 && height > 0 && (int64_t)width * height <= (int64_t (1) << 30);
+}
+
+int drx_framebuffer_clear (float *colors, float *depths, int32_t n_pixels, const float color[4], void *stream)
+{
+  if (n_pixels < 0 || !color || (n_pixels > 0 && !colors))
+    return fail (DRX_ERR_ARG, "drx_framebuffer_clear: bad arguments");
+  if (n_pixels == 0) return DRX_OK;
+  hipStream_t st = as_stream (stream);
+  hipLaunchKernelGGL (k_fill_f4, dim3 (nblk (n_pixels, 256)), dim3 (256), 0, st, (float4 *)colors, (int64_t)n_pixels,
+                      make_float4 (color[0], color[1], color[2], color[3]));
+  if (depths)
+    hipLaunchKernelGGL (k_fill_f32, dim3 (nblk (n_pixels, 256)), dim3 (256), 0, st, depths, (int64_t)n_pixels,
+                        __builtin_huge_valf ());
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_partials_to_framebuffer (const void *partials, int32_t n, int32_t width, int32_t height, float *colors,
+                                 float *depths, int32_t blend, void *stream)
+{
+  if (n < 0 || !image_ok (width, height) || !colors || (n > 0 && !partials) || (blend != 0 && blend != 1))
+    return fail (DRX_ERR_ARG, "drx_partials_to_framebuffer: bad arguments");
+  if (n == 0) return DRX_OK;
+  hipStream_t st = as_stream (stream);
+  Partial *sorted = nullptr;
+  int rc = sort_partials_pixel_depth ((const Partial *)partials, n, (int64_t)width * height, st, &sorted);
+  if (rc) return rc;
+  hipLaunchKernelGGL (k_composite_runs, dim3 (nblk (n, 256)), dim3 (256), 0, st, sorted, n, width * height,
+                      (float4 *)colors, depths, (int)blend);
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_framebuffer_composite_background (float *colors, int32_t n_pixels, const float bg[4], void *stream)
+{
+  if (n_pixels < 0 || !bg || (n_pixels > 0 && !colors))
+    return fail (DRX_ERR_ARG, "drx_framebuffer_composite_background: bad arguments");
+  if (n_pixels == 0) return DRX_OK;
+  hipLaunchKernelGGL (k_background, dim3 (nblk (n_pixels, 256)), dim3 (256), 0, as_stream (stream), (float4 *)colors,
+                      (int)n_pixels, make_float4 (bg[0], bg[1], bg[2], bg[3]));
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_composite_domains (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba,
+                           void *stream)
+{
+  if (n < 0 || !image_ok (width, height) || !out_rgba || (n > 0 && !partials))
+    return fail (DRX_ERR_ARG, "drx_composite_domains: bad arguments");
+  const float zero[4] = { 0.f, 0.f, 0.f, 0.f }, white[4] = { 1.f, 1.f, 1.f, 1.f };
+  const int32_t npix = width * height;
+  int rc = drx_framebuffer_clear (out_rgba, nullptr, npix, zero, stream);
+  if (!rc) rc = drx_partials_to_framebuffer (partials, n, width, height, out_rgba, nullptr, 0, stream);
+  if (!rc) rc = drx_framebuffer_composite_background (out_rgba, npix, white, stream);
+  return rc;
 }
 
 int drx_composite (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba, void *stream)
 {
-  if (n < 0 || width <= 0 || height <= 0 || !out_rgba || (n > 0 && !partials))
+  if (n < 0 || !image_ok (width, height) || !out_rgba || (n > 0 && !partials))
     return fail (DRX_ERR_ARG, "drx_composite: bad arguments");
   hipStream_t st = as_stream (stream);
   const int npix = width * height;
   hipLaunchKernelGGL (k_fill_f32, dim3 (nblk (4 * (int64_t)npix, 256)), dim3 (256), 0, st, out_rgba, 4 * (int64_t)npix, 0.f);
   if (n > 0)
     hipLaunchKernelGGL (k_composite_runs, dim3 (nblk (n, 256)), dim3 (256), 0, st, (const Partial *)partials, n, npix,
-                        (float4 *)out_rgba);
-  hipLaunchKernelGGL (k_background, dim3 (nblk (npix, 256)), dim3 (256), 0, st, (float4 *)out_rgba, npix);
+                        (float4 *)out_rgba, (float *)nullptr, 0);
+  hipLaunchKernelGGL (k_background, dim3 (nblk (npix, 256)), dim3 (256), 0, st, (float4 *)out_rgba, npix,
+                      make_float4 (1.f, 1.f, 1.f, 1.f));
   DRX_LAUNCH_CHECK ();
   return DRX_OK;
 }

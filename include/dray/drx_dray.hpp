@@ -6,12 +6,15 @@
 //   UnstructuredMesh         data_model/unstructured_mesh.hpp:117-135 (locate :84, get_bvh, bounds)
 //   UnstructuredField        data_model/unstructured_field.hpp (eval, range)
 //   ColorTable               color_table.hpp (add_point, add_alpha, set_smooth, correct_opacity, sample)
-//   Camera                   rendering/camera.hpp (set_width/height/fov/pos/look_at/up, create_rays)
+//   Camera                   rendering/camera.hpp (set_width/height/fov/pos/look_at/up, create_rays,
+//                            reset_to_bounds, elevate, azimuth)
 //   DataSet, Collection      data_model/data_set.hpp:20-59, data_model/collection.hpp:16-55
 //   ColorMap                 color_map.hpp (color_table, scalar_range)
 //   Volume                   rendering/volume.hpp:25-60 (Volume(Collection&), field, samples, use_lighting,
 //                            color_map, active_domain, integrate)
 //   PointLocation            queries/point_location.hpp:14-33 (add_var, empty_val, execute)
+//   Framebuffer              rendering/framebuffer.hpp:19-52 (clear, composite_background, save, colors)
+//   Renderer                 rendering/renderer.hpp:22-53 (volume, add_light, clear_lights, render)
 // Errors throw dray::DRayError (error.hpp) carrying drx_last_error().  All compute runs in libdrx's
 // gfx950 kernels; there is no host fallback.
 #pragma once
@@ -105,6 +108,7 @@ struct Range
   float32 max () const { return m_max; }
   bool is_empty () const { return m_min > m_max; }
   float32 length () const { return m_max - m_min; }
+  float32 center () const { return 0.5f * (m_min + m_max); }
   void include (float32 v)
   {
     m_min = m_min < v ? m_min : v;
@@ -248,6 +252,15 @@ public:
   void set_look_at (const Vec3f &p) { m_look_at = p; }
   void set_up (const Vec3f &u) { m_up = u; }
   void set_zoom (float32 z) { m_zoom = z; }
+  float32 get_fov () const { return m_fov_y; }
+  // Camera::reset_to_bounds (camera.cpp:485-514): look at the padded bounds' centre from one diagonal
+  // length away along the current viewing direction; field of view 60 degrees.
+  void reset_to_bounds (const AABB<3> bounds, const float64 xpad = 0., const float64 ypad = 0.,
+                        const float64 zpad = 0.);
+  // Camera::elevate / azimuth (camera.cpp:516-552): orbit the look-at point about the camera's right /
+  // up axis by the given degrees (the up vector is kept).
+  void elevate (const float32 degrees);
+  void azimuth (const float32 degrees);
   int32 get_width () const { return m_width; }
   int32 get_height () const { return m_height; }
   Vec3f get_pos () const { return m_pos; }
@@ -381,6 +394,54 @@ private:
 
 // detail::default_light (rendering/renderer.cpp:141-159): the light the Renderer adds when none is given.
 PointLight default_light (Camera &camera);
+
+// ---- framebuffer (rendering/framebuffer.hpp) ------------------------------------------------------------
+// RGBA colours and depths per pixel (pixel id = x + y * width, row 0 at the bottom of the picture).
+class Framebuffer
+{
+public:
+  Framebuffer ();                                     // 1024 x 1024
+  Framebuffer (const int32 width, const int32 height);
+  int32 width () const { return m_width; }
+  int32 height () const { return m_height; }
+  void clear ();                                      // transparent black, depths +infinity
+  void clear (const Vec<float32, 4> &color);
+  void composite_background ();                       // pixels with alpha < 1 over the background colour
+  void save (const std::string name);                 // name + ".png", 8-bit RGBA
+  void background_color (const Vec<float32, 4> &color) { m_bg_color = color; }
+  void foreground_color (const Vec<float32, 4> &color) { m_fg_color = color; }
+  Vec<float32, 4> background_color () const { return m_bg_color; }
+  Vec<float32, 4> foreground_color () const { return m_fg_color; }
+  Array<Vec<float32, 4>> colors () const { return m_colors; }   // shallow, like dray's
+  Array<float32> depths () const { return m_depths; }
+
+private:
+  int32 m_width, m_height;
+  Vec<float32, 4> m_bg_color = { { 1.f, 1.f, 1.f, 1.f } };
+  Vec<float32, 4> m_fg_color = { { 0.f, 0.f, 0.f, 1.f } };
+  Array<Vec<float32, 4>> m_colors;
+  Array<float32> m_depths;
+};
+
+// ---- renderer (rendering/renderer.hpp), volume branch ------------------------------------------------------
+// Renderer::render (renderer.cpp:229-337) with a volume and no surface traceables: camera rays, a cleared
+// framebuffer, the added lights (or default_light(camera)), Volume::integrate on every domain, then the
+// partials of all domains composited by (pixel, depth) into the framebuffer.  Annotations (colour bars,
+// triad, world annotations) are not drawn.
+class Renderer
+{
+public:
+  void volume (std::shared_ptr<Volume> volume) { m_volume = std::move (volume); }
+  void add_light (const PointLight &light) { m_lights.push_back (light); }
+  void clear_lights () { m_lights.clear (); }
+  void use_lighting (bool use_it) { m_use_lighting = use_it; }   // surfaces only in dray; kept for the API
+  Framebuffer render (Camera &camera);
+
+private:
+  std::shared_ptr<Volume> m_volume;
+  std::vector<PointLight> m_lights;
+  bool m_use_lighting = true;
+};
 
 // Renderer::render's cross-domain volume compositing (rendering/renderer.cpp:307-330): the partials of
 // every domain, sorted by pixel and depth, blended front to back over a white background -> RGBA.

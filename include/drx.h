@@ -26,6 +26,12 @@
  *   drx_composite_domains   Renderer::render volume partial compositing  rendering/renderer.cpp:307-330
  *   drx_composite           Volume::save compositing + Framebuffer::  rendering/volume.cpp:369-416,
  *                           composite_background                      rendering/framebuffer.cpp:146-162
+ *   drx_partials_to_framebuffer  apcomp PartialCompositor::composite  rendering/renderer.cpp:307-330,
+ *                           + detail::partials_to_framebuffer         rendering/renderer.cpp:97-138
+ *   drx_framebuffer_clear   Framebuffer::clear                        rendering/framebuffer.cpp:126-144
+ *   drx_framebuffer_composite_background  Framebuffer::composite_background  rendering/framebuffer.cpp:146-162
+ *   drx_save_png            Framebuffer::save -> PNGEncoder           rendering/framebuffer.cpp:44-51,
+ *                                                                     utils/png_encoder.cpp:56-116
  *   drx_color_table_sample  ColorTable::correct_opacity + sample      color_table.cpp:152-324
  *   drx_mesh_pack           (no counterpart: GridFunction -> element-local SoA conversion done once at
  *                           upload; replaces DeviceMesh::get_elem's indirect reads, device_mesh.hpp:79-96)
@@ -248,6 +254,24 @@ int drx_default_light (const drx_camera *cam, drx_point_light *out_light);
 /* Partials (grouped per pixel, as integrate emits them) -> RGBA image over a white background. */
 int drx_composite (const void *partials, int32_t n, int32_t width, int32_t height, float *out_rgba,
                    void *stream);
+
+/* ---- framebuffer (rendering/framebuffer.hpp) ------------------------------------------------ */
+/* A framebuffer is `colors` (RGBA, 4 floats per pixel) and `depths` (1 float per pixel) in device
+ * memory, pixel id = x + y * width, row 0 at the bottom of the image. */
+/* Framebuffer::clear: colors = color, depths = +infinity (depths nullable). */
+int drx_framebuffer_clear (float *colors, float *depths, int32_t n_pixels, const float color[4], void *stream);
+/* Renderer::render's volume compositing: partials of any number of domains, in any order, are sorted
+ * by (pixel id, depth) (ties keep input order), blended front to back per pixel and written as
+ * detail::partials_to_framebuffer does: blend = 0 replaces the pixel's colour, blend = 1 puts the
+ * volume over the colour the framebuffer holds (colour = volume + (1 - volume alpha) * framebuffer);
+ * depths (nullable) take each pixel's nearest partial depth.  Pixels without partials are untouched. */
+int drx_partials_to_framebuffer (const void *partials, int32_t n, int32_t width, int32_t height, float *colors,
+                                 float *depths, int32_t blend, void *stream);
+/* Framebuffer::composite_background: pixels with alpha < 1 are put over bg (blend_pre_alpha). */
+int drx_framebuffer_composite_background (float *colors, int32_t n_pixels, const float bg[4], void *stream);
+/* Framebuffer::save: host RGBA floats -> 8-bit RGBA PNG file at `path` (rows flipped so row 0 is the
+ * bottom of the picture; each channel (uint8)(v * 255), clamped to [0, 255]). */
+int drx_save_png (const float *rgba, int32_t width, int32_t height, const char *path);
 
 /* ---- learned Newton initializer ----------------------------------------------------------- */
 /* MLP 27 -> 64 -> 64 -> 64 -> 64 -> 3, GELU x*Phi(x) with Phi a cubic smoothstep over [-2, 2], BF16

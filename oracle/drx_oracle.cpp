@@ -1387,6 +1387,137 @@ void orc_composite (const void *parts_v, int n, int w, int h, float *out_rgba)
   }
 }
 
+// Renderer::render's volume compositing (rendering/renderer.cpp:307-330; apcomp's PartialCompositor is not
+// part of the reference sources: partials sorted by (pixel id, depth), ties in input order, each pixel's
+// run blended front to back with pre_mult_alpha_blend_host) followed by detail::partials_to_framebuffer
+// (renderer.cpp:97-138): blend = 0 replaces the pixel; blend = 1 puts the volume over the framebuffer
+// colour (colour = volume + (1 - volume alpha) * framebuffer).  depths (nullable) take the run's first
+// partial's depth.  Pixels without partials are left as they are.
+void orc_partials_to_framebuffer (const void *parts_v, int n, int w, int h, float *colors, float *depths, int blend)
+{
+  const Partial *p = (const Partial *)parts_v;
+  std::vector<Partial> v (p, p + n);
+  std::stable_sort (v.begin (), v.end (), [] (const Partial &a, const Partial &b) {
+    return a.pixel_id != b.pixel_id ? a.pixel_id < b.pixel_id : a.depth < b.depth;
+  });
+  int i = 0;
+  while (i < n)
+  {
+    const int pid = v[i].pixel_id;
+    float c[4];
+    std::memcpy (c, v[i].rgba, sizeof c);
+    const float d = v[i].depth;
+    int k = i + 1;
+    for (; k < n && v[k].pixel_id == pid; ++k)
+    {
+      const float a = 1.f - c[3];
+      for (int q = 0; q < 4; ++q) c[q] = c[q] + v[k].rgba[q] * a;
+    }
+    if (pid >= 0 && pid < w * h)
+    {
+      float *fb = colors + 4 * (i64)pid;
+      if (blend)
+      {
+        const float o = 1.f - c[3];
+        for (int q = 0; q < 4; ++q) c[q] = c[q] + o * fb[q];
+      }
+      std::memcpy (fb, c, sizeof c);
+      if (depths) depths[pid] = d;
+    }
+    i = k;
+  }
+}
+
+// Framebuffer::composite_background (framebuffer.cpp:146-162) with blend_pre_alpha (colors.hpp:25-33).
+void orc_framebuffer_composite_background (float *colors, int npix, const float *bg)
+{
+  for (int i = 0; i < npix; ++i)
+  {
+    float *c = colors + 4 * (i64)i;
+    if (c[3] < 1.f)
+    {
+      const float a = 1.f - c[3];
+      for (int q = 0; q < 4; ++q) c[q] = c[q] + bg[q] * a;
+    }
+  }
+}
+
+// Camera::reset_to_bounds / elevate / azimuth (rendering/camera.cpp:485-552) with rotate()
+// (transform_3d.hpp:126-157) and transform_vector (:53-60), float operation order as in dray.
+// cam = 7 floats in/out: pos[3], look_at[3], fov_y; up[3] read only.  mode 0 reset_to_bounds(bounds6,
+// pads[3]), 1 elevate(arg), 2 azimuth(arg).
+static void orbit_rotate (float deg, float axis[3], float v[3])
+{
+  const float rad = 0.01745329251994329547437168059786927f * deg;
+  vnormalize (axis);
+  const float sn = sinf (rad), cs = cosf (rad);
+  const float m[3][3] = {
+    { axis[0] * axis[0] * (1 - cs) + cs, axis[0] * axis[1] * (1 - cs) - axis[2] * sn, axis[0] * axis[2] * (1 - cs) + axis[1] * sn },
+    { axis[1] * axis[0] * (1 - cs) + axis[2] * sn, axis[1] * axis[1] * (1 - cs) + cs, axis[1] * axis[2] * (1 - cs) - axis[0] * sn },
+    { axis[2] * axis[0] * (1 - cs) - axis[1] * sn, axis[2] * axis[1] * (1 - cs) + axis[0] * sn, axis[2] * axis[2] * (1 - cs) + cs } };
+  float r[3];
+  for (int i = 0; i < 3; ++i)
+  {
+    float d = 0.f;
+    for (int j = 0; j < 3; ++j) d += m[i][j] * v[j];
+    d += 0.f * 0.f;
+    r[i] = d;
+  }
+  std::memcpy (v, r, sizeof r);
+}
+
+void orc_camera_orbit (float *cam, const float *up_in, int mode, float arg, const float *bounds6, const double *pads)
+{
+  float *pos = cam, *look_at = cam + 3;
+  if (mode == 0)
+  {
+    float db[6] = { 3.402823466e+38f, -3.402823466e+38f, 3.402823466e+38f, -3.402823466e+38f, 3.402823466e+38f,
+                    -3.402823466e+38f };
+    for (int d = 0; d < 3; ++d)
+    {
+      const double pad = pads[d] * ((double)bounds6[2 * d + 1] - (double)bounds6[2 * d]);
+      const float hi = (float)((double)bounds6[2 * d + 1] + pad), lo = (float)((double)bounds6[2 * d] - pad);
+      for (float x : { hi, lo })
+      {
+        db[2 * d] = db[2 * d] < x ? db[2 * d] : x;
+        db[2 * d + 1] = db[2 * d + 1] > x ? db[2 * d + 1] : x;
+      }
+    }
+    float pd[3] = { pos[0] - look_at[0], pos[1] - look_at[1], pos[2] - look_at[2] };
+    vnormalize (pd);
+    float c[3], e[3];
+    for (int d = 0; d < 3; ++d)
+    {
+      c[d] = 0.5f * (db[2 * d] + db[2 * d + 1]);
+      e[d] = db[2 * d + 1] - db[2 * d];
+    }
+    float m2 = 0.f;
+    for (int d = 0; d < 3; ++d) m2 += e[d] * e[d];
+    const float diag = sqrtf (m2);
+    for (int d = 0; d < 3; ++d)
+    {
+      look_at[d] = c[d];
+      pos[d] = c[d] + pd[d] * diag * 1.0f;
+    }
+    cam[6] = 60.f;
+    return;
+  }
+  float look[3] = { look_at[0] - pos[0], look_at[1] - pos[1], look_at[2] - pos[2] };
+  float m2 = 0.f;
+  for (int d = 0; d < 3; ++d) m2 += look[d] * look[d];
+  const float dist = sqrtf (m2);
+  float right[3] = { look[1] * up_in[2] - look[2] * up_in[1], look[2] * up_in[0] - look[0] * up_in[2],
+                     look[0] * up_in[1] - look[1] * up_in[0] };
+  float up[3] = { right[1] * look[2] - right[2] * look[1], right[2] * look[0] - right[0] * look[2],
+                  right[0] * look[1] - right[1] * look[0] };
+  vnormalize (look);
+  vnormalize (right);
+  vnormalize (up);
+  if (mode == 1) orbit_rotate (-arg, right, look);
+  else orbit_rotate (arg, up, look);
+  for (int d = 0; d < 3; ++d) pos[d] = look_at[d] - look[d] * dist;
+}
+
 // ColorTable::sample (color_table.cpp:152-166) of a table given as control points, after
 // correct_opacity(factor) (:304-324; factor <= 0 skips the correction).
 // rgb_pts: n_rgb x (pos, r, g, b, a); alpha_pts: n_alpha x (pos, alpha).

@@ -52,6 +52,9 @@ _SIGS = {
     "orc_composite": (None, [P, I32, I32, I32, P]),
     "orc_composite_domains": (None, [P, I32, I32, I32, P]),
     "orc_color_table_sample": (None, [P, I32, P, I32, I32, F32, I32, F32, P]),
+    "orc_partials_to_framebuffer": (None, [P, I32, I32, I32, P, P, I32]),
+    "orc_framebuffer_composite_background": (None, [P, I32, P]),
+    "orc_camera_orbit": (None, [P, P, I32, F32, P, P]),
 }
 
 
@@ -292,6 +295,34 @@ def composite_domains(parts, w, h):
     p = np.ascontiguousarray(parts)
     lib().orc_composite_domains(ptr(p), int(p.shape[0]), w, h, ptr(out))
     return out
+
+
+def partials_to_framebuffer(parts, w, h, colors, depths, blend=False):
+    """Renderer's volume compositing into a framebuffer (colors float32 [w*h, 4], depths float32 [w*h],
+    both updated in place)."""
+    p = np.ascontiguousarray(parts)
+    assert colors.dtype == np.float32 and colors.flags.c_contiguous and colors.size == 4 * w * h
+    assert depths.dtype == np.float32 and depths.flags.c_contiguous and depths.size == w * h
+    lib().orc_partials_to_framebuffer(ptr(p), int(p.shape[0]), w, h, ptr(colors), ptr(depths), 1 if blend else 0)
+
+
+def framebuffer_composite_background(colors, bg=(1.0, 1.0, 1.0, 1.0)):
+    """Framebuffer::composite_background in place on colors float32 [n, 4]."""
+    assert colors.dtype == np.float32 and colors.flags.c_contiguous
+    b = np.ascontiguousarray(np.asarray(bg, np.float32))
+    lib().orc_framebuffer_composite_background(ptr(colors), int(colors.size // 4), ptr(b))
+
+
+def camera_orbit(pos, look_at, up, fov, mode, arg=0.0, bounds6=None, pads=(0.0, 0.0, 0.0)):
+    """Camera::reset_to_bounds (mode "reset"), elevate ("elevate", arg degrees) or azimuth ("azimuth")
+    -> (pos, look_at, fov) as float32."""
+    cam = np.ascontiguousarray(np.array(list(pos) + list(look_at) + [fov], np.float32))
+    u = np.ascontiguousarray(np.asarray(up, np.float32))
+    b = np.ascontiguousarray(np.zeros(6, np.float32) if bounds6 is None else np.asarray(bounds6, np.float32))
+    pd = np.ascontiguousarray(np.asarray(pads, np.float64))
+    m = {"reset": 0, "elevate": 1, "azimuth": 2}[mode]
+    lib().orc_camera_orbit(ptr(cam), ptr(u), m, float(arg), ptr(b), ptr(pd))
+    return cam[0:3].copy(), cam[3:6].copy(), float(cam[6])
 
 
 def composite(parts, w, h):

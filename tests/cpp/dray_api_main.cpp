@@ -4,12 +4,15 @@
 // the Python test to compare.
 //
 //   dray_api_test <in_dir> <out_dir>
+//   dray_api_test --camera <out_dir>      (host only, no GPU: Camera::reset_to_bounds / azimuth / elevate)
 // in_dir:  params.txt  "order n_el n_pts width height samples alpha0 fov px py pz lx ly lz"
 //          ctrl.bin (int32), coords.bin (float32 xyz), field.bin (float32), points.bin (float32 xyz),
 //          pl_points.bin (float32 xyz, PointLocation queries)
 // out_dir: inner.bin leaf.bin ids.bin bounds.bin loc.bin vals.bin rays.bin partials.bin image.bin
-//          lit_image.bin light.bin bounds2.bin range2.bin pl_values.bin two_image.bin stats.txt, and "errors ok" on
-//          stdout once the error paths have thrown as expected.
+//          lit_image.bin light.bin bounds2.bin range2.bin pl_values.bin two_image.bin two_parts0.bin
+//          two_parts1.bin fb_colors.bin fb_depths.bin fb_final.bin render.png stats.txt, and "errors ok" on
+//          stdout once the error paths have thrown as expected.  --camera writes camera.bin: pos, look_at,
+//          fov after reset_to_bounds, then pos after azimuth(-60), then pos after elevate(25).
 #include "dray/drx_dray.hpp"
 
 #include <cstdio>
@@ -65,8 +68,36 @@ template <typename F> static bool throws (F f)
   return false;
 }
 
+// Camera orbits as dray's triple-point test drives them (t_dray_volume_render.cpp:82-83), host only.
+static int camera_only (const std::string &out)
+{
+  Camera c;
+  c.set_pos ({ { 0.3f, 0.7f, 4.f } });
+  c.set_look_at ({ { 0.5f, 0.5f, 0.5f } });
+  c.set_up ({ { 0.f, 1.f, 0.f } });
+  AABB<> b;
+  const float lo[3] = { 0.f, 0.f, -1.f }, hi[3] = { 1.f, 2.f, 1.f };
+  for (int d = 0; d < 3; ++d)
+  {
+    b.m_ranges[d].include (lo[d]);
+    b.m_ranges[d].include (hi[d]);
+  }
+  std::vector<float> v;
+  c.reset_to_bounds (b, 0.1, 0.2, 0.3);
+  for (int d = 0; d < 3; ++d) v.push_back (c.get_pos ()[d]);
+  for (int d = 0; d < 3; ++d) v.push_back (c.get_look_at ()[d]);
+  v.push_back (c.get_fov ());
+  c.azimuth (-60.f);
+  for (int d = 0; d < 3; ++d) v.push_back (c.get_pos ()[d]);
+  c.elevate (25.f);
+  for (int d = 0; d < 3; ++d) v.push_back (c.get_pos ()[d]);
+  write_raw (out + "/camera.bin", v.data (), v.size ());
+  return 0;
+}
+
 int main (int argc, char **argv)
 {
+  if (argc == 3 && std::string (argv[1]) == "--camera") return camera_only (argv[2]);
   if (argc != 3)
   {
     std::fprintf (stderr, "usage: %s <in_dir> <out_dir>\n", argv[0]);
@@ -208,6 +239,23 @@ int main (int argc, char **argv)
       }
       Array<Vec<float32, 4>> timg = composite_domains (dp, width, height);
       write_array (out + "/two_image.bin", timg);
+      write_array (out + "/two_parts0.bin", dp[0]);
+      write_array (out + "/two_parts1.bin", dp[1]);
+
+      // the same through Renderer::render -> Framebuffer::composite_background -> Framebuffer::save
+      auto vr = std::make_shared<Volume> (two);
+      vr->field ("f");
+      vr->use_lighting (false);
+      vr->samples (samples);
+      vr->color_map ().color_table (tab);
+      Renderer renderer;
+      renderer.volume (vr);
+      Framebuffer fb = renderer.render (cam);
+      write_array (out + "/fb_colors.bin", fb.colors ());
+      write_array (out + "/fb_depths.bin", fb.depths ());
+      fb.composite_background ();
+      write_array (out + "/fb_final.bin", fb.colors ());
+      fb.save (out + "/render");
     }
     {
       std::ofstream sf (out + "/stats.txt");
@@ -218,6 +266,7 @@ int main (int argc, char **argv)
     bool ok = true;
     ok &= throws ([] { Camera c; c.set_width (0); });
     ok &= throws ([] { Camera c; c.set_fov (181.f); });
+    ok &= throws ([] { Framebuffer f (0, 4); });
     ok &= throws ([&] { UnstructuredMesh m (gf, 5); });
     ok &= throws ([&] {   // more lights than the library takes
       Volume v (col);

@@ -9,6 +9,7 @@ import pytest
 
 from devil_ray_amd import _lib as L
 from devil_ray_amd import synthetic as syn
+from pngread import read_png
 
 HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "drx.h")
 
@@ -91,3 +92,22 @@ def test_api_fails_loudly_without_gpu():
     from devil_ray_amd import api
     with pytest.raises(L.DrxError):
         api.UnstructuredMesh(syn.hex_mesh(1, 1, 0.0, 0.0, 0))
+
+
+def test_save_png_pixels(tmp_path):
+    """Framebuffer::save: row 0 of the framebuffer is the last PNG row; each channel is
+    (uint8)(v * 255) (truncation), clamped to [0, 255]; 300 x 250 spans several 64 KiB deflate blocks."""
+    lib = L.lib()
+    w, h = 300, 250
+    rng = np.random.default_rng(3)
+    rgba = rng.uniform(-0.1, 1.1, (h, w, 4)).astype(np.float32)
+    rgba[0, 0] = (0.0, 1.0, 0.5, 254.9 / 255)
+    path = str(tmp_path / "img.png")
+    assert lib.drx_save_png(rgba.ctypes.data, w, h, path.encode()) == 0
+    got = read_png(path)
+    exp = np.clip(rgba * np.float32(255), 0, 255).astype(np.uint8)[::-1]
+    assert np.array_equal(got, exp)
+    assert tuple(got[h - 1, 0]) == (0, 255, 127, 254)
+    assert lib.drx_save_png(rgba.ctypes.data, 0, h, path.encode()) == L.DRX_ERR_ARG
+    assert lib.drx_save_png(rgba.ctypes.data, w, h, str(tmp_path / "no" / "dir.png").encode()) == L.DRX_ERR_ARG
+    assert b"cannot open" in lib.drx_last_error()

@@ -14,6 +14,7 @@ import pytest
 from devil_ray_amd import build as B
 from devil_ray_amd import synthetic as syn
 from devil_ray_amd.synthetic import LOCATION_DTYPE, PARTIAL_DTYPE, RAY_DTYPE
+from pngread import read_png
 
 BIN = B.CPP_TEST_BIN
 
@@ -35,6 +36,22 @@ def test_library_exports_dray_classes():
                 "dray::Volume::composite(", "dray::LinearBVHBuilder::construct(", "dray::Camera::create_rays(",
                 "dray::UnstructuredField::eval(", "dray::ColorTable::sample("):
         assert sym in out, f"libdrx.so does not export {sym}"
+
+
+def test_camera_orbits_match_oracle(orc, tmp_path):
+    """Camera::reset_to_bounds(pads), azimuth(-60), elevate(25) in the C++ layer (host code, no GPU)
+    give the oracle's positions bit for bit (its closed forms are checked in test_oracle.py)."""
+    _need_bin()
+    r = subprocess.run([BIN, "--camera", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(tmp_path / "camera.bin", np.float32)
+    up = (0.0, 1.0, 0.0)
+    p, look, fov = orc.camera_orbit((0.3, 0.7, 4.0), (0.5, 0.5, 0.5), up, 30.0, "reset",
+                                    bounds6=(0, 1, 0, 2, -1, 1), pads=(0.1, 0.2, 0.3))
+    p2, _, _ = orc.camera_orbit(p, look, up, fov, "azimuth", -60.0)
+    p3, _, _ = orc.camera_orbit(p2, look, up, fov, "elevate", 25.0)
+    exp = np.array([*p, *look, fov, *p2, *p3], np.float32)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
 
 
 @pytest.mark.gpu
@@ -142,6 +159,23 @@ def test_cpp_api_matches_c_abi(gpu, orc, tmp_path):
     torch.cuda.synchronize()
     assert np.array_equal(np.fromfile(outd / "two_image.bin", np.float32).view(np.uint32),
                           timg.cpu().numpy().view(np.uint32))
+    # Renderer::render over the same collection -> Framebuffer colours and depths (the oracle's
+    # partials_to_framebuffer of the two domains' partials), composite_background -> the image above,
+    # Framebuffer::save -> the PNG of that image
+    tp = [np.fromfile(outd / f"two_parts{d}.bin", PARTIAL_DTYPE) for d in (0, 1)]
+    for d in (0, 1):
+        assert np.array_equal(tp[d].view(np.uint8),
+                              dparts[d][0].cpu().numpy()[: dparts[d][1] * PARTIAL_DTYPE.itemsize])
+    col = np.zeros((w * h, 4), np.float32)
+    dep = np.full(w * h, np.inf, np.float32)
+    orc.partials_to_framebuffer(np.concatenate(tp), w, h, col, dep)
+    assert np.array_equal(np.fromfile(outd / "fb_colors.bin", np.float32).view(np.uint32), col.view(np.uint32).ravel())
+    assert np.array_equal(np.fromfile(outd / "fb_depths.bin", np.float32).view(np.uint32), dep.view(np.uint32))
+    assert np.isfinite(dep).sum() > 100
+    fin = np.fromfile(outd / "fb_final.bin", np.float32)
+    assert np.array_equal(fin.view(np.uint32), timg.cpu().numpy().view(np.uint32))
+    png = read_png(outd / "render.png")
+    assert np.array_equal(png, np.clip(fin.reshape(h, w, 4) * np.float32(255), 0, 255).astype(np.uint8)[::-1])
     st = (outd / "stats.txt").read_text().split()
     assert int(st[0]) == vol.last_stats.located_samples and int(st[3]) == n
     assert LOCATION_DTYPE.itemsize == 16 and RAY_DTYPE.itemsize == 36

@@ -250,6 +250,92 @@ def test_multi_domain_render(gpu, orc):
     assert np.mean(d.max(axis=1) > RGBA_TOL) < 0.01 and d.mean() < 1e-4
 
 
+def test_framebuffer_entry_points(gpu, orc):
+    """drx_partials_to_framebuffer (replace and blend-over-surface), drx_framebuffer_clear and
+    drx_framebuffer_composite_background against the oracle, bit for bit: 30,000 partials from three
+    'domains' in shuffled order, many pixels with several partials, equal depths inside a pixel."""
+    rng = np.random.default_rng(11)
+    w, h, n = 97, 61, 30000
+    p = np.zeros(n, PARTIAL_DTYPE)
+    p["pixel_id"] = rng.integers(0, w * h, n)
+    p["depth"] = rng.integers(0, 40, n).astype(np.float32) * np.float32(0.25)
+    p["rgba"] = rng.uniform(0, 0.3, (n, 4)).astype(np.float32)
+    surf = rng.uniform(0, 1, (w * h, 4)).astype(np.float32)
+    for blend in (False, True):
+        fb = gpu.Framebuffer(w, h)
+        fb.colors.copy_(torch.from_numpy(surf.reshape(-1)).to(fb.colors.device))
+        gpu.partials_to_framebuffer([(gpu.as_bytes(p[: n // 3]), n // 3), (gpu.as_bytes(p[n // 3:]), n - n // 3)], fb,
+                                    blend=blend)
+        col, dep = surf.copy(), np.full(w * h, np.inf, np.float32)
+        orc.partials_to_framebuffer(p, w, h, col, dep, blend=blend)
+        torch.cuda.synchronize()
+        assert np.array_equal(_np(fb.colors).view(np.uint32), col.view(np.uint32).ravel())
+        assert np.array_equal(_np(fb.depths).view(np.uint32), dep.view(np.uint32))
+    fb.background_color((0.2, 0.4, 0.6, 1.0))
+    fb.composite_background()
+    orc.framebuffer_composite_background(col, (0.2, 0.4, 0.6, 1.0))
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(fb.colors).view(np.uint32), col.view(np.uint32).ravel())
+    fb.clear((0.5, 0.0, 0.25, 1.0))
+    torch.cuda.synchronize()
+    assert np.all(_np(fb.colors).reshape(-1, 4) == np.float32([0.5, 0.0, 0.25, 1.0])) and np.all(_np(fb.depths) == np.inf)
+
+
+def test_renderer_two_domains(gpu, orc, tmp_path):
+    """Renderer::render over two domains -> Framebuffer (colours and nearest depths) ->
+    composite_background -> save: the framebuffer equals the oracle's compositing of the GPU partials
+    bit for bit and the oracle's own render within the RGBA tolerance; the PNG holds the quantised image."""
+    from pngread import read_png
+    a = syn.hex_mesh(4, 2, 0.05, 0.01, 7)
+    b = syn.HexMesh(n=a.n, order=a.order, ctrl_idx=a.ctrl_idx,
+                    values=(a.values + np.float32([0.6, 0.0, -0.7])).astype(np.float32), field=(a.field * 0.5).astype(np.float32))
+    w, h = 48, 40
+    cam = gpu.Camera(w, h, 50.0, pos=(0.8, 0.5, 2.8), look_at=(0.8, 0.5, 0.1))
+    vols, obs = [], []
+    for mesh in (a, b):
+        v = gpu.Volume(gpu.UnstructuredMesh(mesh), gpu.UnstructuredField(mesh))
+        v.use_lighting = False
+        v.samples(96)
+        v.table.add_point(0.0, (0.1, 0.3, 0.9))
+        v.table.add_point(1.0, (0.9, 0.2, 0.1))
+        v.table.add_alpha(0.0, 0.3)
+        v.table.set_smooth(True)
+        vols.append(v)
+        obs.append(orc.build_mesh_bvh(mesh))
+    r = gpu.Renderer()
+    r.volume(vols)
+    fb = r.render(cam)
+    colors, depths = _np(fb.colors).reshape(-1, 4).copy(), _np(fb.depths).copy()
+    # the partials the renderer composited, recomputed through the same volumes
+    rays = cam.create_rays()
+    gparts = [v.integrate(rays, tile_width=w, tile_height=h) for v in vols]
+    torch.cuda.synchronize()
+    allg = np.concatenate([_np(p, np.uint8)[: n * 24].view(PARTIAL_DTYPE) for p, n in gparts])
+    col, dep = np.zeros((w * h, 4), np.float32), np.full(w * h, np.inf, np.float32)
+    orc.partials_to_framebuffer(allg, w, h, col, dep)
+    assert np.array_equal(colors.view(np.uint32), col.view(np.uint32)) and np.array_equal(depths, dep)
+    assert np.isfinite(depths).sum() > 300
+    # the oracle's own render of both domains (collection bounds and range, as the renderer set them)
+    cb, (lo, hi) = vols[0].sample_bounds, vols[0].range
+    oparts = []
+    orays = orc.camera_perspective(w, h, 50.0, (0.8, 0.5, 2.8), (0.8, 0.5, 0.1))
+    for mesh, ob, v in zip((a, b), obs, vols):
+        lut = v.table.sample(1024, opacity_factor=10.0 / 96)
+        op, _, _ = orc.integrate(mesh, ob, orc.remove_missed_rays(orays, ob.bounds), orc.sample_dist(cb, 96), lut, lo, hi)
+        oparts.append(orc.compact_partials(op))
+    ocol, odep = np.zeros((w * h, 4), np.float32), np.full(w * h, np.inf, np.float32)
+    orc.partials_to_framebuffer(np.concatenate(oparts), w, h, ocol, odep)
+    d = np.abs(colors - ocol)
+    assert np.mean(d.max(axis=1) > RGBA_TOL) < 0.01 and d.mean() < 1e-4
+    assert np.array_equal(np.isfinite(depths), np.isfinite(odep))
+    fb.composite_background()
+    fb.save(str(tmp_path / "two"))
+    fin = _np(fb.colors).reshape(h, w, 4)
+    assert np.array_equal(fin.reshape(-1, 4), orc.composite_domains(allg, w, h))
+    png = read_png(tmp_path / "two.png")
+    assert np.array_equal(png, np.clip(fin * np.float32(255), 0, 255).astype(np.uint8)[::-1])
+
+
 def test_volume_orthographic_config0_small(gpu, orc):
     mesh = syn.hex_mesh(8, 2, 0.05, 0.005, 1, jitter_scaled=False)
     w = h = 32

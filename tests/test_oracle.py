@@ -526,3 +526,69 @@ def test_composite_domains_matches_single_domain_and_is_order_free(orc):
     assert np.array_equal(orc.composite_domains(parts, w, h), ref)            # already in (pixel, depth) order
     shuffled = parts[np.random.default_rng(0).permutation(len(parts))]
     assert np.array_equal(orc.composite_domains(shuffled, w, h), ref)         # any input order
+
+
+# ---- framebuffer (Renderer volume compositing, composite_background) and camera orbits -------------------
+def test_partials_to_framebuffer_replace_and_blend(orc):
+    """Hand-computed: pixel 1 gets two partials (back first in the input), pixel 0 one, pixel 2 none.
+    blend=False replaces pixels 0/1 and keeps pixel 2; blend=True puts the volume over the framebuffer
+    colour; depths take the nearest partial."""
+    front, back, solo = (0.2, 0.1, 0.0, 0.4), (0.0, 0.3, 0.5, 0.5), (0.1, 0.1, 0.1, 0.2)
+    p = _partials([(1, 2.0, back), (0, 1.5, solo), (1, 1.0, front)])
+    surf = np.array([[0.3, 0.3, 0.3, 1.0]] * 3, np.float32)
+    a = 1 - front[3]
+    vol1 = np.array([front[c] + back[c] * a for c in range(4)], np.float32)
+    for blend in (False, True):
+        col = surf.copy()
+        dep = np.full(3, np.inf, np.float32)
+        orc.partials_to_framebuffer(p, 3, 1, col, dep, blend=blend)
+        if blend:
+            exp1 = vol1 + (1 - vol1[3]) * surf[1]
+            exp0 = np.array(solo, np.float32) + (1 - solo[3]) * surf[0]
+        else:
+            exp1, exp0 = vol1, np.array(solo, np.float32)
+        assert np.allclose(col[1], exp1, atol=1e-7) and np.allclose(col[0], exp0, atol=1e-7)
+        assert np.array_equal(col[2], surf[2])                               # no partials: untouched
+        assert dep[0] == np.float32(1.5) and dep[1] == np.float32(1.0) and dep[2] == np.inf
+
+
+def test_framebuffer_background_and_volume_save_equivalence(orc):
+    """composite_background puts alpha < 1 pixels over a coloured background (closed form) and leaves
+    opaque ones; clear + partials_to_framebuffer + white background equals composite_domains."""
+    col = np.array([[0.2, 0.0, 0.1, 0.5], [0.3, 0.3, 0.3, 1.0], [0.0, 0.0, 0.0, 0.0]], np.float32)
+    bg = (0.5, 0.25, 1.0, 1.0)
+    ref = col.copy()
+    orc.framebuffer_composite_background(col, bg)
+    assert np.allclose(col[0], ref[0] + np.float32(0.5) * np.array(bg, np.float32), atol=1e-7)
+    assert np.array_equal(col[1], ref[1]) and np.allclose(col[2], bg)
+    rng = np.random.default_rng(5)
+    n, w, h = 400, 9, 7
+    rows = [(int(rng.integers(0, w * h)), float(rng.integers(0, 4)), tuple(rng.uniform(0, 0.3, 4))) for _ in range(n)]
+    p = _partials(rows)
+    col = np.zeros((w * h, 4), np.float32)
+    dep = np.full(w * h, np.inf, np.float32)
+    orc.partials_to_framebuffer(p, w, h, col, dep)
+    orc.framebuffer_composite_background(col)
+    assert np.array_equal(col, orc.composite_domains(p, w, h))
+
+
+def test_camera_orbits_closed_forms(orc):
+    """azimuth(90) about +y moves a camera at +z to +x; elevate(90) moves it to +y; 360 degrees return to
+    the start; reset_to_bounds looks at the padded box centre from one diagonal away, fov 60.  A flipped
+    rotation sign sends azimuth(90) to -x and elevate(90) to -y."""
+    pos, look, up = (0.0, 0.0, 5.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0)
+    p, l, f = orc.camera_orbit(pos, look, up, 30.0, "azimuth", 90.0)
+    assert np.allclose(p, (5, 0, 0), atol=1e-5) and np.array_equal(l, np.zeros(3, np.float32)) and f == 30.0
+    p, _, _ = orc.camera_orbit(pos, look, up, 30.0, "elevate", 90.0)
+    assert np.allclose(p, (0, 5, 0), atol=1e-5)
+    for mode in ("azimuth", "elevate"):
+        p, _, _ = orc.camera_orbit(pos, look, up, 30.0, mode, 360.0)
+        assert np.allclose(p, pos, atol=1e-5)
+    p, l, f = orc.camera_orbit((3.0, 0.5, 0.5), (0.0, 0.5, 0.5), up, 30.0, "reset",
+                               bounds6=(0, 1, 0, 2, -1, 1), pads=(0.0, 0.0, 0.0))
+    assert np.allclose(l, (0.5, 1.0, 0.0)) and f == 60.0
+    assert np.allclose(p, np.array([0.5, 1.0, 0.0]) + np.array([1.0, 0, 0]) * np.sqrt(1 + 4 + 4), atol=1e-6)
+    # pads grow each axis by pad x length on both sides
+    p, l, _ = orc.camera_orbit((0.5, 0.5, 9.0), (0.5, 0.5, 0.5), up, 30.0, "reset",
+                               bounds6=(0, 1, 0, 1, 0, 1), pads=(0.5, 0.0, 0.0))
+    assert np.allclose(l, (0.5, 0.5, 0.5)) and np.allclose(p, (0.5, 0.5, 0.5 + np.sqrt(4 + 1 + 1)), atol=1e-6)
