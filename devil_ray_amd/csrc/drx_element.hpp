@@ -516,74 +516,91 @@ struct TravStack
 };
 
 // DeviceMesh::locate (device_mesh.hpp:247-354): depth-first, left child first, right child pushed when
-// the point is in both; the first accepted candidate wins.  Returns the element id or -1; sets *overflow
-// if the 64-entry stack would overflow (dray's stack has the same capacity).
+// the point is in both; the first accepted candidate wins.
+//
+// Resumable form, one candidate per call: cur / sp carry this lane's depth-first state between calls
+// (start a search with cur = 0, sp = 0 and kBarrier in stack entry 0).  The lane walks inner nodes until
+// it holds a leaf (lanes that get there early wait, so the Newton solve runs with the wave converged),
+// then tests that one candidate.  Returns 1 if it was accepted (ref, cell set), 0 if it was rejected (the
+// state then holds the next stack entry: call again with the same point), -1 when the search is
+// exhausted or the 64-entry stack (dray's capacity) would overflow (*overflow set).
+template <int P, bool POLISH>
+__device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStack &todo, const float pt[3],
+                                                  float ref[3], LocateStats &st, bool &overflow, int &cur, int &sp,
+                                                  int &cell)
+{
+  while (cur > -1)
+  {
+    const float4 *nd = cfg.nodes + cur;
+    const float4 a = nd[0], b = nd[1], c = nd[2], ch = nd[3];
+    bool in_l = true, in_r = true;
+    if (pt[0] < a.x) in_l = false;
+    if (pt[1] < a.y) in_l = false;
+    if (pt[2] < a.z) in_l = false;
+    if (pt[0] > a.w) in_l = false;
+    if (pt[1] > b.x) in_l = false;
+    if (pt[2] > b.y) in_l = false;
+    if (pt[0] < b.z) in_r = false;
+    if (pt[1] < b.w) in_r = false;
+    if (pt[2] < c.x) in_r = false;
+    if (pt[0] > c.y) in_r = false;
+    if (pt[1] > c.z) in_r = false;
+    if (pt[2] > c.w) in_r = false;
+    if (!in_l && !in_r)
+    {
+      cur = todo.get (sp);
+      --sp;
+    }
+    else
+    {
+      const int lc = __float_as_int (ch.x), rc = __float_as_int (ch.y);
+      cur = in_l ? lc : rc;
+      if (in_l && in_r)
+      {
+        if (sp + 1 >= kStackDepth)
+        {
+          overflow = true;
+          return -1;
+        }
+        todo.set (++sp, rc);
+      }
+    }
+  }
+  if (cur == kBarrier) return -1;
+  const int leaf = -cur - 1;
+  const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
+  const int el = __float_as_int (l0.x);
+  float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
+  bool found;
+  if constexpr (P == 0)
+    found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
+  else
+    found = try_element_at<P, POLISH> (cfg, cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, pt, x, st);
+  if (found)
+  {
+    ref[0] = x[0];
+    ref[1] = x[1];
+    ref[2] = x[2];
+    cell = el;
+    return 1;
+  }
+  cur = todo.get (sp);
+  --sp;
+  return 0;
+}
+
+// The whole search for one point: the element id or -1.
 template <int P, bool POLISH>
 __device__ __forceinline__ int locate_point (const LocateCfg &cfg, TravStack &todo, const float pt[3], float ref[3],
                                              LocateStats &st, bool &overflow)
 {
-  int cur = 0, sp = 0;
+  int cur = 0, sp = 0, cell = -1;
   todo.set (0, kBarrier);
   for (;;)
   {
-    // Walk inner nodes until this lane holds a leaf (or the stack is exhausted).  Lanes that reach a
-    // leaf early wait here, so the Newton solve below runs with the wave converged.
-    while (cur > -1)
-    {
-      const float4 *nd = cfg.nodes + cur;
-      const float4 a = nd[0], b = nd[1], c = nd[2], ch = nd[3];
-      bool in_l = true, in_r = true;
-      if (pt[0] < a.x) in_l = false;
-      if (pt[1] < a.y) in_l = false;
-      if (pt[2] < a.z) in_l = false;
-      if (pt[0] > a.w) in_l = false;
-      if (pt[1] > b.x) in_l = false;
-      if (pt[2] > b.y) in_l = false;
-      if (pt[0] < b.z) in_r = false;
-      if (pt[1] < b.w) in_r = false;
-      if (pt[2] < c.x) in_r = false;
-      if (pt[0] > c.y) in_r = false;
-      if (pt[1] > c.z) in_r = false;
-      if (pt[2] > c.w) in_r = false;
-      if (!in_l && !in_r)
-      {
-        cur = todo.get (sp);
-        --sp;
-      }
-      else
-      {
-        const int lc = __float_as_int (ch.x), rc = __float_as_int (ch.y);
-        cur = in_l ? lc : rc;
-        if (in_l && in_r)
-        {
-          if (sp + 1 >= kStackDepth)
-          {
-            overflow = true;
-            return -1;
-          }
-          todo.set (++sp, rc);
-        }
-      }
-    }
-    if (cur == kBarrier) return -1;
-    const int leaf = -cur - 1;
-    const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
-    const int el = __float_as_int (l0.x);
-    float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
-    bool found;
-    if constexpr (P == 0)
-      found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
-    else
-      found = try_element_at<P, POLISH> (cfg, cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, pt, x, st);
-    if (found)
-    {
-      ref[0] = x[0];
-      ref[1] = x[1];
-      ref[2] = x[2];
-      return el;
-    }
-    cur = todo.get (sp);
-    --sp;
+    const int r = locate_point_step<P, POLISH> (cfg, todo, pt, ref, st, overflow, cur, sp, cell);
+    if (r > 0) return cell;
+    if (r < 0) return -1;
   }
 }
 

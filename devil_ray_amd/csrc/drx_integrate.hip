@@ -215,6 +215,11 @@ __global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_inte
     float depth = 0.f;
     float rgba[4] = { 0.f, 0.f, 0.f, 0.f };
     bool alive = has_ray;
+    // Binary walk: one candidate per iteration.  A lane whose candidate was rejected keeps its
+    // depth-first state and resumes at the same sample point in the next iteration, while the other
+    // lanes go on with their next samples, so the wavefront never runs a Newton round for a few lanes.
+    int tcur = 0, tsp = 0;
+    bool resume = false;
     for (;;)
     {
       if (alive && !inseg && !(distance < ray.far_)) alive = false;
@@ -236,6 +241,20 @@ __global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_inte
       }
       else if constexpr (WAVE)
         cell = locate_wave<P, POLISH> (a.cfg, ws, pt, ref, st, overflow);
+      else if constexpr (MODE == 0)
+      {
+        if (!resume)
+        {
+          tcur = 0;
+          tsp = 0;
+          todo.set (0, kBarrier);
+        }
+        int c = -1;
+        const int r = locate_point_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c);
+        resume = r == 0;
+        if (resume) continue;
+        cell = r > 0 ? c : -1;
+      }
       else
         cell = locate_any<P, POLISH, MODE == 1> (a.cfg, todo, pt, ref, st, overflow);
       if (overflow)
