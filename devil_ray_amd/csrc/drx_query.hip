@@ -294,6 +294,30 @@ __global__ void k_tile_scatter (const Ray *__restrict__ rays, int n, int w, int 
   if (pid >= 0 && pid < w * h) slots[tile_rank (pid, w, h)] = k;
 }
 
+// Tile keys for longest-first processing: the longest [near, far] segment among the tile's rays, as a
+// descending unsigned key (tiles without rays sort last).
+__global__ void k_tile_keys (const Ray *__restrict__ rays, const int32_t *__restrict__ slots, int n_tiles,
+                             uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
+  float len = -1.f;
+  for (int l = 0; l < 64; ++l)
+  {
+    const int r = slots[((int64_t)t << 6) + l];
+    if (r >= 0) len = fmaxf (len, fmaxf (rays[r].far_ - rays[r].near_, 0.f));
+  }
+  keys[t] = len < 0.f ? 0xFFFFFFFFu : 0x7FFFFFFFu - __float_as_uint (len);
+  vals[t] = t;
+}
+
+__global__ void k_gather_tiles (const int32_t *__restrict__ slots, const int32_t *__restrict__ perm, int64_t n,
+                                int32_t *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = slots[((int64_t)perm[i >> 6] << 6) + (i & 63)];
+}
+
 __global__ void k_flag_nonneg (const int32_t *__restrict__ v, int64_t n, int32_t *__restrict__ flags)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -578,11 +602,30 @@ int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t heig
   const int tiles_x = (width + 7) / 8, tiles_y = (height + 7) / 8;
   const int64_t slots_n = (int64_t)tiles_x * tiles_y * 64;
   if (slots_n > (1ll << 30)) return fail (DRX_ERR_ARG, "drx_ray_tile_order: image too large");
-  int32_t *slots = (int32_t *)arena (SLOT_TREE2, sizeof (int32_t) * 3 * (size_t)slots_n + 64);
-  if (!slots) return DRX_ERR_HIP;
-  int32_t *flags = slots + slots_n, *pos = flags + slots_n;
-  hipLaunchKernelGGL (k_fill_i32q, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots, slots_n, -1);
-  hipLaunchKernelGGL (k_tile_scatter, dim3 (nblk (n, 256)), dim3 (256), 0, st, (const Ray *)rays, n, width, height, slots);
+  int32_t *slots0 = (int32_t *)arena (SLOT_TREE2, sizeof (int32_t) * 4 * (size_t)slots_n + 64);
+  if (!slots0) return DRX_ERR_HIP;
+  int32_t *slots = slots0 + slots_n, *flags = slots + slots_n, *pos = flags + slots_n;
+  hipLaunchKernelGGL (k_fill_i32q, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots0, slots_n, -1);
+  hipLaunchKernelGGL (k_tile_scatter, dim3 (nblk (n, 256)), dim3 (256), 0, st, (const Ray *)rays, n, width, height, slots0);
+  DRX_LAUNCH_CHECK ();
+  // Tiles longest-first (stable): a frame needs more wavefronts than the GPU holds at once, and the ones
+  // dispatched last run on an emptying GPU, so they should be the short ones.
+  const int n_img_tiles = tiles_x * tiles_y;
+  const int n_rt = radix_tiles (n_img_tiles);
+  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n_img_tiles);
+  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n_img_tiles);
+  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n_img_tiles);
+  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n_img_tiles);
+  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_rt + 8));
+  if (!kA || !kB || !vA || !vB || !hist) return DRX_ERR_HIP;
+  hipLaunchKernelGGL (k_tile_keys, dim3 (nblk (n_img_tiles, 256)), dim3 (256), 0, st, (const Ray *)rays, slots0,
+                      n_img_tiles, kA, vA);
+  DRX_LAUNCH_CHECK ();
+  uint32_t *ks;
+  int32_t *vs;
+  int rc0 = radix_sort_pairs (kA, vA, kB, vB, hist, n_img_tiles, 32, st, &ks, &vs);
+  if (rc0) return rc0;
+  hipLaunchKernelGGL (k_gather_tiles, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots0, vs, slots_n, slots);
   hipLaunchKernelGGL (k_flag_nonneg, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots, slots_n, flags);
   DRX_LAUNCH_CHECK ();
   int32_t total = 0;

@@ -230,7 +230,9 @@ int drx_camera_rays (const drx_camera *cam, void *out_rays, void *stream);
 int drx_remove_missed_rays (const void *rays, int32_t n, const float *bounds6_host, void *out_rays,
                             int32_t *out_count, void *stream);
 /* Tile-major processing order for rays that came from a width x height camera (pixel_id = j*w+i):
- * fills out_order[n] with a permutation of 0..n-1 that walks 8x8 pixel tiles. */
+ * fills out_order[n] with a permutation of 0..n-1 that walks 8x8 pixel tiles, tiles ordered by their
+ * longest [near, far] ray segment, longest first (ties in tile order).  Results of the integration do
+ * not depend on the order; it sets which rays share a wavefront and which run first. */
 int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t height, int32_t *out_order,
                         void *stream);
 /* Volume::integrate on active rays: samples = |bounds diagonal|/sample_dist steps, field colour from

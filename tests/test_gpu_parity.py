@@ -391,3 +391,26 @@ def test_wave_traversal_same_as_lane(gpu, name, make):
         assert dc == 0 and np.array_equal(c1, c2)
         # the two entry points inline separately compiled Newton code: references may differ in the last bits
         assert dr <= 1e-6 and (s1 != s2).sum() <= 1e-4 * len(s1)
+
+
+def test_ray_tile_order_longest_first(gpu, orc):
+    """drx_ray_tile_order: a permutation of the active rays in which each 8x8 tile's rays form one
+    contiguous run; tiles come longest [near, far] segment first, ties in tile-index order."""
+    mesh = syn.hex_mesh(4, 2, 0.05, 0.01, 3)
+    ob = orc.build_mesh_bvh(mesh)
+    w, h = 72, 56
+    cam = gpu.Camera(w, h, 40.0, pos=(0.5, 0.6, 2.2), look_at=(0.5, 0.5, 0.5))
+    act, n = gpu.remove_missed_rays(cam.create_rays(), ob.bounds)
+    order = _np(gpu.ray_tile_order(act, n, w, h))[:n]
+    rays = _np(act, np.uint8)[: n * RAY_DTYPE.itemsize].view(RAY_DTYPE)
+    assert n > 500 and np.array_equal(np.sort(order), np.arange(n))
+    pid = rays["pixel_id"][order]
+    tile = (pid // w // 8) * ((w + 7) // 8) + (pid % w) // 8
+    seg = (rays["far"] - rays["near"])[order]
+    starts = np.flatnonzero(np.r_[True, tile[1:] != tile[:-1]])
+    assert len(np.unique(tile)) == len(starts)                       # each tile is one contiguous run
+    longest = np.maximum.reduceat(seg, starts)
+    assert np.all(np.diff(longest) <= 0)
+    t_first = tile[starts]
+    same = np.diff(longest) == 0
+    assert np.all(np.diff(t_first)[same] > 0)                         # ties keep tile-index order
