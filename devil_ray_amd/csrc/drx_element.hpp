@@ -488,6 +488,66 @@ __device__ __forceinline__ bool try_element_mixed (const LocateCfg &cfg, int el,
   }
 }
 
+// Newton's starting point for a candidate leaf (dray hands LocateHack an uninitialised guess,
+// device_mesh.hpp:327-334): the element's affine approximation at its centre, i.e. the trilinear map
+// through its 8 corner control points differentiated at the reference centre, solved for the point and
+// clamped into the candidate's SubRef; the SubRef centre where that approximation is singular.  Nearly
+// affine elements start within the Newton tolerance's reach, so most solves converge in two steps instead
+// of three.  Plain float operations (no contraction), so the oracle computes the same start bit for bit.
+__device__ __forceinline__ void corner_start (const float *g, int p, const float pt[3], const float4 &l0,
+                                              const float4 &l1, float x[3])
+{
+#pragma clang fp contract(off)
+  const float lo[3] = { l0.y, l0.z, l0.w }, hi[3] = { l1.x, l1.y, l1.z };
+  const int n1 = p + 1, np = (n1 * n1 * n1 + 3) & ~3;
+  int idx[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) idx[q] = (q & 1) * p + n1 * (((q >> 1) & 1) * p) + n1 * n1 * ((q >> 2) * p);
+  float r[3], eu[3], ev[3], ew[3];
+  // one coordinate at a time, so only its 8 corner values are live
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    const float *X = g + 4 + d * np;
+    float c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = X[idx[q]];
+    const float m = (((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]))) * 0.125f;
+    r[d] = (pt[d] - g[d]) - m;
+    eu[d] = (((c[1] - c[0]) + (c[3] - c[2])) + ((c[5] - c[4]) + (c[7] - c[6]))) * 0.25f;
+    ev[d] = (((c[2] - c[0]) + (c[3] - c[1])) + ((c[6] - c[4]) + (c[7] - c[5]))) * 0.25f;
+    ew[d] = (((c[4] - c[0]) + (c[5] - c[1])) + ((c[6] - c[2]) + (c[7] - c[3]))) * 0.25f;
+    asm volatile ("" ::: "memory");   // finish this coordinate before loading the next one's corners
+  }
+  // Cramer's rule: x - 1/2 = (det(r, ev, ew), det(eu, r, ew), det(eu, ev, r)) / det(eu, ev, ew)
+  const float vw[3] = { ev[1] * ew[2] - ev[2] * ew[1], ev[2] * ew[0] - ev[0] * ew[2], ev[0] * ew[1] - ev[1] * ew[0] };
+  const float rw[3] = { r[1] * ew[2] - r[2] * ew[1], r[2] * ew[0] - r[0] * ew[2], r[0] * ew[1] - r[1] * ew[0] };
+  const float vr[3] = { ev[1] * r[2] - ev[2] * r[1], ev[2] * r[0] - ev[0] * r[2], ev[0] * r[1] - ev[1] * r[0] };
+  const float det = (eu[0] * vw[0] + eu[1] * vw[1]) + eu[2] * vw[2];
+  const float inv = 1.0f / det;
+  const float dx[3] = { ((r[0] * vw[0] + r[1] * vw[1]) + r[2] * vw[2]) * inv,
+                        ((eu[0] * rw[0] + eu[1] * rw[1]) + eu[2] * rw[2]) * inv,
+                        ((eu[0] * vr[0] + eu[1] * vr[1]) + eu[2] * vr[2]) * inv };
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    const float v = 0.5f + dx[d];
+    // NaN / infinity (singular corners) fall back to the SubRef centre
+    x[d] = (v == v && fabsf (v) < 1e30f) ? fminf (hi[d], fmaxf (lo[d], v)) : (lo[d] + hi[d]) * 0.5f;
+  }
+}
+
+// The Newton start for candidate element `el` of leaf record (l0, l1).
+template <int P>
+__device__ __forceinline__ void candidate_start (const LocateCfg &cfg, int el, const float pt[3], const float4 &l0,
+                                                 const float4 &l1, float x[3])
+{
+  if constexpr (P == 0)
+    corner_start (cfg.mesh + cfg.offsets[el], cfg.orders[el], pt, l0, l1, x);
+  else
+    corner_start (cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, P, pt, l0, l1, x);
+}
+
 #ifndef DRX_STACK_LDS
 #define DRX_STACK_LDS 16   // entries of each lane's traversal stack kept in LDS
 #endif
@@ -570,7 +630,8 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
   const int leaf = -cur - 1;
   const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
   const int el = __float_as_int (l0.x);
-  float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
+  float x[3];
+  candidate_start<P> (cfg, el, pt, l0, l1, x);
   bool found;
   if constexpr (P == 0)
     found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
@@ -609,12 +670,11 @@ __device__ __forceinline__ int locate_point (const LocateCfg &cfg, TravStack &to
 constexpr int32_t kWideEnd = (int32_t)0x80000000;
 
 template <int P, bool POLISH>
-__device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStack &todo, const float pt[3],
-                                                  float ref[3], LocateStats &st, bool &overflow)
+__device__ __forceinline__ int locate_point_wide_step (const LocateCfg &cfg, TravStack &todo, const float pt[3],
+                                                       float ref[3], LocateStats &st, bool &overflow, int32_t &cur,
+                                                       int &sp, int &cell)
 {
-  int32_t cur = 0;
-  int sp = 0;   // number of stack entries
-  for (;;)
+  // resumable like locate_point_step: start with cur = 0, sp = 0 (sp counts stack entries)
   {
     while (cur >= 0)
     {
@@ -666,7 +726,8 @@ __device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStac
     const int leaf = -cur - 1;
     const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
     const int el = __float_as_int (l0.x);
-    float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
+    float x[3];
+    candidate_start<P> (cfg, el, pt, l0, l1, x);
     bool found;
     if constexpr (P == 0)
       found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
@@ -677,9 +738,25 @@ __device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStac
       ref[0] = x[0];
       ref[1] = x[1];
       ref[2] = x[2];
-      return el;
+      cell = el;
+      return 1;
     }
     cur = (sp == 0) ? kWideEnd : todo.get (--sp);
+    return 0;
+  }
+}
+
+template <int P, bool POLISH>
+__device__ __forceinline__ int locate_point_wide (const LocateCfg &cfg, TravStack &todo, const float pt[3],
+                                                  float ref[3], LocateStats &st, bool &overflow)
+{
+  int32_t cur = 0;
+  int sp = 0, cell = -1;
+  for (;;)
+  {
+    const int r = locate_point_wide_step<P, POLISH> (cfg, todo, pt, ref, st, overflow, cur, sp, cell);
+    if (r > 0) return cell;
+    if (r < 0) return -1;
   }
 }
 
@@ -735,7 +812,8 @@ __device__ __forceinline__ int locate_wave (const LocateCfg &cfg, const WaveScra
       const int leaf = ws.cand[q * 64 + lane];
       const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
       const int el = __float_as_int (l0.x);
-      float x[3] = { (l0.y + l1.x) * 0.5f, (l0.z + l1.y) * 0.5f, (l0.w + l1.z) * 0.5f };   // subref_center
+      float x[3];
+      candidate_start<P> (cfg, el, pt, l0, l1, x);
       bool found;
       if constexpr (P == 0)
         found = try_element_mixed<POLISH> (cfg, el, pt, x, st);

@@ -154,17 +154,23 @@ __device__ __forceinline__ float4 lit_color (const IntegrateArgs &a, int cell, c
                             ref, ray);
 }
 
+#ifndef DRX_INTEGRATE_BLOCK
+#define DRX_INTEGRATE_BLOCK 64
+#endif
+constexpr int kIntBlock = DRX_INTEGRATE_BLOCK;   // threads per workgroup of the integrate kernel
+constexpr int kIntWaves = kIntBlock / 64;
+
 template <int P, bool POLISH, int MODE, bool LIT>
-__global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_integrate (IntegrateArgs a)
+__global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) k_integrate (IntegrateArgs a)
 {
   constexpr bool WAVE = MODE == 2;
   constexpr bool LEARNED = MODE == 3;   // learned Newton start; the whole wavefront stays in the loop
-  __shared__ int32_t stack_lds[WAVE ? 1 : kStackLds * 256];
-  __shared__ int32_t w_code[WAVE ? 4 * kWaveStack : 1];
-  __shared__ uint64_t w_mask[WAVE ? 4 * kWaveStack : 1];
-  __shared__ int32_t w_cand[WAVE ? 4 * kCandQ * 64 : 1];
-  __shared__ __attribute__ ((aligned (16))) __bf16 mlp_x[LEARNED ? 4 * 64 * mlp::IN_PAD : 8];
-  __shared__ __attribute__ ((aligned (16))) float mlp_y[LEARNED ? 4 * 64 * 4 : 4];
+  __shared__ int32_t stack_lds[WAVE ? 1 : kStackLds * kIntBlock];
+  __shared__ int32_t w_code[WAVE ? kIntWaves * kWaveStack : 1];
+  __shared__ uint64_t w_mask[WAVE ? kIntWaves * kWaveStack : 1];
+  __shared__ int32_t w_cand[WAVE ? kIntWaves * kCandQ * 64 : 1];
+  __shared__ __attribute__ ((aligned (16))) __bf16 mlp_x[LEARNED ? kIntWaves * 64 * mlp::IN_PAD : 8];
+  __shared__ __attribute__ ((aligned (16))) float mlp_y[LEARNED ? kIntWaves * 64 * 4 : 4];
   WaveScratch ws;
   const int wv = threadIdx.x >> 6;
   {
@@ -215,7 +221,7 @@ __global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_inte
     float depth = 0.f;
     float rgba[4] = { 0.f, 0.f, 0.f, 0.f };
     bool alive = has_ray;
-    // Binary walk: one candidate per iteration.  A lane whose candidate was rejected keeps its
+    // Binary and 8-wide walks: one candidate per iteration.  A lane whose candidate was rejected keeps its
     // depth-first state and resumes at the same sample point in the next iteration, while the other
     // lanes go on with their next samples, so the wavefront never runs a Newton round for a few lanes.
     int tcur = 0, tsp = 0;
@@ -241,16 +247,20 @@ __global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_inte
       }
       else if constexpr (WAVE)
         cell = locate_wave<P, POLISH> (a.cfg, ws, pt, ref, st, overflow);
-      else if constexpr (MODE == 0)
+      else if constexpr (MODE <= 1)
       {
         if (!resume)
         {
           tcur = 0;
           tsp = 0;
-          todo.set (0, kBarrier);
+          if constexpr (MODE == 0) todo.set (0, kBarrier);
         }
         int c = -1;
-        const int r = locate_point_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c);
+        int r;
+        if constexpr (MODE == 1)
+          r = locate_point_wide_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c);
+        else
+          r = locate_point_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c);
         resume = r == 0;
         if (resume) continue;
         cell = r > 0 ? c : -1;
@@ -407,7 +417,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   a.budget = (learned && opts->mlp_newton_steps > 0) ? opts->mlp_newton_steps : 4;
   a.n_lights = lit ? shading->n_lights : 0;
   for (int l = 0; l < a.n_lights; ++l) a.lights[l] = shading->lights[l];
-  const dim3 g (nblk (n_rays, 256)), b (256);
+  const dim3 g (nblk (n_rays, kIntBlock)), b (kIntBlock);
   const int P = mesh->elem_order ? 0 : mesh->order;
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !a.cfg.wide)

@@ -125,9 +125,7 @@ __device__ __forceinline__ int locate_point_mlp (const LocateCfg &cfg, TravStack
       const int leaf = -cur - 1;
       const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
       el = __float_as_int (l0.x);
-      xc[0] = (l0.y + l1.x) * 0.5f;   // subref_center
-      xc[1] = (l0.z + l1.y) * 0.5f;
-      xc[2] = (l0.w + l1.z) * 0.5f;
+      candidate_start<P> (cfg, el, pt, l0, l1, xc);   // the fallback start of the plain method
       if constexpr (P == 0)
       {
         g = cfg.mesh + cfg.offsets[el];

@@ -10,8 +10,9 @@
 // Deviations from dray, deliberate and documented:
 //   * The Morton sort is stable: equal codes keep index order (dray: std::sort, unstable,
 //     src/dray/linear_bvh_builder.cpp:128-129).
-//   * Newton starts from the leaf's SubRef centre (dray passes an uninitialised guess,
-//     src/dray/data_model/device_mesh.hpp:327-334 with use_init_guess = true).
+//   * Newton's start (dray passes an uninitialised guess, src/dray/data_model/device_mesh.hpp:327-334
+//     with use_init_guess = true) is the element's corner-affine estimate clamped into the candidate's
+//     SubRef (corner_start below; the SubRef centre where that is singular).
 //   * The FP64 instance is a clean FP64 restatement: the clamp and the inf-norm use fmin/fmax in
 //     double (dray's FP64 build still rounds through fminf/fmaxf, src/dray/data_model/element.hpp:449,
 //     src/dray/vec.hpp:229-238).
@@ -616,6 +617,44 @@ struct BVHView
 struct LocStats { i64 newton_steps = 0; i64 candidates = 0; };
 
 // a10/a11: DeviceMesh::locate (device_mesh.hpp:247-354)
+// Newton's start for a candidate (dray passes an uninitialised guess, device_mesh.hpp:327-334; the GPU
+// path uses the same rule): the element's affine approximation at its centre -- the trilinear map through
+// its 8 corner control points, differentiated at the reference centre -- solved for the point by Cramer's
+// rule and clamped into the candidate's SubRef; the SubRef centre where that is singular.
+template <typename T>
+static void corner_start (const T *dofs, int p, const T pt[3], const float *sr, T x[3])
+{
+  const int n1 = p + 1;
+  T c[8][3];
+  for (int q = 0; q < 8; ++q)
+  {
+    const int idx = (q & 1) * p + n1 * (((q >> 1) & 1) * p) + n1 * n1 * ((q >> 2) * p);
+    for (int d = 0; d < 3; ++d) c[q][d] = dofs[3 * idx + d];
+  }
+  T r[3], eu[3], ev[3], ew[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    const T m = (((c[0][d] + c[1][d]) + (c[2][d] + c[3][d])) + ((c[4][d] + c[5][d]) + (c[6][d] + c[7][d]))) * T (0.125);
+    r[d] = pt[d] - m;
+    eu[d] = (((c[1][d] - c[0][d]) + (c[3][d] - c[2][d])) + ((c[5][d] - c[4][d]) + (c[7][d] - c[6][d]))) * T (0.25);
+    ev[d] = (((c[2][d] - c[0][d]) + (c[3][d] - c[1][d])) + ((c[6][d] - c[4][d]) + (c[7][d] - c[5][d]))) * T (0.25);
+    ew[d] = (((c[4][d] - c[0][d]) + (c[5][d] - c[1][d])) + ((c[6][d] - c[2][d]) + (c[7][d] - c[3][d]))) * T (0.25);
+  }
+  const T vw[3] = { ev[1] * ew[2] - ev[2] * ew[1], ev[2] * ew[0] - ev[0] * ew[2], ev[0] * ew[1] - ev[1] * ew[0] };
+  const T rw[3] = { r[1] * ew[2] - r[2] * ew[1], r[2] * ew[0] - r[0] * ew[2], r[0] * ew[1] - r[1] * ew[0] };
+  const T vr[3] = { ev[1] * r[2] - ev[2] * r[1], ev[2] * r[0] - ev[0] * r[2], ev[0] * r[1] - ev[1] * r[0] };
+  const T det = (eu[0] * vw[0] + eu[1] * vw[1]) + eu[2] * vw[2];
+  const T inv = T (1) / det;
+  const T dx[3] = { ((r[0] * vw[0] + r[1] * vw[1]) + r[2] * vw[2]) * inv, ((eu[0] * rw[0] + eu[1] * rw[1]) + eu[2] * rw[2]) * inv,
+                    ((eu[0] * vr[0] + eu[1] * vr[1]) + eu[2] * vr[2]) * inv };
+  for (int d = 0; d < 3; ++d)
+  {
+    const T v = T (0.5) + dx[d];
+    const T lo = T (sr[d]), hi = T (sr[3 + d]);
+    x[d] = (v == v && std::fabs (v) < T (1e30)) ? std::min (hi, std::max (lo, v)) : T ((sr[d] + sr[3 + d]) * 0.5f);
+  }
+}
+
 template <typename T>
 static i32 locate_point (const MeshView<T> &mesh, const BVHView &bvh, const T pt[3], T ref_out[3],
                          std::vector<T> &dofs, LocStats *st)
@@ -662,11 +701,11 @@ static i32 locate_point (const MeshView<T> &mesh, const BVHView &bvh, const T pt
       const i32 leaf = -cur - 1;
       const i32 el = bvh.leaf_nodes[leaf];
       const float *sr = bvh.subrefs + (size_t)6 * bvh.aabb_ids[leaf];
-      T x[3];
-      for (int d = 0; d < 3; ++d) x[d] = T ((sr[d] + sr[3 + d]) * 0.5f);   // subref_center
       const int p = mesh.order_of (el);
       dofs.resize ((size_t)3 * npe_of (p));
       mesh.template gather<3> (el, dofs.data ());
+      T x[3];
+      corner_start<T> (dofs.data (), p, pt, sr, x);
       int steps = 0;
       const bool found = newton_inverse<T> (dofs.data (), p, pt, x, &steps);
       if (st) { st->newton_steps += steps; st->candidates += 1; }
