@@ -55,6 +55,86 @@ __global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_loca
   if (cand_out) cand_out[i] = st.candidates;
 }
 
+// Per-lane walks, streamed: each wavefront owns `chunk` consecutive entries of the processing order and
+// every iteration advances each lane's point by one candidate (locate_point_step); a lane whose point is
+// settled (accepted or exhausted) writes its Location and takes the chunk's next unclaimed point.  The
+// wavefront therefore never waits for the lanes whose points need several candidates.  Same candidate
+// sequence and results as one lane per point.
+constexpr int kLocateChunk = 1024;
+
+template <int P, bool POLISH, int MODE>
+__global__ void __launch_bounds__ (64, waves_per_simd (P, POLISH, MODE))
+k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const int32_t *__restrict__ order,
+                 Location *__restrict__ out, int32_t *__restrict__ steps_out, int32_t *__restrict__ cand_out,
+                 int32_t *__restrict__ err)
+{
+  __shared__ int32_t stack_lds[kStackLds * 64];
+  const int lane = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kLocateChunk;
+  const int64_t end = base + kLocateChunk < n ? base + kLocateChunk : n;
+  TravStack todo;
+  todo.lds = stack_lds;
+  todo.bdim = 64;
+  todo.tid = lane;
+  int64_t claimed = base + 64;   // wave-uniform: next unclaimed entry of the chunk
+  int64_t k = base + lane;
+  bool have = k < end;
+  int64_t i = 0;
+  float pt[3] = { 0.f, 0.f, 0.f };
+  int cur = 0, sp = 0;
+  LocateStats st { 0, 0, 0, 0 };
+  auto start = [&] () {
+    i = order ? (int64_t)order[k] : k;
+    pt[0] = pts[3 * i];
+    pt[1] = pts[3 * i + 1];
+    pt[2] = pts[3 * i + 2];
+    cur = 0;
+    sp = 0;
+    if constexpr (MODE == 0) todo.set (0, kBarrier);
+    st = LocateStats { 0, 0, 0, 0 };
+  };
+  if (have) start ();
+  for (;;)
+  {
+    if (__ballot (have) == 0) break;
+    bool done = false;
+    if (have)
+    {
+      float ref[3];
+      int cell = -1;
+      bool overflow = false;
+      int r;
+      if constexpr (MODE == 1)
+        r = locate_point_wide_step<P, POLISH> (cfg, todo, pt, ref, st, overflow, cur, sp, cell);
+      else
+        r = locate_point_step<P, POLISH> (cfg, todo, pt, ref, st, overflow, cur, sp, cell);
+      if (r != 0)
+      {
+        done = true;
+        if (overflow) atomicOr (err, 1);
+        const bool hit = r > 0;
+        Location loc;
+        loc.cell = hit ? cell : -1;
+        loc.ref[0] = hit ? ref[0] : -1.f;
+        loc.ref[1] = hit ? ref[1] : -1.f;
+        loc.ref[2] = hit ? ref[2] : -1.f;
+        out[i] = loc;
+        if (steps_out) steps_out[i] = st.newton;
+        if (cand_out) cand_out[i] = st.candidates;
+      }
+    }
+    // refill: settled lanes take the next entries of the chunk, in lane order
+    const uint64_t m = __ballot (done);
+    if (done)
+    {
+      k = claimed + __popcll (m & ((1ull << lane) - 1));
+      have = k < end;
+      if (have) start ();
+    }
+    claimed += __popcll (m);
+  }
+}
+
 // Mesh::locate with the learned initial guess (binary per-lane traversal).  Lanes past n stay in the
 // kernel as idle participants of their wavefront's MLP rounds.
 template <int P, bool POLISH>
@@ -457,6 +537,7 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   int32_t *cands = opts ? opts->candidates : nullptr;
   const int P = mesh->elem_order ? 0 : mesh->order;
   const dim3 g (nblk (n, 256)), b (256);
+  const dim3 gs ((unsigned)((n + kLocateChunk - 1) / kLocateChunk)), bs (64);
   Location *out = (Location *)out_locations;
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !cfg.wide)
@@ -480,9 +561,9 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
     else if (mode == 2)                                                                             \
       hipLaunchKernelGGL ((k_locate<PP, PO, 2>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
     else if (mode == 1)                                                                             \
-      hipLaunchKernelGGL ((k_locate<PP, PO, 1>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
+      hipLaunchKernelGGL ((k_locate_stream<PP, PO, 1>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err); \
     else                                                                                            \
-      hipLaunchKernelGGL ((k_locate<PP, PO, 0>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
+      hipLaunchKernelGGL ((k_locate_stream<PP, PO, 0>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err); \
   } while (0)
   if (cfg.polish)
     switch (P)
