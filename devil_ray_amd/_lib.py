@@ -29,7 +29,7 @@ class DrxMesh(C.Structure):
 
 class DrxBVH(C.Structure):
     _fields_ = [("inner_nodes", P), ("leaf_nodes", P), ("aabb_ids", P), ("subrefs", P), ("leaf_pack", P),
-                ("wide_nodes", P), ("n_leaves", I32), ("bounds", F32 * 6)]
+                ("wide_nodes", P), ("n_leaves", I32), ("bounds", F32 * 6), ("leaf_starts", P)]
 
 
 class DrxLocateOpts(C.Structure):
@@ -75,6 +75,7 @@ _SIGS = {
     "drx_bvh_split_elements": (I32, [P, P, P, P, I32, I32, P, P, P, P]),
     "drx_bvh_build": (I32, [P, P, I32, P, P, P, P, P, P]),
     "drx_bvh_pack_leaves": (I32, [C.POINTER(DrxBVH), P, P]),
+    "drx_bvh_leaf_starts": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, P]),
     "drx_bvh_widen": (I32, [C.POINTER(DrxBVH), P, I64, C.POINTER(I64), P]),
     "drx_locate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I64, P, C.POINTER(DrxLocateOpts), P]),
     "drx_field_eval": (I32, [C.POINTER(DrxMesh), P, I64, P, P]),

@@ -66,7 +66,9 @@ class BVH:
         self.mcodes = mcodes
         self.leaf_pack = None
         self.wide_nodes = None
+        self.leaf_starts = None
         self.use_wide = True
+        self.use_starts = True
 
     @property
     def n_leaves(self) -> int:
@@ -83,7 +85,15 @@ class BVH:
         v.n_leaves = self.n_leaves
         for q in range(6):
             v.bounds[q] = float(self.bounds[q])
+        v.leaf_starts = self.leaf_starts.data_ptr() if (self.leaf_starts is not None and self.use_starts) else None
         return v
+
+    def compute_leaf_starts(self, mesh_view: L.DrxMesh) -> None:
+        """Per-leaf Newton start maps (drx_bvh_leaf_starts) from the geometry the BVH was built on."""
+        self.leaf_starts = torch.empty(self.n_leaves * 12, dtype=torch.float32, device=_dev())
+        v = self.view()
+        L.check(L.lib().drx_bvh_leaf_starts(C.byref(mesh_view), C.byref(v), _ptr(self.leaf_starts), _stream()),
+                "drx_bvh_leaf_starts")
 
     def pack_leaves(self) -> None:
         dev = _dev()
@@ -221,6 +231,7 @@ class UnstructuredMesh:
             bvh = LinearBVHBuilder().construct(aabbs, prims)
             bvh.subrefs = subrefs
             bvh.pack_leaves()
+            bvh.compute_leaf_starts(self.gf.view())
             bvh.widen()
             self._bvh = bvh
         return self._bvh

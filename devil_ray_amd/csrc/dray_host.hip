@@ -147,6 +147,9 @@ BVH &UnstructuredMesh::get_bvh ()
     drx_bvh v = bvh_view ();
     m_bvh.m_leaf_pack.resize (8 * (size_t)m_bvh.m_leaf_nodes.size ());
     drx_check (drx_bvh_pack_leaves (&v, m_bvh.m_leaf_pack.get_device_ptr (), nullptr), "drx_bvh_pack_leaves");
+    m_bvh.m_leaf_starts.resize (12 * (size_t)m_bvh.m_leaf_nodes.size ());
+    const drx_mesh mv = device_view ();
+    drx_check (drx_bvh_leaf_starts (&mv, &v, m_bvh.m_leaf_starts.get_device_ptr (), nullptr), "drx_bvh_leaf_starts");
     int64_t cnt = 0;
     v = bvh_view ();
     drx_check (drx_bvh_widen (&v, nullptr, 0, &cnt, nullptr), "drx_bvh_widen");
@@ -172,6 +175,7 @@ drx_bvh UnstructuredMesh::bvh_view ()
     v.bounds[2 * d] = m_bvh.m_bounds.m_ranges[d].m_min;
     v.bounds[2 * d + 1] = m_bvh.m_bounds.m_ranges[d].m_max;
   }
+  v.leaf_starts = m_bvh.m_leaf_starts.size () ? m_bvh.m_leaf_starts.get_device_ptr_const () : nullptr;
   return v;
 }
 

@@ -437,6 +437,7 @@ struct LocateCfg
   const float4 *nodes;      // inner nodes, 4 float4 each
   const float4 *wide;       // 8-wide nodes, 16 float4 each (nullptr: binary traversal)
   const float4 *leaves;     // packed leaves, 2 float4 each
+  const float4 *starts;     // optional Newton start maps, 3 float4 per leaf (drx_bvh_leaf_starts)
   const float *mesh;        // packed geometry blocks
   const int64_t *offsets;   // mixed order: per-element block offset (nullptr for uniform)
   const int32_t *orders;    // mixed order: per-element order
@@ -537,12 +538,25 @@ __device__ __forceinline__ void corner_start (const float *g, int p, const float
   }
 }
 
-// The Newton start for candidate element `el` of leaf record (l0, l1).
+// The Newton start for candidate element `el` of leaf `leaf` (leaf record l0, l1): the leaf's stored
+// start map when the BVH carries one (read in the same round as the leaf record), else corner_start.
 template <int P>
-__device__ __forceinline__ void candidate_start (const LocateCfg &cfg, int el, const float pt[3], const float4 &l0,
-                                                 const float4 &l1, float x[3])
+__device__ __forceinline__ void candidate_start (const LocateCfg &cfg, int el, int leaf, const float pt[3],
+                                                 const float4 &l0, const float4 &l1, float x[3])
 {
-  if constexpr (P == 0)
+  if (cfg.starts)
+  {
+    const float4 *m = cfg.starts + 3 * (int64_t)leaf;
+    const float4 r0 = m[0], r1 = m[1], r2 = m[2];
+    const float v[3] = { ((r0.x * pt[0] + r0.y * pt[1]) + r0.z * pt[2]) + r0.w,
+                         ((r1.x * pt[0] + r1.y * pt[1]) + r1.z * pt[2]) + r1.w,
+                         ((r2.x * pt[0] + r2.y * pt[1]) + r2.z * pt[2]) + r2.w };
+    const float lo[3] = { l0.y, l0.z, l0.w }, hi[3] = { l1.x, l1.y, l1.z };
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      x[d] = (v[d] == v[d] && fabsf (v[d]) < 1e30f) ? fminf (hi[d], fmaxf (lo[d], v[d])) : (lo[d] + hi[d]) * 0.5f;
+  }
+  else if constexpr (P == 0)
     corner_start (cfg.mesh + cfg.offsets[el], cfg.orders[el], pt, l0, l1, x);
   else
     corner_start (cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, P, pt, l0, l1, x);
@@ -631,7 +645,7 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
   const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
   const int el = __float_as_int (l0.x);
   float x[3];
-  candidate_start<P> (cfg, el, pt, l0, l1, x);
+  candidate_start<P> (cfg, el, leaf, pt, l0, l1, x);
   bool found;
   if constexpr (P == 0)
     found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
@@ -727,7 +741,7 @@ __device__ __forceinline__ int locate_point_wide_step (const LocateCfg &cfg, Tra
     const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
     const int el = __float_as_int (l0.x);
     float x[3];
-    candidate_start<P> (cfg, el, pt, l0, l1, x);
+    candidate_start<P> (cfg, el, leaf, pt, l0, l1, x);
     bool found;
     if constexpr (P == 0)
       found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
@@ -813,7 +827,7 @@ __device__ __forceinline__ int locate_wave (const LocateCfg &cfg, const WaveScra
       const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
       const int el = __float_as_int (l0.x);
       float x[3];
-      candidate_start<P> (cfg, el, pt, l0, l1, x);
+      candidate_start<P> (cfg, el, leaf, pt, l0, l1, x);
       bool found;
       if constexpr (P == 0)
         found = try_element_mixed<POLISH> (cfg, el, pt, x, st);

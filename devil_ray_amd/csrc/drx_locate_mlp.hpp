@@ -125,7 +125,7 @@ __device__ __forceinline__ int locate_point_mlp (const LocateCfg &cfg, TravStack
       const int leaf = -cur - 1;
       const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
       el = __float_as_int (l0.x);
-      candidate_start<P> (cfg, el, pt, l0, l1, xc);   // the fallback start of the plain method
+      candidate_start<P> (cfg, el, leaf, pt, l0, l1, xc);   // the fallback start of the plain method
       if constexpr (P == 0)
       {
         g = cfg.mesh + cfg.offsets[el];

@@ -55,6 +55,46 @@ __global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_loca
   if (cand_out) cand_out[i] = st.candidates;
 }
 
+// Newton start map of every leaf (drx_bvh_leaf_starts): the inverse of its element's corner-affine
+// approximation, x0 = M p + b in absolute coordinates (b NaN when singular).
+__global__ void k_leaf_starts (const float *__restrict__ mesh, const int64_t *__restrict__ offsets,
+                               const int32_t *__restrict__ orders, int stride, int order, const int32_t *__restrict__ leaf_el,
+                               int n, float4 *__restrict__ out)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int el = leaf_el[i];
+  const float *g = offsets ? mesh + offsets[el] : mesh + (int64_t)el * stride;
+  const int p = orders ? orders[el] : order;
+  const int n1 = p + 1, np = (n1 * n1 * n1 + 3) & ~3;
+  float m[3], eu[3], ev[3], ew[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    const float *X = g + 4 + d * np;
+    float c[8];
+    for (int q = 0; q < 8; ++q) c[q] = X[(q & 1) * p + n1 * (((q >> 1) & 1) * p) + n1 * n1 * ((q >> 2) * p)];
+    m[d] = (((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]))) * 0.125f + g[d];
+    eu[d] = (((c[1] - c[0]) + (c[3] - c[2])) + ((c[5] - c[4]) + (c[7] - c[6]))) * 0.25f;
+    ev[d] = (((c[2] - c[0]) + (c[3] - c[1])) + ((c[6] - c[4]) + (c[7] - c[5]))) * 0.25f;
+    ew[d] = (((c[4] - c[0]) + (c[5] - c[1])) + ((c[6] - c[2]) + (c[7] - c[3]))) * 0.25f;
+  }
+  // M = [eu ev ew]^-1 = adjugate / det: row d of the inverse is (column vector cross products) / det
+  const float r0[3] = { ev[1] * ew[2] - ev[2] * ew[1], ev[2] * ew[0] - ev[0] * ew[2], ev[0] * ew[1] - ev[1] * ew[0] };
+  const float r1[3] = { ew[1] * eu[2] - ew[2] * eu[1], ew[2] * eu[0] - ew[0] * eu[2], ew[0] * eu[1] - ew[1] * eu[0] };
+  const float r2[3] = { eu[1] * ev[2] - eu[2] * ev[1], eu[2] * ev[0] - eu[0] * ev[2], eu[0] * ev[1] - eu[1] * ev[0] };
+  const float det = (eu[0] * r0[0] + eu[1] * r0[1]) + eu[2] * r0[2];
+  const float inv = 1.0f / det;
+  const bool ok = det != 0.f && fabsf (inv) < 1e30f;
+  const float *rows[3] = { r0, r1, r2 };
+  for (int d = 0; d < 3; ++d)
+  {
+    const float a = rows[d][0] * inv, b = rows[d][1] * inv, c = rows[d][2] * inv;
+    const float off = ok ? 0.5f - ((a * m[0] + b * m[1]) + c * m[2]) : __builtin_nanf ("");
+    out[3 * (int64_t)i + d] = make_float4 (ok ? a : 0.f, ok ? b : 0.f, ok ? c : 0.f, off);
+  }
+}
+
 // Per-lane walks, streamed: each wavefront owns `chunk` consecutive entries of the processing order and
 // every iteration advances each lane's point by one candidate (locate_point_step); a lane whose point is
 // settled (accepted or exhausted) writes its Location and takes the chunk's next unclaimed point.  The
@@ -514,6 +554,20 @@ __global__ void k_background (float4 *__restrict__ img, int npix, float4 bg)
 using namespace drx;
 
 extern "C" {
+
+int drx_bvh_leaf_starts (const drx_mesh *mesh, const drx_bvh *bvh, float *out_leaf_starts, void *stream)
+{
+  int rc = check_mesh (mesh, 3, "drx_bvh_leaf_starts");
+  if (rc) return rc;
+  if (!bvh || !bvh->leaf_nodes || bvh->n_leaves < 0 || !out_leaf_starts)
+    return fail (DRX_ERR_ARG, "drx_bvh_leaf_starts: bvh needs leaf_nodes; output required");
+  if (bvh->n_leaves == 0) return DRX_OK;
+  hipLaunchKernelGGL (k_leaf_starts, dim3 (nblk (bvh->n_leaves, 256)), dim3 (256), 0, as_stream (stream), mesh->data,
+                      mesh->elem_offset, mesh->elem_order, mesh->stride, mesh->order, bvh->leaf_nodes, bvh->n_leaves,
+                      (float4 *)out_leaf_starts);
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
 
 int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n, void *out_locations,
                 const drx_locate_opts *opts, void *stream)

@@ -160,6 +160,7 @@ struct BVH
   Array<int32> m_aabb_ids;
   // traversal copies (libdrx): fused leaf records and the 8-wide collapse
   Array<float32> m_leaf_pack;
+  Array<float32> m_leaf_starts;   // Newton start map per leaf (drx_bvh_leaf_starts)
   Array<float32> m_wide_nodes;
 };
 

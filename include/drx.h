@@ -92,6 +92,7 @@ typedef struct drx_bvh
   const float *wide_nodes;    /* optional 8-wide collapse (drx_bvh_widen); used for traversal when set */
   int32_t n_leaves;
   float bounds[6];            /* host copy of m_bounds: xmin, xmax, ymin, ymax, zmin, zmax */
+  const float *leaf_starts;   /* optional, 12 floats per leaf: Newton start map (drx_bvh_leaf_starts) */
 } drx_bvh;
 
 /* Location (location.hpp:16-21): {int32 cell (-1 = not found), float ref[3]} = 16 bytes. */
@@ -213,6 +214,14 @@ int drx_bvh_build (const float *aabbs, const int32_t *prims, int32_t n, float *o
                    void *stream);
 /* Fused leaf records for traversal: 8 floats per leaf {el_id bits, subref lo xyz, hi xyz, 0}. */
 int drx_bvh_pack_leaves (const drx_bvh *bvh, float *out_leaf_pack, void *stream);
+/* Newton start data per leaf (no dray counterpart; dray starts Newton from an uninitialised guess,
+ * device_mesh.hpp:327-334): 12 floats per leaf, rows {M_d0, M_d1, M_d2, b_d} for d = 0..2 of the affine
+ * map x0 = M p + b that inverts the leaf element's corner-affine approximation (the trilinear map
+ * through its 8 corner control points, differentiated at the reference centre); b is NaN where that
+ * approximation is singular.  A candidate's Newton starts at x0 clamped into its SubRef (SubRef centre
+ * if NaN).  Optional: without it the same start is computed from the element's corners at each
+ * candidate, one more dependent memory access. */
+int drx_bvh_leaf_starts (const drx_mesh *mesh, const drx_bvh *bvh, float *out_leaf_starts, void *stream);
 /* 8-wide traversal copy of the binary BVH: children in binary left-to-right order, exact float boxes,
  * so traversal visits the same candidate leaves in the same order as dray's binary depth-first walk.
  * Node = 64 floats: lo_x[8] lo_y[8] lo_z[8] hi_x[8] hi_y[8] hi_z[8] child[8] (int32 bits) pad[8].
