@@ -277,6 +277,104 @@ class UnstructuredField:
 
 
 # ---------------------------------------------------------------------------------------------
+class DataSet:
+    """dray::DataSet (data_model/data_set.hpp:20-59): one domain's mesh and its named fields."""
+
+    def __init__(self, mesh: "UnstructuredMesh"):
+        self.mesh = mesh
+        self._fields = {}
+
+    def add_field(self, name: str, field: "UnstructuredField"):
+        self._fields[name] = field
+
+    def has_field(self, name: str) -> bool:
+        return name in self._fields
+
+    def field(self, name: str) -> "UnstructuredField":
+        if name not in self._fields:
+            raise L.DrxError(f"DataSet: no field named '{name}'")
+        return self._fields[name]
+
+    def fields(self):
+        return list(self._fields)
+
+
+class Collection:
+    """dray::Collection (data_model/collection.hpp:16-55) on one rank: every domain is local."""
+
+    def __init__(self):
+        self.domains = []
+
+    def add_domain(self, ds: DataSet):
+        self.domains.append(ds)
+
+    def domain(self, i: int) -> DataSet:
+        return self.domains[i]
+
+    def size(self) -> int:
+        return len(self.domains)
+
+    def has_field(self, name: str) -> bool:
+        """True when every domain has the field (Collection::has_field)."""
+        return len(self.domains) > 0 and all(d.has_field(name) for d in self.domains)
+
+    def range(self, name: str):
+        """Union of the domains' field ranges (Collection::range)."""
+        rs = [d.field(name).range() for d in self.domains if d.has_field(name)]
+        if not rs:
+            raise L.DrxError(f"Collection: no domain has field '{name}'")
+        return float(np.float32(min(r[0] for r in rs))), float(np.float32(max(r[1] for r in rs)))
+
+    def bounds(self) -> np.ndarray:
+        """Union of the domains' mesh bounds (Collection::bounds), [xmin, xmax, ymin, ymax, zmin, zmax]."""
+        b = np.stack([d.mesh.bounds() for d in self.domains]).astype(np.float32)
+        return np.array([b[:, 0].min(), b[:, 1].max(), b[:, 2].min(), b[:, 3].max(), b[:, 4].min(), b[:, 5].max()],
+                        np.float32)
+
+    def field_list(self) -> str:
+        names = sorted({n for d in self.domains for n in d.fields()})
+        return "[" + " ".join(names) + "]"
+
+
+class PointLocation:
+    """dray::PointLocation (queries/point_location.cpp:176-287) on one rank: every domain locates every
+    point; a point located in a domain takes that domain's field values (later domains overwrite
+    earlier ones), the others keep empty_val."""
+
+    def __init__(self):
+        self.m_vars = []
+        self.m_empty_val = 0.0
+
+    def add_var(self, var: str):
+        self.m_vars.append(var)
+
+    def empty_val(self, v: float):
+        self.m_empty_val = float(v)
+
+    def execute(self, collection: Collection, points: torch.Tensor) -> dict:
+        """points: float32 device tensor of n*3 values -> {"points", "values" (one device tensor per valid
+        variable), "vars", "empty_val"}.  Raises DrxError as dray does for missing variables or points."""
+        if not self.m_vars:
+            raise L.DrxError("PointLocation: must specify at least 1 variables.")
+        n = int(points.numel() // 3)
+        if n == 0:
+            raise L.DrxError("PointLocation: must specify at least 1 point.")
+        valid = [v for v in self.m_vars if collection.has_field(v)]
+        if not valid:
+            raise L.DrxError("PointLocation: must specify at least one valid field. Known fields "
+                             + collection.field_list())
+        values = [torch.full((n,), self.m_empty_val, dtype=torch.float32, device=_dev()) for _ in valid]
+        for ds in collection.domains:
+            locs = ds.mesh.locate(points)
+            cells = locs.view(torch.int32).view(n, 4)[:, 0]
+            if not bool((cells != -1).any()):
+                continue
+            for f, name in enumerate(valid):
+                ds.field(name).eval(locs, values[f])
+        return {"points": points, "values": values, "vars": list(self.m_vars), "empty_val": self.m_empty_val}
+
+
+# ---------------------------------------------------------------------------------------------
 class ColorTable:
     """dray::ColorTable (color_table.cpp): rgb + alpha control points, smooth flag (default False)."""
 

@@ -448,3 +448,44 @@ def test_leaf_start_maps_same_results(gpu, orc):
     el = _np(bvh.leaf_nodes)
     exp = (p[None, :] * 3 - np.stack([el % 3, (el // 3) % 3, el // 9], 1))
     assert np.abs(x0 - exp).max() < 1e-5
+
+
+def test_point_location_collection(gpu):
+    """PointLocation over a two-domain Collection (a mesh and a copy shifted by (+0.6, +0.5, 0) whose
+    field is f + 10; the copies overlap and leave two corners of the union box uncovered): a point takes
+    the last domain that locates it, else empty_val; Collection range / bounds are the unions."""
+    a = syn.hex_mesh(4, 2, 0.05, 0.01, 7)
+    b = syn.HexMesh(n=a.n, order=a.order, ctrl_idx=a.ctrl_idx,
+                    values=(a.values + np.float32([0.6, 0.5, 0.0])).astype(np.float32),
+                    field=(a.field + np.float32(10.0)).astype(np.float32))
+    col = gpu.Collection()
+    meshes = []
+    for mesh in (a, b):
+        m, f = gpu.UnstructuredMesh(mesh), gpu.UnstructuredField(mesh)
+        ds = gpu.DataSet(m)
+        ds.add_field("f", f)
+        col.add_domain(ds)
+        meshes.append((m, f))
+    bb = col.bounds()
+    pts = syn.query_points(20000, bb, seed=9)
+    dp = gpu.to_device(pts.reshape(-1))
+    pl = gpu.PointLocation()
+    pl.add_var("f")
+    pl.add_var("nope")
+    pl.empty_val(-7.0)
+    res = pl.execute(col, dp)
+    got = _np(res["values"][0])
+    l0, l1 = meshes[0][0].locate(dp), meshes[1][0].locate(dp)
+    v0, v1 = _np(meshes[0][1].eval(l0)), _np(meshes[1][1].eval(l1))
+    c0 = _np(l0, np.uint8).view(LOCATION_DTYPE)["cell"]
+    c1 = _np(l1, np.uint8).view(LOCATION_DTYPE)["cell"]
+    exp = np.where(c1 >= 0, v1, np.where(c0 >= 0, v0, np.float32(-7.0))).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    assert (c0 >= 0).sum() > 1000 and (c1 >= 0).sum() > 1000 and (exp == -7).sum() > 100
+    assert len(res["values"]) == 1 and res["vars"] == ["f", "nope"]
+    lo, hi = col.range("f")
+    assert lo == float(a.field.min()) and hi == float(np.float32(a.field.max() + np.float32(10.0)))
+    ba, bbm = meshes[0][0].bounds(), meshes[1][0].bounds()
+    assert bb[0] == ba[0] and bb[1] == bbm[1] and bb[2] == ba[2] and bb[3] == bbm[3]
+    both = (c0 >= 0) & (c1 >= 0)
+    assert both.sum() > 100 and np.all(got[both] == v1[both])
