@@ -589,6 +589,84 @@ struct TravStack
   }
 };
 
+// Element staging in LDS: the lanes of a wavefront (rays of one 8x8 tile, or neighbouring query points)
+// mostly test the same few elements.  After traversal each lane holds a candidate; the wavefront takes the
+// distinct candidate elements in lane order, up to kStage<P> of them, copies each element's packed block
+// into its LDS area in one cooperative round (one float4 per lane) and returns, per lane, the LDS copy of
+// its element (or its global block when its element did not fit).  Newton then reads control points from
+// LDS instead of re-reading them from the cache hierarchy every iteration; the bits are the same.
+template <int P> struct Stage
+{
+  static constexpr int F4 = (Ord<P>::GSTRIDE + 3) / 4;   // float4 per packed element block
+  static constexpr int K = P <= 3 ? 4 : 2;               // elements staged per wavefront
+  static constexpr int FLOATS = K * F4 * 4;              // LDS floats per wavefront
+};
+
+__device__ __forceinline__ void stage_sync ()
+{
+  __builtin_amdgcn_fence (__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier ();
+  __builtin_amdgcn_fence (__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int P>
+__device__ __forceinline__ const float *stage_element (const LocateCfg &cfg, int el, const float *g, float *stage)
+{
+  constexpr int F4 = Stage<P>::F4, K = Stage<P>::K;
+  constexpr int MIN_ACTIVE = 16;                     // fewer active lanes: read from global memory
+  constexpr int R = (F4 + MIN_ACTIVE - 1) / MIN_ACTIVE;   // copy rounds per element at >= MIN_ACTIVE lanes
+  // Only the lanes active here take part: lanes whose ray or search has ended are masked off, so the
+  // copy is spread over the active lanes by their rank among them.
+  const uint64_t active = __ballot (1);
+  const int nact = __popcll (active);
+  if (nact < MIN_ACTIVE) return g;
+  const int lane = __lane_id ();
+  const int rank = __popcll (active & ((1ull << lane) - 1));
+  uint64_t todo = __ballot (el >= 0);
+  int slot = -1;
+  int ids[K];
+  int n = 0;
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+  {
+    ids[s] = -1;
+    if (todo)
+    {
+      const int leader = __ffsll ((unsigned long long)todo) - 1;
+      const int e = __builtin_amdgcn_readlane (el, leader);
+      const uint64_t mine = __ballot (el == e);
+      if (el == e) slot = s;
+      todo &= ~mine;
+      ids[s] = e;
+      n = s + 1;
+    }
+  }
+  if (n == 0) return g;
+  stage_sync ();   // earlier readers of this wavefront's stage are done
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+    if (s < n)
+    {
+      const float4 *src = (const float4 *)(cfg.mesh + (int64_t)ids[s] * Ord<P>::GSTRIDE);
+      float4 *dst = (float4 *)(stage + s * F4 * 4);
+      float4 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+      {
+        const int q = rank + r * nact;
+        if (q < F4) v[r] = src[q];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+      {
+        const int q = rank + r * nact;
+        if (q < F4) dst[q] = v[r];
+      }
+    }
+  stage_sync ();
+  return slot >= 0 ? stage + slot * F4 * 4 : g;
+}
+
 // DeviceMesh::locate (device_mesh.hpp:247-354): depth-first, left child first, right child pushed when
 // the point is in both; the first accepted candidate wins.
 //
@@ -601,7 +679,7 @@ struct TravStack
 template <int P, bool POLISH>
 __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStack &todo, const float pt[3],
                                                   float ref[3], LocateStats &st, bool &overflow, int &cur, int &sp,
-                                                  int &cell)
+                                                  int &cell, float *stage = nullptr)
 {
   while (cur > -1)
   {
@@ -650,7 +728,11 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
   if constexpr (P == 0)
     found = try_element_mixed<POLISH> (cfg, el, pt, x, st);
   else
-    found = try_element_at<P, POLISH> (cfg, cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, pt, x, st);
+  {
+    const float *g = cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE;
+    if (stage) g = stage_element<P> (cfg, el, g, stage);
+    found = try_element_at<P, POLISH> (cfg, g, pt, x, st);
+  }
   if (found)
   {
     ref[0] = x[0];

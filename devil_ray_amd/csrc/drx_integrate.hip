@@ -171,6 +171,8 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
   __shared__ int32_t w_cand[WAVE ? kIntWaves * kCandQ * 64 : 1];
   __shared__ __attribute__ ((aligned (16))) __bf16 mlp_x[LEARNED ? kIntWaves * 64 * mlp::IN_PAD : 8];
   __shared__ __attribute__ ((aligned (16))) float mlp_y[LEARNED ? kIntWaves * 64 * 4 : 4];
+  constexpr bool STAGE = MODE == 0 && P >= 1;   // element blocks staged in LDS for Newton
+  __shared__ __attribute__ ((aligned (16))) float stage_lds[STAGE ? kIntWaves * Stage<(P >= 1 ? P : 1)>::FLOATS : 4];
   WaveScratch ws;
   const int wv = threadIdx.x >> 6;
   {
@@ -260,7 +262,8 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
         if constexpr (MODE == 1)
           r = locate_point_wide_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c);
         else
-          r = locate_point_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c);
+          r = locate_point_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c,
+                                            STAGE ? stage_lds + wv * Stage<(P >= 1 ? P : 1)>::FLOATS : nullptr);
         resume = r == 0;
         if (resume) continue;
         cell = r > 0 ? c : -1;
