@@ -595,10 +595,13 @@ struct TravStack
 // into its LDS area in one cooperative round (one float4 per lane) and returns, per lane, the LDS copy of
 // its element (or its global block when its element did not fit).  Newton then reads control points from
 // LDS instead of re-reading them from the cache hierarchy every iteration; the bits are the same.
+#ifndef DRX_STAGE_K
+#define DRX_STAGE_K 4   // elements of order <= 3 staged per wavefront
+#endif
 template <int P> struct Stage
 {
   static constexpr int F4 = (Ord<P>::GSTRIDE + 3) / 4;   // float4 per packed element block
-  static constexpr int K = P <= 3 ? 4 : 2;               // elements staged per wavefront
+  static constexpr int K = P <= 3 ? DRX_STAGE_K : 2;     // elements staged per wavefront
   static constexpr int FLOATS = K * F4 * 4;              // LDS floats per wavefront
 };
 
