@@ -171,7 +171,10 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
   __shared__ int32_t w_cand[WAVE ? kIntWaves * kCandQ * 64 : 1];
   __shared__ __attribute__ ((aligned (16))) __bf16 mlp_x[LEARNED ? kIntWaves * 64 * mlp::IN_PAD : 8];
   __shared__ __attribute__ ((aligned (16))) float mlp_y[LEARNED ? kIntWaves * 64 * 4 : 4];
-  constexpr bool STAGE = MODE == 0 && P >= 1;   // element blocks staged in LDS for Newton
+  // Element blocks staged in LDS for Newton: order 3 only.  Measured per order (binary walk): order 3
+  // 15.5 -> 15.2 ms on the 64^3 frame; order 4 (two slots fit) 211 -> 234 ms and order 2 on a frame of one
+  // wavefront per SIMD 4.3 -> 5.1 ms, where the copy and the wave synchronisation cost more than they save.
+  constexpr bool STAGE = MODE == 0 && P == 3;
   __shared__ __attribute__ ((aligned (16))) float stage_lds[STAGE ? kIntWaves * Stage<(P >= 1 ? P : 1)>::FLOATS : 4];
   WaveScratch ws;
   const int wv = threadIdx.x >> 6;
