@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy (x2.0, x1.0); the summary with its commands is
 # profiles/k_integrate_p3_64cubed_1024sq_512spp.md.  Keyed by (config, kernel) so that other workloads or
 # kernel variants report null rather than a stale figure.
-PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,false>"): 36.05e9 + 6.50e9}
+PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,false>"): 31.01e9 + 3.37e9}
 
 
 def _cpu_model() -> str:
@@ -124,17 +124,8 @@ def main() -> int:
     torch.cuda.synchronize()
     bvh_ms_warm = e0.elapsed_time(e1)
 
-    vol = api.Volume(m, f)
-    vol.use_lighting = False
-    vol.samples(samples)
-    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
-    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
-    vol.table.add_alpha(0.0, syn.table_alpha_for_corrected(0.02, samples))
-    vol.table.set_smooth(True)
+    vol, cam, centre = _volume_setup(api, syn, m, f, bvh, res, samples)
     vol.opts = api.LocateOptions(fp64_polish=args.polish, cond_threshold=1e4, traversal=args.traversal)
-    c = bvh.bounds
-    centre = (0.5 * (c[0] + c[1]), 0.5 * (c[2] + c[3]), 0.5 * (c[4] + c[5]))
-    cam = api.Camera(res, res, 45.0, pos=(centre[0], centre[1], centre[2] + 2.5), look_at=centre)
     if args.config == 0:
         ortho = api.as_bytes(syn.orthographic_rays(res, res))
 
@@ -240,6 +231,22 @@ def main() -> int:
     out["mesh_generation_s"] = t_gen
     print(json.dumps(out))
     return 0
+
+
+def _volume_setup(api, syn, m, f, bvh, res, samples):
+    """The volume of configs 0, 1 and 4 (unlit, grey -> red transfer function, opacity 0.02 per sample)
+    and the perspective camera at distance 2.5 from the mesh centre, 45 degree field of view."""
+    vol = api.Volume(m, f)
+    vol.use_lighting = False
+    vol.samples(samples)
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, syn.table_alpha_for_corrected(0.02, samples))
+    vol.table.set_smooth(True)
+    c = bvh.bounds
+    centre = (0.5 * (c[0] + c[1]), 0.5 * (c[2] + c[3]), 0.5 * (c[4] + c[5]))
+    cam = api.Camera(res, res, 45.0, pos=(centre[0], centre[1], centre[2] + 2.5), look_at=centre)
+    return vol, cam, centre
 
 
 def _oracle_bvh(bvh):
@@ -442,16 +449,18 @@ def bench_mlp(args):
     # ---- held-out evaluation ----
     hist_len = 41
     agg = {k: {"ms": 0.0, "steps": 0, "hist": np.zeros(hist_len, np.int64), "err": 0.0}
-           for k in ("plain", "plain_wide", "learned")}
+           for k in ("plain", "plain_centre", "plain_wide", "learned")}
     mism_total, n_total, pred_err = 0, 0, []
     for s in range(32, 40):
         mesh = syn.hex_mesh(64, 3, _amp_of_seed(s), 0.01, 1000 + s)
         m = api.UnstructuredMesh(mesh)
         pts_h = syn.query_points(args.mlp_eval_points, m.bounds(), seed=s)
         pts = api.to_device(pts_h.reshape(-1))
-        # plain = the library default (binary walk, as the learned start uses); plain_wide = the 8-wide walk
-        opts = {"plain": api.LocateOptions(), "plain_wide": api.LocateOptions(traversal=1),
-                "learned": api.LocateOptions(mlp=packed)}
+        # plain = the library default (binary walk, as the learned start uses, Newton from the corner-affine
+        # start); plain_centre = the same from the SubRef centre (dray's subref_center start); plain_wide =
+        # the 8-wide walk
+        opts = {"plain": api.LocateOptions(), "plain_centre": api.LocateOptions(start="centre"),
+                "plain_wide": api.LocateOptions(traversal=1), "learned": api.LocateOptions(mlp=packed)}
         res = {}
         for k, o in opts.items():
             for _ in range(args.warmup):
@@ -485,6 +494,41 @@ def bench_mlp(args):
     pe = np.concatenate(pred_err)
     value = n_total / (agg["learned"]["ms"] / 1e3)
     plain_rate = n_total / (agg["plain"]["ms"] / 1e3)
+    centre_rate = n_total / (agg["plain_centre"]["ms"] / 1e3)
+
+    # ---- volume render of the 64^3 order-3 frame (1024^2, 512 samples/ray, FP64 polish on) per start ----
+    res_px, samples = 1024, 512
+    mesh = syn.hex_mesh(64, 3, 0.1, 0.01, 1234)   # warp 0.1; seed outside the training and held-out seeds
+    m, f = api.UnstructuredMesh(mesh), api.UnstructuredField(mesh)
+    vol, cam, _ = _volume_setup(api, syn, m, f, m.get_bvh(), res_px, samples)
+    rays = cam.create_rays()
+    render, imgs = {}, {}
+    starts = {"plain": api.LocateOptions(fp64_polish=True),
+              "plain_centre": api.LocateOptions(fp64_polish=True, start="centre"),
+              "learned": api.LocateOptions(fp64_polish=True, mlp=packed)}
+    for k, o in starts.items():
+        vol.opts = o
+        for _ in range(max(args.warmup, 1)):
+            vol.integrate(rays, tile_width=res_px, tile_height=res_px)
+        ks = []
+        for _ in range(max(args.steps, 1)):
+            parts, cnt = vol.integrate(rays, tile_width=res_px, tile_height=res_px)
+            ks.append(vol.last_stats.kernel_ms)
+        st = vol.last_stats
+        imgs[k] = api.composite(parts, cnt, res_px, res_px).cpu().numpy()
+        k_ms = float(np.mean(ks))
+        render[k] = {"integrate_kernel_ms": k_ms, "located_samples": int(st.located_samples),
+                     "samples_per_s": st.located_samples / (k_ms / 1e3),
+                     "newton_steps_per_located": st.newton_steps / max(1, st.located_samples)}
+        print(f"[config 3] render {k}: {k_ms:.2f} ms", file=sys.stderr, flush=True)
+    for k in render:
+        render[k]["rgba_max_diff_vs_plain"] = float(np.abs(imgs[k] - imgs["plain"]).max())
+    render["speedup_learned_vs_plain_centre"] = (render["plain_centre"]["integrate_kernel_ms"]
+                                                 / render["learned"]["integrate_kernel_ms"])
+    render["speedup_learned_vs_plain"] = render["plain"]["integrate_kernel_ms"] / render["learned"]["integrate_kernel_ms"]
+    render["speedup_plain_vs_plain_centre"] = (render["plain_centre"]["integrate_kernel_ms"]
+                                               / render["plain"]["integrate_kernel_ms"])
+    del m, f, vol
     out = {
         "metric": "learned-init vs plain locate @1e-5 err (held-out seeds)", "value": value, "unit": "queries/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": agg["learned"]["ms"] / 8,
@@ -495,11 +539,13 @@ def bench_mlp(args):
                    "training_pairs": n_pairs, "epochs": args.mlp_epochs, "batch": batch, "train_steps": steps,
                    "eval_points_per_mesh": args.mlp_eval_points, "newton_budget_from_guess": 4},
         "plain_queries_per_s": plain_rate, "speedup_vs_plain": value / plain_rate,
+        "plain_centre_queries_per_s": centre_rate, "speedup_vs_plain_centre": value / centre_rate,
         "plain_wide_queries_per_s": n_total / (agg["plain_wide"]["ms"] / 1e3),
         "newton_steps_per_query": {k: agg[k]["steps"] / n_total for k in agg},
         "newton_steps_hist": {k: agg[k]["hist"].tolist() for k in agg},
         "max_ref_err_vs_fp64": {k: agg[k]["err"] for k in agg},
         "located_id_mismatch_learned_vs_plain": mism_total,
+        "render_64cubed_1024sq_512spp_polish": render,
         "guess_error_inf": {"max": float(pe.max()), "mean": float(pe.mean()), "p99": float(np.percentile(pe, 99)),
                             "frac_below_1e-3": float((pe < 1e-3).mean())},
         "training": {"ms": train_ms, "samples_per_s": train_rate, "mse_first": float(loss[:10].mean()),

@@ -18,6 +18,7 @@ F32 = C.c_float
 
 DRX_OK = 0
 DRX_ERR_ARG = 1
+DRX_INIT_AFFINE, DRX_INIT_MLP, DRX_INIT_SUBREF_CENTRE = 0, 1, 2   # drx_init_mode
 SPLITS_PER_ELEMENT = 33
 MAX_ORDER = 4
 

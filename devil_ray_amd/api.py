@@ -183,10 +183,14 @@ class LocateOptions:
     coherent_order: bool = False  # locate: process queries in Morton order
     mlp: Optional[torch.Tensor] = None   # packed learned initializer (LearnedInitializer.pack())
     mlp_newton_steps: int = 4            # Newton budget from the learned guess before the plain fallback
+    start: str = "affine"   # plain Newton start: "affine" (corner-affine estimate) or "centre" (SubRef centre)
 
     def c(self, steps=None, cands=None) -> L.DrxLocateOpts:
         o = L.DrxLocateOpts()
-        o.init_mode = 0
+        modes = {"affine": L.DRX_INIT_AFFINE, "centre": L.DRX_INIT_SUBREF_CENTRE}
+        if self.start not in modes:
+            raise ValueError(f"LocateOptions.start must be 'affine' or 'centre', not {self.start!r}")
+        o.init_mode = modes[self.start]
         o.fp64_polish = 1 if self.fp64_polish else 0
         o.cond_threshold = float(self.cond_threshold)
         o.newton_steps = steps.data_ptr() if steps is not None else None
@@ -261,11 +265,13 @@ class UnstructuredField:
         vals = mesh.field if values is None else values
         self.host_values = vals
         self.gf = DeviceGridFunction(mesh.ctrl_idx, vals, 1, mesh.order, mesh.orders, mesh.offsets)
+        # the device copy is a snapshot of vals, so its range is fixed here (not recomputed per render)
+        v = np.asarray(vals, np.float32)
+        self._range = (float(v.min()), float(v.max())) if v.size else (float("inf"), float("-inf"))
 
     def range(self):
-        """get_range (unstructured_field.cpp:35-90): min/max over the control values."""
-        v = self.host_values.astype(np.float32)
-        return float(v.min()), float(v.max())
+        """get_range (unstructured_field.cpp:35-90): min/max over the control values (float32)."""
+        return self._range
 
     def eval(self, locations: torch.Tensor, values: Optional[torch.Tensor] = None) -> torch.Tensor:
         n = int(locations.numel() // 16)

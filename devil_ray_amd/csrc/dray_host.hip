@@ -188,7 +188,7 @@ Array<Location> UnstructuredMesh::locate (Array<Vec<Float, 3>> &wpoints)
   drx_bvh bv = bvh_view ();
   Array<Location> locs (wpoints.size ());
   drx_locate_opts o {};
-  o.init_mode = DRX_INIT_SUBREF_CENTRE;
+  o.init_mode = DRX_INIT_AFFINE;
   o.cond_threshold = 1e4f;
   o.traversal = DRX_TRAVERSE_AUTO;
   if (wpoints.size ())

@@ -438,6 +438,7 @@ struct LocateCfg
   const float4 *wide;       // 8-wide nodes, 16 float4 each (nullptr: binary traversal)
   const float4 *leaves;     // packed leaves, 2 float4 each
   const float4 *starts;     // optional Newton start maps, 3 float4 per leaf (drx_bvh_leaf_starts)
+  bool centre;              // start at the SubRef centre (used only when starts is null)
   const float *mesh;        // packed geometry blocks
   const int64_t *offsets;   // mixed order: per-element block offset (nullptr for uniform)
   const int32_t *orders;    // mixed order: per-element order
@@ -539,7 +540,8 @@ __device__ __forceinline__ void corner_start (const float *g, int p, const float
 }
 
 // The Newton start for candidate element `el` of leaf `leaf` (leaf record l0, l1): the leaf's stored
-// start map when the BVH carries one (read in the same round as the leaf record), else corner_start.
+// start map when the BVH carries one (read in the same round as the leaf record), else the SubRef centre
+// when the caller asked for dray's subref_center start (data_model/subref.hpp:52-56), else corner_start.
 template <int P>
 __device__ __forceinline__ void candidate_start (const LocateCfg &cfg, int el, int leaf, const float pt[3],
                                                  const float4 &l0, const float4 &l1, float x[3])
@@ -555,6 +557,12 @@ __device__ __forceinline__ void candidate_start (const LocateCfg &cfg, int el, i
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       x[d] = (v[d] == v[d] && fabsf (v[d]) < 1e30f) ? fminf (hi[d], fmaxf (lo[d], v[d])) : (lo[d] + hi[d]) * 0.5f;
+  }
+  else if (cfg.centre)
+  {
+    x[0] = (l0.y + l1.x) * 0.5f;
+    x[1] = (l0.z + l1.y) * 0.5f;
+    x[2] = (l0.w + l1.z) * 0.5f;
   }
   else if constexpr (P == 0)
     corner_start (cfg.mesh + cfg.offsets[el], cfg.orders[el], pt, l0, l1, x);

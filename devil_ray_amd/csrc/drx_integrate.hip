@@ -377,7 +377,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   if (n_rays < 0 || lut_size < 1 || !lut) return fail (DRX_ERR_ARG, "drx_volume_integrate: bad rays/lut");
   if (!(field_min <= field_max)) return fail (DRX_ERR_ARG, "ColorMap scalar range never set");
   const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
-  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
+  if (opts && opts->init_mode != DRX_INIT_AFFINE && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: unknown init_mode %d", opts->init_mode);
   if (learned && !opts->mlp)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");

@@ -579,7 +579,7 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   if (n == 0) return DRX_OK;
   if (!points || !out_locations) return fail (DRX_ERR_ARG, "drx_locate: null points/output");
   const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
-  if (opts && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
+  if (opts && opts->init_mode != DRX_INIT_AFFINE && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
     return fail (DRX_ERR_ARG, "drx_locate: unknown init_mode %d", opts->init_mode);
   if (learned && !opts->mlp) return fail (DRX_ERR_ARG, "drx_locate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
   hipStream_t st = as_stream (stream);

@@ -93,7 +93,8 @@ static inline LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx
   c.nodes = (const float4 *)b->inner_nodes;
   c.wide = (const float4 *)b->wide_nodes;
   c.leaves = (const float4 *)b->leaf_pack;
-  c.starts = (const float4 *)b->leaf_starts;
+  c.centre = o && o->init_mode == DRX_INIT_SUBREF_CENTRE;
+  c.starts = c.centre ? nullptr : (const float4 *)b->leaf_starts;
   c.mesh = m->data;
   c.offsets = m->elem_offset;
   c.orders = m->elem_order;

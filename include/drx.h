@@ -101,8 +101,13 @@ typedef struct drx_bvh
 
 enum drx_init_mode
 {
-  DRX_INIT_SUBREF_CENTRE = 0, /* Newton starts at the candidate leaf's SubRef centre */
-  DRX_INIT_MLP = 1            /* learned initial guess (opts->mlp), plain SubRef-centre Newton as fallback */
+  DRX_INIT_AFFINE = 0,        /* default: Newton starts at the element's corner-affine estimate clamped into
+                                 the candidate's SubRef (the leaf's start map when bvh->leaf_starts is set,
+                                 drx_bvh_leaf_starts); the SubRef centre where that estimate is singular */
+  DRX_INIT_MLP = 1,           /* learned initial guess (opts->mlp); plain Newton from the DRX_INIT_AFFINE
+                                 start decides acceptance when the guess has not converged */
+  DRX_INIT_SUBREF_CENTRE = 2  /* Newton starts at the candidate's SubRef centre, subref_center
+                                 (data_model/subref.hpp:52-56): dray's start when use_init_guess is false */
 };
 
 typedef struct drx_locate_opts

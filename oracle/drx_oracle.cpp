@@ -12,7 +12,8 @@
 //     src/dray/linear_bvh_builder.cpp:128-129).
 //   * Newton's start (dray passes an uninitialised guess, src/dray/data_model/device_mesh.hpp:327-334
 //     with use_init_guess = true) is the element's corner-affine estimate clamped into the candidate's
-//     SubRef (corner_start below; the SubRef centre where that is singular).
+//     SubRef (corner_start below; the SubRef centre where that is singular), or, after
+//     orc_set_newton_start(2), the SubRef centre itself (subref_center, data_model/subref.hpp:52-56).
 //   * The FP64 instance is a clean FP64 restatement: the clamp and the inf-norm use fmin/fmax in
 //     double (dray's FP64 build still rounds through fminf/fmaxf, src/dray/data_model/element.hpp:449,
 //     src/dray/vec.hpp:229-238).
@@ -655,6 +656,11 @@ static void corner_start (const T *dofs, int p, const T pt[3], const float *sr, 
   }
 }
 
+// Newton start rule of locate_point: 0 = corner_start, 2 = the candidate's SubRef centre, subref_center
+// (data_model/subref.hpp:52-56), which is dray's start when use_init_guess is false.  Set by
+// orc_set_newton_start; read-only while a locate runs.
+static int g_start_mode = 0;
+
 template <typename T>
 static i32 locate_point (const MeshView<T> &mesh, const BVHView &bvh, const T pt[3], T ref_out[3],
                          std::vector<T> &dofs, LocStats *st)
@@ -705,7 +711,10 @@ static i32 locate_point (const MeshView<T> &mesh, const BVHView &bvh, const T pt
       dofs.resize ((size_t)3 * npe_of (p));
       mesh.template gather<3> (el, dofs.data ());
       T x[3];
-      corner_start<T> (dofs.data (), p, pt, sr, x);
+      if (g_start_mode == 2)
+        for (int d = 0; d < 3; ++d) x[d] = T ((sr[d] + sr[3 + d]) * 0.5f);
+      else
+        corner_start<T> (dofs.data (), p, pt, sr, x);
       int steps = 0;
       const bool found = newton_inverse<T> (dofs.data (), p, pt, x, &steps);
       if (st) { st->newton_steps += steps; st->candidates += 1; }
@@ -1133,6 +1142,13 @@ int orc_locate_f64 (const i32 *ctrl_idx, const double *values, const i32 *orders
                     const float *inner, const i32 *leaf_nodes, const i32 *aabb_ids, const float *subrefs,
                     const double *pts, i64 npts, i32 *out_cells, double *out_refs, i32 *out_steps, i64 *stats2, int nthreads)
 { ORC_LOCATE_BODY (double) }
+
+int orc_set_newton_start (int mode)
+{
+  if (mode != 0 && mode != 2) return 1;
+  g_start_mode = mode;
+  return 0;
+}
 
 // Brute force: try every element (no BVH) in index order, Newton from the element centre, first
 // accept wins.  Used on tiny meshes to show the BVH traversal loses no hits.

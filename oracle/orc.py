@@ -34,6 +34,7 @@ _SIGS = {
     "orc_newton_f32": (I32, [P, I32, P, P, P]),
     "orc_newton_f64": (I32, [P, I32, P, P, P]),
     "orc_locate_f32": (I32, [P, P, P, P, I32, I32, P, P, P, P, P, I64, P, P, P, P, I32]),
+    "orc_set_newton_start": (I32, [I32]),
     "orc_locate_f64": (I32, [P, P, P, P, I32, I32, P, P, P, P, P, I64, P, P, P, P, I32]),
     "orc_locate_brute_f64": (I32, [P, P, P, P, I32, I32, P, I64, P, P]),
     "orc_refine_f64": (I32, [P, P, P, P, I32, I32, P, P, P, I64, P, I32]),
@@ -135,6 +136,14 @@ def build_mesh_bvh(mesh, nthreads=0):
     b = bvh_build(aabbs, prims)
     b.aabbs, b.prims, b.subrefs = aabbs, prims, subrefs
     return b
+
+
+def set_newton_start(mode: str):
+    """Newton start rule of locate/integrate: "affine" (corner_start, the default) or "centre" (the
+    candidate's SubRef centre, data_model/subref.hpp:52-56)."""
+    code = {"affine": 0, "centre": 2}.get(mode)
+    if code is None or lib().orc_set_newton_start(code) != 0:
+        raise ValueError(f"unknown Newton start {mode!r}")
 
 
 def locate(mesh, bvh, pts, dtype=np.float32, nthreads=0, want_steps=False):

@@ -243,6 +243,37 @@ def test_locate_affine_closed_form(orc):
     assert np.all(c == -1) and np.all(r == -1)
 
 
+def test_locate_start_modes_on_affine_mesh(orc):
+    """On a lattice (identity-map) mesh the corner-affine start is the answer itself, so each located
+    point takes exactly one Newton step; the SubRef-centre start (subref_center) takes two: the first moves
+    from the centre to the answer (more than the 1e-4 convergence step), the second confirms it.  Both
+    give the closed-form element and reference point."""
+    n = 4
+    mesh = syn.hex_mesh(n, 2, 0.0, 0.0, 0)
+    bvh = orc.build_mesh_bvh(mesh)
+    rng = np.random.default_rng(4)
+    pts = (0.002 + 0.996 * rng.random((3000, 3))).astype(np.float32)
+    q = pts.astype(np.float64) * n
+    idx = np.floor(q).astype(int)
+    frac = q - idx
+    interior = np.all((frac > 2e-4) & (frac < 1 - 2e-4), axis=1)
+    exp_cell = idx[:, 0] + n * (idx[:, 1] + n * idx[:, 2])
+    res = {}
+    try:
+        for mode in ("affine", "centre"):
+            orc.set_newton_start(mode)
+            cells, refs, steps, _ = orc.locate(mesh, bvh, pts, want_steps=True)
+            assert np.array_equal(cells[interior], exp_cell[interior])
+            assert np.abs(refs[interior] - frac[interior]).max() < 1e-5
+            res[mode] = steps[interior]
+    finally:
+        orc.set_newton_start("affine")
+    assert np.all(res["affine"] == 1)
+    assert np.all((res["centre"] == 1) | (res["centre"] == 2)) and (res["centre"] == 2).mean() > 0.99
+    with pytest.raises(ValueError):
+        orc.set_newton_start("mlp")
+
+
 @pytest.mark.parametrize("make", [lambda: syn.hex_mesh(3, 2, 0.05, 0.02, 4),
                                   lambda: syn.hex_mesh(2, 4, 0.25, 0.01, 2),
                                   lambda: syn.mixed_hex_mesh(3, 0.15, 0.05, 8)])
