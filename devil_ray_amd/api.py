@@ -472,6 +472,7 @@ class Volume:
         self.opts = LocateOptions()
         self.last_stats = None
         self.sample_bounds = None   # dray's m_bounds (the Collection's bounds) for the sample distance
+        self._lut = (None, None)    # (colour-table state, device LUT): resampled only when the state changes
 
     def samples(self, n: int):
         self.m_samples = int(n)
@@ -500,7 +501,11 @@ class Volume:
         else:
             active, n_act = active
         lo, hi = self.range if self.range is not None else self.field_.range()
-        lut = to_device(self.table.sample(1024, opacity_factor=10.0 / self.m_samples).reshape(-1))
+        key = (tuple(map(tuple, self.table.rgb)), tuple(map(tuple, self.table.alpha)), self.table.smooth,
+               self.m_samples)
+        if self._lut[0] != key:
+            self._lut = (key, to_device(self.table.sample(1024, opacity_factor=10.0 / self.m_samples).reshape(-1)))
+        lut = self._lut[1]
         sd = sample_distance(bounds if self.sample_bounds is None else self.sample_bounds, self.m_samples)
         order = ray_tile_order(active, n_act, tile_width, tile_height) if (tile_width and n_act) else None
         out = torch.empty(max(5 * n_act, 1) * PARTIAL_DTYPE.itemsize, dtype=torch.uint8, device=_dev())

@@ -452,7 +452,7 @@ struct LocateStats
   int newton;
   int candidates;
   int polished;
-  int fallbacks;   // learned-start Newton runs that did not converge and were redone from the SubRef centre
+  int fallbacks;   // learned-start Newton runs that did not converge and were redone from the plain start
 };
 
 template <int P, bool POLISH>

@@ -201,6 +201,42 @@ def test_locate_centre_start_parity(gpu, orc, name, make):
     assert ost.sum() > 1.1 * ost_affine.sum()
 
 
+def test_fp64_polish_fires(gpu, orc):
+    """With cond_threshold below 1 every accepted point takes the two FP64 Newton corrections: located
+    cells are unchanged, and reference points match the FP64 oracle (iterated to 1e-14 on the same float32
+    control points) to float32 output rounding, closer than the FP32 solution.  In a render every located
+    sample is polished."""
+    mesh = syn.hex_mesh(3, 4, 0.25, 0.01, 3)
+    print("[fp64 polish] p4_n3 warp 0.25", flush=True)
+    m, bvh = _gpu_bvh(gpu, mesh)
+    pts = syn.query_points(40000, bvh.bounds, seed=13)
+    d = gpu.to_device(pts.reshape(-1))
+    l32 = _np(m.locate(d, gpu.LocateOptions()), LOCATION_DTYPE)
+    l64 = _np(m.locate(d, gpu.LocateOptions(fp64_polish=True, cond_threshold=0.5)), LOCATION_DTYPE)
+    assert np.array_equal(l32["cell"], l64["cell"])
+    idx = np.nonzero(l32["cell"] >= 0)[0]
+    ex = orc.refine_f64(mesh, pts[idx], l32["cell"][idx], l32["ref"][idx].astype(np.float64))
+    e32 = np.abs(l32["ref"][idx].astype(np.float64) - ex).max(axis=1)
+    e64 = np.abs(l64["ref"][idx].astype(np.float64) - ex).max(axis=1)
+    print(f"  located={len(idx)} fp32 max {e32.max():.3g} mean {e32.mean():.3g}; "
+          f"polished max {e64.max():.3g} mean {e64.mean():.3g}", flush=True)
+    assert e64.max() < 1e-7
+    assert e64.mean() < e32.mean() and e32.max() > 2 * e64.max()
+    f = gpu.UnstructuredField(mesh)
+    vol = gpu.Volume(m, f)
+    vol.use_lighting = False
+    vol.samples(64)
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, 0.05)
+    vol.opts = gpu.LocateOptions(fp64_polish=True, cond_threshold=0.5)
+    rays = orc.camera_perspective(32, 32, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5))
+    vol.integrate(gpu.as_bytes(rays), tile_width=32, tile_height=32)
+    st = vol.last_stats
+    print(f"  render: located {st.located_samples} polished {st.polished}", flush=True)
+    assert st.located_samples > 0 and st.polished == st.located_samples
+
+
 @pytest.mark.parametrize("name,make", MESHES[:4], ids=[m[0] for m in MESHES[:4]])
 def test_volume_integrate_parity(gpu, orc, name, make):
     mesh = make()
