@@ -30,7 +30,8 @@ class DrxMesh(C.Structure):
 
 class DrxBVH(C.Structure):
     _fields_ = [("inner_nodes", P), ("leaf_nodes", P), ("aabb_ids", P), ("subrefs", P), ("leaf_pack", P),
-                ("wide_nodes", P), ("n_leaves", I32), ("bounds", F32 * 6), ("leaf_starts", P)]
+                ("wide_nodes", P), ("n_leaves", I32), ("bounds", F32 * 6), ("leaf_starts", P), ("qnodes", P),
+                ("leaf_boxes", P), ("qgrid", F32 * 6)]
 
 
 class DrxLocateOpts(C.Structure):
@@ -77,6 +78,7 @@ _SIGS = {
     "drx_bvh_build": (I32, [P, P, I32, P, P, P, P, P, P]),
     "drx_bvh_pack_leaves": (I32, [C.POINTER(DrxBVH), P, P]),
     "drx_bvh_leaf_starts": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, P]),
+    "drx_bvh_quantize": (I32, [C.POINTER(DrxBVH), P, P, P, P]),
     "drx_bvh_widen": (I32, [C.POINTER(DrxBVH), P, I64, C.POINTER(I64), P]),
     "drx_locate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I64, P, C.POINTER(DrxLocateOpts), P]),
     "drx_field_eval": (I32, [C.POINTER(DrxMesh), P, I64, P, P]),

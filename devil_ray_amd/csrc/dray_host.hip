@@ -150,6 +150,11 @@ BVH &UnstructuredMesh::get_bvh ()
     m_bvh.m_leaf_starts.resize (12 * (size_t)m_bvh.m_leaf_nodes.size ());
     const drx_mesh mv = device_view ();
     drx_check (drx_bvh_leaf_starts (&mv, &v, m_bvh.m_leaf_starts.get_device_ptr (), nullptr), "drx_bvh_leaf_starts");
+    m_bvh.m_qnodes.resize (8 * (size_t)(m_bvh.m_leaf_nodes.size () - 1));
+    m_bvh.m_leaf_boxes.resize (8 * (size_t)m_bvh.m_leaf_nodes.size ());
+    drx_check (drx_bvh_quantize (&v, m_bvh.m_qnodes.get_device_ptr (), m_bvh.m_leaf_boxes.get_device_ptr (),
+                                 m_bvh.m_qgrid, nullptr),
+               "drx_bvh_quantize");
     int64_t cnt = 0;
     v = bvh_view ();
     drx_check (drx_bvh_widen (&v, nullptr, 0, &cnt, nullptr), "drx_bvh_widen");
@@ -162,7 +167,7 @@ BVH &UnstructuredMesh::get_bvh ()
 
 drx_bvh UnstructuredMesh::bvh_view ()
 {
-  drx_bvh v;
+  drx_bvh v {};
   v.inner_nodes = (const float *)m_bvh.m_inner_nodes.get_device_ptr_const ();
   v.leaf_nodes = m_bvh.m_leaf_nodes.get_device_ptr_const ();
   v.aabb_ids = m_bvh.m_aabb_ids.get_device_ptr_const ();
@@ -176,6 +181,12 @@ drx_bvh UnstructuredMesh::bvh_view ()
     v.bounds[2 * d + 1] = m_bvh.m_bounds.m_ranges[d].m_max;
   }
   v.leaf_starts = m_bvh.m_leaf_starts.size () ? m_bvh.m_leaf_starts.get_device_ptr_const () : nullptr;
+  if (m_bvh.m_qnodes.size ())
+  {
+    v.qnodes = m_bvh.m_qnodes.get_device_ptr_const ();
+    v.leaf_boxes = m_bvh.m_leaf_boxes.get_device_ptr_const ();
+    for (int q = 0; q < 6; ++q) v.qgrid[q] = m_bvh.m_qgrid[q];
+  }
   return v;
 }
 

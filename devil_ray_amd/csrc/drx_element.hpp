@@ -432,9 +432,22 @@ __device__ __forceinline__ void polish_f64 (const float *__restrict__ g, const f
   for (int i = 0; i < 3; ++i) x[i] = (float)xd[i];
 }
 
+// Grid cell of a coordinate for the compact traversal nodes (drx_bvh_quantize): monotone in v, so a
+// point inside a box lies inside the box's cell range.  The same expression quantizes the boxes and the
+// query points.
+__host__ __device__ __forceinline__ uint32_t grid_cell (float v, float origin, float cells_per_unit)
+{
+#pragma clang fp contract(off)
+  const float c = floorf ((v - origin) * cells_per_unit);
+  return c > 0.f ? (c < 65535.f ? (uint32_t)c : 65535u) : 0u;
+}
+
 struct LocateCfg
 {
   const float4 *nodes;      // inner nodes, 4 float4 each
+  const uint4 *qnodes;      // optional compact inner nodes, 2 uint4 each (then leaf_boxes and the grid are set)
+  const float4 *leaf_boxes; // with qnodes: exact leaf boxes, 2 float4 each
+  float q_org[3], q_inv[3]; // with qnodes: grid origin and cells per unit length
   const float4 *wide;       // 8-wide nodes, 16 float4 each (nullptr: binary traversal)
   const float4 *leaves;     // packed leaves, 2 float4 each
   const float4 *starts;     // optional Newton start maps, 3 float4 per leaf (drx_bvh_leaf_starts)
@@ -692,45 +705,86 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
                                                   float ref[3], LocateStats &st, bool &overflow, int &cur, int &sp,
                                                   int &cell, float *stage = nullptr)
 {
-  while (cur > -1)
+  // compact nodes: the point's grid cells, compared with each child box's cell range
+  const bool compact = cfg.qnodes != nullptr;
+  uint32_t qp[3] = { 0u, 0u, 0u };
+  if (compact)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) qp[d] = grid_cell (pt[d], cfg.q_org[d], cfg.q_inv[d]);
+  int leaf;
+  for (;;)
   {
-    const float4 *nd = cfg.nodes + cur;
-    const float4 a = nd[0], b = nd[1], c = nd[2], ch = nd[3];
-    bool in_l = true, in_r = true;
-    if (pt[0] < a.x) in_l = false;
-    if (pt[1] < a.y) in_l = false;
-    if (pt[2] < a.z) in_l = false;
-    if (pt[0] > a.w) in_l = false;
-    if (pt[1] > b.x) in_l = false;
-    if (pt[2] > b.y) in_l = false;
-    if (pt[0] < b.z) in_r = false;
-    if (pt[1] < b.w) in_r = false;
-    if (pt[2] < c.x) in_r = false;
-    if (pt[0] > c.y) in_r = false;
-    if (pt[1] > c.z) in_r = false;
-    if (pt[2] > c.w) in_r = false;
-    if (!in_l && !in_r)
+    while (cur > -1)
     {
-      cur = todo.get (sp);
-      --sp;
-    }
-    else
-    {
-      const int lc = __float_as_int (ch.x), rc = __float_as_int (ch.y);
-      cur = in_l ? lc : rc;
-      if (in_l && in_r)
+      bool in_l = true, in_r = true;
+      int lc, rc;
+      if (compact)
       {
-        if (sp + 1 >= kStackDepth)
+        const uint4 A = cfg.qnodes[cur], B = cfg.qnodes[cur + 1];
+        if (qp[0] < (A.x & 0xFFFFu)) in_l = false;
+        if (qp[1] < (A.x >> 16)) in_l = false;
+        if (qp[2] < (A.y & 0xFFFFu)) in_l = false;
+        if (qp[0] > (A.y >> 16)) in_l = false;
+        if (qp[1] > (A.z & 0xFFFFu)) in_l = false;
+        if (qp[2] > (A.z >> 16)) in_l = false;
+        if (qp[0] < (A.w & 0xFFFFu)) in_r = false;
+        if (qp[1] < (A.w >> 16)) in_r = false;
+        if (qp[2] < (B.x & 0xFFFFu)) in_r = false;
+        if (qp[0] > (B.x >> 16)) in_r = false;
+        if (qp[1] > (B.y & 0xFFFFu)) in_r = false;
+        if (qp[2] > (B.y >> 16)) in_r = false;
+        lc = (int)B.z;
+        rc = (int)B.w;
+      }
+      else
+      {
+        const float4 *nd = cfg.nodes + cur;
+        const float4 a = nd[0], b = nd[1], c = nd[2], ch = nd[3];
+        if (pt[0] < a.x) in_l = false;
+        if (pt[1] < a.y) in_l = false;
+        if (pt[2] < a.z) in_l = false;
+        if (pt[0] > a.w) in_l = false;
+        if (pt[1] > b.x) in_l = false;
+        if (pt[2] > b.y) in_l = false;
+        if (pt[0] < b.z) in_r = false;
+        if (pt[1] < b.w) in_r = false;
+        if (pt[2] < c.x) in_r = false;
+        if (pt[0] > c.y) in_r = false;
+        if (pt[1] > c.z) in_r = false;
+        if (pt[2] > c.w) in_r = false;
+        lc = __float_as_int (ch.x);
+        rc = __float_as_int (ch.y);
+      }
+      if (!in_l && !in_r)
+      {
+        cur = todo.get (sp);
+        --sp;
+      }
+      else
+      {
+        cur = in_l ? lc : rc;
+        if (in_l && in_r)
         {
-          overflow = true;
-          return -1;
+          if (sp + 1 >= kStackDepth)
+          {
+            overflow = true;
+            return -1;
+          }
+          todo.set (++sp, rc);
         }
-        todo.set (++sp, rc);
       }
     }
+    if (cur == kBarrier) return -1;
+    leaf = -cur - 1;
+    if (!compact) break;
+    // the cell ranges are conservative: the leaf's exact box (dray's child-box test) decides
+    const float4 b0 = cfg.leaf_boxes[2 * (int64_t)leaf], b1 = cfg.leaf_boxes[2 * (int64_t)leaf + 1];
+    const bool in = !(pt[0] < b0.x) && !(pt[1] < b0.y) && !(pt[2] < b0.z) && !(pt[0] > b0.w) && !(pt[1] > b1.x) &&
+                    !(pt[2] > b1.y);
+    if (in) break;
+    cur = todo.get (sp);
+    --sp;
   }
-  if (cur == kBarrier) return -1;
-  const int leaf = -cur - 1;
   const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
   const int el = __float_as_int (l0.x);
   float x[3];

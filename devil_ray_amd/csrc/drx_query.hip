@@ -95,6 +95,47 @@ __global__ void k_leaf_starts (const float *__restrict__ mesh, const int64_t *__
   }
 }
 
+// Compact traversal nodes (drx_bvh_quantize): one thread per inner node quantizes both child boxes to
+// grid cells and records the exact box of each child that is a leaf.
+__global__ void k_quantize_nodes (const float4 *__restrict__ inner, int n_inner, float ox, float oy, float oz, float ix,
+                                  float iy, float iz, uint4 *__restrict__ qn, float4 *__restrict__ leaf_boxes)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inner) return;
+  const float4 a = inner[4 * (int64_t)i], b = inner[4 * (int64_t)i + 1], c = inner[4 * (int64_t)i + 2],
+               ch = inner[4 * (int64_t)i + 3];
+  const float lo[2][3] = { { a.x, a.y, a.z }, { b.z, b.w, c.x } };
+  const float hi[2][3] = { { a.w, b.x, b.y }, { c.y, c.z, c.w } };
+  const float org[3] = { ox, oy, oz }, inv[3] = { ix, iy, iz };
+  uint32_t w[6];
+  for (int k = 0; k < 2; ++k)
+  {
+    uint32_t ql[3], qh[3];
+    for (int d = 0; d < 3; ++d)
+    {
+      ql[d] = grid_cell (lo[k][d], org[d], inv[d]);
+      qh[d] = grid_cell (hi[k][d], org[d], inv[d]);
+    }
+    w[3 * k] = ql[0] | (ql[1] << 16);
+    w[3 * k + 1] = ql[2] | (qh[0] << 16);
+    w[3 * k + 2] = qh[1] | (qh[2] << 16);
+  }
+  const int child[2] = { __float_as_int (ch.x), __float_as_int (ch.y) };
+  int q[2];
+  for (int k = 0; k < 2; ++k)
+  {
+    q[k] = child[k] >= 0 ? child[k] / 2 : child[k];   // float4 offset 4 * index -> uint4 offset 2 * index
+    if (child[k] < 0)
+    {
+      const int64_t leaf = -(int64_t)child[k] - 1;
+      leaf_boxes[2 * leaf] = make_float4 (lo[k][0], lo[k][1], lo[k][2], hi[k][0]);
+      leaf_boxes[2 * leaf + 1] = make_float4 (hi[k][1], hi[k][2], 0.f, 0.f);
+    }
+  }
+  qn[2 * (int64_t)i] = make_uint4 (w[0], w[1], w[2], w[3]);
+  qn[2 * (int64_t)i + 1] = make_uint4 (w[4], w[5], (uint32_t)q[0], (uint32_t)q[1]);
+}
+
 // Per-lane walks, streamed: each wavefront owns `chunk` consecutive entries of the processing order and
 // every iteration advances each lane's point by one candidate (locate_point_step); a lane whose point is
 // settled (accepted or exhausted) writes its Location and takes the chunk's next unclaimed point.  The
@@ -565,6 +606,29 @@ int drx_bvh_leaf_starts (const drx_mesh *mesh, const drx_bvh *bvh, float *out_le
   hipLaunchKernelGGL (k_leaf_starts, dim3 (nblk (bvh->n_leaves, 256)), dim3 (256), 0, as_stream (stream), mesh->data,
                       mesh->elem_offset, mesh->elem_order, mesh->stride, mesh->order, bvh->leaf_nodes, bvh->n_leaves,
                       (float4 *)out_leaf_starts);
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_bvh_quantize (const drx_bvh *bvh, uint32_t *out_qnodes, float *out_leaf_boxes, float *out_grid, void *stream)
+{
+  if (!bvh || !bvh->inner_nodes || bvh->n_leaves < 2 || !out_qnodes || !out_leaf_boxes || !out_grid)
+    return fail (DRX_ERR_ARG, "drx_bvh_quantize: bvh needs inner_nodes and >= 2 leaves; outputs required");
+  // 65,000 cells over the bounds plus a 200-cell margin on each side
+  for (int d = 0; d < 3; ++d)
+  {
+    const float lo = bvh->bounds[2 * d], hi = bvh->bounds[2 * d + 1];
+    if (!(lo <= hi) || !std::isfinite (lo) || !std::isfinite (hi))
+      return fail (DRX_ERR_ARG, "drx_bvh_quantize: bounds not finite");
+    const double ext = (double)hi - (double)lo;
+    const double cell = ext > 0.0 ? ext / 65000.0 : 1.0;
+    out_grid[d] = (float)((double)lo - 200.0 * cell);
+    out_grid[3 + d] = (float)(1.0 / cell);
+  }
+  const int n_inner = bvh->n_leaves - 1;
+  hipLaunchKernelGGL (k_quantize_nodes, dim3 (nblk (n_inner, 256)), dim3 (256), 0, as_stream (stream),
+                      (const float4 *)bvh->inner_nodes, n_inner, out_grid[0], out_grid[1], out_grid[2], out_grid[3],
+                      out_grid[4], out_grid[5], (uint4 *)out_qnodes, (float4 *)out_leaf_boxes);
   DRX_LAUNCH_CHECK ();
   return DRX_OK;
 }

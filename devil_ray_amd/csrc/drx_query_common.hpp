@@ -93,6 +93,14 @@ static inline LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx
   c.nodes = (const float4 *)b->inner_nodes;
   c.wide = (const float4 *)b->wide_nodes;
   c.leaves = (const float4 *)b->leaf_pack;
+  c.qnodes = (const uint4 *)b->qnodes;
+  c.leaf_boxes = (const float4 *)b->leaf_boxes;
+  for (int d = 0; d < 3; ++d)
+  {
+    c.q_org[d] = b->qgrid[d];
+    c.q_inv[d] = b->qgrid[3 + d];
+  }
+  if (c.qnodes && !c.leaf_boxes) c.qnodes = nullptr;
   c.centre = o && o->init_mode == DRX_INIT_SUBREF_CENTRE;
   c.starts = c.centre ? nullptr : (const float4 *)b->leaf_starts;
   c.mesh = m->data;

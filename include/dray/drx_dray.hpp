@@ -31,6 +31,7 @@ namespace dray
 {
 
 typedef int32_t int32;
+typedef uint32_t uint32;
 typedef float float32;
 typedef double float64;
 typedef float Float;
@@ -161,6 +162,9 @@ struct BVH
   // traversal copies (libdrx): fused leaf records and the 8-wide collapse
   Array<float32> m_leaf_pack;
   Array<float32> m_leaf_starts;   // Newton start map per leaf (drx_bvh_leaf_starts)
+  Array<uint32> m_qnodes;         // compact traversal nodes (drx_bvh_quantize)
+  Array<float32> m_leaf_boxes;    // exact leaf boxes for the compact walk
+  float32 m_qgrid[6] = { 0, 0, 0, 0, 0, 0 };
   Array<float32> m_wide_nodes;
 };
 

@@ -93,6 +93,10 @@ typedef struct drx_bvh
   int32_t n_leaves;
   float bounds[6];            /* host copy of m_bounds: xmin, xmax, ymin, ymax, zmin, zmax */
   const float *leaf_starts;   /* optional, 12 floats per leaf: Newton start map (drx_bvh_leaf_starts) */
+  const uint32_t *qnodes;     /* optional, 8 words per inner node: compact traversal nodes (drx_bvh_quantize);
+                                 binary walks use them, with leaf_boxes and qgrid, when set */
+  const float *leaf_boxes;    /* with qnodes: 8 floats per leaf, its exact box {lo xyz, hi x, hi yz, 0, 0} */
+  float qgrid[6];             /* with qnodes: grid origin xyz, grid cells per unit length xyz */
 } drx_bvh;
 
 /* Location (location.hpp:16-21): {int32 cell (-1 = not found), float ref[3]} = 16 bytes. */
@@ -227,6 +231,16 @@ int drx_bvh_pack_leaves (const drx_bvh *bvh, float *out_leaf_pack, void *stream)
  * if NaN).  Optional: without it the same start is computed from the element's corners at each
  * candidate, one more dependent memory access. */
 int drx_bvh_leaf_starts (const drx_mesh *mesh, const drx_bvh *bvh, float *out_leaf_starts, void *stream);
+/* Compact traversal nodes (no dray counterpart; dray walks the 64-byte nodes above, bvh.hpp:25-38,
+ * device_mesh.hpp:262-320).  Per inner node 8 uint32: {L lo x | L lo y << 16, L lo z | L hi x << 16,
+ * L hi y | L hi z << 16, R lo x | R lo y << 16, R lo z | R hi x << 16, R hi y | R hi z << 16, lchild, rchild},
+ * where each box corner is its cell on a 16-bit grid over the bounds, cell(v) = clamp(floor((v - origin)
+ * * cells_per_unit), 0, 65535) in float32; lchild/rchild >= 0: 2 * inner index (uint4 offset), < 0: leaf
+ * -(child) - 1.  Since cell() is monotone, a point in a box lies in its cell range, so a walk that tests
+ * cells visits every subtree dray visits; it tests each leaf's exact float32 box (out_leaf_boxes, 8 floats
+ * per leaf) before Newton, so the candidates and their order are dray's.  out_grid (host, 6 floats):
+ * origin xyz, cells per unit xyz; copy it into drx_bvh.qgrid.  Needs inner_nodes, n_leaves, bounds. */
+int drx_bvh_quantize (const drx_bvh *bvh, uint32_t *out_qnodes, float *out_leaf_boxes, float *out_grid, void *stream);
 /* 8-wide traversal copy of the binary BVH: children in binary left-to-right order, exact float boxes,
  * so traversal visits the same candidate leaves in the same order as dray's binary depth-first walk.
  * Node = 64 floats: lo_x[8] lo_y[8] lo_z[8] hi_x[8] hi_y[8] hi_z[8] child[8] (int32 bits) pad[8].

@@ -237,6 +237,95 @@ def test_fp64_polish_fires(gpu, orc):
     assert st.located_samples > 0 and st.polished == st.located_samples
 
 
+@pytest.mark.parametrize("name,make", MESHES, ids=[m[0] for m in MESHES])
+def test_compact_nodes_same_candidates(gpu, name, make):
+    """drx_bvh_quantize: the binary walk over the 32-byte compact nodes (cell ranges, then each leaf's exact
+    box) gives dray's candidates in dray's order: identical cells, reference points (bitwise), Newton
+    steps and candidate counts per point to the walk over dray's 64-byte nodes."""
+    mesh = make()
+    print(f"[compact nodes] {name}", flush=True)
+    m, bvh = _gpu_bvh(gpu, mesh)
+    assert bvh.qnodes is not None
+    pts = syn.query_points(60000, bvh.bounds, seed=21)
+    d = gpu.to_device(pts.reshape(-1))
+    out = {}
+    for use in (False, True):
+        bvh.use_qnodes = use
+        loc, steps, cands = m.locate(d, gpu.LocateOptions(), stats=True)
+        out[use] = (_np(loc, LOCATION_DTYPE), _np(steps), _np(cands))
+    bvh.use_qnodes = True
+    (a, sa, ca), (b, sb, cb) = out[False], out[True]
+    assert np.array_equal(a["cell"], b["cell"])
+    assert np.array_equal(a["ref"].view(np.uint32), b["ref"].view(np.uint32))
+    assert np.array_equal(sa, sb) and np.array_equal(ca, cb)
+    print(f"  located {(a['cell'] >= 0).sum()} candidates {ca.sum()}", flush=True)
+
+
+def test_compact_nodes_layout(gpu, orc):
+    """Compact node k: the cells of both child boxes of dray's node k under cell(v) = clamp(floor((v - origin)
+    * cells_per_unit)), children as uint4 offsets (2 * inner index) or dray's leaf code; every leaf's exact
+    box; and points drawn inside each child box fall inside its cell range."""
+    mesh = syn.hex_mesh(4, 2, 0.05, 0.005, 1, jitter_scaled=False)
+    m, bvh = _gpu_bvh(gpu, mesh)
+    inner = _np(bvh.inner_nodes).reshape(-1, 16)
+    q = _np(bvh.qnodes).view(np.uint32).reshape(-1, 8)
+    boxes = _np(bvh.leaf_boxes).reshape(-1, 8)
+    org, inv = bvh.qgrid[:3].astype(np.float32), bvh.qgrid[3:].astype(np.float32)
+
+    def cell(v, dd):
+        c = np.floor((np.float32(v) - org[dd]).astype(np.float32) * inv[dd]).astype(np.float32)
+        return np.clip(c, 0, 65535).astype(np.uint32)
+
+    lo = [inner[:, 0:3], inner[:, 6:9]]
+    hi = [inner[:, 3:6], inner[:, 9:12]]
+    ch = inner[:, 12:14].view(np.int32)
+    for k in range(2):
+        ql = [cell(lo[k][:, dd], dd) for dd in range(3)]
+        qh = [cell(hi[k][:, dd], dd) for dd in range(3)]
+        w = q[:, 3 * k:3 * k + 3]
+        assert np.array_equal(w[:, 0], ql[0] | (ql[1] << 16))
+        assert np.array_equal(w[:, 1], ql[2] | (qh[0] << 16))
+        assert np.array_equal(w[:, 2], qh[1] | (qh[2] << 16))
+        c = ch[:, k]
+        assert np.array_equal(q[:, 6 + k].view(np.int32), np.where(c >= 0, c // 2, c))
+        leaf = -c[c < 0] - 1
+        ex = np.concatenate([lo[k][c < 0], hi[k][c < 0]], axis=1)
+        assert np.array_equal(boxes[leaf][:, :6].view(np.uint32), ex.view(np.uint32))
+        # containment: random points inside each child box (and its corners) land in its cell range
+        rng = np.random.default_rng(k)
+        t = rng.random((len(inner), 3)).astype(np.float32)
+        for frac in (np.zeros_like(t), np.ones_like(t), t):
+            p = (lo[k] + frac * (hi[k] - lo[k])).astype(np.float32)
+            p = np.minimum(np.maximum(p, lo[k]), hi[k])
+            for dd in range(3):
+                cp = cell(p[:, dd], dd)
+                assert np.all(ql[dd] <= cp) and np.all(cp <= qh[dd])
+    assert len(np.unique(-ch[ch < 0] - 1)) == bvh.n_leaves
+
+
+def test_compact_nodes_render_identical(gpu, orc):
+    """The order-3 render (the stage-in-LDS kernel) over compact nodes is bit-identical to dray's nodes."""
+    mesh = syn.hex_mesh(5, 3, 0.1, 0.01, 2)
+    m, bvh = _gpu_bvh(gpu, mesh)
+    f = gpu.UnstructuredField(mesh)
+    vol = gpu.Volume(m, f)
+    vol.use_lighting = False
+    vol.samples(128)
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, 0.05)
+    rays = gpu.as_bytes(orc.camera_perspective(64, 64, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)))
+    res = {}
+    for use in (False, True):
+        bvh.use_qnodes = use
+        parts, n = vol.integrate(rays, tile_width=64, tile_height=64)
+        st = vol.last_stats
+        res[use] = (_np(parts, np.uint8)[: n * 24].copy(), st.located_samples, st.newton_steps, st.candidates)
+    bvh.use_qnodes = True
+    assert np.array_equal(res[False][0], res[True][0])
+    assert res[False][1:] == res[True][1:]
+
+
 @pytest.mark.parametrize("name,make", MESHES[:4], ids=[m[0] for m in MESHES[:4]])
 def test_volume_integrate_parity(gpu, orc, name, make):
     mesh = make()
