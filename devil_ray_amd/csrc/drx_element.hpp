@@ -691,19 +691,16 @@ __device__ __forceinline__ const float *stage_element (const LocateCfg &cfg, int
   return slot >= 0 ? stage + slot * F4 * 4 : g;
 }
 
-// DeviceMesh::locate (device_mesh.hpp:247-354): depth-first, left child first, right child pushed when
-// the point is in both; the first accepted candidate wins.
-//
-// Resumable form, one candidate per call: cur / sp carry this lane's depth-first state between calls
-// (start a search with cur = 0, sp = 0 and kBarrier in stack entry 0).  The lane walks inner nodes until
-// it holds a leaf (lanes that get there early wait, so the Newton solve runs with the wave converged),
-// then tests that one candidate.  Returns 1 if it was accepted (ref, cell set), 0 if it was rejected (the
-// state then holds the next stack entry: call again with the same point), -1 when the search is
-// exhausted or the 64-entry stack (dray's capacity) would overflow (*overflow set).
-template <int P, bool POLISH>
-__device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStack &todo, const float pt[3],
-                                                  float ref[3], LocateStats &st, bool &overflow, int &cur, int &sp,
-                                                  int &cell, float *stage = nullptr)
+constexpr int kWalkDone = -1, kWalkOverflow = -2;
+
+// DeviceMesh::locate's traversal (device_mesh.hpp:262-320) from the state (cur, sp) to the next candidate
+// leaf: inner nodes are visited depth-first, left child first, the right child pushed when the point is
+// in both child boxes.  Over the compact nodes (cfg.qnodes) the point's grid cells are compared with
+// each child's cell range, a superset of dray's box test, and each reached leaf's exact box then decides,
+// so the leaves returned are dray's candidates in dray's order.  Returns the leaf index with cur = its
+// code, kWalkDone when the stack is exhausted, kWalkOverflow when the 64-entry stack would overflow.
+__device__ __forceinline__ int next_candidate_leaf (const LocateCfg &cfg, TravStack &todo, const float pt[3], int &cur,
+                                                    int &sp)
 {
   // compact nodes: the point's grid cells, compared with each child box's cell range
   const bool compact = cfg.qnodes != nullptr;
@@ -767,14 +764,13 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
         {
           if (sp + 1 >= kStackDepth)
           {
-            overflow = true;
-            return -1;
+            return kWalkOverflow;
           }
           todo.set (++sp, rc);
         }
       }
     }
-    if (cur == kBarrier) return -1;
+    if (cur == kBarrier) return kWalkDone;
     leaf = -cur - 1;
     if (!compact) break;
     // the cell ranges are conservative: the leaf's exact box (dray's child-box test) decides
@@ -784,6 +780,29 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
     if (in) break;
     cur = todo.get (sp);
     --sp;
+  }
+  return leaf;
+}
+
+// DeviceMesh::locate (device_mesh.hpp:247-354): depth-first, left child first, right child pushed when
+// the point is in both; the first accepted candidate wins.
+//
+// Resumable form, one candidate per call: cur / sp carry this lane's depth-first state between calls
+// (start a search with cur = 0, sp = 0 and kBarrier in stack entry 0).  The lane walks inner nodes until
+// it holds a leaf (lanes that get there early wait, so the Newton solve runs with the wave converged),
+// then tests that one candidate.  Returns 1 if it was accepted (ref, cell set), 0 if it was rejected (the
+// state then holds the next stack entry: call again with the same point), -1 when the search is
+// exhausted or the 64-entry stack (dray's capacity) would overflow (*overflow set).
+template <int P, bool POLISH>
+__device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStack &todo, const float pt[3],
+                                                  float ref[3], LocateStats &st, bool &overflow, int &cur, int &sp,
+                                                  int &cell, float *stage = nullptr)
+{
+  const int leaf = next_candidate_leaf (cfg, todo, pt, cur, sp);
+  if (leaf < 0)
+  {
+    if (leaf == kWalkOverflow) overflow = true;
+    return -1;
   }
   const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
   const int el = __float_as_int (l0.x);

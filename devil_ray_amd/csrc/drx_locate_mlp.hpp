@@ -1,8 +1,9 @@
 // Point location with the learned Newton initializer.  Same traversal and candidate order as
 // locate_point (DeviceMesh::locate, device_mesh.hpp:247-354); at each candidate the wavefront runs the
 // MLP for all its lanes at once (MFMA needs the whole wave), then each lane runs Newton from its
-// learned guess with a small step budget and, if that run has not converged, the plain Newton from the
-// SubRef centre, which also decides acceptance in that case.
+// learned guess with a small step budget and, if that run has not converged, the plain method's Newton
+// from its own start (candidate_start), which also decides acceptance in that case.  The walk is the
+// plain method's (next_candidate_leaf, over the compact nodes when the BVH has them).
 #pragma once
 
 #include "drx_element.hpp"
@@ -73,47 +74,12 @@ __device__ __forceinline__ int locate_point_mlp (const LocateCfg &cfg, TravStack
   int result = -1;
   for (;;)
   {
+    int leaf = -1;
     if (!done)
     {
-      while (cur > -1)
-      {
-        const float4 *nd = cfg.nodes + cur;
-        const float4 a = nd[0], b = nd[1], c = nd[2], ch = nd[3];
-        bool in_l = true, in_r = true;
-        if (pt[0] < a.x) in_l = false;
-        if (pt[1] < a.y) in_l = false;
-        if (pt[2] < a.z) in_l = false;
-        if (pt[0] > a.w) in_l = false;
-        if (pt[1] > b.x) in_l = false;
-        if (pt[2] > b.y) in_l = false;
-        if (pt[0] < b.z) in_r = false;
-        if (pt[1] < b.w) in_r = false;
-        if (pt[2] < c.x) in_r = false;
-        if (pt[0] > c.y) in_r = false;
-        if (pt[1] > c.z) in_r = false;
-        if (pt[2] > c.w) in_r = false;
-        if (!in_l && !in_r)
-        {
-          cur = todo.get (sp);
-          --sp;
-        }
-        else
-        {
-          const int lc = __float_as_int (ch.x), rc = __float_as_int (ch.y);
-          cur = in_l ? lc : rc;
-          if (in_l && in_r)
-          {
-            if (sp + 1 >= kStackDepth)
-            {
-              overflow = true;
-              cur = kBarrier;
-              break;
-            }
-            todo.set (++sp, rc);
-          }
-        }
-      }
-      if (cur == kBarrier) done = true;
+      leaf = next_candidate_leaf (cfg, todo, pt, cur, sp);
+      if (leaf == kWalkOverflow) overflow = true;
+      if (leaf < 0) done = true;
     }
     const bool has = !done;
     if (__ballot (has) == 0) break;
@@ -122,7 +88,6 @@ __device__ __forceinline__ int locate_point_mlp (const LocateCfg &cfg, TravStack
     int el = -1, order = P;
     if (has)
     {
-      const int leaf = -cur - 1;
       const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
       el = __float_as_int (l0.x);
       candidate_start<P> (cfg, el, leaf, pt, l0, l1, xc);   // the fallback start of the plain method
