@@ -665,7 +665,8 @@ class LearnedInitializer:
     """Learned Newton initial guess: MLP 27 -> 64 x4 (GELU) -> 3, BF16 on MFMA (drx_mlp_* in libdrx).
 
     Not part of dray: dray starts Newton from LocateHack::eval_inverse's guess
-    (data_model/device_mesh.hpp:131-165); plain SubRef-centre Newton stays the fallback.
+    (data_model/device_mesh.hpp:131-165).  Where Newton from the learned guess has not converged within its
+    budget, the plain method's Newton from its own start (the corner-affine map) decides.
     """
 
     def __init__(self, seed: int = 0):

@@ -21,7 +21,7 @@ __device__ __forceinline__ bool try_element_learned (const LocateCfg &cfg, const
   bool found = false;
   st.candidates += 1;
   // pass 0: from the learned guess within `budget` steps; pass 1 (only if pass 0 did not converge):
-  // dray's Newton from the SubRef centre with the full 20-step budget
+  // the plain method's Newton from its own start xc (candidate_start) with dray's 20-step budget
 #pragma unroll 1
   for (int pass = 0; pass < 2; ++pass)
   {
