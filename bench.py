@@ -318,6 +318,10 @@ def bench_locate(args):
     variants = {"binary": api.LocateOptions(traversal=3), "morton+binary": api.LocateOptions(traversal=3, coherent_order=True),
                 "morton+wide": api.LocateOptions(traversal=1, coherent_order=True),
                 "morton+wave": api.LocateOptions(traversal=2, coherent_order=True)}
+    # autotuner (untimed): chunk size x launch form of the streamed walks, recorded as their defaults
+    tune = {k: m.locate_autotune(pts, variants[k]) for k in ("morton+binary", "morton+wide")}
+    print(f"[config 2] autotune {json.dumps({k: (v['launch'], v['chunk']) for k, v in tune.items()})}",
+          file=sys.stderr, flush=True)
     timings = {}
     locs = {}
     for name, opt in variants.items():
@@ -341,6 +345,7 @@ def bench_locate(args):
         "config": {"workload": "batched locate, 2^26 points, 48^3 mixed-order (1..4) hex mesh, warp 0.15",
                    "elements": 48 ** 3, "points": n_pts, "bvh_leaves": bvh.n_leaves, "best_variant": best},
         "ms_per_variant": {k: 1000 * v for k, v in timings.items()}, "variants_identical": same,
+        "autotune": tune,
         "hit_fraction": float((ref["cell"] >= 0).mean()),
         "roofline": {"bound": "hbm", "achieved": value * (28 + 12 * 56) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": value * (28 + 12 * 56) / 1e9 / HBM_PEAK_GBS, "traffic": None,

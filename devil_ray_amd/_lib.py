@@ -37,7 +37,7 @@ class DrxBVH(C.Structure):
 class DrxLocateOpts(C.Structure):
     _fields_ = [("init_mode", I32), ("fp64_polish", I32), ("cond_threshold", F32), ("newton_steps", P),
                 ("candidates", P), ("traversal", I32), ("coherent_order", I32), ("mlp", P),
-                ("mlp_newton_steps", I32)]
+                ("mlp_newton_steps", I32), ("launch", I32), ("chunk", I32)]
 
 
 class DrxMlpTrainOpts(C.Structure):
@@ -79,6 +79,8 @@ _SIGS = {
     "drx_bvh_pack_leaves": (I32, [C.POINTER(DrxBVH), P, P]),
     "drx_bvh_leaf_starts": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, P]),
     "drx_bvh_quantize": (I32, [C.POINTER(DrxBVH), P, P, P, P]),
+    "drx_locate_autotune": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I64, P, C.POINTER(DrxLocateOpts),
+                                  C.POINTER(I32), C.POINTER(I32), P, P]),
     "drx_bvh_widen": (I32, [C.POINTER(DrxBVH), P, I64, C.POINTER(I64), P]),
     "drx_locate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I64, P, C.POINTER(DrxLocateOpts), P]),
     "drx_field_eval": (I32, [C.POINTER(DrxMesh), P, I64, P, P]),

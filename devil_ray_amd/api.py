@@ -206,6 +206,8 @@ class LocateOptions:
     mlp: Optional[torch.Tensor] = None   # packed learned initializer (LearnedInitializer.pack())
     mlp_newton_steps: int = 4            # Newton budget from the learned guess before the plain fallback
     start: str = "affine"   # plain Newton start: "affine" (corner-affine estimate) or "centre" (SubRef centre)
+    launch: int = 0         # streamed walks: 0 tuned (drx_locate_autotune) else chunked, 1 chunked, 2 persistent
+    chunk: int = 0          # points per wavefront chunk (0: tuned, else 1024)
 
     def c(self, steps=None, cands=None) -> L.DrxLocateOpts:
         o = L.DrxLocateOpts()
@@ -219,6 +221,8 @@ class LocateOptions:
         o.candidates = cands.data_ptr() if cands is not None else None
         o.traversal = int(self.traversal)
         o.coherent_order = 1 if self.coherent_order else 0
+        o.launch = int(self.launch)
+        o.chunk = int(self.chunk)
         if self.mlp is not None:
             o.init_mode = 1
             o.mlp = self.mlp.data_ptr()
@@ -279,6 +283,23 @@ class UnstructuredMesh:
         L.check(L.lib().drx_locate(C.byref(mv), C.byref(bv), _ptr(points), n, _ptr(out), C.byref(o), _stream()),
                 "drx_locate")
         return (out, steps, cands) if stats else out
+
+    def locate_autotune(self, points: torch.Tensor, opts: Optional[LocateOptions] = None) -> dict:
+        """drx_locate_autotune: time every chunk size x launch form of the streamed walk on these points and
+        options, and make the fastest the default (launch 0, chunk 0) for this device, order, polish and walk."""
+        n = int(points.numel() // 3)
+        out = torch.empty(max(n, 1) * 16, dtype=torch.uint8, device=_dev())
+        o = (opts or LocateOptions()).c()
+        launch, chunk = C.c_int32(0), C.c_int32(0)
+        ms = (C.c_float * 10)()
+        mv, bv = self.gf.view(), self.get_bvh().view()
+        L.check(L.lib().drx_locate_autotune(C.byref(mv), C.byref(bv), _ptr(points), n, _ptr(out), C.byref(o),
+                                            C.byref(launch), C.byref(chunk), C.cast(ms, C.c_void_p), _stream()),
+                "drx_locate_autotune")
+        chunks = (256, 512, 1024, 2048, 4096)
+        return {"launch": {1: "chunked", 2: "persistent"}[launch.value], "chunk": chunk.value,
+                "ms": {f"{'persistent' if l else 'chunked'}/{c}": float(ms[2 * i + l])
+                       for i, c in enumerate(chunks) for l in (0, 1)}}
 
 
 class UnstructuredField:

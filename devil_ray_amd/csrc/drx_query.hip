@@ -136,30 +136,39 @@ __global__ void k_quantize_nodes (const float4 *__restrict__ inner, int n_inner,
   qn[2 * (int64_t)i + 1] = make_uint4 (w[4], w[5], (uint32_t)q[0], (uint32_t)q[1]);
 }
 
-// Per-lane walks, streamed: each wavefront owns `chunk` consecutive entries of the processing order and
-// every iteration advances each lane's point by one candidate (locate_point_step); a lane whose point is
-// settled (accepted or exhausted) writes its Location and takes the chunk's next unclaimed point.  The
-// wavefront therefore never waits for the lanes whose points need several candidates.  Same candidate
-// sequence and results as one lane per point.
-constexpr int kLocateChunk = 1024;
+// Per-lane walks, streamed: each wavefront works through `chunk` consecutive entries of the processing
+// order and every iteration advances each lane's point by one candidate (locate_point_step); a lane whose
+// point is settled (accepted or exhausted) writes its Location and takes the next unclaimed entry.  The
+// wavefront therefore never waits for the lanes whose points need several candidates.  Without a queue
+// wavefront b owns chunk b; with one (persistent launch, a grid of about the resident wavefronts) each
+// wavefront takes the next chunk from the queue counter whenever its current chunk is used up, so its
+// lanes refill across chunks until the last chunk is gone.  Same candidate sequence and results as one
+// lane per point, whatever the chunk size and launch form.
+constexpr int kLocateChunk = 1024;   // entries per chunk unless tuned (drx_locate_autotune)
 
 template <int P, bool POLISH, int MODE>
 __global__ void __launch_bounds__ (64, waves_per_simd (P, POLISH, MODE))
 k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const int32_t *__restrict__ order,
                  Location *__restrict__ out, int32_t *__restrict__ steps_out, int32_t *__restrict__ cand_out,
-                 int32_t *__restrict__ err)
+                 int32_t *__restrict__ err, int chunk, int32_t *__restrict__ queue)
 {
   __shared__ int32_t stack_lds[kStackLds * 64];
   const int lane = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * kLocateChunk;
-  const int64_t end = base + kLocateChunk < n ? base + kLocateChunk : n;
   TravStack todo;
   todo.lds = stack_lds;
   todo.bdim = 64;
   todo.tid = lane;
-  int64_t claimed = base + 64;   // wave-uniform: next unclaimed entry of the chunk
-  int64_t k = base + lane;
-  bool have = k < end;
+  // wave-uniform pool of unclaimed entries [claimed, end); `more`: further chunks may come from the queue
+  int64_t claimed = 0, end = 0;
+  bool more = true;
+  if (!queue)
+  {
+    claimed = (int64_t)blockIdx.x * chunk;
+    end = claimed + chunk < n ? claimed + chunk : n;
+    more = false;
+  }
+  int64_t k = 0;
+  bool have = false;
   int64_t i = 0;
   float pt[3] = { 0.f, 0.f, 0.f };
   int cur = 0, sp = 0;
@@ -174,11 +183,43 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
     if constexpr (MODE == 0) todo.set (0, kBarrier);
     st = LocateStats { 0, 0, 0, 0 };
   };
-  if (have) start ();
+  bool done = true;   // every lane starts without a point
   for (;;)
   {
+    // refill: lanes without a point take the pool's next entries in lane order, fetching chunks from the
+    // queue while the pool is empty (at most 64 / chunk + 1 fetches for one refill)
+    uint64_t m = __ballot (done);
+    while (m)
+    {
+      if (claimed >= end)
+      {
+        if (!more) break;
+        int c = 0;
+        if (lane == 0) c = atomicAdd (queue, 1);
+        c = __builtin_amdgcn_readfirstlane (c);
+        claimed = (int64_t)c * chunk;
+        if (claimed >= n)
+        {
+          more = false;
+          break;
+        }
+        end = claimed + chunk < n ? claimed + chunk : n;
+      }
+      const int64_t avail = end - claimed;
+      const int rank = __popcll (m & ((1ull << lane) - 1));
+      const int take = __popcll (m) < avail ? __popcll (m) : (int)avail;
+      if (((m >> lane) & 1ull) && rank < take)
+      {
+        k = claimed + rank;
+        have = true;
+        start ();
+      }
+      claimed += take;
+      // drop the `take` lowest lanes of m
+      for (int t = 0; t < take; ++t) m &= m - 1;
+    }
     if (__ballot (have) == 0) break;
-    bool done = false;
+    done = false;
     if (have)
     {
       float ref[3];
@@ -192,6 +233,7 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
       if (r != 0)
       {
         done = true;
+        have = false;
         if (overflow) atomicOr (err, 1);
         const bool hit = r > 0;
         Location loc;
@@ -204,15 +246,6 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
         if (cand_out) cand_out[i] = st.candidates;
       }
     }
-    // refill: settled lanes take the next entries of the chunk, in lane order
-    const uint64_t m = __ballot (done);
-    if (done)
-    {
-      k = claimed + __popcll (m & ((1ull << lane) - 1));
-      have = k < end;
-      if (have) start ();
-    }
-    claimed += __popcll (m);
   }
 }
 
@@ -592,6 +625,40 @@ __global__ void k_background (float4 *__restrict__ img, int npix, float4 bg)
 
 } // namespace drx
 
+namespace drx
+{
+namespace
+{
+// Launch settings drx_locate_autotune picked, per device, order (0: mixed), FP64 polish and walk (0 binary,
+// 1 8-wide); chunk 0 = not tuned.  Written by the tuner, read by drx_locate on the same host thread.
+struct LocateTune
+{
+  int launch;
+  int chunk;
+};
+constexpr int kTuneDevices = 16;
+LocateTune g_tune[kTuneDevices][5][2][2];
+int g_cus[kTuneDevices];
+
+int compute_units (int dev)
+{
+  if (dev < 0 || dev >= kTuneDevices) return 256;
+  if (!g_cus[dev])
+  {
+    int v = 0;
+    g_cus[dev] = (hipDeviceGetAttribute (&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
+  }
+  return g_cus[dev];
+}
+
+int stream_walk (const drx_bvh *bvh, const drx_locate_opts *opts)
+{
+  const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
+  return (trav == DRX_TRAVERSE_LANE && bvh->wide_nodes) ? 1 : 0;
+}
+} // namespace
+} // namespace drx
+
 using namespace drx;
 
 extern "C" {
@@ -646,16 +713,46 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   if (opts && opts->init_mode != DRX_INIT_AFFINE && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
     return fail (DRX_ERR_ARG, "drx_locate: unknown init_mode %d", opts->init_mode);
   if (learned && !opts->mlp) return fail (DRX_ERR_ARG, "drx_locate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
+  int launch = opts ? opts->launch : DRX_LAUNCH_AUTO;
+  int chunk = opts ? opts->chunk : 0;
+  if (launch < DRX_LAUNCH_AUTO || launch > DRX_LAUNCH_PERSISTENT || chunk < 0)
+    return fail (DRX_ERR_ARG, "drx_locate: launch %d / chunk %d", launch, chunk);
   hipStream_t st = as_stream (stream);
   int32_t *err = (int32_t *)arena (SLOT_FLAGS, 64);
   if (!err) return DRX_ERR_HIP;
-  hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, err, (int64_t)1, 0);
+  int32_t *queue = err + 1;   // chunk counter of the persistent launch
+  hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, err, (int64_t)2, 0);
   const LocateCfg cfg = make_cfg (mesh, bvh, opts);
   int32_t *steps = opts ? opts->newton_steps : nullptr;
   int32_t *cands = opts ? opts->candidates : nullptr;
   const int P = mesh->elem_order ? 0 : mesh->order;
+  int dev = 0;
+  DRX_HIP (hipGetDevice (&dev));
+  const int walk = stream_walk (bvh, opts);
+  if ((launch == DRX_LAUNCH_AUTO || chunk == 0) && dev >= 0 && dev < kTuneDevices)
+  {
+    const LocateTune &t = g_tune[dev][P][cfg.polish ? 1 : 0][walk];
+    if (t.chunk > 0)
+    {
+      if (launch == DRX_LAUNCH_AUTO) launch = t.launch;
+      if (chunk == 0) chunk = t.chunk;
+    }
+  }
+  if (launch == DRX_LAUNCH_AUTO) launch = DRX_LAUNCH_CHUNKED;
+  if (chunk == 0) chunk = kLocateChunk;
+  if (chunk < 64 || chunk > (1 << 20)) return fail (DRX_ERR_ARG, "drx_locate: chunk %d outside [64, 2^20]", chunk);
+  const int64_t n_chunks = (n + chunk - 1) / chunk;
+  if (n_chunks > (1ll << 31) - 1) return fail (DRX_ERR_ARG, "drx_locate: too many chunks");
+  int64_t grid_s = n_chunks;
+  if (launch == DRX_LAUNCH_PERSISTENT)
+  {
+    const int64_t resident = (int64_t)compute_units (dev) * 4 * waves_per_simd (P, cfg.polish, walk);
+    grid_s = n_chunks < resident ? n_chunks : resident;
+  }
+  else
+    queue = nullptr;
   const dim3 g (nblk (n, 256)), b (256);
-  const dim3 gs ((unsigned)((n + kLocateChunk - 1) / kLocateChunk)), bs (64);
+  const dim3 gs ((unsigned)grid_s), bs (64);
   Location *out = (Location *)out_locations;
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !cfg.wide)
@@ -679,9 +776,11 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
     else if (mode == 2)                                                                             \
       hipLaunchKernelGGL ((k_locate<PP, PO, 2>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
     else if (mode == 1)                                                                             \
-      hipLaunchKernelGGL ((k_locate_stream<PP, PO, 1>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err); \
+      hipLaunchKernelGGL ((k_locate_stream<PP, PO, 1>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err, \
+                          chunk, queue);                                                            \
     else                                                                                            \
-      hipLaunchKernelGGL ((k_locate_stream<PP, PO, 0>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err); \
+      hipLaunchKernelGGL ((k_locate_stream<PP, PO, 0>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err, \
+                          chunk, queue);                                                            \
   } while (0)
   if (cfg.polish)
     switch (P)
@@ -707,6 +806,72 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   DRX_HIP (hipMemcpyAsync (&herr, err, sizeof (int32_t), hipMemcpyDeviceToHost, st));
   DRX_HIP (hipStreamSynchronize (st));
   if (herr) return fail (DRX_ERR_OVERFLOW, "drx_locate: BVH traversal stack (%d entries) overflowed", kStackDepth);
+  return DRX_OK;
+}
+
+int drx_locate_autotune (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
+                         void *out_locations, const drx_locate_opts *opts, int32_t *out_launch, int32_t *out_chunk,
+                         float *out_ms, void *stream)
+{
+  if (!out_launch || !out_chunk) return fail (DRX_ERR_ARG, "drx_locate_autotune: null output");
+  if (n <= 0) return fail (DRX_ERR_ARG, "drx_locate_autotune: no points");
+  if (opts && (opts->init_mode == DRX_INIT_MLP || opts->traversal == DRX_TRAVERSE_WAVE))
+    return fail (DRX_ERR_ARG, "drx_locate_autotune: only the streamed walks (binary, 8-wide per lane) are tuned");
+  int rc = check_mesh (mesh, 3, "drx_locate_autotune");
+  if (rc) return rc;
+  if ((rc = check_bvh (bvh, "drx_locate_autotune"))) return rc;
+  drx_locate_opts o {};
+  if (opts)
+    o = *opts;
+  else
+  {
+    o.cond_threshold = 1e4f;
+    o.traversal = DRX_TRAVERSE_AUTO;
+  }
+  int dev = 0;
+  DRX_HIP (hipGetDevice (&dev));
+  hipStream_t st = as_stream (stream);
+  static const int kChunks[5] = { 256, 512, 1024, 2048, 4096 };
+  hipEvent_t e0, e1;
+  DRX_HIP (hipEventCreate (&e0));
+  if (hipEventCreate (&e1) != hipSuccess)
+  {
+    (void)hipEventDestroy (e0);
+    return fail (DRX_ERR_HIP, "drx_locate_autotune: hipEventCreate");
+  }
+  float best = INFINITY;
+  int best_launch = DRX_LAUNCH_CHUNKED, best_chunk = kLocateChunk;
+  for (int c = 0; c < 5 && rc == DRX_OK; ++c)
+    for (int l = 0; l < 2 && rc == DRX_OK; ++l)
+    {
+      o.launch = l ? DRX_LAUNCH_PERSISTENT : DRX_LAUNCH_CHUNKED;
+      o.chunk = kChunks[c];
+      rc = drx_locate (mesh, bvh, points, n, out_locations, &o, stream);   // warm-up
+      if (rc) break;
+      if (hipEventRecord (e0, st) != hipSuccess) rc = fail (DRX_ERR_HIP, "drx_locate_autotune: hipEventRecord");
+      for (int r = 0; r < 2 && rc == DRX_OK; ++r) rc = drx_locate (mesh, bvh, points, n, out_locations, &o, stream);
+      float ms = 0.f;
+      if (rc == DRX_OK && (hipEventRecord (e1, st) != hipSuccess || hipEventSynchronize (e1) != hipSuccess ||
+                           hipEventElapsedTime (&ms, e0, e1) != hipSuccess))
+        rc = fail (DRX_ERR_HIP, "drx_locate_autotune: event timing");
+      if (rc) break;
+      ms *= 0.5f;
+      if (out_ms) out_ms[2 * c + l] = ms;
+      if (ms < best)
+      {
+        best = ms;
+        best_launch = o.launch;
+        best_chunk = o.chunk;
+      }
+    }
+  (void)hipEventDestroy (e0);
+  (void)hipEventDestroy (e1);
+  if (rc) return rc;
+  const int P = mesh->elem_order ? 0 : mesh->order;
+  const bool polish = o.fp64_polish != 0;
+  if (dev >= 0 && dev < kTuneDevices) g_tune[dev][P][polish ? 1 : 0][stream_walk (bvh, &o)] = LocateTune { best_launch, best_chunk };
+  *out_launch = best_launch;
+  *out_chunk = best_chunk;
   return DRX_OK;
 }
 

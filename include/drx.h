@@ -126,7 +126,17 @@ typedef struct drx_locate_opts
   const void *mlp;            /* DRX_INIT_MLP: packed initializer (drx_mlp_pack), device */
   int32_t mlp_newton_steps;   /* DRX_INIT_MLP: Newton budget from the learned guess before the plain
                                  fallback (<= 0: 4) */
+  int32_t launch;             /* drx_launch: how drx_locate's streamed walks (binary, 8-wide per lane) are
+                                 launched; the results are the same either way */
+  int32_t chunk;              /* points per wavefront chunk for those walks (0: tuned, else 1024) */
 } drx_locate_opts;
+
+enum drx_launch
+{
+  DRX_LAUNCH_AUTO = 0,        /* the form drx_locate_autotune picked for this device, order and walk, else chunked */
+  DRX_LAUNCH_CHUNKED = 1,     /* one wavefront per chunk of points */
+  DRX_LAUNCH_PERSISTENT = 2   /* about one wavefront per resident slot; wavefronts take chunks from a queue */
+};
 
 /* Training schedule of the learned initializer (drx_mlp_train): Adam, cosine decay lr -> lr_final over
  * total_steps.  Step k reads rows [sample0 + k*batch, +batch) of the data set, wrapping around. */
@@ -250,6 +260,16 @@ int drx_bvh_widen (const drx_bvh *bvh, float *out_wide, int64_t capacity_nodes, 
 /* ---- queries ------------------------------------------------------------------------------- */
 int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
                 void *out_locations, const drx_locate_opts *opts, void *stream);
+/* Autotuner for drx_locate's streamed walks (no dray counterpart; dray launches one thread per point,
+ * unstructured_mesh.cpp:123-148): times every chunk size in {256, 512, 1024, 2048, 4096} with each launch
+ * form (one warm-up and two timed calls each, HIP events on `stream`) on the caller's points and options,
+ * and records the fastest for the current device, the mesh's order, opts->fp64_polish and the walk, where
+ * DRX_LAUNCH_AUTO and chunk 0 then use it.  out_locations (n Locations) is overwritten.  out_launch,
+ * out_chunk: the choice; out_ms (nullable, 10 floats): ms per call of each candidate, chunk-major, chunked
+ * before persistent.  Not for the learned start or the wave walk (DRX_ERR_ARG). */
+int drx_locate_autotune (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
+                         void *out_locations, const drx_locate_opts *opts, int32_t *out_launch, int32_t *out_chunk,
+                         float *out_ms, void *stream);
 int drx_field_eval (const drx_mesh *field, const void *locations, int64_t n, float *out_values, void *stream);
 
 /* ---- rendering ----------------------------------------------------------------------------- */

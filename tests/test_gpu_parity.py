@@ -326,6 +326,48 @@ def test_compact_nodes_render_identical(gpu, orc):
     assert res[False][1:] == res[True][1:]
 
 
+@pytest.mark.parametrize("name,make", [MESHES[2], MESHES[4]], ids=["p3_n5", "mixed_n4"])
+def test_locate_launch_forms_identical(gpu, name, make):
+    """drx_locate's streamed walks give identical locations, Newton steps and candidate counts for every
+    chunk size and launch form (chunked; persistent with a chunk queue), for both walks and with Morton
+    order, on a point count that is no multiple of the chunk."""
+    mesh = make()
+    print(f"[launch forms] {name}", flush=True)
+    m, bvh = _gpu_bvh(gpu, mesh)
+    pts = syn.query_points(40037, bvh.bounds, seed=31)
+    d = gpu.to_device(pts.reshape(-1))
+    for trav, coherent in ((0, False), (1, False), (0, True)):
+        ref = None
+        for launch, chunk in ((1, 1024), (1, 64), (1, 256), (1, 4096), (2, 64), (2, 1024), (2, 4096)):
+            opts = gpu.LocateOptions(traversal=trav, coherent_order=coherent, launch=launch, chunk=chunk)
+            loc, steps, cands = m.locate(d, opts, stats=True)
+            got = (_np(loc), _np(steps), _np(cands))
+            if ref is None:
+                ref = got
+            else:
+                assert all(np.array_equal(a, b) for a, b in zip(ref, got)), (trav, coherent, launch, chunk)
+    for bad in (gpu.LocateOptions(launch=3), gpu.LocateOptions(chunk=32), gpu.LocateOptions(chunk=-1)):
+        with pytest.raises(gpu.L.DrxError):
+            m.locate(d, bad)
+
+
+def test_locate_autotune(gpu):
+    """drx_locate_autotune times the 10 candidates, records the fastest as the default for this order and
+    walk, and locate with the default options then returns the same results as before tuning."""
+    mesh = syn.mixed_hex_mesh(4, 0.15, 0.05, 5)
+    m, bvh = _gpu_bvh(gpu, mesh)
+    d = gpu.to_device(syn.query_points(200000, bvh.bounds, seed=41).reshape(-1))
+    before = _np(m.locate(d))
+    t = m.locate_autotune(d)
+    print(f"[autotune] {t}", flush=True)
+    assert t["launch"] in ("chunked", "persistent") and t["chunk"] in (256, 512, 1024, 2048, 4096)
+    assert len(t["ms"]) == 10 and all(v > 0 for v in t["ms"].values())
+    assert t["ms"][f"{t['launch']}/{t['chunk']}"] == min(t["ms"].values())
+    assert np.array_equal(before, _np(m.locate(d)))
+    with pytest.raises(gpu.L.DrxError):
+        m.locate_autotune(d, gpu.LocateOptions(traversal=2))
+
+
 @pytest.mark.parametrize("name,make", MESHES[:4], ids=[m[0] for m in MESHES[:4]])
 def test_volume_integrate_parity(gpu, orc, name, make):
     mesh = make()
