@@ -451,6 +451,16 @@ def bench_mlp(args):
     inf_tflops = inf_rate * MLP_FLOP / 1e12
     del X, T
 
+    # ---- launch tuning (untimed): chunk size x launch form for the plain and learned walks ----
+    mesh = syn.hex_mesh(64, 3, _amp_of_seed(32), 0.01, 1000 + 32)
+    m = api.UnstructuredMesh(mesh)
+    pts = api.to_device(syn.query_points(args.mlp_eval_points, m.bounds(), seed=32).reshape(-1))
+    tune = {k: m.locate_autotune(pts, o) for k, o in (("plain", api.LocateOptions()),
+                                                       ("learned", api.LocateOptions(mlp=packed)))}
+    print(f"[config 3] autotune {json.dumps({k: (v['launch'], v['chunk']) for k, v in tune.items()})}",
+          file=sys.stderr, flush=True)
+    del m, pts
+
     # ---- held-out evaluation ----
     hist_len = 41
     agg = {k: {"ms": 0.0, "steps": 0, "hist": np.zeros(hist_len, np.int64), "err": 0.0}
@@ -550,6 +560,7 @@ def bench_mlp(args):
         "newton_steps_hist": {k: agg[k]["hist"].tolist() for k in agg},
         "max_ref_err_vs_fp64": {k: agg[k]["err"] for k in agg},
         "located_id_mismatch_learned_vs_plain": mism_total,
+        "autotune": tune,
         "render_64cubed_1024sq_512spp_polish": render,
         "guess_error_inf": {"max": float(pe.max()), "mean": float(pe.mean()), "p99": float(np.percentile(pe, 99)),
                             "frac_below_1e-3": float((pe < 1e-3).mean())},

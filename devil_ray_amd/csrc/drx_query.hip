@@ -146,8 +146,57 @@ __global__ void k_quantize_nodes (const float4 *__restrict__ inner, int n_inner,
 // lane per point, whatever the chunk size and launch form.
 constexpr int kLocateChunk = 1024;   // entries per chunk unless tuned (drx_locate_autotune)
 
+// Wave-uniform pool of unclaimed processing-order entries [claimed, end) for the streamed kernels; `more`:
+// further chunks may come from the queue.
+struct EntryPool
+{
+  int64_t claimed, end, n;
+  int chunk;
+  int32_t *queue;
+  bool more;
+
+  __device__ __forceinline__ EntryPool (int64_t n_, int chunk_, int32_t *queue_)
+      : claimed (0), end (0), n (n_), chunk (chunk_), queue (queue_), more (queue_ != nullptr)
+  {
+    if (!queue)
+    {
+      claimed = (int64_t)blockIdx.x * chunk;
+      end = claimed + chunk < n ? claimed + chunk : n;
+    }
+  }
+
+  // The lanes in m (wave-uniform mask of lanes without a point) take the next entries in lane order,
+  // fetching chunks from the queue while the pool is empty; a lane that gets entry k calls take(k).
+  template <typename F> __device__ __forceinline__ void refill (uint64_t m, int lane, F &&take)
+  {
+    while (m)
+    {
+      if (claimed >= end)
+      {
+        if (!more) break;
+        int c = 0;
+        if (lane == 0) c = atomicAdd (queue, 1);
+        c = __builtin_amdgcn_readfirstlane (c);
+        claimed = (int64_t)c * chunk;
+        if (claimed >= n)
+        {
+          more = false;
+          break;
+        }
+        end = claimed + chunk < n ? claimed + chunk : n;
+      }
+      const int64_t avail = end - claimed;
+      const int rank = __popcll (m & ((1ull << lane) - 1));
+      const int cnt = __popcll (m) < avail ? __popcll (m) : (int)avail;
+      if (((m >> lane) & 1ull) && rank < cnt) take (claimed + rank);
+      claimed += cnt;
+      for (int t = 0; t < cnt; ++t) m &= m - 1;   // drop the cnt lowest lanes of m
+    }
+  }
+};
+
 template <int P, bool POLISH, int MODE>
-__global__ void __launch_bounds__ (64, waves_per_simd (P, POLISH, MODE))
+__global__ void __launch_bounds__ (64, locate_waves_per_simd (P, POLISH, MODE))
 k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const int32_t *__restrict__ order,
                  Location *__restrict__ out, int32_t *__restrict__ steps_out, int32_t *__restrict__ cand_out,
                  int32_t *__restrict__ err, int chunk, int32_t *__restrict__ queue)
@@ -158,15 +207,7 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
   todo.lds = stack_lds;
   todo.bdim = 64;
   todo.tid = lane;
-  // wave-uniform pool of unclaimed entries [claimed, end); `more`: further chunks may come from the queue
-  int64_t claimed = 0, end = 0;
-  bool more = true;
-  if (!queue)
-  {
-    claimed = (int64_t)blockIdx.x * chunk;
-    end = claimed + chunk < n ? claimed + chunk : n;
-    more = false;
-  }
+  EntryPool pool (n, chunk, queue);
   int64_t k = 0;
   bool have = false;
   int64_t i = 0;
@@ -186,38 +227,12 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
   bool done = true;   // every lane starts without a point
   for (;;)
   {
-    // refill: lanes without a point take the pool's next entries in lane order, fetching chunks from the
-    // queue while the pool is empty (at most 64 / chunk + 1 fetches for one refill)
-    uint64_t m = __ballot (done);
-    while (m)
-    {
-      if (claimed >= end)
-      {
-        if (!more) break;
-        int c = 0;
-        if (lane == 0) c = atomicAdd (queue, 1);
-        c = __builtin_amdgcn_readfirstlane (c);
-        claimed = (int64_t)c * chunk;
-        if (claimed >= n)
-        {
-          more = false;
-          break;
-        }
-        end = claimed + chunk < n ? claimed + chunk : n;
-      }
-      const int64_t avail = end - claimed;
-      const int rank = __popcll (m & ((1ull << lane) - 1));
-      const int take = __popcll (m) < avail ? __popcll (m) : (int)avail;
-      if (((m >> lane) & 1ull) && rank < take)
-      {
-        k = claimed + rank;
-        have = true;
-        start ();
-      }
-      claimed += take;
-      // drop the `take` lowest lanes of m
-      for (int t = 0; t < take; ++t) m &= m - 1;
-    }
+    // refill: lanes without a point take the next entries of the processing order
+    pool.refill (__ballot (done), lane, [&] (int64_t e) {
+      k = e;
+      have = true;
+      start ();
+    });
     if (__ballot (have) == 0) break;
     done = false;
     if (have)
@@ -249,52 +264,69 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
   }
 }
 
-// Mesh::locate with the learned initial guess (binary per-lane traversal).  Lanes past n stay in the
-// kernel as idle participants of their wavefront's MLP rounds.
+// Mesh::locate with the learned initial guess, streamed like k_locate_stream: one wavefront per block, one
+// candidate per lane per iteration with a wavefront-wide MLP round for the lanes that hold one, settled
+// lanes refilled from the chunk (or the queue).  Same candidates, guesses and results as one lane per
+// point: a lane's MLP output is its own MFMA column, independent of the other lanes in the round.
 template <int P, bool POLISH>
-__global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, 3)) k_locate_mlp (LocateCfg cfg, MlpNet net, int budget,
-                                                                       const float *__restrict__ pts, int64_t n,
-                                                                       const int32_t *__restrict__ order,
-                                                                       Location *__restrict__ out,
-                                                                       int32_t *__restrict__ steps_out,
-                                                                       int32_t *__restrict__ cand_out,
-                                                                       int32_t *__restrict__ err)
+__global__ void __launch_bounds__ (64, locate_waves_per_simd (P, POLISH, 3))
+k_locate_mlp_stream (LocateCfg cfg, MlpNet net, int budget, const float *__restrict__ pts, int64_t n,
+                     const int32_t *__restrict__ order, Location *__restrict__ out, int32_t *__restrict__ steps_out,
+                     int32_t *__restrict__ cand_out, int32_t *__restrict__ err, int chunk, int32_t *__restrict__ queue)
 {
-  __shared__ int32_t stack_lds[kStackLds * 256];
-  __shared__ __attribute__ ((aligned (16))) __bf16 lx[4 * 64 * mlp::IN_PAD];
-  __shared__ __attribute__ ((aligned (16))) float ly[4 * 64 * 4];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = t < n;
-  if (__ballot (active) == 0) return;
-  const int64_t i = active ? (order ? (int64_t)order[t] : t) : 0;
+  __shared__ int32_t stack_lds[kStackLds * 64];
+  __shared__ __attribute__ ((aligned (16))) __bf16 lx[64 * mlp::IN_PAD];
+  __shared__ __attribute__ ((aligned (16))) float ly[64 * 4];
+  const int lane = threadIdx.x;
   TravStack todo;
   todo.lds = stack_lds;
-  todo.bdim = blockDim.x;
-  todo.tid = threadIdx.x;
+  todo.bdim = 64;
+  todo.tid = lane;
+  EntryPool pool (n, chunk, queue);
+  int64_t k = 0, i = 0;
+  bool have = false, done = true;
   float pt[3] = { 0.f, 0.f, 0.f };
-  if (active)
-  {
-    pt[0] = pts[3 * i];
-    pt[1] = pts[3 * i + 1];
-    pt[2] = pts[3 * i + 2];
-  }
-  float ref[3] = { -1.f, -1.f, -1.f };
+  int cur = 0, sp = 0;
   LocateStats st { 0, 0, 0, 0 };
-  bool overflow = false;
-  const int wv = threadIdx.x >> 6;
-  const int cell = locate_point_mlp<P, POLISH> (cfg, todo, pt, ref, st, overflow, active, net, budget,
-                                                lx + wv * 64 * mlp::IN_PAD, ly + wv * 64 * 4);
-  if (!active) return;
-  if (overflow) atomicOr (err, 1);
-  Location loc;
-  loc.cell = cell;
-  loc.ref[0] = cell >= 0 ? ref[0] : -1.f;
-  loc.ref[1] = cell >= 0 ? ref[1] : -1.f;
-  loc.ref[2] = cell >= 0 ? ref[2] : -1.f;
-  out[i] = loc;
-  if (steps_out) steps_out[i] = st.newton;
-  if (cand_out) cand_out[i] = st.candidates;
+  for (;;)
+  {
+    pool.refill (__ballot (done), lane, [&] (int64_t e) {
+      k = e;
+      have = true;
+      i = order ? (int64_t)order[k] : k;
+      pt[0] = pts[3 * i];
+      pt[1] = pts[3 * i + 1];
+      pt[2] = pts[3 * i + 2];
+      cur = 0;
+      sp = 0;
+      todo.set (0, kBarrier);
+      st = LocateStats { 0, 0, 0, 0 };
+    });
+    if (__ballot (have) == 0) break;
+    done = false;
+    float ref[3];
+    int cell = -1;
+    bool overflow = false;
+    const int r = locate_point_mlp_step<P, POLISH> (cfg, todo, pt, ref, st, overflow, have, cur, sp, cell, net, budget,
+                                                    lx, ly);
+    if (have && r != 0)
+    {
+      done = true;
+      have = false;
+      if (overflow) atomicOr (err, 1);
+      const bool hit = r > 0;
+      Location loc;
+      loc.cell = hit ? cell : -1;
+      loc.ref[0] = hit ? ref[0] : -1.f;
+      loc.ref[1] = hit ? ref[1] : -1.f;
+      loc.ref[2] = hit ? ref[2] : -1.f;
+      out[i] = loc;
+      if (steps_out) steps_out[i] = st.newton;
+      if (cand_out) cand_out[i] = st.candidates;
+    }
+  }
 }
+
 
 template <int P>
 __global__ void __launch_bounds__ (256) k_field_eval (const float *__restrict__ field, const int64_t *__restrict__ offs,
@@ -630,14 +662,15 @@ namespace drx
 namespace
 {
 // Launch settings drx_locate_autotune picked, per device, order (0: mixed), FP64 polish and walk (0 binary,
-// 1 8-wide); chunk 0 = not tuned.  Written by the tuner, read by drx_locate on the same host thread.
+// 1 8-wide, 2 learned start); chunk 0 = not tuned.  Written by the tuner, read by drx_locate on the same
+// host thread.
 struct LocateTune
 {
   int launch;
   int chunk;
 };
 constexpr int kTuneDevices = 16;
-LocateTune g_tune[kTuneDevices][5][2][2];
+LocateTune g_tune[kTuneDevices][5][2][3];
 int g_cus[kTuneDevices];
 
 int compute_units (int dev)
@@ -653,6 +686,7 @@ int compute_units (int dev)
 
 int stream_walk (const drx_bvh *bvh, const drx_locate_opts *opts)
 {
+  if (opts && opts->init_mode == DRX_INIT_MLP) return 2;
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   return (trav == DRX_TRAVERSE_LANE && bvh->wide_nodes) ? 1 : 0;
 }
@@ -729,7 +763,8 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   int dev = 0;
   DRX_HIP (hipGetDevice (&dev));
   const int walk = stream_walk (bvh, opts);
-  if ((launch == DRX_LAUNCH_AUTO || chunk == 0) && dev >= 0 && dev < kTuneDevices)
+  const bool tunable = learned || !(opts && opts->traversal == DRX_TRAVERSE_WAVE);
+  if (tunable && (launch == DRX_LAUNCH_AUTO || chunk == 0) && dev >= 0 && dev < kTuneDevices)
   {
     const LocateTune &t = g_tune[dev][P][cfg.polish ? 1 : 0][walk];
     if (t.chunk > 0)
@@ -746,7 +781,7 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   int64_t grid_s = n_chunks;
   if (launch == DRX_LAUNCH_PERSISTENT)
   {
-    const int64_t resident = (int64_t)compute_units (dev) * 4 * waves_per_simd (P, cfg.polish, walk);
+    const int64_t resident = (int64_t)compute_units (dev) * 4 * locate_waves_per_simd (P, cfg.polish, learned ? 3 : walk);
     grid_s = n_chunks < resident ? n_chunks : resident;
   }
   else
@@ -771,8 +806,8 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   do                                                                                                \
   {                                                                                                 \
     if (learned)                                                                                    \
-      hipLaunchKernelGGL ((k_locate_mlp<PP, PO>), g, b, 0, st, cfg, net, budget, points, n, order, out, steps, \
-                          cands, err);                                                              \
+      hipLaunchKernelGGL ((k_locate_mlp_stream<PP, PO>), gs, bs, 0, st, cfg, net, budget, points, n, order, out, \
+                          steps, cands, err, chunk, queue);                                         \
     else if (mode == 2)                                                                             \
       hipLaunchKernelGGL ((k_locate<PP, PO, 2>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
     else if (mode == 1)                                                                             \
@@ -815,8 +850,8 @@ int drx_locate_autotune (const drx_mesh *mesh, const drx_bvh *bvh, const float *
 {
   if (!out_launch || !out_chunk) return fail (DRX_ERR_ARG, "drx_locate_autotune: null output");
   if (n <= 0) return fail (DRX_ERR_ARG, "drx_locate_autotune: no points");
-  if (opts && (opts->init_mode == DRX_INIT_MLP || opts->traversal == DRX_TRAVERSE_WAVE))
-    return fail (DRX_ERR_ARG, "drx_locate_autotune: only the streamed walks (binary, 8-wide per lane) are tuned");
+  if (opts && opts->init_mode != DRX_INIT_MLP && opts->traversal == DRX_TRAVERSE_WAVE)
+    return fail (DRX_ERR_ARG, "drx_locate_autotune: the wave walk is not streamed");
   int rc = check_mesh (mesh, 3, "drx_locate_autotune");
   if (rc) return rc;
   if ((rc = check_bvh (bvh, "drx_locate_autotune"))) return rc;

@@ -58,6 +58,16 @@ __host__ __device__ constexpr int waves_per_simd (int P, bool polish, int mode)
   return DRX_WAVES_PER_EU > 0 ? DRX_WAVES_PER_EU : ((P >= 1 && P <= 3 && !polish && mode != 3) ? 5 : 4);
 }
 
+// Streamed batched-locate kernels (k_locate_stream, k_locate_mlp_stream): the integrate kernels' choice
+// unless DRX_LOCATE_WAVES fixes it.
+#ifndef DRX_LOCATE_WAVES
+#define DRX_LOCATE_WAVES 0
+#endif
+__host__ __device__ constexpr int locate_waves_per_simd (int P, bool polish, int mode)
+{
+  return DRX_LOCATE_WAVES > 0 ? DRX_LOCATE_WAVES : waves_per_simd (P, polish, mode);
+}
+
 static __global__ void k_fill_i32q (int32_t *p, int64_t n, int32_t v)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -126,8 +126,8 @@ typedef struct drx_locate_opts
   const void *mlp;            /* DRX_INIT_MLP: packed initializer (drx_mlp_pack), device */
   int32_t mlp_newton_steps;   /* DRX_INIT_MLP: Newton budget from the learned guess before the plain
                                  fallback (<= 0: 4) */
-  int32_t launch;             /* drx_launch: how drx_locate's streamed walks (binary, 8-wide per lane) are
-                                 launched; the results are the same either way */
+  int32_t launch;             /* drx_launch: how drx_locate's streamed walks (binary, 8-wide per lane,
+                                 learned start) are launched; the results are the same either way */
   int32_t chunk;              /* points per wavefront chunk for those walks (0: tuned, else 1024) */
 } drx_locate_opts;
 
@@ -263,10 +263,10 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
 /* Autotuner for drx_locate's streamed walks (no dray counterpart; dray launches one thread per point,
  * unstructured_mesh.cpp:123-148): times every chunk size in {256, 512, 1024, 2048, 4096} with each launch
  * form (one warm-up and two timed calls each, HIP events on `stream`) on the caller's points and options,
- * and records the fastest for the current device, the mesh's order, opts->fp64_polish and the walk, where
- * DRX_LAUNCH_AUTO and chunk 0 then use it.  out_locations (n Locations) is overwritten.  out_launch,
- * out_chunk: the choice; out_ms (nullable, 10 floats): ms per call of each candidate, chunk-major, chunked
- * before persistent.  Not for the learned start or the wave walk (DRX_ERR_ARG). */
+ * and records the fastest for the current device, the mesh's order, opts->fp64_polish and the walk (binary,
+ * 8-wide per lane, or the learned start's), where DRX_LAUNCH_AUTO and chunk 0 then use it.  out_locations
+ * (n Locations) is overwritten.  out_launch, out_chunk: the choice; out_ms (nullable, 10 floats): ms per
+ * call of each candidate, chunk-major, chunked before persistent.  Not for the wave walk (DRX_ERR_ARG). */
 int drx_locate_autotune (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
                          void *out_locations, const drx_locate_opts *opts, int32_t *out_launch, int32_t *out_chunk,
                          float *out_ms, void *stream);

@@ -346,6 +346,15 @@ def test_locate_launch_forms_identical(gpu, name, make):
                 ref = got
             else:
                 assert all(np.array_equal(a, b) for a, b in zip(ref, got)), (trav, coherent, launch, chunk)
+    for n in (1, 63, 64, 65, 1000):   # grids of one wavefront, partial chunks, a partial last refill
+        dn = gpu.to_device(pts[:n].reshape(-1))
+        ref = None
+        for launch, chunk in ((1, 1024), (1, 64), (2, 64), (2, 128), (2, 4096)):
+            got = _np(m.locate(dn, gpu.LocateOptions(launch=launch, chunk=chunk)))
+            if ref is None:
+                ref = got
+            else:
+                assert np.array_equal(ref, got), (n, launch, chunk)
     for bad in (gpu.LocateOptions(launch=3), gpu.LocateOptions(chunk=32), gpu.LocateOptions(chunk=-1)):
         with pytest.raises(gpu.L.DrxError):
             m.locate(d, bad)

@@ -226,6 +226,24 @@ def test_learned_locate_matches_plain(gpu, orc, trained, which):
         assert sl.mean() < sp.mean(), "a trained initializer should need fewer Newton steps"
 
 
+def test_learned_locate_launch_forms_identical(gpu, trained):
+    """The streamed learned locate gives the same locations, Newton steps and candidate counts for every
+    chunk size and launch form (a lane's MLP output does not depend on the other lanes of its round)."""
+    packed = trained[0].pack()
+    mesh = syn.hex_mesh(5, 3, 0.12, 0.01, 778)
+    m = gpu.UnstructuredMesh(mesh)
+    for n in (37, 20011):
+        dp = gpu.to_device(syn.query_points(n, m.bounds(), seed=9).reshape(-1))
+        ref = None
+        for launch, chunk in ((1, 1024), (1, 64), (2, 64), (2, 512), (1, 4096)):
+            loc, st, ca = m.locate(dp, gpu.LocateOptions(mlp=packed, launch=launch, chunk=chunk), stats=True)
+            got = (loc.cpu().numpy(), st.cpu().numpy(), ca.cpu().numpy())
+            if ref is None:
+                ref = got
+            else:
+                assert all(np.array_equal(a, b) for a, b in zip(ref, got)), (n, launch, chunk)
+
+
 def test_learned_integrate_matches_plain(gpu, trained):
     """Volume::integrate with the learned start renders the same image as with the plain start."""
     li = trained[0]
