@@ -246,9 +246,20 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
       int cell;
       if constexpr (LEARNED)
       {
-        cell = locate_point_mlp<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, alive, a.net, a.budget,
-                                            mlp_x + wv * 64 * mlp::IN_PAD, mlp_y + wv * 64 * 4);
+        // one candidate per iteration as in the plain walks; every lane joins the wavefront's MLP round
+        if (alive && !resume)
+        {
+          tcur = 0;
+          tsp = 0;
+          todo.set (0, kBarrier);
+        }
+        int c = -1;
+        const int r = locate_point_mlp_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, alive, tcur, tsp, c, a.net,
+                                                        a.budget, mlp_x + wv * 64 * mlp::IN_PAD, mlp_y + wv * 64 * 4);
         if (!alive) continue;
+        resume = r == 0;
+        if (resume) continue;
+        cell = r > 0 ? c : -1;
       }
       else if constexpr (WAVE)
         cell = locate_wave<P, POLISH> (a.cfg, ws, pt, ref, st, overflow);

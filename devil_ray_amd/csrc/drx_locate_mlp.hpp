@@ -127,79 +127,4 @@ __device__ __forceinline__ int locate_point_mlp_step (const LocateCfg &cfg, Trav
   return 0;
 }
 
-// Lanes with active == false take part in the wavefront's MLP rounds only.  lx / ly: this wave's LDS
-// scratch for mlp_wave_infer.
-template <int P, bool POLISH>
-__device__ __forceinline__ int locate_point_mlp (const LocateCfg &cfg, TravStack &todo, const float pt[3], float ref[3],
-                                                 LocateStats &st, bool &overflow, bool active, const MlpNet &net,
-                                                 int budget, __bf16 *lx, float *ly)
-{
-  int cur = 0, sp = 0;
-  if (active) todo.set (0, kBarrier);
-  bool done = !active;
-  int result = -1;
-  for (;;)
-  {
-    int leaf = -1;
-    if (!done)
-    {
-      leaf = next_candidate_leaf (cfg, todo, pt, cur, sp);
-      if (leaf == kWalkOverflow) overflow = true;
-      if (leaf < 0) done = true;
-    }
-    const bool has = !done;
-    if (__ballot (has) == 0) break;
-    float f[mlp::IN_PAD], base[3] = { 0.5f, 0.5f, 0.5f }, xc[3] = { 0.5f, 0.5f, 0.5f };
-    const float *g = cfg.mesh;
-    int el = -1, order = P;
-    if (has)
-    {
-      const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
-      el = __float_as_int (l0.x);
-      candidate_start<P> (cfg, el, leaf, pt, l0, l1, xc);   // the fallback start of the plain method
-      if constexpr (P == 0)
-      {
-        g = cfg.mesh + cfg.offsets[el];
-        order = cfg.orders[el];
-        mlp_features_mixed (order, g, pt, f, base);
-      }
-      else
-      {
-        g = cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE;
-        mlp_features<P> (g, pt, f, base);
-      }
-    }
-    else
-    {
-#pragma unroll
-      for (int k = 0; k < mlp::IN_PAD; ++k) f[k] = 0.f;
-    }
-    float corr[3];
-    mlp_wave_infer (net, f, has, corr, lx, ly);
-    if (has)
-    {
-      float x[3] = { base[0] + corr[0], base[1] + corr[1], base[2] + corr[2] };
-      bool found;
-      if constexpr (P == 0)
-        found = try_element_learned_mixed<POLISH> (cfg, order, g, pt, x, xc, budget, st);
-      else
-        found = try_element_learned<P, POLISH> (cfg, g, pt, x, xc, budget, st);
-      if (found)
-      {
-        ref[0] = x[0];
-        ref[1] = x[1];
-        ref[2] = x[2];
-        result = el;
-        done = true;
-      }
-      else
-      {
-        cur = todo.get (sp);
-        --sp;
-      }
-    }
-  }
-  return result;
-}
-
 } // namespace drx
