@@ -442,6 +442,20 @@ __host__ __device__ __forceinline__ uint32_t grid_cell (float v, float origin, f
   return c > 0.f ? (c < 65535.f ? (uint32_t)c : 65535u) : 0u;
 }
 
+// polish_f64 as an out-of-line call: the FP64 correction runs only for accepted points whose condition
+// estimate exceeds the threshold, and kept out of line its double-precision temporaries do not raise the
+// register budget of the walk around it.  Arguments and result by value, so that the caller's point stays
+// in registers.
+template <int P>
+__device__ __attribute__ ((noinline)) float3 polish_f64_call (const float *__restrict__ g, float px, float py, float pz,
+                                                              float x0, float x1, float x2)
+{
+  const float pt[3] = { px, py, pz };
+  float x[3] = { x0, x1, x2 };
+  polish_f64<P> (g, pt, x);
+  return make_float3 (x[0], x[1], x[2]);
+}
+
 struct LocateCfg
 {
   const float4 *nodes;      // inner nodes, 4 float4 each
@@ -481,7 +495,10 @@ __device__ __forceinline__ bool try_element_at (const LocateCfg &cfg, const floa
   {
     if (found && !(no.cond <= cfg.cond_threshold))
     {
-      polish_f64<P> (g, pt, x);
+      const float3 r = polish_f64_call<P> (g, pt[0], pt[1], pt[2], x[0], x[1], x[2]);
+      x[0] = r.x;
+      x[1] = r.y;
+      x[2] = r.z;
       st.polished += 1;
     }
   }

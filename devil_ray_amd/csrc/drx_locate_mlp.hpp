@@ -40,7 +40,10 @@ __device__ __forceinline__ bool try_element_learned (const LocateCfg &cfg, const
   {
     if (found && !(no.cond <= cfg.cond_threshold))
     {
-      polish_f64<P> (g, pt, x);
+      const float3 r = polish_f64_call<P> (g, pt[0], pt[1], pt[2], x[0], x[1], x[2]);
+      x[0] = r.x;
+      x[1] = r.y;
+      x[2] = r.z;
       st.polished += 1;
     }
   }

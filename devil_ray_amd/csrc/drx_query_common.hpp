@@ -50,12 +50,12 @@ static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) /
 #endif
 
 // Minimum waves per SIMD for the locate / integrate kernels (sets the VGPR budget: 5 -> 96, 4 -> 128).
-// Measured on MI355X: order <= 3 FP32 walks run fastest at 5 (config 1: 19.97 ms vs 22.37 ms at 4, some
-// spills included); order 4, FP64 polish and the learned start need the 128-register budget (config 4:
-// 297.5 ms at 4 vs 323 ms at 5).
+// Measured on MI355X: order <= 3 walks run fastest at 5 (config 1: 19.97 ms vs 22.37 ms at 4, some spills
+// included; with FP64 polish, whose correction is an out-of-line call, 14.36 ms vs 16.40 ms at 4); order 4
+// and the learned start need the 128-register budget (config 4: 297.5 ms at 4 vs 323 ms at 5).
 __host__ __device__ constexpr int waves_per_simd (int P, bool polish, int mode)
 {
-  return DRX_WAVES_PER_EU > 0 ? DRX_WAVES_PER_EU : ((P >= 1 && P <= 3 && !polish && mode != 3) ? 5 : 4);
+  return DRX_WAVES_PER_EU > 0 ? DRX_WAVES_PER_EU : ((P >= 1 && P <= 3 && mode != 3) ? 5 : 4);
 }
 
 // Streamed batched-locate kernels (k_locate_stream, k_locate_mlp_stream): the integrate kernels' choice
