@@ -53,9 +53,13 @@ static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) /
 // Measured on MI355X: order <= 3 walks run fastest at 5 (config 1: 19.97 ms vs 22.37 ms at 4, some spills
 // included; with FP64 polish, whose correction is an out-of-line call, 14.36 ms vs 16.40 ms at 4); order 4
 // and the learned start need the 128-register budget (config 4: 297.5 ms at 4 vs 323 ms at 5).
+#ifndef DRX_LEARNED_WAVES
+#define DRX_LEARNED_WAVES 4   // learned-start integrate kernels (mode 3); 3 (no spills) measured slower
+#endif
 __host__ __device__ constexpr int waves_per_simd (int P, bool polish, int mode)
 {
-  return DRX_WAVES_PER_EU > 0 ? DRX_WAVES_PER_EU : ((P >= 1 && P <= 3 && mode != 3) ? 5 : 4);
+  return DRX_WAVES_PER_EU > 0 ? DRX_WAVES_PER_EU
+                              : (mode == 3 ? DRX_LEARNED_WAVES : ((P >= 1 && P <= 3) ? 5 : 4));
 }
 
 // Streamed batched-locate kernels (k_locate_stream, k_locate_mlp_stream): the integrate kernels' choice
@@ -65,7 +69,7 @@ __host__ __device__ constexpr int waves_per_simd (int P, bool polish, int mode)
 #endif
 __host__ __device__ constexpr int locate_waves_per_simd (int P, bool polish, int mode)
 {
-  return DRX_LOCATE_WAVES > 0 ? DRX_LOCATE_WAVES : waves_per_simd (P, polish, mode);
+  return DRX_LOCATE_WAVES > 0 ? DRX_LOCATE_WAVES : (mode == 3 ? 4 : waves_per_simd (P, polish, mode));
 }
 
 static __global__ void k_fill_i32q (int32_t *p, int64_t n, int32_t v)
