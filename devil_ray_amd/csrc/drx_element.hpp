@@ -715,9 +715,11 @@ constexpr int kWalkDone = -1, kWalkOverflow = -2;
 // in both child boxes.  Over the compact nodes (cfg.qnodes) the point's grid cells are compared with
 // each child's cell range, a superset of dray's box test, and each reached leaf's exact box then decides,
 // so the leaves returned are dray's candidates in dray's order.  Returns the leaf index with cur = its
-// code, kWalkDone when the stack is exhausted, kWalkOverflow when the 64-entry stack would overflow.
+// code and its packed record in l0, l1 (read together with the exact box over the compact nodes, so a
+// candidate costs one memory round trip), kWalkDone when the stack is exhausted, kWalkOverflow when the
+// 64-entry stack would overflow.
 __device__ __forceinline__ int next_candidate_leaf (const LocateCfg &cfg, TravStack &todo, const float pt[3], int &cur,
-                                                    int &sp)
+                                                    int &sp, float4 &l0, float4 &l1)
 {
   // compact nodes: the point's grid cells, compared with each child box's cell range
   const bool compact = cfg.qnodes != nullptr;
@@ -789,9 +791,16 @@ __device__ __forceinline__ int next_candidate_leaf (const LocateCfg &cfg, TravSt
     }
     if (cur == kBarrier) return kWalkDone;
     leaf = -cur - 1;
-    if (!compact) break;
+    if (!compact)
+    {
+      l0 = cfg.leaves[2 * (int64_t)leaf];
+      l1 = cfg.leaves[2 * (int64_t)leaf + 1];
+      break;
+    }
     // the cell ranges are conservative: the leaf's exact box (dray's child-box test) decides
     const float4 b0 = cfg.leaf_boxes[2 * (int64_t)leaf], b1 = cfg.leaf_boxes[2 * (int64_t)leaf + 1];
+    l0 = cfg.leaves[2 * (int64_t)leaf];
+    l1 = cfg.leaves[2 * (int64_t)leaf + 1];
     const bool in = !(pt[0] < b0.x) && !(pt[1] < b0.y) && !(pt[2] < b0.z) && !(pt[0] > b0.w) && !(pt[1] > b1.x) &&
                     !(pt[2] > b1.y);
     if (in) break;
@@ -815,13 +824,13 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
                                                   float ref[3], LocateStats &st, bool &overflow, int &cur, int &sp,
                                                   int &cell, float *stage = nullptr)
 {
-  const int leaf = next_candidate_leaf (cfg, todo, pt, cur, sp);
+  float4 l0, l1;
+  const int leaf = next_candidate_leaf (cfg, todo, pt, cur, sp, l0, l1);
   if (leaf < 0)
   {
     if (leaf == kWalkOverflow) overflow = true;
     return -1;
   }
-  const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
   const int el = __float_as_int (l0.x);
   float x[3];
   candidate_start<P> (cfg, el, leaf, pt, l0, l1, x);

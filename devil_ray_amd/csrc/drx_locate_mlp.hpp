@@ -76,9 +76,10 @@ __device__ __forceinline__ int locate_point_mlp_step (const LocateCfg &cfg, Trav
                                                       __bf16 *lx, float *ly)
 {
   int leaf = -1;
+  float4 l0 = make_float4 (0.f, 0.f, 0.f, 0.f), l1 = l0;
   if (have)
   {
-    leaf = next_candidate_leaf (cfg, todo, pt, cur, sp);
+    leaf = next_candidate_leaf (cfg, todo, pt, cur, sp, l0, l1);
     if (leaf == kWalkOverflow) overflow = true;
   }
   const bool has = leaf >= 0;
@@ -88,7 +89,6 @@ __device__ __forceinline__ int locate_point_mlp_step (const LocateCfg &cfg, Trav
   int el = -1, order = P;
   if (has)
   {
-    const float4 l0 = cfg.leaves[2 * (int64_t)leaf], l1 = cfg.leaves[2 * (int64_t)leaf + 1];
     el = __float_as_int (l0.x);
     candidate_start<P> (cfg, el, leaf, pt, l0, l1, xc);   // the fallback start of the plain method
     if constexpr (P == 0)
