@@ -572,17 +572,18 @@ __device__ __forceinline__ void corner_start (const float *g, int p, const float
 // The Newton start for candidate element `el` of leaf `leaf` (leaf record l0, l1): the leaf's stored
 // start map when the BVH carries one (read in the same round as the leaf record), else the SubRef centre
 // when the caller asked for dray's subref_center start (data_model/subref.hpp:52-56), else corner_start.
+// candidate_start_rows takes the start map rows already read (map[], used when cfg.starts is set).
 template <int P>
-__device__ __forceinline__ void candidate_start (const LocateCfg &cfg, int el, int leaf, const float pt[3],
-                                                 const float4 &l0, const float4 &l1, float x[3])
+__device__ __forceinline__ void candidate_start_rows (const LocateCfg &cfg, int el, const float4 (&map)[3],
+                                                      const float pt[3], const float4 &l0, const float4 &l1, float x[3])
 {
   if (cfg.starts)
   {
-    const float4 *m = cfg.starts + 3 * (int64_t)leaf;
-    const float4 r0 = m[0], r1 = m[1], r2 = m[2];
-    const float v[3] = { ((r0.x * pt[0] + r0.y * pt[1]) + r0.z * pt[2]) + r0.w,
-                         ((r1.x * pt[0] + r1.y * pt[1]) + r1.z * pt[2]) + r1.w,
-                         ((r2.x * pt[0] + r2.y * pt[1]) + r2.z * pt[2]) + r2.w };
+    // an explicit fma chain, so the start's rounding does not depend on how the compiler contracts
+    const float4 r0 = map[0], r1 = map[1], r2 = map[2];
+    const float v[3] = { fmaf (r0.x, pt[0], fmaf (r0.y, pt[1], fmaf (r0.z, pt[2], r0.w))),
+                         fmaf (r1.x, pt[0], fmaf (r1.y, pt[1], fmaf (r1.z, pt[2], r1.w))),
+                         fmaf (r2.x, pt[0], fmaf (r2.y, pt[1], fmaf (r2.z, pt[2], r2.w))) };
     const float lo[3] = { l0.y, l0.z, l0.w }, hi[3] = { l1.x, l1.y, l1.z };
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -598,6 +599,21 @@ __device__ __forceinline__ void candidate_start (const LocateCfg &cfg, int el, i
     corner_start (cfg.mesh + cfg.offsets[el], cfg.orders[el], pt, l0, l1, x);
   else
     corner_start (cfg.mesh + (int64_t)el * Ord<P>::GSTRIDE, P, pt, l0, l1, x);
+}
+
+template <int P>
+__device__ __forceinline__ void candidate_start (const LocateCfg &cfg, int el, int leaf, const float pt[3],
+                                                 const float4 &l0, const float4 &l1, float x[3])
+{
+  float4 map[3];
+  if (cfg.starts)
+  {
+    const float4 *m = cfg.starts + 3 * (int64_t)leaf;
+    map[0] = m[0];
+    map[1] = m[1];
+    map[2] = m[2];
+  }
+  candidate_start_rows<P> (cfg, el, map, pt, l0, l1, x);
 }
 
 #ifndef DRX_STACK_LDS
@@ -715,11 +731,11 @@ constexpr int kWalkDone = -1, kWalkOverflow = -2;
 // in both child boxes.  Over the compact nodes (cfg.qnodes) the point's grid cells are compared with
 // each child's cell range, a superset of dray's box test, and each reached leaf's exact box then decides,
 // so the leaves returned are dray's candidates in dray's order.  Returns the leaf index with cur = its
-// code and its packed record in l0, l1 (read together with the exact box over the compact nodes, so a
-// candidate costs one memory round trip), kWalkDone when the stack is exhausted, kWalkOverflow when the
-// 64-entry stack would overflow.
+// code, its packed record in l0, l1 and its Newton start map in map[] when the BVH has start maps (read
+// together with the exact box over the compact nodes, so a candidate costs one memory round trip),
+// kWalkDone when the stack is exhausted, kWalkOverflow when the 64-entry stack would overflow.
 __device__ __forceinline__ int next_candidate_leaf (const LocateCfg &cfg, TravStack &todo, const float pt[3], int &cur,
-                                                    int &sp, float4 &l0, float4 &l1)
+                                                    int &sp, float4 &l0, float4 &l1, float4 (&map)[3])
 {
   // compact nodes: the point's grid cells, compared with each child box's cell range
   const bool compact = cfg.qnodes != nullptr;
@@ -791,16 +807,18 @@ __device__ __forceinline__ int next_candidate_leaf (const LocateCfg &cfg, TravSt
     }
     if (cur == kBarrier) return kWalkDone;
     leaf = -cur - 1;
-    if (!compact)
-    {
-      l0 = cfg.leaves[2 * (int64_t)leaf];
-      l1 = cfg.leaves[2 * (int64_t)leaf + 1];
-      break;
-    }
-    // the cell ranges are conservative: the leaf's exact box (dray's child-box test) decides
-    const float4 b0 = cfg.leaf_boxes[2 * (int64_t)leaf], b1 = cfg.leaf_boxes[2 * (int64_t)leaf + 1];
     l0 = cfg.leaves[2 * (int64_t)leaf];
     l1 = cfg.leaves[2 * (int64_t)leaf + 1];
+    if (cfg.starts)
+    {
+      const float4 *m = cfg.starts + 3 * (int64_t)leaf;
+      map[0] = m[0];
+      map[1] = m[1];
+      map[2] = m[2];
+    }
+    if (!compact) break;
+    // the cell ranges are conservative: the leaf's exact box (dray's child-box test) decides
+    const float4 b0 = cfg.leaf_boxes[2 * (int64_t)leaf], b1 = cfg.leaf_boxes[2 * (int64_t)leaf + 1];
     const bool in = !(pt[0] < b0.x) && !(pt[1] < b0.y) && !(pt[2] < b0.z) && !(pt[0] > b0.w) && !(pt[1] > b1.x) &&
                     !(pt[2] > b1.y);
     if (in) break;
@@ -824,8 +842,8 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
                                                   float ref[3], LocateStats &st, bool &overflow, int &cur, int &sp,
                                                   int &cell, float *stage = nullptr)
 {
-  float4 l0, l1;
-  const int leaf = next_candidate_leaf (cfg, todo, pt, cur, sp, l0, l1);
+  float4 l0, l1, map[3];
+  const int leaf = next_candidate_leaf (cfg, todo, pt, cur, sp, l0, l1, map);
   if (leaf < 0)
   {
     if (leaf == kWalkOverflow) overflow = true;
@@ -833,7 +851,7 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
   }
   const int el = __float_as_int (l0.x);
   float x[3];
-  candidate_start<P> (cfg, el, leaf, pt, l0, l1, x);
+  candidate_start_rows<P> (cfg, el, map, pt, l0, l1, x);
   bool found;
   if constexpr (P == 0)
     found = try_element_mixed<POLISH> (cfg, el, pt, x, st);

@@ -76,10 +76,10 @@ __device__ __forceinline__ int locate_point_mlp_step (const LocateCfg &cfg, Trav
                                                       __bf16 *lx, float *ly)
 {
   int leaf = -1;
-  float4 l0 = make_float4 (0.f, 0.f, 0.f, 0.f), l1 = l0;
+  float4 l0 = make_float4 (0.f, 0.f, 0.f, 0.f), l1 = l0, map[3] = { l0, l0, l0 };
   if (have)
   {
-    leaf = next_candidate_leaf (cfg, todo, pt, cur, sp, l0, l1);
+    leaf = next_candidate_leaf (cfg, todo, pt, cur, sp, l0, l1, map);
     if (leaf == kWalkOverflow) overflow = true;
   }
   const bool has = leaf >= 0;
@@ -90,7 +90,7 @@ __device__ __forceinline__ int locate_point_mlp_step (const LocateCfg &cfg, Trav
   if (has)
   {
     el = __float_as_int (l0.x);
-    candidate_start<P> (cfg, el, leaf, pt, l0, l1, xc);   // the fallback start of the plain method
+    candidate_start_rows<P> (cfg, el, map, pt, l0, l1, xc);   // the fallback start of the plain method
     if constexpr (P == 0)
     {
       g = cfg.mesh + cfg.offsets[el];
