@@ -155,7 +155,7 @@ def main() -> int:
     k_ms = float(np.mean(kernel_ms))
     bytes_per_sample = 16 * (p + 1) ** 3
     achieved_gbs = located_per_frame * bytes_per_sample / (k_ms * 1e-3) / 1e9
-    walk = {0: 0, 3: 0, 1: 1, 2: 2}[args.traversal]
+    walk = int(st.walk)   # the walk the library ran (the automatic choice depends on the frame's size)
     kernel_name = f"k_integrate<{p},{'true' if args.polish else 'false'},{walk},false>"
 
     out = {

@@ -47,7 +47,7 @@ class DrxMlpTrainOpts(C.Structure):
 
 class DrxIntegrateStats(C.Structure):
     _fields_ = [("located_samples", I64), ("newton_steps", I64), ("candidates", I64), ("polished", I64),
-                ("kernel_ms", F32), ("total_ms", F32)]
+                ("kernel_ms", F32), ("total_ms", F32), ("walk", I32)]
 
 
 class DrxCamera(C.Structure):

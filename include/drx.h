@@ -153,7 +153,9 @@ typedef struct drx_mlp_train_opts
 
 enum drx_traversal
 {
-  DRX_TRAVERSE_AUTO = 0,      /* library default: DRX_TRAVERSE_BINARY (fastest measured on MI355X) */
+  DRX_TRAVERSE_AUTO = 0,      /* library default: DRX_TRAVERSE_BINARY (fastest measured on MI355X), except
+                                 drx_volume_integrate with fewer active rays than 64 per SIMD and an 8-wide
+                                 collapse: DRX_TRAVERSE_WAVE (latency-bound frames) */
   DRX_TRAVERSE_LANE = 1,      /* every lane walks the 8-wide collapse (bvh->wide_nodes) depth-first */
   DRX_TRAVERSE_WAVE = 2,      /* the wavefront walks the union of its lanes' paths over the 8-wide
                                  collapse with uniform node reads */
@@ -168,6 +170,7 @@ typedef struct drx_integrate_stats
   int64_t polished;           /* samples that took the FP64 correction */
   float kernel_ms;            /* duration of the integrate kernel alone (HIP events on `stream`) */
   float total_ms;             /* the whole call: integrate + partial compaction (HIP events) */
+  int32_t walk;               /* the walk that ran: 0 binary, 1 8-wide per lane, 2 wave-coherent, 3 learned start */
 } drx_integrate_stats;
 
 /* PointLight (rendering/point_light.hpp:15-33); dray's defaults: amb 0.4, diff 0.75, spec 0.8, spec_pow 15. */

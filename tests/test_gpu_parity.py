@@ -314,6 +314,7 @@ def test_compact_nodes_render_identical(gpu, orc):
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)
+    vol.opts = gpu.LocateOptions(traversal=3)   # the binary walk, which reads the compact nodes
     rays = gpu.as_bytes(orc.camera_perspective(64, 64, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)))
     res = {}
     for use in (False, True):
@@ -375,6 +376,33 @@ def test_locate_autotune(gpu):
     assert np.array_equal(before, _np(m.locate(d)))
     with pytest.raises(gpu.L.DrxError):
         m.locate_autotune(d, gpu.LocateOptions(traversal=2))
+
+
+@pytest.mark.parametrize("name,make", [MESHES[1], MESHES[2], MESHES[4]], ids=["p2_n4", "p3_n5", "mixed_n4"])
+def test_render_walks_identical(gpu, orc, name, make):
+    """Volume::integrate gives bit-identical partials and the same located-sample, Newton-step and
+    candidate counts with every walk: automatic (the wave walk for a frame this small), binary over the
+    compact nodes, 8-wide per lane, wave-coherent."""
+    mesh = make()
+    m, bvh = _gpu_bvh(gpu, mesh)
+    f = gpu.UnstructuredField(mesh)
+    vol = gpu.Volume(m, f)
+    vol.use_lighting = False
+    vol.samples(96)
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, 0.05)
+    rays = gpu.as_bytes(orc.camera_perspective(48, 48, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)))
+    res = {}
+    for trav in (0, 3, 1, 2):
+        vol.opts = gpu.LocateOptions(traversal=trav)
+        parts, n = vol.integrate(rays, tile_width=48, tile_height=48)
+        st = vol.last_stats
+        res[trav] = (_np(parts, np.uint8)[: n * 24].copy(), st.located_samples, st.newton_steps, st.candidates)
+    print(f"[render walks] {name}: located {res[3][1]}", flush=True)
+    for trav in (0, 1, 2):
+        assert np.array_equal(res[trav][0], res[3][0]), trav
+        assert res[trav][1:] == res[3][1:], trav
 
 
 @pytest.mark.parametrize("name,make", MESHES[:4], ids=[m[0] for m in MESHES[:4]])
