@@ -502,13 +502,19 @@ __global__ void k_gather_active (const Ray *__restrict__ rays, int n, Bounds6 mb
   out[pos[i]] = r;
 }
 
-// 8x8 tile-major rank of a pixel in a w x h image.
+// Ray tiles of 64 pixels, kTileW x kTileH (one wavefront each).
+#ifndef DRX_TILE_LOG2_W
+#define DRX_TILE_LOG2_W 3   // 8 x 8 (64^3 frame: 13.9 ms; 16 x 4: 14.5 ms; 4 x 16: 14.7 ms)
+#endif
+constexpr int kTileLW = DRX_TILE_LOG2_W, kTileW = 1 << kTileLW, kTileLH = 6 - kTileLW, kTileH = 1 << kTileLH;
+
+// Tile-major rank of a pixel in a w x h image.
 __device__ __forceinline__ int64_t tile_rank (int pid, int w, int h)
 {
   const int i = pid % w, j = pid / w;
-  const int tx = i >> 3, ty = j >> 3;
-  const int tiles_x = (w + 7) >> 3;
-  return ((int64_t)(ty * tiles_x + tx) << 6) + ((j & 7) << 3) + (i & 7);
+  const int tx = i >> kTileLW, ty = j >> kTileLH;
+  const int tiles_x = (w + kTileW - 1) >> kTileLW;
+  return ((int64_t)(ty * tiles_x + tx) << 6) + ((j & (kTileH - 1)) << kTileLW) + (i & (kTileW - 1));
 }
 
 
@@ -998,7 +1004,7 @@ int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t heig
   if (n == 0) return DRX_OK;
   if (!rays || !out_order) return fail (DRX_ERR_ARG, "drx_ray_tile_order: null pointer");
   hipStream_t st = as_stream (stream);
-  const int tiles_x = (width + 7) / 8, tiles_y = (height + 7) / 8;
+  const int tiles_x = (width + kTileW - 1) / kTileW, tiles_y = (height + kTileH - 1) / kTileH;
   const int64_t slots_n = (int64_t)tiles_x * tiles_y * 64;
   if (slots_n > (1ll << 30)) return fail (DRX_ERR_ARG, "drx_ray_tile_order: image too large");
   int32_t *slots0 = (int32_t *)arena (SLOT_TREE2, sizeof (int32_t) * 4 * (size_t)slots_n + 64);
