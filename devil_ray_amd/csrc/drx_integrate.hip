@@ -444,17 +444,9 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   {
     // fewer wavefronts than SIMDs: nothing hides a sample's chain of dependent reads, and the wave walk's
     // uniform node reads shorten it (64^3-mesh frames: binary; an 8^3 frame of 24,025 rays: 4.36 -> 3.97 ms)
-    static int cus[16];
     int dev = 0;
-    if (hipGetDevice (&dev) == hipSuccess && dev >= 0 && dev < 16)
-    {
-      if (!cus[dev])
-      {
-        int v = 0;
-        cus[dev] = (hipDeviceGetAttribute (&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
-      }
-      if ((int64_t)n_rays <= (int64_t)64 * 4 * cus[dev]) walk = DRX_TRAVERSE_WAVE;
-    }
+    if (hipGetDevice (&dev) == hipSuccess && (int64_t)n_rays <= (int64_t)64 * 4 * compute_units (dev))
+      walk = DRX_TRAVERSE_WAVE;
   }
   const int mode = learned ? 3 : walk == DRX_TRAVERSE_WAVE ? 2 : (walk == DRX_TRAVERSE_LANE && a.cfg.wide) ? 1 : 0;
   hipEvent_t ev[3];

@@ -677,18 +677,6 @@ struct LocateTune
 };
 constexpr int kTuneDevices = 16;
 LocateTune g_tune[kTuneDevices][5][2][3];
-int g_cus[kTuneDevices];
-
-int compute_units (int dev)
-{
-  if (dev < 0 || dev >= kTuneDevices) return 256;
-  if (!g_cus[dev])
-  {
-    int v = 0;
-    g_cus[dev] = (hipDeviceGetAttribute (&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
-  }
-  return g_cus[dev];
-}
 
 int stream_walk (const drx_bvh *bvh, const drx_locate_opts *opts)
 {

@@ -62,6 +62,19 @@ __host__ __device__ constexpr int waves_per_simd (int P, bool polish, int mode)
                               : (mode == 3 ? DRX_LEARNED_WAVES : ((P >= 1 && P <= 3) ? 5 : 4));
 }
 
+// Compute units of device `dev` (cached per device; 256 if the query fails).
+inline int compute_units (int dev)
+{
+  static int cus[16];
+  if (dev < 0 || dev >= 16) return 256;
+  if (!cus[dev])
+  {
+    int v = 0;
+    cus[dev] = (hipDeviceGetAttribute (&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ? v : 256;
+  }
+  return cus[dev];
+}
+
 // Streamed batched-locate kernels (k_locate_stream, k_locate_mlp_stream): the integrate kernels' choice
 // unless DRX_LOCATE_WAVES fixes it.
 #ifndef DRX_LOCATE_WAVES
