@@ -61,10 +61,16 @@ void *arena (int slot, size_t bytes)
   if (s.bytes >= bytes) return s.ptr;
   if (s.ptr)
   {
-    hipDeviceSynchronize ();
-    hipFree (s.ptr);
+    // the old buffer may still be in use by queued work: wait, then free it
+    const hipError_t es = hipDeviceSynchronize ();
+    const hipError_t ef = hipFree (s.ptr);
     s.ptr = nullptr;
     s.bytes = 0;
+    if (es != hipSuccess || ef != hipSuccess)
+    {
+      set_error ("arena: growing slot %d: %s", slot, hipGetErrorString (es != hipSuccess ? es : ef));
+      return nullptr;
+    }
   }
   const size_t want = bytes + bytes / 8;   // grow with headroom
   hipError_t e = hipMalloc (&s.ptr, want);
@@ -84,11 +90,17 @@ int arena_release ()
   DRX_HIP (hipGetDevice (&dev));
   DRX_HIP (hipDeviceSynchronize ());
   std::lock_guard<std::mutex> lock (g_arena_mutex);
+  hipError_t first = hipSuccess;
   for (int i = 0; i < SLOT_COUNT; ++i)
   {
-    if (g_slots[dev][i].ptr) hipFree (g_slots[dev][i].ptr);
+    if (g_slots[dev][i].ptr)
+    {
+      const hipError_t e = hipFree (g_slots[dev][i].ptr);
+      if (e != hipSuccess && first == hipSuccess) first = e;
+    }
     g_slots[dev][i] = Slot ();
   }
+  if (first != hipSuccess) return fail (DRX_ERR_HIP, "drx_release: hipFree: %s", hipGetErrorString (first));
   return DRX_OK;
 }
 
