@@ -654,7 +654,14 @@ int radix_sort_pairs (uint32_t *kA, int32_t *vA, uint32_t *kB, int32_t *vB, int3
   return DRX_OK;
 }
 
-// Morton codes of query points in the normalised frame of `bounds` (same quantisation as the LBVH).
+// Morton codes of query points in the normalised frame of `bounds` (the LBVH's 30-bit quantisation),
+// kept to their top kPointMortonBits bits: the order only groups nearby points for coherent walks (any
+// order gives the same results), and 24 bits (256 cells per axis) sort in three radix passes, not four.
+#ifndef DRX_POINT_MORTON_BITS
+#define DRX_POINT_MORTON_BITS 24
+#endif
+constexpr int kPointMortonBits = DRX_POINT_MORTON_BITS;
+
 __global__ void k_point_morton (const float *__restrict__ pts, int n, float minx, float miny, float minz, float ix,
                                 float iy, float iz, uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
 {
@@ -669,7 +676,7 @@ __global__ void k_point_morton (const float *__restrict__ pts, int n, float minx
     x = fminf (fmaxf (x * 1024.0f, 0.0f), 1023.0f);
     q[d] = (uint32_t)x;
   }
-  keys[i] = (expand_bits32 (q[2]) << 2) | (expand_bits32 (q[1]) << 1) | expand_bits32 (q[0]);
+  keys[i] = ((expand_bits32 (q[2]) << 2) | (expand_bits32 (q[1]) << 1) | expand_bits32 (q[0])) >> (30 - kPointMortonBits);
   vals[i] = i;
 }
 
@@ -694,7 +701,7 @@ int morton_order_points (const float *pts, int n, const float *bounds6, hipStrea
   DRX_LAUNCH_CHECK ();
   uint32_t *ks;
   int32_t *vs;
-  int rc = radix_sort_pairs (kA, vA, kB, vB, hist, n, 30, st, &ks, &vs);
+  int rc = radix_sort_pairs (kA, vA, kB, vB, hist, n, kPointMortonBits, st, &ks, &vs);
   if (rc) return rc;
   *order_out = vs;
   return DRX_OK;
