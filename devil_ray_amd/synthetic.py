@@ -72,7 +72,9 @@ class HexMesh:
 
     @property
     def n_el(self) -> int:
-        return self.n ** 3
+        if self.orders is not None:
+            return int(len(self.orders))
+        return int(len(self.ctrl_idx) // (self.order + 1) ** 3)
 
     def elem_dofs(self, e: int) -> np.ndarray:
         if self.orders is None:
@@ -81,6 +83,23 @@ class HexMesh:
         p = int(self.orders[e])
         o = int(self.offsets[e])
         return self.ctrl_idx[o:o + (p + 1) ** 3]
+
+
+def elements_from_nets(nets, p: int, field=None) -> HexMesh:
+    """A mesh of independent order-p elements, one control net [(p+1)^3, 3] each (dof i + (p+1)(j + (p+1)k),
+    u fastest); no control points are shared.  field: per-element nets [(p+1)^3] (default zero)."""
+    nets = np.asarray(nets, dtype=np.float32).reshape(-1, (p + 1) ** 3, 3)
+    values = np.ascontiguousarray(nets.reshape(-1, 3))
+    f = np.zeros(len(values), np.float32) if field is None else np.asarray(field, np.float32).reshape(-1)
+    return HexMesh(n=0, order=p, ctrl_idx=np.arange(len(values), dtype=np.int32), values=values, field=f)
+
+
+def box_net(lo, hi, p: int) -> np.ndarray:
+    """The control net of the affine map of [0,1]^3 onto the box [lo, hi] (a lattice net), float64."""
+    t = np.arange(p + 1, dtype=np.float64) / p
+    k, j, i = np.meshgrid(t, t, t, indexing="ij")
+    u = np.stack([i.reshape(-1), j.reshape(-1), k.reshape(-1)], axis=1)
+    return np.asarray(lo, np.float64) + u * (np.asarray(hi, np.float64) - np.asarray(lo, np.float64))
 
 
 def _element_lattice_index(n: int, p: int) -> np.ndarray:
@@ -240,3 +259,48 @@ def grey_to_red_table():
 def table_alpha_for_corrected(opacity: float, samples: int) -> float:
     """Table alpha a such that 1-(1-a)^(10/samples) == opacity (per-sample corrected opacity)."""
     return 1.0 - (1.0 - opacity) ** (samples / 10.0)
+
+
+# ---- furnace query points ------------------------------------------------------------------------
+_MINSTD_A = 48271
+_MINSTD_M = 2147483647
+
+
+def minstd_outputs(n: int, seed: int = 0) -> np.ndarray:
+    """The first n outputs of std::minstd_rand (x_{k+1} = 48271 x_k mod 2^31 - 1; a seed of 0 mod m starts
+    the state at 1, as the standard's linear_congruential_engine does for an increment of 0), uint64.
+    Computed in blocks: x_k = x_0 a^k mod m with a^k = a^(jB) a^(i+1) for k = jB + i + 1."""
+    m, a = _MINSTD_M, _MINSTD_A
+    x0 = seed % m or 1
+    B = 1 << 16
+    pw = np.empty(B, np.uint64)
+    v = 1
+    for i in range(B):
+        v = v * a % m
+        pw[i] = v                      # a^(i+1)
+    nb = -(-n // B)
+    starts = np.empty(nb, np.uint64)
+    s, aB = x0, pow(a, B, m)
+    for j in range(nb):
+        starts[j] = s                  # x_0 a^(jB)
+        s = s * aB % m
+    out = (starts[:, None] * pw[None, :]) % np.uint64(m)
+    return out.reshape(-1)[:n]
+
+
+def furnace_points(n_pts: int, bounds6) -> np.ndarray:
+    """The furnace point-location driver's query points (utilities/furnace/point_location.cpp:49-68): one
+    std::minstd_rand seeded 0 and three std::uniform_real_distribution<float> over the bounds' axes, drawn
+    x, y, z per point.  libstdc++'s uniform_real_distribution<float> takes one engine output per draw
+    (generate_canonical<float, 24>: u = float(x - 1) / float(2^31 - 2), where float(2^31 - 2) = 2^31, and
+    1 - 2^-24 if that rounds to 1) and
+    returns u * (b - a) + a in float32.  float32 [n, 3]."""
+    b = np.asarray(bounds6, np.float32)
+    x = minstd_outputs(3 * n_pts).reshape(n_pts, 3)
+    u = (x - np.uint64(1)).astype(np.float32) / np.float32(float(np.float32(_MINSTD_M - 1)))
+    u = np.where(u >= np.float32(1), np.nextafter(np.float32(1), np.float32(0)), u).astype(np.float32)
+    out = np.empty((n_pts, 3), np.float32)
+    for c in range(3):
+        lo, hi = b[2 * c], b[2 * c + 1]
+        out[:, c] = u[:, c] * (hi - lo) + lo
+    return out

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from devil_ray_amd import synthetic as syn
+from parity_util import band_edge_distance, dfs_leaves
 
 
 # ---------------------------------------------------------------- element evaluation (a13)
@@ -218,8 +219,71 @@ def test_split_affine_identity_element(orc):
     lo, hi = subrefs[:, :3].astype(np.float64), subrefs[:, 3:].astype(np.float64)
     c, half = (lo + hi) / 2, (hi - lo) / 2 * 1.000001
     assert np.allclose(aabbs[:, 0::2], c - half, atol=1e-7) and np.allclose(aabbs[:, 1::2], c + half, atol=1e-7)
-    # first split is along u (ties -> lowest axis), lower half stays in slot 0
-    assert np.array_equal(subrefs[1], [0.5, 0, 0, 1, 1, 1]) or subrefs[1][0] == 0.5
+
+
+# The 33 subrefs [u0, v0, w0, u1, v1, w1] of construct_bvh's split (mesh_utils.cpp:586-639) worked out by hand
+# for two exactly representable elements.  The rule: split the box of largest volume (surface area when the
+# volume is 0), ties to the lowest index (strict >); cut its subref at 0.5 along its longest reference axis
+# (ties to the lowest axis); the lower half stays in place, the upper half is appended.
+# Identity map (a lattice net): every box is its subref, so the volumes halve in generations: split 1 cuts u,
+# splits 2-3 cut v of boxes 0 and 1, splits 4-7 cut w of boxes 0-3, ...; after 31 splits the 32 boxes are
+# [1/4 x 1/4 x 1/2] bricks and split 32 cuts box 0 along w, so box 0 ends as [0,1/4]^3 and box 32 as
+# [0,1/4]^2 x [1/4,1/2].
+_IDENTITY_SUBREFS = [
+    (0, 0, 0, 1, 1, 1), (2, 0, 0, 3, 1, 2), (0, 2, 0, 1, 3, 2), (2, 2, 0, 3, 3, 2), (0, 0, 2, 1, 1, 4),
+    (2, 0, 2, 3, 1, 4), (0, 2, 2, 1, 3, 4), (2, 2, 2, 3, 3, 4), (1, 0, 0, 2, 1, 2), (3, 0, 0, 4, 1, 2),
+    (1, 2, 0, 2, 3, 2), (3, 2, 0, 4, 3, 2), (1, 0, 2, 2, 1, 4), (3, 0, 2, 4, 1, 4), (1, 2, 2, 2, 3, 4),
+    (3, 2, 2, 4, 3, 4), (0, 1, 0, 1, 2, 2), (2, 1, 0, 3, 2, 2), (0, 3, 0, 1, 4, 2), (2, 3, 0, 3, 4, 2),
+    (0, 1, 2, 1, 2, 4), (2, 1, 2, 3, 2, 4), (0, 3, 2, 1, 4, 4), (2, 3, 2, 3, 4, 4), (1, 1, 0, 2, 2, 2),
+    (3, 1, 0, 4, 2, 2), (1, 3, 0, 2, 4, 2), (3, 3, 0, 4, 4, 2), (1, 1, 2, 2, 2, 4), (3, 1, 2, 4, 2, 4),
+    (1, 3, 2, 2, 4, 4), (3, 3, 2, 4, 4, 4), (0, 0, 1, 1, 1, 2)]
+# Flat element (identity in x, y; every control point at z = 0): every box has volume 0, so its surface area
+# 2 |du| |dv| decides, and a cut along w leaves both halves with their parent's area.  After split 4 (box 0
+# along w) all five boxes tie at area 1/2, so split 5 takes box 0 again and cuts it along u (box 5 starts as
+# [1/4,1/2] x [0,1/2] x [0,1/2]), where the identity element takes box 1, the largest volume; the two orders
+# differ from box 5 on.
+_FLAT_SUBREFS = [
+    (0, 0, 0, 1, 1, 1), (2, 0, 0, 3, 1, 2), (0, 2, 0, 1, 3, 2), (2, 2, 0, 3, 3, 2), (0, 0, 2, 1, 1, 4),
+    (1, 0, 0, 2, 1, 2), (2, 0, 2, 3, 1, 4), (3, 0, 0, 4, 1, 2), (0, 2, 2, 1, 3, 4), (1, 2, 0, 2, 3, 2),
+    (2, 2, 2, 3, 3, 4), (3, 2, 0, 4, 3, 2), (1, 0, 2, 2, 1, 4), (3, 0, 2, 4, 1, 4), (1, 2, 2, 2, 3, 4),
+    (3, 2, 2, 4, 3, 4), (0, 1, 0, 1, 2, 2), (2, 1, 0, 3, 2, 2), (0, 3, 0, 1, 4, 2), (2, 3, 0, 3, 4, 2),
+    (0, 1, 2, 1, 2, 4), (1, 1, 0, 2, 2, 2), (2, 1, 2, 3, 2, 4), (3, 1, 0, 4, 2, 2), (0, 3, 2, 1, 4, 4),
+    (1, 3, 0, 2, 4, 2), (2, 3, 2, 3, 4, 4), (3, 3, 0, 4, 4, 2), (1, 1, 2, 2, 2, 4), (3, 1, 2, 4, 2, 4),
+    (1, 3, 2, 2, 4, 4), (3, 3, 2, 4, 4, 4), (0, 0, 1, 1, 1, 2)]
+
+
+def _quarters(t):
+    return np.asarray(t, np.float32) * np.float32(0.25)
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_split_identity_element_known_answer(orc, p):
+    """The hand-derived 33 subrefs of the identity element, in order; each box is its subref x1.000001."""
+    mesh = syn.elements_from_nets([syn.box_net((0, 0, 0), (1, 1, 1), p)], p)
+    aabbs, prims, subrefs = orc.split_elements(mesh)
+    exp = _quarters(_IDENTITY_SUBREFS)
+    assert np.array_equal(subrefs, exp)
+    assert np.array_equal(prims, np.zeros(33, np.int32))
+    lo, hi = exp[:, :3].astype(np.float64), exp[:, 3:].astype(np.float64)
+    c, half = (lo + hi) / 2, (hi - lo) / 2 * 1.000001
+    assert np.abs(aabbs[:, 0::2] - (c - half)).max() < 1e-7 and np.abs(aabbs[:, 1::2] - (c + half)).max() < 1e-7
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_split_flat_element_uses_surface_area(orc, p):
+    """A flat element (zero volume everywhere) is split by surface area, ties to the lowest index: the
+    hand-derived order differs from the identity element's from box 5 on, and the z extent stays 0."""
+    net = syn.box_net((0, 0, 0), (1, 1, 1), p)
+    net[:, 2] = 0.0
+    mesh = syn.elements_from_nets([net], p)
+    aabbs, _, subrefs = orc.split_elements(mesh)
+    exp = _quarters(_FLAT_SUBREFS)
+    assert np.array_equal(subrefs, exp)
+    assert not np.array_equal(exp, _quarters(_IDENTITY_SUBREFS))
+    assert np.all(aabbs[:, 4] == 0) and np.all(aabbs[:, 5] == 0)
+    uv = exp[:, [0, 1, 3, 4]].astype(np.float64)
+    c, half = (uv[:, :2] + uv[:, 2:]) / 2, (uv[:, 2:] - uv[:, :2]) / 2 * 1.000001
+    assert np.abs(aabbs[:, [0, 2]] - (c - half)).max() < 1e-7 and np.abs(aabbs[:, [1, 3]] - (c + half)).max() < 1e-7
 
 
 # ---------------------------------------------------------------- locate (a10-a12, a17)
@@ -292,6 +356,57 @@ def test_locate_bvh_loses_no_hits(orc, make):
         p = mesh.order if mesh.orders is None else int(mesh.orders[e])
         x = orc.eval_d(dofs, p, refs[i])[0]
         assert np.abs(x - pts[i]).max() < 1e-6
+
+
+def _dfs_winners(bvh, boxes, pts):
+    """The element of the first leaf, in depth-first left-first visiting order, whose axis-aligned affine
+    element accepts the point (ref coordinates inside [-1e-4, 1 + 1e-4]); None where a ref coordinate is
+    within 1e-6 of that band's edge in an element the walk reaches (rounding may decide there)."""
+    out = []
+    for p in pts:
+        win = -1
+        for leaf in dfs_leaves(bvh.inner, p):
+            e = int(bvh.leaf_nodes[leaf])
+            lo, hi = boxes[e]
+            r = (p.astype(np.float64) - lo) / (hi - lo)
+            if band_edge_distance(r[None, :])[0] < 1e-6:
+                win = None
+                break
+            if np.all(r >= -1e-4) and np.all(r <= 1 + 1e-4):
+                win = e
+                break
+        out.append(win)
+    return out
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_locate_first_accepting_candidate_wins(orc, p):
+    """DeviceMesh::locate tries the candidate leaves depth-first, left child first, and the first element
+    whose Newton solve is accepted wins (device_mesh.hpp:307-316).  Two overlapping affine elements (every
+    point of the overlap lies inside both) and two elements sharing a face (points within 5e-5 of the face
+    lie inside both acceptance bands): the oracle's winners equal an independent walk over its own emitted
+    nodes, and both elements win somewhere, so a last-wins, right-first or lowest-id rule fails."""
+    cases = [
+        [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)), ((0.5, 0.25, 0.0), (1.5, 1.25, 1.0))],
+        [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)), ((1.0, 0.0, 0.0), (2.0, 1.0, 1.0))],
+    ]
+    rng = np.random.default_rng(17)
+    for boxes in cases:
+        mesh = syn.elements_from_nets([syn.box_net(lo, hi, p) for lo, hi in boxes], p)
+        bvh = orc.build_mesh_bvh(mesh)
+        b = [(np.array(lo, np.float64), np.array(hi, np.float64)) for lo, hi in boxes]
+        if boxes[1][0][0] == 0.5:
+            pts = np.stack([0.5 + 0.5 * rng.random(1500), 0.25 + 0.75 * rng.random(1500), rng.random(1500)], 1)
+        else:
+            pts = np.stack([1.0 + 5e-5 * (2 * rng.random(1500) - 1), rng.random(1500), rng.random(1500)], 1)
+        pts = pts.astype(np.float32)
+        exp = _dfs_winners(bvh, b, pts)
+        keep = np.array([e is not None for e in exp])
+        exp = np.array([e for e in exp if e is not None], np.int32)
+        assert keep.sum() > 1400 and (exp == 0).sum() > 50 and (exp == 1).sum() > 50, ((exp == 0).sum(), (exp == 1).sum())
+        for dt in (np.float32, np.float64):
+            cells, _ = orc.locate(mesh, bvh, pts, dtype=dt)
+            assert np.array_equal(cells[keep], exp)
 
 
 def test_refine_is_exact_inverse(orc):
