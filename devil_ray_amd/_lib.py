@@ -21,6 +21,7 @@ DRX_ERR_ARG = 1
 DRX_INIT_AFFINE, DRX_INIT_MLP, DRX_INIT_SUBREF_CENTRE = 0, 1, 2   # drx_init_mode
 SPLITS_PER_ELEMENT = 33
 MAX_ORDER = 4
+LOCATE_NOALLOC_MAX_ORDERED = 262144   # DRX_LOCATE_NOALLOC_MAX_ORDERED
 
 
 class DrxMesh(C.Structure):
@@ -83,6 +84,7 @@ _SIGS = {
                                   C.POINTER(I32), C.POINTER(I32), P, P]),
     "drx_bvh_widen": (I32, [C.POINTER(DrxBVH), P, I64, C.POINTER(I64), P]),
     "drx_locate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I64, P, C.POINTER(DrxLocateOpts), P]),
+    "drx_locate_noalloc": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I64, P, C.POINTER(DrxLocateOpts), P]),
     "drx_field_eval": (I32, [C.POINTER(DrxMesh), P, I64, P, P]),
     "drx_camera_rays": (I32, [C.POINTER(DrxCamera), P, P]),
     "drx_remove_missed_rays": (I32, [P, I32, P, P, C.POINTER(I32), P]),

@@ -87,15 +87,32 @@ static_assert (sizeof (Location) == 16, "Location layout");
 inline int npe_of (int p) { return (p + 1) * (p + 1) * (p + 1); }
 inline int npe_pad_of (int p) { return (npe_of (p) + 3) & ~3; }
 
-// Exclusive scan of int32 flags (device), n elements; writes scan[n] and *total (host).
-int exclusive_scan_i32 (const int32_t *in, int32_t *out, int32_t n, int32_t *total_host, hipStream_t st);
+// Exclusive scan of int32 flags (device), n elements; writes scan[n] and *total (host).  The block sums
+// live in `sums` (scan_sums_len(n) int32) when given, else in the arena's scan slot.
+int exclusive_scan_i32 (const int32_t *in, int32_t *out, int32_t n, int32_t *total_host, hipStream_t st,
+                        int32_t *sums = nullptr);
+int32_t scan_sums_len (int32_t n);
 
 // Stable LSD radix sort of (key, value) pairs over the low `bits` bits (8-bit digits); ping-pongs between
-// A and B; *keys_out / *vals_out point at whichever holds the result.
+// A and B; *keys_out / *vals_out point at whichever holds the result.  hist holds radix_hist_len(n) int32;
+// scan_sums (nullable: arena) scan_sums_len(256 * radix_tiles(n)) int32.
 int radix_sort_pairs (uint32_t *kA, int32_t *vA, uint32_t *kB, int32_t *vB, int32_t *hist, int n, int bits,
-                      hipStream_t st, uint32_t **keys_out, int32_t **vals_out);
+                      hipStream_t st, uint32_t **keys_out, int32_t **vals_out, int32_t *scan_sums = nullptr);
 int radix_tiles (int n);
-// Permutation of n query points in Morton order over bounds6 (host); lives in the arena's sort slots.
-int morton_order_points (const float *pts, int n, const float *bounds6, hipStream_t st, const int32_t **order_out);
+inline int64_t radix_hist_len (int n) { return 2 * 256 * (int64_t)radix_tiles (n) + 8; }
+
+// Caller-provided buffers for morton_order_points (n keys/values each, hist and scan sums as above).
+struct SortScratch
+{
+  uint32_t *kA, *kB;
+  int32_t *vA, *vB, *hist, *scan_sums;
+};
+// Permutation of n query points in Morton order over bounds6 (host); lives in `ws` when given, else in
+// the arena's sort slots.
+int morton_order_points (const float *pts, int n, const float *bounds6, hipStream_t st, const int32_t **order_out,
+                         const SortScratch *ws = nullptr);
+
+// Resolves the current device's address of drx_locate_noalloc's static scratch array (once per device).
+int locate_noalloc_setup ();
 
 } // namespace drx

@@ -631,7 +631,7 @@ static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) /
 int radix_tiles (int n) { return (n + kSortTile - 1) / kSortTile; }
 
 int radix_sort_pairs (uint32_t *kA, int32_t *vA, uint32_t *kB, int32_t *vB, int32_t *hist, int n, int bits,
-                      hipStream_t st, uint32_t **keys_out, int32_t **vals_out)
+                      hipStream_t st, uint32_t **keys_out, int32_t **vals_out, int32_t *scan_sums)
 {
   const int n_tiles = radix_tiles (n);
   uint32_t *ks = kA, *kd = kB;
@@ -641,7 +641,7 @@ int radix_sort_pairs (uint32_t *kA, int32_t *vA, uint32_t *kB, int32_t *vB, int3
   {
     hipLaunchKernelGGL (k_radix_hist, dim3 (n_tiles), dim3 (kSortBlock), 0, st, ks, n, shift, n_tiles, hist);
     DRX_LAUNCH_CHECK ();
-    int rc = exclusive_scan_i32 (hist, offs, 256 * n_tiles, nullptr, st);
+    int rc = exclusive_scan_i32 (hist, offs, 256 * n_tiles, nullptr, st, scan_sums);
     if (rc != DRX_OK) return rc;
     hipLaunchKernelGGL (k_radix_scatter, dim3 (n_tiles), dim3 (kSortBlock), 0, st, ks, vs, n, shift, n_tiles, offs, kd,
                         vd);
@@ -680,14 +680,23 @@ __global__ void k_point_morton (const float *__restrict__ pts, int n, float minx
   vals[i] = i;
 }
 
-int morton_order_points (const float *pts, int n, const float *bounds6, hipStream_t st, const int32_t **order_out)
+int morton_order_points (const float *pts, int n, const float *bounds6, hipStream_t st, const int32_t **order_out,
+                         const SortScratch *ws)
 {
-  const int n_tiles = radix_tiles (n);
-  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n);
-  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n);
-  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n);
-  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n);
-  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8));
+  uint32_t *kA, *kB;
+  int32_t *vA, *vB, *hist, *sums = nullptr;
+  if (ws)
+  {
+    kA = ws->kA, kB = ws->kB, vA = ws->vA, vB = ws->vB, hist = ws->hist, sums = ws->scan_sums;
+  }
+  else
+  {
+    kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n);
+    kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n);
+    vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n);
+    vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n);
+    hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (size_t)radix_hist_len (n));
+  }
   if (!kA || !kB || !vA || !vB || !hist) return DRX_ERR_HIP;
   float minc[3], inv[3];
   for (int d = 0; d < 3; ++d)
@@ -701,7 +710,7 @@ int morton_order_points (const float *pts, int n, const float *bounds6, hipStrea
   DRX_LAUNCH_CHECK ();
   uint32_t *ks;
   int32_t *vs;
-  int rc = radix_sort_pairs (kA, vA, kB, vB, hist, n, kPointMortonBits, st, &ks, &vs);
+  int rc = radix_sort_pairs (kA, vA, kB, vB, hist, n, kPointMortonBits, st, &ks, &vs, sums);
   if (rc) return rc;
   *order_out = vs;
   return DRX_OK;

@@ -6,6 +6,7 @@
 #include "drx_wave.hpp"
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace drx
@@ -685,6 +686,50 @@ int stream_walk (const drx_bvh *bvh, const drx_locate_opts *opts)
   return (trav == DRX_TRAVERSE_LANE && bvh->wide_nodes) ? 1 : 0;
 }
 } // namespace
+// ---- allocation-free locate ----------------------------------------------------------------------
+// drx_locate_noalloc keeps its scratch in this statically sized device array instead of the arena: the
+// two counters of locate_impl in the first 256 bytes, then the Morton-order buffers for up to
+// DRX_LOCATE_NOALLOC_MAX_ORDERED points.  Every GPU holds its own copy (loaded with the code object,
+// kept until the process ends).  Host code must not pass the array's name to a kernel (on the host it is
+// a stand-in, not a device address): g_scratch_dev[dev] holds the device address hipGetSymbolAddress
+// returned for each device.
+constexpr int64_t kNaOrdered = DRX_LOCATE_NOALLOC_MAX_ORDERED;
+constexpr int64_t kNaHeaderWords = 64;
+constexpr int64_t kNaHistWords = 2 * 256 * ((kNaOrdered + 4095) / 4096) + 8;
+constexpr int64_t kNaSumsWords = (256 * ((kNaOrdered + 4095) / 4096) + 2047) / 2048 + 1;
+constexpr int64_t kNaWords = kNaHeaderWords + 4 * kNaOrdered + ((kNaHistWords + 63) & ~63) + ((kNaSumsWords + 63) & ~63);
+static __device__ __align__ (256) int32_t scratch[kNaWords];
+
+constexpr int kNaDevices = 16;
+static int32_t *g_scratch_dev[kNaDevices];
+static std::mutex g_scratch_mutex;
+
+// Resolves the current device's copy of `scratch` (once per device).  Called by drx_init and by the first
+// drx_locate_noalloc on a device drx_init did not set up.
+int locate_noalloc_setup ()
+{
+  int dev = 0;
+  DRX_HIP (hipGetDevice (&dev));
+  if (dev < 0 || dev >= kNaDevices) return fail (DRX_ERR_STATE, "drx_locate_noalloc: device %d >= %d", dev, kNaDevices);
+  std::lock_guard<std::mutex> lock (g_scratch_mutex);
+  if (g_scratch_dev[dev]) return DRX_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress (&p, HIP_SYMBOL (scratch));
+  if (e != hipSuccess || !p)
+    return fail (DRX_ERR_HIP, "drx_locate_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                 hipGetErrorString (e));
+  g_scratch_dev[dev] = (int32_t *)p;
+  return DRX_OK;
+}
+
+static int32_t *scratch_of_current_device ()
+{
+  int dev = 0;
+  if (hipGetDevice (&dev) != hipSuccess || dev < 0 || dev >= kNaDevices) return nullptr;
+  std::lock_guard<std::mutex> lock (g_scratch_mutex);
+  return g_scratch_dev[dev];
+}
+
 } // namespace drx
 
 using namespace drx;
@@ -728,15 +773,13 @@ int drx_bvh_quantize (const drx_bvh *bvh, uint32_t *out_qnodes, float *out_leaf_
   return DRX_OK;
 }
 
-int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n, void *out_locations,
-                const drx_locate_opts *opts, void *stream)
+// The body of drx_locate and drx_locate_noalloc.  `counters` (nullptr: the arena's flag slot) holds two
+// int32: the traversal-stack overflow flag and the persistent launch's chunk counter; `ws` (nullptr: the
+// arena's sort slots) the buffers of the Morton order.  Arguments are checked by the caller.
+static int locate_impl (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
+                        void *out_locations, const drx_locate_opts *opts, void *stream, int32_t *counters,
+                        const SortScratch *ws)
 {
-  int rc = check_mesh (mesh, 3, "drx_locate");
-  if (rc) return rc;
-  if ((rc = check_bvh (bvh, "drx_locate"))) return rc;
-  if (n < 0) return fail (DRX_ERR_ARG, "drx_locate: n < 0");
-  if (n == 0) return DRX_OK;
-  if (!points || !out_locations) return fail (DRX_ERR_ARG, "drx_locate: null points/output");
   const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
   if (opts && opts->init_mode != DRX_INIT_AFFINE && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
     return fail (DRX_ERR_ARG, "drx_locate: unknown init_mode %d", opts->init_mode);
@@ -746,7 +789,7 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   if (launch < DRX_LAUNCH_AUTO || launch > DRX_LAUNCH_PERSISTENT || chunk < 0)
     return fail (DRX_ERR_ARG, "drx_locate: launch %d / chunk %d", launch, chunk);
   hipStream_t st = as_stream (stream);
-  int32_t *err = (int32_t *)arena (SLOT_FLAGS, 64);
+  int32_t *err = counters ? counters : (int32_t *)arena (SLOT_FLAGS, 64);
   if (!err) return DRX_ERR_HIP;
   int32_t *queue = err + 1;   // chunk counter of the persistent launch
   hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, err, (int64_t)2, 0);
@@ -791,7 +834,7 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   if (opts && opts->coherent_order)
   {
     if (n > (1ll << 31) - 1) return fail (DRX_ERR_ARG, "drx_locate: coherent_order supports < 2^31 points");
-    int rc2 = morton_order_points (points, (int)n, bvh->bounds, st, &order);
+    int rc2 = morton_order_points (points, (int)n, bvh->bounds, st, &order, ws);
     if (rc2) return rc2;
   }
   const MlpNet net = mlp_net (learned ? opts->mlp : nullptr);
@@ -836,6 +879,56 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
   DRX_HIP (hipStreamSynchronize (st));
   if (herr) return fail (DRX_ERR_OVERFLOW, "drx_locate: BVH traversal stack (%d entries) overflowed", kStackDepth);
   return DRX_OK;
+}
+
+static int locate_check (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
+                         void *out_locations, const char *who)
+{
+  int rc = check_mesh (mesh, 3, who);
+  if (rc) return rc;
+  if ((rc = check_bvh (bvh, who))) return rc;
+  if (n < 0) return fail (DRX_ERR_ARG, "%s: n < 0", who);
+  if (n > 0 && (!points || !out_locations)) return fail (DRX_ERR_ARG, "%s: null points/output", who);
+  return DRX_OK;
+}
+
+
+int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n, void *out_locations,
+                const drx_locate_opts *opts, void *stream)
+{
+  int rc = locate_check (mesh, bvh, points, n, out_locations, "drx_locate");
+  if (rc || n == 0) return rc;
+  return locate_impl (mesh, bvh, points, n, out_locations, opts, stream, nullptr, nullptr);
+}
+
+int drx_locate_noalloc (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
+                        void *out_locations, const drx_locate_opts *opts, void *stream)
+{
+  int rc = locate_check (mesh, bvh, points, n, out_locations, "drx_locate_noalloc");
+  if (rc || n == 0) return rc;
+  if (opts && opts->coherent_order && n > kNaOrdered)
+    return fail (DRX_ERR_ARG, "drx_locate_noalloc: coherent_order over %lld points exceeds the scratch limit of %lld "
+                 "(DRX_LOCATE_NOALLOC_MAX_ORDERED); split the batch or use drx_locate",
+                 (long long)n, (long long)kNaOrdered);
+  const int64_t hist_need = radix_hist_len ((int)(n < kNaOrdered ? n : kNaOrdered));
+  const int64_t sums_need = scan_sums_len (256 * radix_tiles ((int)(n < kNaOrdered ? n : kNaOrdered)));
+  if (hist_need > kNaHistWords || sums_need > kNaSumsWords)
+    return fail (DRX_ERR_STATE, "drx_locate_noalloc: scratch layout too small (%lld/%lld hist, %lld/%lld sums)",
+                 (long long)hist_need, (long long)kNaHistWords, (long long)sums_need, (long long)kNaSumsWords);
+  int32_t *base = scratch_of_current_device ();
+  if (!base)
+  {
+    if ((rc = locate_noalloc_setup ())) return rc;
+    base = scratch_of_current_device ();
+  }
+  SortScratch ws;
+  ws.kA = (uint32_t *)(base + kNaHeaderWords);
+  ws.kB = ws.kA + kNaOrdered;
+  ws.vA = (int32_t *)(ws.kB + kNaOrdered);
+  ws.vB = ws.vA + kNaOrdered;
+  ws.hist = ws.vB + kNaOrdered;
+  ws.scan_sums = ws.hist + ((kNaHistWords + 63) & ~63);
+  return locate_impl (mesh, bvh, points, n, out_locations, opts, stream, base, &ws);
 }
 
 int drx_locate_autotune (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,

@@ -181,7 +181,9 @@ __global__ void __launch_bounds__ (kScanBlock) k_scan_down (const int32_t *in, i
   }
 }
 
-int exclusive_scan_i32 (const int32_t *in, int32_t *out, int32_t n, int32_t *total_host, hipStream_t st)
+int32_t scan_sums_len (int32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+int exclusive_scan_i32 (const int32_t *in, int32_t *out, int32_t n, int32_t *total_host, hipStream_t st, int32_t *sums)
 {
   if (n <= 0)
   {
@@ -189,7 +191,7 @@ int exclusive_scan_i32 (const int32_t *in, int32_t *out, int32_t n, int32_t *tot
     return DRX_OK;
   }
   const int32_t nb = (n + kScanTile - 1) / kScanTile;
-  int32_t *sums = (int32_t *)arena (SLOT_SCAN, sizeof (int32_t) * (nb + 1));
+  if (!sums) sums = (int32_t *)arena (SLOT_SCAN, sizeof (int32_t) * (nb + 1));
   if (!sums) return DRX_ERR_HIP;
   hipLaunchKernelGGL (k_scan_reduce, dim3 (nb), dim3 (kScanBlock), 0, st, in, n, sums);
   hipLaunchKernelGGL (k_scan_sums, dim3 (1), dim3 (kScanBlock), 0, st, sums, nb);
@@ -274,7 +276,7 @@ int drx_init (int device)
   if (std::strncmp (prop.gcnArchName, "gfx950", 6) != 0)
     return fail (DRX_ERR_UNSUPPORTED, "drx_init: device %d is %s; this library is built for gfx950 only", device,
                  prop.gcnArchName);
-  return DRX_OK;
+  return locate_noalloc_setup ();
 }
 
 const char *drx_last_error (void) { return g_last_error.c_str (); }

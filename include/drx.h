@@ -270,6 +270,17 @@ int drx_locate (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, i
  * 8-wide per lane, or the learned start's), where DRX_LAUNCH_AUTO and chunk 0 then use it.  out_locations
  * (n Locations) is overwritten.  out_launch, out_chunk: the choice; out_ms (nullable, 10 floats): ms per
  * call of each candidate, chunk-major, chunked before persistent.  Not for the wave walk (DRX_ERR_ARG). */
+/* Allocation-free drx_locate.  No counterpart in dray: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as drx_locate; it calls no hipMalloc/hipFree and keeps its scratch (the
+ * traversal-overflow flag, the persistent launch's chunk counter and, with opts->coherent_order, the Morton
+ * sort buffers) in a static device array of 4.13 MiB that each GPU the library runs on holds until
+ * the process ends.  opts->coherent_order is limited to n <= DRX_LOCATE_NOALLOC_MAX_ORDERED points
+ * (DRX_ERR_ARG beyond it); without it any n is accepted.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (two streams, two host threads) race and must be serialised by the caller. */
+#define DRX_LOCATE_NOALLOC_MAX_ORDERED 262144
+int drx_locate_noalloc (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
+                        void *out_locations, const drx_locate_opts *opts, void *stream);
 int drx_locate_autotune (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
                          void *out_locations, const drx_locate_opts *opts, int32_t *out_launch, int32_t *out_chunk,
                          float *out_ms, void *stream);
