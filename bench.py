@@ -10,8 +10,8 @@ integrate (locate + field eval + LUT + blend, <= 5 partials/ray) -> partial comp
 
 Prints one JSON line.  value = located samples/s over the whole frame (inputs resident in HBM).
 roofline: the integrate kernel, algorithmic bytes = 16 B x (p+1)^3 per located sample (fp32
-geometry + field of the located element, one compulsory fetch, no reuse), timed with HIP events on
-the stream the kernel runs on.  cpu_baseline: the CPU oracle (dray's OpenMP path restated) on a
+geometry + field of the located element, one compulsory fetch, no reuse; 20 B x (p+1)^3 for the 8^3
+order-2 frame, whose scalar field is FP64), timed with HIP events on the stream the kernel runs on.  cpu_baseline: the CPU oracle (dray's OpenMP path restated) on a
 bounded, evenly spread subset of the same frame's active rays, timed on this host's cores.
 """
 from __future__ import annotations
@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy (x2.0, x1.0); the summary with its commands is
 # profiles/k_integrate_p3_64cubed_1024sq_512spp.md.  Keyed by (config, kernel) so that other workloads or
 # kernel variants report null rather than a stale figure.
-PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,false>"): 29.99e9 + 3.34e9}
+PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,0>"): 29.99e9 + 3.34e9}
 
 
 def _cpu_model() -> str:
@@ -69,8 +69,9 @@ def main() -> int:
     ap.add_argument("--mlp-epochs", type=float, default=6.0, help="config 3: passes over the training pairs")
     ap.add_argument("--mlp-eval-points", type=int, default=1 << 22, help="config 3: queries per held-out mesh")
     ap.add_argument("--traversal", type=int, default=0, help="0 auto, 1 per lane (8-wide), 2 wave-coherent, 3 per lane (binary)")
-    ap.add_argument("--error-sample", type=int, default=0,
-                    help="report ref-coordinate / RGBA error vs the FP64 oracle on this many points / rays/10")
+    ap.add_argument("--error-sample", type=int, default=20000,
+                    help="report ref-coordinate / RGBA error vs the FP64 oracle and Newton-step histograms on this "
+                         "many points / rays/10, outside the timed region (0: off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); rays shard across GPUs "
@@ -81,7 +82,6 @@ def main() -> int:
         args.n, args.order, args.res, args.samples = 8, 2, 256, 200
     elif args.config == 4:
         args.n, args.order, args.res, args.samples, args.polish = 128, 4, 2048, 1000, True
-        args.error_sample = args.error_sample or 20000
     elif args.config == 2:
         return bench_locate(args)
     elif args.config == 3:
@@ -101,14 +101,15 @@ def main() -> int:
     n, p, res, samples = args.n, args.order, args.res, args.samples
     t0 = time.time()
     if args.config == 0:
-        mesh = syn.hex_mesh(n, p, 0.05, 0.005, 1234, jitter_scaled=False)
+        mesh = syn.hex_mesh(n, p, 0.05, 0.005, 1234, jitter_scaled=False, keep_field64=True)
     elif args.config == 4:
         mesh = syn.hex_mesh(n, p, 0.25, 0.01, 1234)
     else:
         mesh = syn.hex_mesh(n, p, 0.1, 0.01, 1234)
     t_gen = time.time() - t0
+    fp64_field = args.config == 0   # config 0's FP64 scalar field (dray's Float = double field)
     m = api.UnstructuredMesh(mesh)
-    f = api.UnstructuredField(mesh)
+    f = api.UnstructuredField(mesh, fp64=fp64_field)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -153,10 +154,11 @@ def main() -> int:
     value = located / t_total
     mrays = res * res / (t_total / args.steps) / 1e6
     k_ms = float(np.mean(kernel_ms))
-    bytes_per_sample = 16 * (p + 1) ** 3
+    # fp32 geometry (12 B) + the field (4 B, or 8 B for an FP64 field) per control point of the located element
+    bytes_per_sample = (12 + (8 if fp64_field else 4)) * (p + 1) ** 3
     achieved_gbs = located_per_frame * bytes_per_sample / (k_ms * 1e-3) / 1e9
     walk = int(st.walk)   # the walk the library ran (the automatic choice depends on the frame's size)
-    kernel_name = f"k_integrate<{p},{'true' if args.polish else 'false'},{walk},false>"
+    kernel_name = f"k_integrate<{p},{'true' if args.polish else 'false'},{walk},{2 if fp64_field else 0}>"
 
     out = {
         "metric": "located samples/s (volume render; also Mrays/s)",
@@ -169,13 +171,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" + ("+f64polish" if args.polish else ""),
+        "dtype": "f32" + ("+f64polish" if args.polish else "") + ("+f64field" if fp64_field else ""),
         "data": "synthetic (seeded warped Bernstein hex mesh; no dataset)",
         "config": {"workload": f"volume render, order-{p} hex mesh {n}^3 elements, {res}x{res} "
                                f"{'orthographic' if args.config == 0 else 'perspective'}, "
-                               f"{samples} samples/ray, unlit, grey->red TF opacity 0.02",
+                               f"{samples} samples/ray, unlit, grey->red TF opacity 0.02"
+                               + (", FP64 scalar field" if fp64_field else ""),
                    "elements": n ** 3, "order": p, "bvh_leaves": bvh.n_leaves, "width": res, "height": res,
-                   "samples": samples, "fp64_polish": bool(args.polish)},
+                   "samples": samples, "fp64_polish": bool(args.polish), "fp64_field": fp64_field},
         "mrays_per_s": mrays,
         "located_samples_per_frame": located_per_frame,
         "newton_steps_per_located": st.newton_steps / max(1, st.located_samples),
@@ -202,7 +205,10 @@ def main() -> int:
         ob.aabb_ids = bvh.aabb_ids.cpu().numpy()
         ob.subrefs = bvh.subrefs.cpu().numpy().reshape(-1, 6)
         ob.bounds = np.asarray(bvh.bounds, np.float32)
-        rays = orc.camera_perspective(res, res, 45.0, (centre[0], centre[1], centre[2] + 2.5), centre)
+        if args.config == 0:
+            rays = syn.orthographic_rays(res, res)   # the frame the GPU timed
+        else:
+            rays = orc.camera_perspective(res, res, 45.0, (centre[0], centre[1], centre[2] + 2.5), centre)
         act = orc.remove_missed_rays(rays, ob.bounds)
         lut = vol.table.sample(1024, opacity_factor=10.0 / samples)
         lo, hi = f.range()
@@ -212,13 +218,13 @@ def main() -> int:
         stride = max(1, len(act) // 2000)
         pilot = act[::stride].copy()
         t = time.perf_counter()
-        _, _, pst = orc.integrate(mesh, ob, pilot, sd, lut, lo, hi, nthreads=nthreads)
+        _, _, pst = orc.integrate(mesh, ob, pilot, sd, lut, lo, hi, nthreads=nthreads, field64=fp64_field)
         tp = time.perf_counter() - t
         want = int(len(pilot) * args.cpu_seconds / max(tp, 1e-3))
         stride = max(1, len(act) // max(want, 1))
         sub = act[::stride].copy()
         t = time.perf_counter()
-        _, _, cst = orc.integrate(mesh, ob, sub, sd, lut, lo, hi, nthreads=nthreads)
+        _, _, cst = orc.integrate(mesh, ob, sub, sd, lut, lo, hi, nthreads=nthreads, field64=fp64_field)
         tc = time.perf_counter() - t
         cpu_rate = cst[0] / tc
         out["cpu_baseline"] = {"value": cpu_rate, "unit": "samples/s", "cores": nthreads, "kind": "port",
@@ -261,30 +267,46 @@ def _oracle_bvh(bvh):
 
 
 def error_report(args, api, syn, mesh, m, f, vol, bvh, centre, res, samples):
-    """Reference-coordinate error of GPU locate (same options as the render) against the FP64 oracle's exact
-    inverse, and per-ray RGBA error of GPU integrate against an FP64 oracle integrate, on sampled inputs."""
-    import torch
+    """Outside the timed region: GPU locate (with and without the FP64 polish) on sampled points against the
+    FP64 oracle's exact inverse map (reference-coordinate error, Newton-step histograms per point, polished
+    counts, located-id differences against the FP32 and FP64 oracles over the same BVH, the FP64 Jacobian
+    condition numbers of the located points), and per-ray RGBA of GPU integrate against an FP64 oracle
+    integrate on a spread subset of the frame's rays."""
     from oracle import orc
     from devil_ray_amd.synthetic import LOCATION_DTYPE, PARTIAL_DTYPE
     ob = _oracle_bvh(bvh)
     pts = syn.query_points(args.error_sample, bvh.bounds, seed=99)
-    res_ = {}
+    dp = api.to_device(pts.reshape(-1))
+    res_ = {"points": int(len(pts))}
     for polish in (False, True):
-        loc = m.locate(api.to_device(pts.reshape(-1)), api.LocateOptions(fp64_polish=polish, cond_threshold=1e4))
+        loc, steps, _, pol = m.locate(dp, api.LocateOptions(fp64_polish=polish, cond_threshold=1e4), stats=True,
+                                      polished=True)
         l = loc.cpu().numpy().view(LOCATION_DTYPE)
+        stn = steps.cpu().numpy()
         ok = l["cell"] >= 0
         ex = orc.refine_f64(mesh, pts[ok], l["cell"][ok], l["ref"][ok].astype(np.float64))
         err = np.abs(l["ref"][ok].astype(np.float64) - ex).max(axis=1)
-        res_["polish" if polish else "fp32"] = {"max": float(err.max()), "mean": float(err.mean()),
-                                                 "n_over_1e-5": int((err > 1e-5).sum()), "located": int(ok.sum())}
-    oc, _, steps, _ = orc.locate(mesh, ob, pts, dtype=np.float64, want_steps=True)
-    l = m.locate(api.to_device(pts.reshape(-1)), api.LocateOptions(fp64_polish=True)).cpu().numpy().view(LOCATION_DTYPE)
-    res_["located_id_mismatch_vs_fp64_oracle"] = int((l["cell"] != oc).sum())
-    res_["newton_steps_hist_fp64_oracle"] = np.bincount(steps, minlength=21)[:21].tolist()
+        res_["polish" if polish else "fp32"] = {
+            "max": float(err.max()), "mean": float(err.mean()), "n_over_1e-5": int((err > 1e-5).sum()),
+            "located": int(ok.sum()), "polished": int(pol.cpu().numpy().sum()),
+            "newton_steps_hist_gpu": np.bincount(np.minimum(stn, 40), minlength=41).tolist()}
+        if polish:
+            cells, refs = l["cell"], l["ref"]
+            cond = orc.jacobian_cond_f64(mesh, cells[ok], ex)
+            res_["cond_fp64_located"] = {"max": float(cond.max()), "p99.99": float(np.percentile(cond, 99.99)),
+                                         "median": float(np.median(cond)), "n_over_1e4": int((cond > 1e4).sum()),
+                                         "hist_log10": np.histogram(np.log10(cond), bins=np.arange(0, 9))[0].tolist()}
+    oc32, _, st32, _ = orc.locate(mesh, ob, pts, want_steps=True)
+    oc64, _, st64, _ = orc.locate(mesh, ob, pts, dtype=np.float64, want_steps=True)
+    res_["located_id_mismatch_vs_fp32_oracle"] = int((cells != oc32).sum())
+    res_["located_id_mismatch_vs_fp64_oracle"] = int((cells != oc64).sum())
+    res_["newton_steps_hist_fp32_oracle"] = np.bincount(np.minimum(st32, 40), minlength=41).tolist()
+    res_["newton_steps_hist_fp64_oracle"] = np.bincount(np.minimum(st64, 40), minlength=41).tolist()
     # RGBA on a spread subset of rays
-    rays = orc.camera_perspective(res, res, 45.0, (centre[0], centre[1], centre[2] + 2.5), centre)
     if args.config == 0:
         rays = syn.orthographic_rays(res, res)
+    else:
+        rays = orc.camera_perspective(res, res, 45.0, (centre[0], centre[1], centre[2] + 2.5), centre)
     act = orc.remove_missed_rays(rays, ob.bounds)
     sub = act[:: max(1, len(act) // max(1, args.error_sample // 10))].copy()
     lut = vol.table.sample(1024, opacity_factor=10.0 / samples)
@@ -297,7 +319,8 @@ def error_report(args, api, syn, mesh, m, f, vol, bvh, centre, res, samples):
     gimg = orc.composite(gp, res, res)
     pix = sub["pixel_id"]
     d = np.abs(gimg[pix] - oimg[pix])
-    res_["rgba_vs_fp64"] = {"rays": int(len(sub)), "max": float(d.max()), "mean": float(d.mean())}
+    res_["rgba_vs_fp64"] = {"rays": int(len(sub)), "max": float(d.max()), "mean": float(d.mean()),
+                            "partials_gpu": int(cnt), "partials_fp64_oracle": int(len(op))}
     return res_
 
 

@@ -15,6 +15,7 @@ P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
 F32 = C.c_float
+F64 = C.c_double
 
 DRX_OK = 0
 DRX_ERR_ARG = 1
@@ -26,7 +27,7 @@ LOCATE_NOALLOC_MAX_ORDERED = 262144   # DRX_LOCATE_NOALLOC_MAX_ORDERED
 
 class DrxMesh(C.Structure):
     _fields_ = [("data", P), ("elem_offset", P), ("elem_order", P), ("n_el", I32), ("order", I32),
-                ("ncomp", I32), ("stride", I32)]
+                ("ncomp", I32), ("stride", I32), ("data64", P)]
 
 
 class DrxBVH(C.Structure):
@@ -38,7 +39,7 @@ class DrxBVH(C.Structure):
 class DrxLocateOpts(C.Structure):
     _fields_ = [("init_mode", I32), ("fp64_polish", I32), ("cond_threshold", F32), ("newton_steps", P),
                 ("candidates", P), ("traversal", I32), ("coherent_order", I32), ("mlp", P),
-                ("mlp_newton_steps", I32), ("launch", I32), ("chunk", I32)]
+                ("mlp_newton_steps", I32), ("launch", I32), ("chunk", I32), ("polished", P)]
 
 
 class DrxMlpTrainOpts(C.Structure):
@@ -75,6 +76,7 @@ _SIGS = {
     "drx_mesh_stride": (I32, [I32, I32]),
     "drx_mesh_pack_layout": (I64, [P, I32, I32, P]),
     "drx_mesh_pack": (I32, [P, P, I32, P, P, P, I32, I32, P, P]),
+    "drx_mesh_pack_f64": (I32, [P, P, P, P, P, I32, I32, P, P]),
     "drx_bvh_split_elements": (I32, [P, P, P, P, I32, I32, P, P, P, P]),
     "drx_bvh_build": (I32, [P, P, I32, P, P, P, P, P, P]),
     "drx_bvh_pack_leaves": (I32, [C.POINTER(DrxBVH), P, P]),
@@ -89,8 +91,8 @@ _SIGS = {
     "drx_camera_rays": (I32, [C.POINTER(DrxCamera), P, P]),
     "drx_remove_missed_rays": (I32, [P, I32, P, P, C.POINTER(I32), P]),
     "drx_ray_tile_order": (I32, [P, I32, I32, I32, P, P]),
-    "drx_volume_integrate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I32, P, I32, F32,
-                                   F32, F32, P, C.POINTER(DrxLocateOpts), C.POINTER(DrxShading), P,
+    "drx_volume_integrate": (I32, [C.POINTER(DrxMesh), C.POINTER(DrxMesh), C.POINTER(DrxBVH), P, I32, P, I32, F64,
+                                   F64, F32, P, C.POINTER(DrxLocateOpts), C.POINTER(DrxShading), P,
                                    C.POINTER(I32), C.POINTER(DrxIntegrateStats), P]),
     "drx_default_light": (I32, [C.POINTER(DrxCamera), C.POINTER(DrxPointLight)]),
     "drx_composite": (I32, [P, I32, I32, I32, P, P]),

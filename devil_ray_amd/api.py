@@ -158,13 +158,18 @@ class DeviceGridFunction:
     """dray GridFunction (ctrl_idx + values) uploaded, plus the element-local packed blocks."""
 
     def __init__(self, ctrl_idx: np.ndarray, values: np.ndarray, ncomp: int, order: int,
-                 orders: Optional[np.ndarray] = None, offsets: Optional[np.ndarray] = None):
+                 orders: Optional[np.ndarray] = None, offsets: Optional[np.ndarray] = None, fp64: bool = False):
+        """fp64 (scalar fields only): keep the values in double (dray's Float = double build) and pack
+        FP64 blocks (drx_mesh_pack_f64)."""
         dev = _dev()
+        if fp64 and ncomp != 1:
+            raise L.DrxError("FP64 blocks are for scalar fields only")
+        self.fp64 = fp64
         self.ncomp = ncomp
         self.order = int(order) if orders is None else int(orders.max())
         self.n_el = int(len(orders)) if orders is not None else int(len(ctrl_idx) // (order + 1) ** 3)
         self.ctrl_idx = to_device(ctrl_idx.astype(np.int32))
-        self.values = to_device(values.astype(np.float32).reshape(-1))
+        self.values = to_device(values.astype(np.float64 if fp64 else np.float32).reshape(-1))
         self.orders = to_device(orders.astype(np.int32)) if orders is not None else None
         self.offsets = to_device(offsets.astype(np.int64)) if offsets is not None else None
         lib = L.lib()
@@ -180,6 +185,12 @@ class DeviceGridFunction:
             total = lib.drx_mesh_pack_layout(o32.ctypes.data, self.n_el, ncomp, h_off.ctypes.data)
             self.stride = 0
             self.pack_offsets = to_device(h_off)
+        if fp64:
+            self.packed = torch.empty(max(int(total), 1), dtype=torch.float64, device=dev)
+            L.check(lib.drx_mesh_pack_f64(_ptr(self.ctrl_idx), _ptr(self.values), _ptr(self.orders),
+                                          _ptr(self.offsets), _ptr(self.pack_offsets), int(order), self.n_el,
+                                          _ptr(self.packed), _stream()), "drx_mesh_pack_f64")
+            return
         self.packed = torch.empty(max(int(total), 1), dtype=torch.float32, device=dev)
         L.check(lib.drx_mesh_pack(_ptr(self.ctrl_idx), _ptr(self.values), ncomp, _ptr(self.orders), _ptr(self.offsets),
                                   _ptr(self.pack_offsets), int(order), self.n_el, _ptr(self.packed), _stream()),
@@ -187,7 +198,11 @@ class DeviceGridFunction:
 
     def view(self) -> L.DrxMesh:
         v = L.DrxMesh()
-        v.data = self.packed.data_ptr()
+        if self.fp64:
+            v.data = None
+            v.data64 = self.packed.data_ptr()
+        else:
+            v.data = self.packed.data_ptr()
         v.elem_offset = self.pack_offsets.data_ptr() if self.pack_offsets is not None else None
         v.elem_order = self.orders.data_ptr() if self.orders is not None else None
         v.n_el = self.n_el
@@ -209,7 +224,7 @@ class LocateOptions:
     launch: int = 0         # streamed walks: 0 tuned (drx_locate_autotune) else chunked, 1 chunked, 2 persistent
     chunk: int = 0          # points per wavefront chunk (0: tuned, else 1024)
 
-    def c(self, steps=None, cands=None) -> L.DrxLocateOpts:
+    def c(self, steps=None, cands=None, polished=None) -> L.DrxLocateOpts:
         o = L.DrxLocateOpts()
         modes = {"affine": L.DRX_INIT_AFFINE, "centre": L.DRX_INIT_SUBREF_CENTRE}
         if self.start not in modes:
@@ -219,6 +234,7 @@ class LocateOptions:
         o.cond_threshold = float(self.cond_threshold)
         o.newton_steps = steps.data_ptr() if steps is not None else None
         o.candidates = cands.data_ptr() if cands is not None else None
+        o.polished = polished.data_ptr() if polished is not None else None
         o.traversal = int(self.traversal)
         o.coherent_order = 1 if self.coherent_order else 0
         o.launch = int(self.launch)
@@ -270,19 +286,26 @@ class UnstructuredMesh:
     def bounds(self) -> np.ndarray:
         return self.get_bvh().bounds
 
-    def locate(self, points: torch.Tensor, opts: Optional[LocateOptions] = None, stats: bool = False):
-        """Mesh::locate: points float32 [n,3] (device) -> Location records (uint8 tensor, 16 B each)."""
+    def locate(self, points: torch.Tensor, opts: Optional[LocateOptions] = None, stats: bool = False,
+               polished: bool = False):
+        """Mesh::locate: points float32 [n,3] (device) -> Location records (uint8 tensor, 16 B each).
+        stats: also per-point Newton steps and candidate counts; polished: also the per-point flag of the
+        FP64 correction (appended last)."""
         bvh = self.get_bvh()
         n = int(points.numel() // 3)
         dev = _dev()
         out = torch.empty(max(n, 1) * 16, dtype=torch.uint8, device=dev)
         steps = torch.zeros(max(n, 1), dtype=torch.int32, device=dev) if stats else None
         cands = torch.zeros(max(n, 1), dtype=torch.int32, device=dev) if stats else None
-        o = (opts or LocateOptions()).c(steps, cands)
+        pol = torch.zeros(max(n, 1), dtype=torch.int32, device=dev) if polished else None
+        o = (opts or LocateOptions()).c(steps, cands, pol)
         mv, bv = self.gf.view(), bvh.view()
         L.check(L.lib().drx_locate(C.byref(mv), C.byref(bv), _ptr(points), n, _ptr(out), C.byref(o), _stream()),
                 "drx_locate")
-        return (out, steps, cands) if stats else out
+        res = (out, steps, cands) if stats else (out,)
+        if polished:
+            res = res + (pol,)
+        return res if len(res) > 1 else out
 
     def locate_autotune(self, points: torch.Tensor, opts: Optional[LocateOptions] = None) -> dict:
         """drx_locate_autotune: time every chunk size x launch form of the streamed walk on these points and
@@ -305,16 +328,24 @@ class UnstructuredMesh:
 class UnstructuredField:
     """UnstructuredField<Hex scalar> (data_model/unstructured_field.cpp)."""
 
-    def __init__(self, mesh: HexMesh, values: Optional[np.ndarray] = None):
-        vals = mesh.field if values is None else values
+    def __init__(self, mesh: HexMesh, values: Optional[np.ndarray] = None, fp64: bool = False):
+        """fp64: an FP64 field (dray's Float = double): values default to mesh.field64, kept and evaluated
+        in double; its colour range is the double min/max."""
+        if values is None:
+            values = mesh.field64 if fp64 else mesh.field
+            if values is None:
+                raise L.DrxError("UnstructuredField(fp64=True) needs FP64 values (mesh.field64)")
+        vals = values
         self.host_values = vals
-        self.gf = DeviceGridFunction(mesh.ctrl_idx, vals, 1, mesh.order, mesh.orders, mesh.offsets)
+        self.fp64 = fp64
+        self.gf = DeviceGridFunction(mesh.ctrl_idx, vals, 1, mesh.order, mesh.orders, mesh.offsets, fp64=fp64)
         # the device copy is a snapshot of vals, so its range is fixed here (not recomputed per render)
-        v = np.asarray(vals, np.float32)
+        v = np.asarray(vals, np.float64 if fp64 else np.float32)
         self._range = (float(v.min()), float(v.max())) if v.size else (float("inf"), float("-inf"))
 
     def range(self):
-        """get_range (unstructured_field.cpp:35-90): min/max over the control values (float32)."""
+        """get_range (unstructured_field.cpp:35-90): min/max over the control values (in the field's
+        precision: float32, or double for an FP64 field)."""
         return self._range
 
     def eval(self, locations: torch.Tensor, values: Optional[torch.Tensor] = None) -> torch.Tensor:

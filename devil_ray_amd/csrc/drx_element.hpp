@@ -187,23 +187,25 @@ __device__ __forceinline__ void eval_geom_regs (const float (&C)[3][Ord<P>::NPE]
   }
 }
 
-template <int P>
-__device__ __forceinline__ float eval_scalar (const float *__restrict__ f, const float x[3])
+// Scalar field value at reference point x; FT = float, or double for an FP64 field (evaluated in double
+// at the float32 reference point).
+template <int P, typename FT = float>
+__device__ __forceinline__ FT eval_scalar (const FT *__restrict__ f, const float x[3])
 {
   constexpr int N1 = Ord<P>::N1;
-  float Bu[N1], dBu[N1], Bv[N1], dBv[N1], Bw[N1], dBw[N1];
-  bernstein<P, float> (x[0], Bu, dBu);
-  bernstein<P, float> (x[1], Bv, dBv);
-  bernstein<P, float> (x[2], Bw, dBw);
-  float val = 0.f;
+  FT Bu[N1], dBu[N1], Bv[N1], dBv[N1], Bw[N1], dBw[N1];
+  bernstein<P, FT> ((FT)x[0], Bu, dBu);
+  bernstein<P, FT> ((FT)x[1], Bv, dBv);
+  bernstein<P, FT> ((FT)x[2], Bw, dBw);
+  FT val = 0;
 #pragma unroll
   for (int k = 0; k < N1; ++k)
   {
-    float sv = 0.f;
+    FT sv = 0;
 #pragma unroll
     for (int j = 0; j < N1; ++j)
     {
-      float a = 0.f;
+      FT a = 0;
       const int row = N1 * (j + N1 * k);
 #pragma unroll
       for (int i = 0; i < N1; ++i) a += Bu[i] * f[row + i];
@@ -472,6 +474,7 @@ struct LocateCfg
   int stride;               // uniform: floats per element block
   bool polish;
   float cond_threshold;
+  int32_t *polished_out;    // optional, per point: 1 where the FP64 correction ran (batched locate only)
 };
 
 struct LocateStats

@@ -18,6 +18,7 @@ struct IntegrateArgs
 {
   LocateCfg cfg;
   const float *field;
+  const double *field64;       // FP64 field (SHADE 2)
   const int64_t *field_offsets;
   const Ray *rays;
   const int32_t *order;
@@ -25,6 +26,7 @@ struct IntegrateArgs
   const float4 *lut;
   int lut_size;
   float fmin, fmax, inv_range;
+  double fmin64, fmax64, inv_range64;   // the colour map's range for an FP64 field
   float sample_dist;
   Partial *slots;
   unsigned long long *stats;   // located, newton, candidates, polished
@@ -35,12 +37,12 @@ struct IntegrateArgs
   drx_point_light lights[DRX_MAX_LIGHTS];
 };
 
-template <int P>
-__device__ __forceinline__ float field_at (const IntegrateArgs &a, int cell, const float ref[3])
+template <int P, typename FT = float>
+__device__ __forceinline__ FT field_at (const IntegrateArgs &a, const FT *field, int cell, const float ref[3])
 {
   if constexpr (P == 0)
   {
-    const float *f = a.field + a.field_offsets[cell];
+    const FT *f = field + a.field_offsets[cell];
     switch (a.cfg.orders[cell])
     {
     case 1: return eval_scalar<1> (f, ref);
@@ -50,7 +52,7 @@ __device__ __forceinline__ float field_at (const IntegrateArgs &a, int cell, con
     }
   }
   else
-    return eval_scalar<P> (a.field + (int64_t)cell * Ord<P>::FSTRIDE, ref);
+    return eval_scalar<P> (field + (int64_t)cell * Ord<P>::FSTRIDE, ref);
 }
 
 // MODE: 0 binary per-lane traversal, 1 8-wide per-lane, 2 8-wide wave-coherent (identical results).
@@ -160,9 +162,11 @@ __device__ __forceinline__ float4 lit_color (const IntegrateArgs &a, int cell, c
 constexpr int kIntBlock = DRX_INTEGRATE_BLOCK;   // threads per workgroup of the integrate kernel
 constexpr int kIntWaves = kIntBlock / 64;
 
-template <int P, bool POLISH, int MODE, bool LIT>
+// SHADE: 0 VolumeShader::color over an FP32 field, 1 shaded_color (lit), 2 color over an FP64 field.
+template <int P, bool POLISH, int MODE, int SHADE>
 __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) k_integrate (IntegrateArgs a)
 {
+  constexpr bool LIT = SHADE == 1;
   constexpr bool WAVE = MODE == 2;
   constexpr bool LEARNED = MODE == 3;   // learned Newton start; the whole wavefront stays in the loop
   __shared__ int32_t stack_lds[WAVE ? 1 : kStackLds * kIntBlock];
@@ -332,8 +336,11 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
       {
         if constexpr (LIT)
           blend_nc (rgba, lit_color<P> (a, cell, ref, ray));
+        else if constexpr (SHADE == 2)
+          blend_nc (rgba, color_lookup64 (a.lut, a.lut_size, a.fmin64, a.fmax64, a.inv_range64,
+                                          field_at<P, double> (a, a.field64, cell, ref)));
         else
-          blend_nc (rgba, color_lookup (a.lut, a.lut_size, a.fmin, a.fmax, a.inv_range, field_at<P> (a, cell, ref)));
+          blend_nc (rgba, color_lookup (a.lut, a.lut_size, a.fmin, a.fmax, a.inv_range, field_at<P> (a, a.field, cell, ref)));
         distance += a.sample_dist;
       }
     }
@@ -372,7 +379,7 @@ using namespace drx;
 extern "C" {
 
 int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx_bvh *bvh, const void *active_rays,
-                          int32_t n_rays, const float *lut, int32_t lut_size, float field_min, float field_max,
+                          int32_t n_rays, const float *lut, int32_t lut_size, double field_min, double field_max,
                           float sample_dist, const int32_t *ray_order, const drx_locate_opts *opts,
                           const drx_shading *shading, void *out_partials, int32_t *out_count,
                           drx_integrate_stats *stats, void *stream)
@@ -393,6 +400,9 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   if (learned && !opts->mlp)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
   const bool lit = shading && shading->use_lighting;
+  const bool f64 = field->data64 != nullptr;
+  if (f64 && (lit || learned))
+    return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: an FP64 field is rendered unlit with the plain Newton start");
   if (lit && (shading->n_lights < 0 || shading->n_lights > DRX_MAX_LIGHTS || (shading->n_lights > 0 && !shading->lights)))
     return fail (DRX_ERR_ARG, "drx_volume_integrate: %d lights (at most %d, host array)", shading->n_lights,
                  DRX_MAX_LIGHTS);
@@ -412,19 +422,25 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, (int32_t *)dstats, (int64_t)16, 0);
   IntegrateArgs a;
   a.cfg = make_cfg (mesh, bvh, opts);
+  a.cfg.polished_out = nullptr;   // per-point flags belong to batched locate
   a.field = field->data;
+  a.field64 = field->data64;
   a.field_offsets = field->elem_offset;
   a.rays = (const Ray *)active_rays;
   a.order = ray_order;
   a.n_rays = n_rays;
   a.lut = (const float4 *)lut;
   a.lut_size = lut_size;
-  a.fmin = field_min;
-  a.fmax = field_max;
+  a.fmin = (float)field_min;
+  a.fmax = (float)field_max;
   {
-    // rcp_safe(range length) on the host, float32 (device_color_map.hpp:45, math.hpp:213-217)
-    const float len = field_max - field_min;
+    // rcp_safe(range length) on the host in the colour map's Float (device_color_map.hpp:45, math.hpp:213-217)
+    const float len = a.fmax - a.fmin;
     a.inv_range = 1.0f / ((std::fabs (len) < 1e-8f) ? (std::signbit (len) ? -1e-8f : 1e-8f) : len);
+    const double len64 = field_max - field_min;
+    a.fmin64 = field_min;
+    a.fmax64 = field_max;
+    a.inv_range64 = 1.0 / ((std::fabs (len64) < 1e-8) ? (std::signbit (len64) ? -1e-8 : 1e-8) : len64);
   }
   a.sample_dist = sample_dist;
   a.slots = slots;
@@ -464,13 +480,25 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     else                                                                 \
       hipLaunchKernelGGL ((k_integrate<PP, PO, 0, LT>), g, b, 0, st, a); \
   } while (0)
+#define DRX_INT64(PP, PO)                                                \
+  do                                                                     \
+  {                                                                      \
+    if (mode == 2)                                                       \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 2, 2>), g, b, 0, st, a);  \
+    else if (mode == 1)                                                  \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 1, 2>), g, b, 0, st, a);  \
+    else                                                                 \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 0, 2>), g, b, 0, st, a);  \
+  } while (0)
 #define DRX_INT(PP, PO)          \
   do                             \
   {                              \
-    if (lit)                     \
-      DRX_INT2 (PP, PO, true);   \
+    if (f64)                     \
+      DRX_INT64 (PP, PO);        \
+    else if (lit)                \
+      DRX_INT2 (PP, PO, 1);      \
     else                         \
-      DRX_INT2 (PP, PO, false);  \
+      DRX_INT2 (PP, PO, 0);      \
   } while (0)
   if (a.cfg.polish)
     switch (P)
@@ -491,6 +519,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     default: DRX_INT (4, false); break;
     }
 #undef DRX_INT
+#undef DRX_INT64
 #undef DRX_INT2
   DRX_LAUNCH_CHECK ();
   DRX_HIP (hipEventRecord (ev[1], st));

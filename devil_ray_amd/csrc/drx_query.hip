@@ -54,6 +54,7 @@ __global__ void __launch_bounds__ (256, waves_per_simd (P, POLISH, MODE)) k_loca
   out[i] = loc;
   if (steps_out) steps_out[i] = st.newton;
   if (cand_out) cand_out[i] = st.candidates;
+        if (cfg.polished_out) cfg.polished_out[i] = st.polished;
 }
 
 // Newton start map of every leaf (drx_bvh_leaf_starts): the inverse of its element's corner-affine
@@ -260,6 +261,7 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
         out[i] = loc;
         if (steps_out) steps_out[i] = st.newton;
         if (cand_out) cand_out[i] = st.candidates;
+        if (cfg.polished_out) cfg.polished_out[i] = st.polished;
       }
     }
   }
@@ -324,13 +326,14 @@ k_locate_mlp_stream (LocateCfg cfg, MlpNet net, int budget, const float *__restr
       out[i] = loc;
       if (steps_out) steps_out[i] = st.newton;
       if (cand_out) cand_out[i] = st.candidates;
+        if (cfg.polished_out) cfg.polished_out[i] = st.polished;
     }
   }
 }
 
 
-template <int P>
-__global__ void __launch_bounds__ (256) k_field_eval (const float *__restrict__ field, const int64_t *__restrict__ offs,
+template <int P, typename FT>
+__global__ void __launch_bounds__ (256) k_field_eval (const FT *__restrict__ field, const int64_t *__restrict__ offs,
                                                       const int32_t *__restrict__ orders, const Location *__restrict__ locs,
                                                       int64_t n, float *__restrict__ out)
 {
@@ -340,17 +343,17 @@ __global__ void __launch_bounds__ (256) k_field_eval (const float *__restrict__ 
   if (l.cell == -1) return;   // Field::eval leaves misses untouched
   if constexpr (P == 0)
   {
-    const float *f = field + offs[l.cell];
+    const FT *f = field + offs[l.cell];
     switch (orders[l.cell])
     {
-    case 1: out[i] = eval_scalar<1> (f, l.ref); break;
-    case 2: out[i] = eval_scalar<2> (f, l.ref); break;
-    case 3: out[i] = eval_scalar<3> (f, l.ref); break;
-    default: out[i] = eval_scalar<4> (f, l.ref); break;
+    case 1: out[i] = (float)eval_scalar<1> (f, l.ref); break;
+    case 2: out[i] = (float)eval_scalar<2> (f, l.ref); break;
+    case 3: out[i] = (float)eval_scalar<3> (f, l.ref); break;
+    default: out[i] = (float)eval_scalar<4> (f, l.ref); break;
     }
   }
   else
-    out[i] = eval_scalar<P> (field + (int64_t)l.cell * Ord<P>::FSTRIDE, l.ref);
+    out[i] = (float)eval_scalar<P> (field + (int64_t)l.cell * Ord<P>::FSTRIDE, l.ref);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1007,14 +1010,24 @@ int drx_field_eval (const drx_mesh *field, const void *locations, int64_t n, flo
   hipStream_t st = as_stream (stream);
   const Location *l = (const Location *)locations;
   const dim3 g (nblk (n, 256)), b (256);
-  switch (field->elem_order ? 0 : field->order)
-  {
-  case 0: hipLaunchKernelGGL (k_field_eval<0>, g, b, 0, st, field->data, field->elem_offset, field->elem_order, l, n, out_values); break;
-  case 1: hipLaunchKernelGGL (k_field_eval<1>, g, b, 0, st, field->data, nullptr, nullptr, l, n, out_values); break;
-  case 2: hipLaunchKernelGGL (k_field_eval<2>, g, b, 0, st, field->data, nullptr, nullptr, l, n, out_values); break;
-  case 3: hipLaunchKernelGGL (k_field_eval<3>, g, b, 0, st, field->data, nullptr, nullptr, l, n, out_values); break;
-  default: hipLaunchKernelGGL (k_field_eval<4>, g, b, 0, st, field->data, nullptr, nullptr, l, n, out_values); break;
+#define DRX_FE(FT, PTR)                                                                                               \
+  switch (field->elem_order ? 0 : field->order)                                                                       \
+  {                                                                                                                   \
+  case 0: hipLaunchKernelGGL ((k_field_eval<0, FT>), g, b, 0, st, PTR, field->elem_offset, field->elem_order, l, n, out_values); break; \
+  case 1: hipLaunchKernelGGL ((k_field_eval<1, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
+  case 2: hipLaunchKernelGGL ((k_field_eval<2, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
+  case 3: hipLaunchKernelGGL ((k_field_eval<3, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
+  default: hipLaunchKernelGGL ((k_field_eval<4, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;  \
   }
+  if (field->data64)
+  {
+    DRX_FE (double, field->data64)
+  }
+  else
+  {
+    DRX_FE (float, field->data)
+  }
+#undef DRX_FE
   DRX_LAUNCH_CHECK ();
   return DRX_OK;
 }

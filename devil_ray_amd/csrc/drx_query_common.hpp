@@ -30,6 +30,20 @@ __device__ __forceinline__ void blend_nc (float f[4], const float4 b)   // rende
   f[3] = a + f[3];
 }
 
+// DeviceColorMap::color for Float = double (device_color_map.hpp:61-77): clamp and normalise in double,
+// the normalised value rounded to float32 before it is scaled to an index.
+__device__ __forceinline__ float4 color_lookup64 (const float4 *__restrict__ lut, int size, double mn, double mx,
+                                                  double inv, double scalar)
+{
+#pragma clang fp contract(off)
+  const double s = fmin (mx, fmax (mn, scalar));
+  const double s2 = (scalar != scalar) ? mn : s;
+  const float normalized = (float)((s2 - mn) * inv);
+  int idx = (int)(normalized * (float)(size - 1));
+  idx = min (max (idx, 0), size - 1);
+  return lut[idx];
+}
+
 // DeviceColorMap::color (device_color_map.hpp:61-77)
 __device__ __forceinline__ float4 color_lookup (const float4 *__restrict__ lut, int size, float mn, float mx, float inv,
                                                 float scalar)
@@ -93,8 +107,9 @@ static __global__ void k_fill_i32q (int32_t *p, int64_t n, int32_t v)
 
 static inline int check_mesh (const drx_mesh *m, int ncomp, const char *what)
 {
-  if (!m || !m->data) return fail (DRX_ERR_ARG, "%s: null mesh", what);
+  if (!m || !(m->data || (ncomp == 1 && m->data64))) return fail (DRX_ERR_ARG, "%s: null mesh", what);
   if (m->ncomp != ncomp) return fail (DRX_ERR_ARG, "%s: expected ncomp %d, got %d", what, ncomp, m->ncomp);
+  if (ncomp == 3 && m->data64) return fail (DRX_ERR_ARG, "%s: FP64 blocks are for scalar fields only", what);
   if ((m->elem_order == nullptr) != (m->elem_offset == nullptr))
     return fail (DRX_ERR_ARG, "%s: mixed-order meshes need elem_order and elem_offset", what);
   if (!m->elem_order)
@@ -136,6 +151,7 @@ static inline LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx
   c.stride = m->stride;
   c.polish = o ? (o->fp64_polish != 0) : false;
   c.cond_threshold = o ? o->cond_threshold : 1e4f;
+  c.polished_out = o ? o->polished : nullptr;
   return c;
 }
 

@@ -259,6 +259,24 @@ __global__ void __launch_bounds__ (64) k_mesh_pack (const int32_t *__restrict__ 
   }
 }
 
+// FP64 scalar field: one wavefront per element, lane d copies dofs d, d+64, ... (zero padding up to npe_pad).
+__global__ void __launch_bounds__ (64) k_mesh_pack_f64 (const int32_t *__restrict__ ctrl_idx,
+                                                         const double *__restrict__ values,
+                                                         const int32_t *__restrict__ orders,
+                                                         const int64_t *__restrict__ offsets,
+                                                         const int64_t *__restrict__ out_offsets, int order, int n_el,
+                                                         double *__restrict__ out)
+{
+  const int e = blockIdx.x;
+  if (e >= n_el) return;
+  const int p = orders ? orders[e] : order;
+  const int npe = (p + 1) * (p + 1) * (p + 1);
+  const int npe_pad = (npe + 3) & ~3;
+  const int64_t in_off = offsets ? offsets[e] : (int64_t)e * npe;
+  const int64_t o = out_offsets ? out_offsets[e] : (int64_t)e * npe_pad;
+  for (int d = threadIdx.x; d < npe_pad; d += 64) out[o + d] = (d < npe) ? values[ctrl_idx[in_off + d]] : 0.0;
+}
+
 } // namespace drx
 
 using namespace drx;
@@ -316,6 +334,22 @@ int drx_mesh_pack (const int32_t *ctrl_idx, const float *values, int32_t ncomp, 
     return fail (DRX_ERR_ARG, "drx_mesh_pack: mixed-order meshes need orders, offsets and out_offsets");
   hipLaunchKernelGGL (k_mesh_pack, dim3 (n_el), dim3 (64), 0, as_stream (stream), ctrl_idx, values, ncomp, orders,
                       offsets, out_offsets, order, n_el, out);
+  DRX_LAUNCH_CHECK ();
+  return DRX_OK;
+}
+
+int drx_mesh_pack_f64 (const int32_t *ctrl_idx, const double *values, const int32_t *orders, const int64_t *offsets,
+                       const int64_t *out_offsets, int32_t order, int32_t n_el, double *out, void *stream)
+{
+  if (n_el < 0) return fail (DRX_ERR_ARG, "drx_mesh_pack_f64: n_el < 0");
+  if (n_el == 0) return DRX_OK;
+  if (!ctrl_idx || !values || !out) return fail (DRX_ERR_ARG, "drx_mesh_pack_f64: null pointer");
+  if (!orders && (order < 1 || order > DRX_MAX_ORDER))
+    return fail (DRX_ERR_ARG, "drx_mesh_pack_f64: order %d outside 1..%d", order, DRX_MAX_ORDER);
+  if ((orders == nullptr) != (out_offsets == nullptr) || (orders == nullptr) != (offsets == nullptr))
+    return fail (DRX_ERR_ARG, "drx_mesh_pack_f64: mixed-order meshes need orders, offsets and out_offsets");
+  hipLaunchKernelGGL (k_mesh_pack_f64, dim3 (n_el), dim3 (64), 0, as_stream (stream), ctrl_idx, values, orders, offsets,
+                      out_offsets, order, n_el, out);
   DRX_LAUNCH_CHECK ();
   return DRX_OK;
 }

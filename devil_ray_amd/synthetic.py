@@ -69,6 +69,7 @@ class HexMesh:
     field: np.ndarray           # float32 [n_pts] (same ctrl_idx)
     orders: Optional[np.ndarray] = None    # int32 [n_el] for mixed-order meshes
     offsets: Optional[np.ndarray] = None   # int64 [n_el] start of element e in ctrl_idx
+    field64: Optional[np.ndarray] = None   # float64 [n_pts]: the field unrounded (an FP64 field), if kept
 
     @property
     def n_el(self) -> int:
@@ -115,11 +116,12 @@ def _element_lattice_index(n: int, p: int) -> np.ndarray:
     return (gi + m * (gj + m * gk)).reshape(-1)
 
 
-def hex_mesh(n: int, p: int, amp: float, jitter: float, seed: int, jitter_scaled: bool = True) -> HexMesh:
+def hex_mesh(n: int, p: int, amp: float, jitter: float, seed: int, jitter_scaled: bool = True,
+             keep_field64: bool = False) -> HexMesh:
     """Uniform order-p mesh of n^3 elements on the unit cube (shared global lattice).
 
     jitter_scaled: jitter is ``jitter * (h/p)`` with ``h = 1/n`` (control-point spacing units); otherwise
-    it is absolute.
+    it is absolute.  keep_field64: also keep the field's coefficients unrounded (float64) as ``field64``.
     """
     m = p * n + 1
     g = np.arange(m, dtype=np.float64) / (p * n)
@@ -138,9 +140,10 @@ def hex_mesh(n: int, p: int, amp: float, jitter: float, seed: int, jitter_scaled
             d[(idx == 0) | (idx == m - 1)] = 0.0
             pts[:, c] += d
     values = pts.astype(np.float32)
-    field = field_function(values.astype(np.float64)).astype(np.float32)
+    f64 = field_function(values.astype(np.float64))
+    field = f64.astype(np.float32)
     ctrl = _element_lattice_index(n, p).astype(np.int32)
-    return HexMesh(n=n, order=p, ctrl_idx=ctrl, values=values, field=field)
+    return HexMesh(n=n, order=p, ctrl_idx=ctrl, values=values, field=field, field64=f64 if keep_field64 else None)
 
 
 def mixed_hex_mesh(n: int, amp: float, jitter: float, seed: int, orders=(1, 2, 3, 4)) -> HexMesh:

@@ -76,7 +76,9 @@ typedef struct drx_mesh
   int32_t n_el;
   int32_t order;              /* uniform order; for mixed meshes the maximum order */
   int32_t ncomp;              /* 3 = geometry, 1 = scalar field */
-  int32_t stride;             /* floats per element block (uniform order) */
+  int32_t stride;             /* floats per element block (uniform order); doubles for an FP64 field */
+  const double *data64;       /* scalar fields only, nullable: FP64 blocks [F[npe_pad]] of doubles
+                                 (drx_mesh_pack_f64), used instead of `data` (which may then be NULL) */
 } drx_mesh;
 
 /* BVH in dray's layout (bvh.hpp:15-39): inner node k = 16 floats
@@ -129,6 +131,8 @@ typedef struct drx_locate_opts
   int32_t launch;             /* drx_launch: how drx_locate's streamed walks (binary, 8-wide per lane,
                                  learned start) are launched; the results are the same either way */
   int32_t chunk;              /* points per wavefront chunk for those walks (0: tuned, else 1024) */
+  int32_t *polished;          /* device, nullable: drx_locate writes 1 where the point's accepted solution took
+                                 the FP64 correction (condition estimate > cond_threshold), else 0 */
 } drx_locate_opts;
 
 enum drx_launch
@@ -222,6 +226,12 @@ int drx_mesh_pack (const int32_t *ctrl_idx, const float *values, int32_t ncomp, 
                    const int64_t *offsets, const int64_t *out_offsets, int32_t order, int32_t n_el,
                    float *out, void *stream);
 
+/* An FP64 scalar field (dray built with Float = double, types.hpp:29-33): dray GridFunction (ctrl_idx + one
+ * double per control point) -> packed double blocks [F[npe_pad]] in `out`; orders/offsets/out_offsets as in
+ * drx_mesh_pack (offsets counted in doubles; drx_mesh_pack_layout with ncomp 1 gives them). */
+int drx_mesh_pack_f64 (const int32_t *ctrl_idx, const double *values, const int32_t *orders, const int64_t *offsets,
+                       const int64_t *out_offsets, int32_t order, int32_t n_el, double *out, void *stream);
+
 /* ---- LBVH ---------------------------------------------------------------------------------- */
 /* 33 sub-boxes per element, bit-exact with dray's float32 split: out_aabbs 6*33*n_el floats,
  * out_prims 33*n_el, out_subrefs 6*33*n_el.  Reads dray's GridFunction (ctrl_idx + Vec3 values). */
@@ -284,6 +294,8 @@ int drx_locate_noalloc (const drx_mesh *mesh, const drx_bvh *bvh, const float *p
 int drx_locate_autotune (const drx_mesh *mesh, const drx_bvh *bvh, const float *points, int64_t n,
                          void *out_locations, const drx_locate_opts *opts, int32_t *out_launch, int32_t *out_chunk,
                          float *out_ms, void *stream);
+/* Field::eval: out_values[i] = field at locations[i] (untouched where cell == -1); an FP64 field
+ * (field->data64) is evaluated in double and rounded to float32 on output. */
 int drx_field_eval (const drx_mesh *field, const void *locations, int64_t n, float *out_values, void *stream);
 
 /* ---- rendering ----------------------------------------------------------------------------- */
@@ -298,13 +310,15 @@ int drx_remove_missed_rays (const void *rays, int32_t n, const float *bounds6_ho
 int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t height, int32_t *out_order,
                         void *stream);
 /* Volume::integrate on active rays: samples = |bounds diagonal|/sample_dist steps, field colour from
- * the (already opacity-corrected) LUT of lut_size RGBA entries over [field_min, field_max], lit per
+ * the (already opacity-corrected) LUT of lut_size RGBA entries over [field_min, field_max] (the colour map's
+ * Float range: rounded to float32 for an FP32 field, used in double for an FP64 field, field->data64, which
+ * is evaluated in double as dray's Float = double build does; unlit only), lit per
  * `shading` (NULL = unlit), front-to-back blend, <= 5 partials per ray.  out_partials must hold
  * 5*n_rays partials; on return it holds the compacted partials (depth != inf, ray order) and *out_count
  * (host) their number.  ray_order may be NULL.  stats (host, nullable).  Synchronises. */
 int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx_bvh *bvh,
                           const void *active_rays, int32_t n_rays, const float *lut, int32_t lut_size,
-                          float field_min, float field_max, float sample_dist, const int32_t *ray_order,
+                          double field_min, double field_max, float sample_dist, const int32_t *ray_order,
                           const drx_locate_opts *opts, const drx_shading *shading, void *out_partials,
                           int32_t *out_count, drx_integrate_stats *stats, void *stream);
 /* Renderer::render's cross-domain volume compositing (rendering/renderer.cpp:307-330, done there by

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -893,11 +894,27 @@ static void shaded_color (const MeshView<T> &mesh, const MeshView<T> &field, int
   for (int c = 0; c < 4; ++c) out[c] = acc[c];
 }
 
-template <typename T>
-static void integrate_ray (const MeshView<T> &mesh, const MeshView<T> &field, const BVHView &bvh,
-                           const Ray &ray, T sample_dist, const ColorMapT<T> &cmap, const Light *lights,
-                           int n_lights, Partial *out5, std::vector<T> &scratch, RayStats &rs)
+// T: geometry, reference coordinates and distances; F: the scalar field and the colour map (F = T as in
+// dray, where both are Float; F = double over T = float for an FP64 field on FP32 geometry).
+// mesh64 (nullable): locate each sample point (computed in T) with the FP64 walk and Newton on the same
+// geometry in double, the reference point then rounded to T: which samples are located, and where, as
+// exact arithmetic decides at the sample points of a T-precision march.
+template <typename T, typename F = T>
+static void integrate_ray (const MeshView<T> &mesh, const MeshView<F> &field, const BVHView &bvh,
+                           const Ray &ray, T sample_dist, const ColorMapT<F> &cmap, const Light *lights,
+                           int n_lights, Partial *out5, std::vector<T> &scratch, RayStats &rs,
+                           const MeshView<double> *mesh64 = nullptr)
 {
+  std::vector<F> fscratch;
+  std::vector<double> scratch64;
+  auto locate = [&] (const T pt[3], T ref[3], LocStats *ls) -> i32 {
+    if (!mesh64) return locate_point<T> (mesh, bvh, pt, ref, scratch, ls);
+    const double p64[3] = { double (pt[0]), double (pt[1]), double (pt[2]) };
+    double r64[3] = { 0, 0, 0 };
+    const i32 c = locate_point<double> (*mesh64, bvh, p64, r64, scratch64, ls);
+    for (int d = 0; d < 3; ++d) ref[d] = T (r64[d]);
+    return c;
+  };
   for (int s = 0; s < 5; ++s) { out5[s].pixel_id = -1; out5[s].depth = inf32 (); for (int c = 0; c < 4; ++c) out5[s].rgba[c] = 0.f; }
   T distance = T (ray.near_) + sample_dist;
   int segment = 0;
@@ -914,7 +931,7 @@ static void integrate_ray (const MeshView<T> &mesh, const MeshView<T> &field, co
     {
       T pt[3];
       for (int d = 0; d < 3; ++d) pt[d] = T (ray.orig[d]) + distance * T (ray.dir[d]);
-      cell = locate_point<T> (mesh, bvh, pt, ref, scratch, &ls);
+      cell = locate (pt, ref, &ls);
       if (cell != -1) { found = true; rs.located++; }
       else distance += sample_dist;
     }
@@ -924,15 +941,23 @@ static void integrate_ray (const MeshView<T> &mesh, const MeshView<T> &field, co
     do
     {
       float sc[4];
-      if (lights)   // use_lighting: VolumeShader::shaded_color, else VolumeShader::color (volume.cpp:162-171)
-        shaded_color<T> (mesh, field, cell, ref, ray, cmap, lights, n_lights, scratch, sc);
+      if constexpr (std::is_same<T, F>::value)
+      {
+        if (lights)   // use_lighting: VolumeShader::shaded_color, else VolumeShader::color (volume.cpp:162-171)
+          shaded_color<T> (mesh, field, cell, ref, ray, cmap, lights, n_lights, scratch, sc);
+        else
+          cmap.color (field_eval<T> (field, cell, ref, scratch), sc);
+      }
       else
-        cmap.color (field_eval<T> (field, cell, ref, scratch), sc);
+      {
+        const F rf[3] = { F (ref[0]), F (ref[1]), F (ref[2]) };
+        cmap.color (field_eval<F> (field, cell, rf, fscratch), sc);
+      }
       blend (partial.rgba, sc);
       distance += sample_dist;
       T pt[3];
       for (int d = 0; d < 3; ++d) pt[d] = T (ray.orig[d]) + distance * T (ray.dir[d]);
-      cell = locate_point<T> (mesh, bvh, pt, ref, scratch, &ls);
+      cell = locate (pt, ref, &ls);
       found = cell != -1;
       if (found) rs.located++;
     } while (distance < T (ray.far_) && found && partial.rgba[3] < 0.95f);
@@ -1270,14 +1295,16 @@ float orc_sample_dist (const float *bounds6, int samples)
   return sqrtf (s) / float (samples);
 }
 
-#define ORC_INTEGRATE_BODY(T)                                                                          \
+#define ORC_INTEGRATE_BODY(T, F)                                                                       \
   MeshView<T> m { ctrl_idx, values, orders, offsets, order, n_el };                                   \
-  MeshView<T> f { f_ctrl_idx, f_values, orders, f_offsets, order, n_el };                             \
+  MeshView<double> m64v { ctrl_idx, values64, orders, offsets, order, n_el };                         \
+  const MeshView<double> *m64 = values64 ? &m64v : nullptr;                                           \
+  MeshView<F> f { f_ctrl_idx, f_values, orders, f_offsets, order, n_el };                             \
   BVHView bv { inner, leaf_nodes, aabb_ids, subrefs };                                                \
-  ColorMapT<T> cm;                                                                                    \
-  cm.lut = lut; cm.size = lut_size; cm.mn = T (fmin_); cm.mx = T (fmax_);                             \
-  { const T len = cm.mx - cm.mn;                                                                     \
-    cm.inv_range = T (1) / ((std::fabs (len) < T (1e-8)) ? (std::signbit (len) ? T (-1e-8) : T (1e-8)) : len); } \
+  ColorMapT<F> cm;                                                                                    \
+  cm.lut = lut; cm.size = lut_size; cm.mn = F (fmin_); cm.mx = F (fmax_);                             \
+  { const F len = cm.mx - cm.mn;                                                                     \
+    cm.inv_range = F (1) / ((std::fabs (len) < F (1e-8)) ? (std::signbit (len) ? F (-1e-8) : F (1e-8)) : len); } \
   const Ray *rays = (const Ray *)rays_v;                                                              \
   Partial *parts = (Partial *)out_partials;                                                           \
   const int nt = nthreads_or_default (nthreads);                                                      \
@@ -1289,8 +1316,8 @@ float orc_sample_dist (const float *bounds6, int samples)
     for (i64 i = 0; i < n_rays; ++i)                                                                  \
     {                                                                                                 \
       RayStats rs;                                                                                    \
-      integrate_ray<T> (m, f, bv, rays[i], T (sample_dist), cm, (const Light *)lights, n_lights,       \
-                        parts + 5 * i, scratch, rs);                                                  \
+      integrate_ray<T, F> (m, f, bv, rays[i], T (sample_dist), cm, (const Light *)lights, n_lights,    \
+                        parts + 5 * i, scratch, rs, m64);                                             \
       if (ray_located) ray_located[i] = (i32)rs.located;                                              \
       tl += rs.located; tn += rs.newton; tc += rs.candidates;                                         \
     }                                                                                                 \
@@ -1306,7 +1333,22 @@ int orc_integrate_f32 (const i32 *ctrl_idx, const float *values, const i32 *f_ct
                        const void *rays_v, i64 n_rays, float sample_dist, const float *lut, int lut_size,
                        double fmin_, double fmax_, void *out_partials, i32 *ray_located, i64 *stats3, int nthreads,
                        const float *lights, int n_lights)
-{ ORC_INTEGRATE_BODY (float) }
+{
+  const double *values64 = nullptr;
+  ORC_INTEGRATE_BODY (float, float)
+}
+
+// The FP32 march and colouring, each sample located with the FP64 walk and Newton on values64 (the same
+// geometry in double).
+int orc_integrate_f32_locate64 (const i32 *ctrl_idx, const float *values, const i32 *f_ctrl_idx, const float *f_values,
+                                const i32 *orders, const i64 *offsets, const i64 *f_offsets, int order, int n_el,
+                                const float *inner, const i32 *leaf_nodes, const i32 *aabb_ids, const float *subrefs,
+                                const void *rays_v, i64 n_rays, float sample_dist, const float *lut, int lut_size,
+                                double fmin_, double fmax_, void *out_partials, i32 *ray_located, i64 *stats3,
+                                int nthreads, const float *lights, int n_lights, const double *values64)
+{
+  ORC_INTEGRATE_BODY (float, float)
+}
 
 int orc_integrate_f64 (const i32 *ctrl_idx, const double *values, const i32 *f_ctrl_idx, const double *f_values,
                        const i32 *orders, const i64 *offsets, const i64 *f_offsets, int order, int n_el,
@@ -1314,7 +1356,24 @@ int orc_integrate_f64 (const i32 *ctrl_idx, const double *values, const i32 *f_c
                        const void *rays_v, i64 n_rays, float sample_dist, const float *lut, int lut_size,
                        double fmin_, double fmax_, void *out_partials, i32 *ray_located, i64 *stats3, int nthreads,
                        const float *lights, int n_lights)
-{ ORC_INTEGRATE_BODY (double) }
+{
+  const double *values64 = nullptr;
+  ORC_INTEGRATE_BODY (double, double)
+}
+
+// FP32 geometry with an FP64 scalar field (and an FP64 colour-map range): unlit only (lights ignored).
+int orc_integrate_f32_field64 (const i32 *ctrl_idx, const float *values, const i32 *f_ctrl_idx, const double *f_values,
+                               const i32 *orders, const i64 *offsets, const i64 *f_offsets, int order, int n_el,
+                               const float *inner, const i32 *leaf_nodes, const i32 *aabb_ids, const float *subrefs,
+                               const void *rays_v, i64 n_rays, float sample_dist, const float *lut, int lut_size,
+                               double fmin_, double fmax_, void *out_partials, i32 *ray_located, i64 *stats3,
+                               int nthreads, const float *lights, int n_lights)
+{
+  lights = nullptr;
+  (void)n_lights;
+  const double *values64 = nullptr;
+  ORC_INTEGRATE_BODY (float, double)
+}
 
 // One shaded_color sample (for testing the lit path on its own): element `cell` at reference point ref,
 // ray direction dir; out = RGBA.

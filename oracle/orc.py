@@ -46,6 +46,10 @@ _SIGS = {
                                 P, I32]),
     "orc_integrate_f64": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P, P, I32,
                                 P, I32]),
+    "orc_integrate_f32_field64": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P,
+                                        P, I32, P, I32]),
+    "orc_integrate_f32_locate64": (I32, [P, P, P, P, P, P, P, I32, I32, P, P, P, P, P, I64, F32, P, I32, F64, F64, P, P,
+                                         P, I32, P, I32, P]),
     "orc_default_light": (None, [P, P, P, P]),
     "orc_shade_f32": (None, [P, P, P, P, P, P, P, I32, I32, I32, P, P, P, I32, F64, F64, P, I32, P]),
     "orc_shade_f64": (None, [P, P, P, P, P, P, P, I32, I32, I32, P, P, P, I32, F64, F64, P, I32, P]),
@@ -88,8 +92,9 @@ def max_threads() -> int:
 
 
 # ---- convenience wrappers ---------------------------------------------------------------------
-def _mesh_args(mesh, dtype=np.float32, field=False):
-    vals = np.ascontiguousarray((mesh.field if field else mesh.values).astype(dtype))
+def _mesh_args(mesh, dtype=np.float32, field=False, field64=False):
+    src = (mesh.field64 if field64 else mesh.field) if field else mesh.values
+    vals = np.ascontiguousarray(src.astype(dtype))
     ctrl = np.ascontiguousarray(mesh.ctrl_idx)
     orders = None if mesh.orders is None else np.ascontiguousarray(mesh.orders.astype(np.int32))
     offs = None if mesh.offsets is None else np.ascontiguousarray(mesh.offsets.astype(np.int64))
@@ -195,6 +200,21 @@ def eval_d(dofs: np.ndarray, p: int, ref, dtype=np.float64):
     return val, der
 
 
+def jacobian_cond_f64(mesh, cells, refs) -> np.ndarray:
+    """inf-norm condition number ||J|| ||J^-1|| (numpy.linalg.cond) of each element's FP64 Jacobian at the
+    given reference point (columns dphi/du, dphi/dv, dphi/dw, element.hpp:458-459): the quantity the
+    FP64-polish gate estimates.  Unlocated points (cell < 0) get nan."""
+    cells = np.asarray(cells)
+    out = np.full(len(cells), np.nan)
+    vals = mesh.values.astype(np.float64)
+    for i in np.nonzero(cells >= 0)[0]:
+        e = int(cells[i])
+        p = int(mesh.orders[e]) if mesh.orders is not None else mesh.order
+        _, der = eval_d(vals[mesh.elem_dofs(e)], p, np.asarray(refs[i], np.float64))
+        out[i] = np.linalg.cond(der.T, p=np.inf)
+    return out
+
+
 def lup_solve(A, b, dtype=np.float64):
     A_ = np.ascontiguousarray(np.asarray(A, dtype=dtype))
     b_ = np.ascontiguousarray(np.asarray(b, dtype=dtype))
@@ -271,24 +291,39 @@ def shade(mesh, cell, ref, ray_dir, lut, fmin, fmax, lights, dtype=np.float64, f
     return out
 
 
-def integrate(mesh, bvh, active_rays, sample_dist_, lut, fmin, fmax, dtype=np.float32, nthreads=0, lights=None):
+def integrate(mesh, bvh, active_rays, sample_dist_, lut, fmin, fmax, dtype=np.float32, nthreads=0, lights=None,
+              field64=False, locate64=False):
     """Returns (partials[5*n] uncompacted, per-ray located counts, stats[located, newton, candidates]).
-    lights: None = unlit (VolumeShader::color); else float32 [k, 13] PointLights (shaded_color)."""
+    lights: None = unlit (VolumeShader::color); else float32 [k, 13] PointLights (shaded_color).
+    field64 (unlit, dtype float32): geometry in float32, the field (mesh.field64) and colour map in double.
+    locate64 (dtype float32): the FP32 march, each sample point located by the FP64 walk and Newton."""
     from devil_ray_amd.synthetic import PARTIAL_DTYPE
     n = int(active_rays.shape[0])
     ctrl, vals, orders, offs = _mesh_args(mesh, dtype)
-    fctrl, fvals, _, foffs = _mesh_args(mesh, dtype, field=True)
+    fctrl, fvals, _, foffs = _mesh_args(mesh, np.float64 if field64 else dtype, field=True, field64=field64)
     parts = np.zeros(5 * n, dtype=PARTIAL_DTYPE)
     located = np.zeros(n, np.int32)
     stats = np.zeros(3, np.int64)
     lut_ = np.ascontiguousarray(lut.astype(np.float32))
     rays = np.ascontiguousarray(active_rays)
     lt = None if lights is None else np.ascontiguousarray(np.asarray(lights, np.float32).reshape(-1, 13))
-    fn = lib().orc_integrate_f32 if dtype == np.float32 else lib().orc_integrate_f64
+    extra = ()
+    if field64:
+        if dtype != np.float32 or lights is not None or locate64:
+            raise ValueError("field64 integrates unlit over float32 geometry")
+        fn = lib().orc_integrate_f32_field64
+    elif locate64:
+        if dtype != np.float32:
+            raise ValueError("locate64 is the float32 march with FP64 location")
+        fn = lib().orc_integrate_f32_locate64
+        v64 = np.ascontiguousarray(mesh.values.astype(np.float64))
+        extra = (ptr(v64),)
+    else:
+        fn = lib().orc_integrate_f32 if dtype == np.float32 else lib().orc_integrate_f64
     fn(ptr(ctrl), ptr(vals), ptr(fctrl), ptr(fvals), ptr(orders), ptr(offs), ptr(foffs), mesh.order, mesh.n_el,
        ptr(bvh.inner), ptr(bvh.leaf_nodes), ptr(bvh.aabb_ids), ptr(bvh.subrefs), ptr(rays), n, float(sample_dist_),
        ptr(lut_), lut_.shape[0], float(fmin), float(fmax), ptr(parts), ptr(located), ptr(stats), nthreads,
-       ptr(lt), 0 if lt is None else lt.shape[0])
+       ptr(lt), 0 if lt is None else lt.shape[0], *extra)
     return parts, located, stats
 
 

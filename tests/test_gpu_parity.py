@@ -1,9 +1,10 @@
 """GPU parity: libdrx (through its C ABI) against the CPU oracle on identical seeded inputs.
 
 Bars: BVH (boxes, Morton codes, permutation, emitted nodes) bit-exact; camera rays and ray set-up
-bit-exact; located element ids identical except points the FP64 oracle places within 1e-5 of the
-[-1e-4, 1+1e-4] acceptance band edge; reference coordinates within 1e-5 of the FP64-refined oracle;
-composited RGBA within 2e-3 (one LUT step of a 1024-entry table is ~1e-3 of the colour range).
+bit-exact; located element ids identical except points the FP64 inverse map places within 1e-6 (plus the
+FP32 solution's own error) of the [-1e-4, 1+1e-4] acceptance band edge of an element one side accepted
+(parity_util.explain_mismatches); reference coordinates within 1e-5 of the
+FP64-refined oracle; renders: equal partial counts and located-sample totals, RGBA within RGBA_TOL.
 """
 import numpy as np
 import pytest
@@ -11,11 +12,13 @@ import torch
 
 from devil_ray_amd import synthetic as syn
 from devil_ray_amd.synthetic import LOCATION_DTYPE, PARTIAL_DTYPE, RAY_DTYPE
+from parity_util import explain_mismatches
 
 pytestmark = pytest.mark.gpu
 
 REF_TOL = 1e-5
-RGBA_TOL = 2e-3
+RGBA_TOL = 1e-4      # per channel; FP32 GPU vs FP32 oracle differ by rounding only (observed <= 3e-5)
+LIT_TOL = 2e-4       # lit renders: the shading gradient goes through J^-1, which amplifies rounding
 
 
 def _np(t, dtype=None):
@@ -94,14 +97,8 @@ def test_locate_parity(gpu, orc, name, make):
     for polish in (False, True):
         loc = _np(m.locate(gpu.to_device(pts.reshape(-1)), gpu.LocateOptions(fp64_polish=polish)), LOCATION_DTYPE)
         gc, gr = loc["cell"], loc["ref"]
-        mism = np.nonzero(gc != oc)[0]
-        # a mismatch must be explained by the acceptance band or a slow (non-quadratic) Newton run
-        if len(mism):
-            ex = orc.refine_f64(mesh, pts[mism], np.where(oc[mism] >= 0, oc[mism], gc[mism]),
-                                np.where(oc[mism, None] >= 0, orf[mism], gr[mism]))
-            edge = np.minimum(np.abs(ex + 1e-4), np.abs(ex - (1 + 1e-4))).min(axis=1)
-            ok = (edge < 1e-5) | (ost[mism] >= 12)
-            assert ok.all(), f"unexplained located-id mismatches: {mism[~ok][:10]}"
+        mism, explained, edge, by64 = explain_mismatches(orc, mesh, ob, pts, gc, gr, oc, orf)
+        assert explained.all(), f"unexplained located-id mismatches: {mism[~explained][:10]} excess {edge[~explained][:10]}"
         assert len(mism) <= max(3, 1e-3 * len(pts)), f"{len(mism)} located-id mismatches"
         both = np.nonzero((gc == oc) & (gc >= 0))[0]
         ex = orc.refine_f64(mesh, pts[both], gc[both], orf[both].astype(np.float64))
@@ -183,13 +180,8 @@ def test_locate_centre_start_parity(gpu, orc, name, make):
     loc, steps, _ = m.locate(gpu.to_device(pts.reshape(-1)), gpu.LocateOptions(start="centre"), stats=True)
     loc = _np(loc, LOCATION_DTYPE)
     gc, gr, gst = loc["cell"], loc["ref"], _np(steps)
-    mism = np.nonzero(gc != oc)[0]
-    if len(mism):
-        ex = orc.refine_f64(mesh, pts[mism], np.where(oc[mism] >= 0, oc[mism], gc[mism]),
-                            np.where(oc[mism, None] >= 0, orf[mism], gr[mism]))
-        edge = np.minimum(np.abs(ex + 1e-4), np.abs(ex - (1 + 1e-4))).min(axis=1)
-        ok = (edge < 1e-5) | (ost[mism] >= 12)
-        assert ok.all(), f"unexplained located-id mismatches: {mism[~ok][:10]}"
+    mism, explained, edge, by64 = explain_mismatches(orc, mesh, ob, pts, gc, gr, oc, orf)
+    assert explained.all(), f"unexplained located-id mismatches: {mism[~explained][:10]} excess {edge[~explained][:10]}"
     assert len(mism) <= max(3, 1e-3 * len(pts)), f"{len(mism)} located-id mismatches"
     both = np.nonzero((gc == oc) & (gc >= 0))[0]
     ex = orc.refine_f64(mesh, pts[both], gc[both], orf[both].astype(np.float64))
@@ -235,6 +227,55 @@ def test_fp64_polish_fires(gpu, orc):
     st = vol.last_stats
     print(f"  render: located {st.located_samples} polished {st.polished}", flush=True)
     assert st.located_samples > 0 and st.polished == st.located_samples
+
+
+def _gate_mesh():
+    """Three order-2 elements far apart: a slab 1e-5 thick (cond(J) ~ 1e5 everywhere), a unit cube
+    (cond 1) and a wedge whose thickness grows from 1e-5 to 1e-3 along x (cond from ~1e5 down to ~1e3,
+    crossing 1e4 near x = 0.09), with heights h(u) = 1e-5 + u (1e-3 - 1e-5)."""
+    p = 2
+    slab = syn.box_net((0, 0, 0), (1, 1, 1e-5), p)
+    cube = syn.box_net((3, 0, 0), (4, 1, 1), p)
+    wedge = syn.box_net((6, 0, 0), (7, 1, 1), p)
+    u = (wedge[:, 0] - 6.0)
+    wedge[:, 2] = wedge[:, 2] * (1e-5 + u * (1e-3 - 1e-5))
+    return syn.elements_from_nets(np.stack([slab, cube, wedge]), p)
+
+
+def test_fp64_polish_gate(gpu, orc):
+    """The FP64 correction runs exactly for accepted points whose Jacobian condition number exceeds the
+    threshold (1e4): the per-point polished flags equal (cond_64 > 1e4) for the FP64 Jacobian's inf-norm
+    condition number at the located point (numpy.linalg.cond), except points whose cond_64 lies within 2%
+    of the threshold (the FP32 estimate's rounding); the unit cube polishes nothing, the slab everything."""
+    mesh = _gate_mesh()
+    m = gpu.UnstructuredMesh(mesh)
+    m.get_bvh()
+    rng = np.random.default_rng(17)
+    k = 3000
+    u = rng.random((3, k, 3))
+    slab = np.stack([u[0, :, 0], u[0, :, 1], u[0, :, 2] * 1e-5], 1)
+    cube = np.stack([3 + u[1, :, 0], u[1, :, 1], u[1, :, 2]], 1)
+    h = 1e-5 + u[2, :, 0] * (1e-3 - 1e-5)
+    wedge = np.stack([6 + u[2, :, 0], u[2, :, 1], u[2, :, 2] * h], 1)
+    pts = np.concatenate([slab, cube, wedge]).astype(np.float32)
+    loc, pol = m.locate(gpu.to_device(pts.reshape(-1)), gpu.LocateOptions(fp64_polish=True, cond_threshold=1e4),
+                        polished=True)
+    loc, pol = _np(loc, LOCATION_DTYPE), _np(pol)
+    cells = loc["cell"]
+    assert np.array_equal(cells, np.repeat(np.arange(3, dtype=np.int32), k)), "points left their elements"
+    cond = orc.jacobian_cond_f64(mesh, cells, loc["ref"].astype(np.float64))
+    want = cond > 1e4
+    near = np.abs(cond / 1e4 - 1) < 0.02
+    bad = (pol.astype(bool) != want) & ~near
+    print(f"[polish gate] polished {pol.sum()} of {len(pts)}; cond>1e4 (FP64) {want.sum()}; near threshold "
+          f"{near.sum()}; slab cond {cond[:k].min():.3g}..{cond[:k].max():.3g}, wedge {cond[2 * k:].min():.3g}.."
+          f"{cond[2 * k:].max():.3g}", flush=True)
+    assert not bad.any(), f"{bad.sum()} points polished against the FP64 condition number, e.g. {np.nonzero(bad)[0][:5]}"
+    assert pol[k:2 * k].sum() == 0 and pol[:k].all()
+    assert 0.03 * k < pol[2 * k:].sum() < 0.2 * k   # the wedge crosses the threshold
+    # without the polish option nothing is flagged
+    _, pol0 = m.locate(gpu.to_device(pts.reshape(-1)), gpu.LocateOptions(), polished=True)
+    assert _np(pol0).sum() == 0
 
 
 @pytest.mark.parametrize("name,make", MESHES, ids=[m[0] for m in MESHES])
@@ -413,12 +454,11 @@ def test_volume_integrate_parity(gpu, orc, name, make):
     rays = orc.camera_perspective(w, h, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5))
     gp, img, op, oimg, gst, ost = _render_both(gpu, orc, mesh, rays, w, h, samples=128)
     print(f"  partials gpu={len(gp)} oracle={len(op)} located gpu={gst.located_samples} oracle={ost[0]}", flush=True)
-    assert abs(len(gp) - len(op)) <= max(2, len(op) // 500)
-    assert abs(gst.located_samples - ost[0]) <= max(10, ost[0] // 1000)
+    assert len(gp) == len(op) and np.array_equal(gp["pixel_id"], op["pixel_id"])
+    assert gst.located_samples == ost[0]
     d = np.abs(img - oimg)
     print(f"  rgba max diff {d.max():.3g} mean {d.mean():.3g}", flush=True)
-    assert np.mean(d.max(axis=1) > RGBA_TOL) < 0.01
-    assert d.mean() < 1e-4
+    assert d.max() <= RGBA_TOL
 
 
 @pytest.mark.parametrize("name,make", [MESHES[1], MESHES[2], MESHES[4]], ids=["p2_n4", "p3_n5", "mixed_n4"])
@@ -438,9 +478,9 @@ def test_volume_lit_parity(gpu, orc, name, make):
         d = np.abs(img - oimg)
         print(f"  {name} lights={len(lights)} partials {len(gp)}/{len(op)} rgba max {d.max():.3g} mean {d.mean():.3g}",
               flush=True)
-        assert abs(len(gp) - len(op)) <= max(2, len(op) // 500)
-        assert np.mean(d.max(axis=1) > RGBA_TOL) < 0.01
-        assert d.mean() < 2e-4
+        assert len(gp) == len(op) and np.array_equal(gp["pixel_id"], op["pixel_id"])
+        assert gst.located_samples == ost[0]
+        assert d.max() <= LIT_TOL
         assert img[:, :3].max() > 0.05   # the lit image is not black
 
 
@@ -488,7 +528,7 @@ def test_multi_domain_render(gpu, orc):
                           _np(gparts[1][0], np.uint8)[: gparts[1][1] * 24].view(PARTIAL_DTYPE)["pixel_id"])
     print(f"  pixels covered by both domains {len(both)}; rgba max {d.max():.3g} mean {d.mean():.3g}", flush=True)
     assert len(both) > 50
-    assert np.mean(d.max(axis=1) > RGBA_TOL) < 0.01 and d.mean() < 1e-4
+    assert d.max() <= RGBA_TOL
 
 
 def test_framebuffer_entry_points(gpu, orc):
@@ -567,7 +607,7 @@ def test_renderer_two_domains(gpu, orc, tmp_path):
     ocol, odep = np.zeros((w * h, 4), np.float32), np.full(w * h, np.inf, np.float32)
     orc.partials_to_framebuffer(np.concatenate(oparts), w, h, ocol, odep)
     d = np.abs(colors - ocol)
-    assert np.mean(d.max(axis=1) > RGBA_TOL) < 0.01 and d.mean() < 1e-4
+    assert d.max() <= RGBA_TOL
     assert np.array_equal(np.isfinite(depths), np.isfinite(odep))
     fb.composite_background()
     fb.save(str(tmp_path / "two"))
@@ -584,8 +624,9 @@ def test_volume_orthographic_config0_small(gpu, orc):
     gp, img, op, oimg, gst, ost = _render_both(gpu, orc, mesh, rays, w, h, samples=200, tile=False)
     assert len(gp) == len(op)
     assert np.array_equal(gp["pixel_id"], op["pixel_id"])
+    assert gst.located_samples == ost[0]
     d = np.abs(img - oimg)
-    assert d.max() < 5e-3 and d.mean() < 1e-4
+    assert d.max() <= RGBA_TOL
 
 
 @pytest.mark.parametrize("name,make", MESHES, ids=[m[0] for m in MESHES])

@@ -738,3 +738,37 @@ def test_camera_orbits_closed_forms(orc):
     p, l, _ = orc.camera_orbit((0.5, 0.5, 9.0), (0.5, 0.5, 0.5), up, 30.0, "reset",
                                bounds6=(0, 1, 0, 1, 0, 1), pads=(0.5, 0.0, 0.0))
     assert np.allclose(l, (0.5, 0.5, 0.5)) and np.allclose(p, (0.5, 0.5, 0.5 + np.sqrt(4 + 1 + 1)), atol=1e-6)
+
+
+def test_explain_mismatches_rule(orc):
+    """The parity bar for located ids: a difference is excused only at the acceptance band edge or where the
+    GPU agrees with the FP64 oracle.  A planted wrong GPU id (the right element's neighbour, for a point deep
+    inside its element) is flagged, and the same wrong id on the FP32 oracle's side is excused; a point the
+    FP64 map puts 5e-7 outside the band in one element and inside its neighbour, reported by one side in
+    each, is excused; the same point 1e-4 further out is not."""
+    from parity_util import explain_mismatches
+    mesh = syn.hex_mesh(3, 2, 0.0, 0.0, 1)        # affine 3^3 lattice: element size 1/3, ref = 3 x - cell
+    ob = orc.build_mesh_bvh(mesh)
+    pts = np.array([[0.5, 0.5, 0.5], [0.16, 0.5, 0.83]], np.float32)
+    oc, orf = orc.locate(mesh, ob, pts)
+    assert (oc >= 0).all()
+    gc = oc.copy()
+    gc[0] = oc[0] + 1                              # planted mistake
+    mism, ok, _, by64 = explain_mismatches(orc, mesh, ob, pts, gc, orf, oc, orf)
+    assert list(mism) == [0] and not ok.any() and not by64.any()
+    # the other way round (the GPU right, an FP32 oracle answer the exact map rejects): excused because the
+    # FP64 oracle agrees with the GPU
+    mism, ok, _, by64 = explain_mismatches(orc, mesh, ob, pts, oc, orf, gc, orf)
+    assert list(mism) == [0] and ok.all() and by64.all()
+    # x = 1/3 + 1e-4/3 + 5e-7/3 lies 5e-7 past u = 1 + 1e-4 in element 0 (u = 3x) and at u = 1e-4 + 5e-7
+    # inside element 1 (u = 3x - 1)
+    for shift, want in ((5e-7, True), (1e-4, False)):
+        x = (1 + 1e-4 + shift) / 3.0
+        p = np.array([[x, 0.5, 0.5]], np.float64)
+        p32 = p.astype(np.float32)
+        e0, e1 = 0 + 3 * (1 + 3 * 1), 1 + 3 * (1 + 3 * 1)   # cells (0,1,1) and (1,1,1): i + 3 (j + 3 k)
+        r0 = np.array([[3 * x, 1.5 - 1, 1.5 - 1]], np.float32)
+        r1 = np.array([[3 * x - 1, 0.5, 0.5]], np.float32)
+        mism, ok, excess, by64 = explain_mismatches(orc, mesh, ob, p32, np.array([e0], np.int32), r0,
+                                                    np.array([e1], np.int32), r1)
+        assert list(mism) == [0] and bool(ok[0]) == want and not by64.any(), (shift, excess)
