@@ -95,10 +95,47 @@ BVH LinearBVHBuilder::construct (Array<AABB<>> aabbs, Array<int32> prim_ids)
   return bvh;
 }
 
-// ---- UnstructuredMesh ----------------------------------------------------------------------------------
-UnstructuredMesh::UnstructuredMesh (const GridFunction<3> &dof_data, int32 poly_order)
+// ---- element names, dispatch failures (elem_attr.hpp:394-427, dispatcher.cpp) --------------------------
+template <typename ElemT> std::string element_name ()
+{
+  std::string s = ElemT::get_dim () == 3 ? "3D_" : ElemT::get_dim () == 2 ? "2D_" : "";
+  s += ElemT::get_etype () == ElemType::Tensor ? "Tensor" : "Simplex";
+  return s + "_C" + std::to_string (ElemT::get_ncomp ()) + "_P" + std::to_string (ElemT::get_P ());
+}
+template std::string element_name<Hex3> ();
+template std::string element_name<Hex_P1> ();
+template std::string element_name<Hex_P2> ();
+template std::string element_name<HexScalar> ();
+template std::string element_name<HexScalar_P1> ();
+template std::string element_name<HexScalar_P2> ();
+
+namespace detail
+{
+void cast_mesh_failed (Mesh *mesh, const char *file, unsigned long long line)
+{
+  throw DRayError (std::string ("Cast of mesh '") + (mesh ? mesh->type_name () : std::string ("null")) +
+                   "' failed (" + file + ":" + std::to_string (line) + ")");
+}
+void cast_field_failed (Field *field, const char *file, unsigned long long line)
+{
+  throw DRayError (std::string ("Cast of field '") + (field ? field->type_name () : std::string ("null")) +
+                   "' failed (" + file + ":" + std::to_string (line) + ")");
+}
+} // namespace detail
+
+static void check_fixed_order (int32 P, int32 poly_order, const char *what)
+{
+  if (P >= 1 && poly_order != P)
+    throw DRayError (std::string (what) + ": order " + std::to_string (poly_order) + " given to an order-" +
+                     std::to_string (P) + " element type");
+}
+
+// ---- UnstructuredMesh<ElemT> ------------------------------------------------------------------------------
+template <class ElemT>
+UnstructuredMesh<ElemT>::UnstructuredMesh (const GridFunction<3> &dof_data, int32 poly_order)
 : m_dof_data (dof_data), m_poly_order (poly_order)
 {
+  check_fixed_order (ElemT::get_P (), poly_order, "UnstructuredMesh");
   if (poly_order < 1 || poly_order > DRX_MAX_ORDER)
     throw DRayError ("UnstructuredMesh: order " + std::to_string (poly_order) + " not supported on the GPU path");
   const int32 npe = (poly_order + 1) * (poly_order + 1) * (poly_order + 1);
@@ -107,7 +144,9 @@ UnstructuredMesh::UnstructuredMesh (const GridFunction<3> &dof_data, int32 poly_
   if (m_dof_data.m_size_el == 0) m_dof_data.m_size_el = (int32)(m_dof_data.m_ctrl_idx.size () / npe);
 }
 
-drx_mesh UnstructuredMesh::device_view ()
+template <class ElemT> std::string UnstructuredMesh<ElemT>::type_name () const { return element_name<ElemT> (); }
+
+template <class ElemT> drx_mesh UnstructuredMesh<ElemT>::device_view ()
 {
   const int32 stride = drx_mesh_stride (m_poly_order, 3);
   if (m_packed.size () == 0)
@@ -118,7 +157,7 @@ drx_mesh UnstructuredMesh::device_view ()
                               nullptr, m_poly_order, cells (), m_packed.get_device_ptr (), nullptr),
                "UnstructuredMesh: drx_mesh_pack");
   }
-  drx_mesh v;
+  drx_mesh v {};
   v.data = m_packed.get_device_ptr_const ();
   v.elem_offset = nullptr;
   v.elem_order = nullptr;
@@ -129,7 +168,7 @@ drx_mesh UnstructuredMesh::device_view ()
   return v;
 }
 
-BVH &UnstructuredMesh::get_bvh ()
+template <class ElemT> BVH &UnstructuredMesh<ElemT>::get_bvh ()
 {
   if (!m_is_constructed)
   {
@@ -165,7 +204,7 @@ BVH &UnstructuredMesh::get_bvh ()
   return m_bvh;
 }
 
-drx_bvh UnstructuredMesh::bvh_view ()
+template <class ElemT> drx_bvh UnstructuredMesh<ElemT>::bvh_view ()
 {
   drx_bvh v {};
   v.inner_nodes = (const float *)m_bvh.m_inner_nodes.get_device_ptr_const ();
@@ -190,9 +229,9 @@ drx_bvh UnstructuredMesh::bvh_view ()
   return v;
 }
 
-AABB<> UnstructuredMesh::bounds () { return get_bvh ().m_bounds; }
+template <class ElemT> AABB<> UnstructuredMesh<ElemT>::bounds () { return get_bvh ().m_bounds; }
 
-Array<Location> UnstructuredMesh::locate (Array<Vec<Float, 3>> &wpoints)
+template <class ElemT> Array<Location> UnstructuredMesh<ElemT>::locate (Array<Vec<Float, 3>> &wpoints)
 {
   get_bvh ();
   drx_mesh mv = device_view ();
@@ -209,16 +248,28 @@ Array<Location> UnstructuredMesh::locate (Array<Vec<Float, 3>> &wpoints)
   return locs;
 }
 
-// ---- UnstructuredField -----------------------------------------------------------------------------------
-UnstructuredField::UnstructuredField (const GridFunction<1> &dof_data, int32 poly_order, const std::string &name)
-: m_dof_data (dof_data), m_poly_order (poly_order), m_name (name)
+
+template class UnstructuredMesh<Hex3>;
+template class UnstructuredMesh<Hex_P1>;
+template class UnstructuredMesh<Hex_P2>;
+
+// ---- UnstructuredField<ElemT> ------------------------------------------------------------------------------
+template <class ElemT>
+UnstructuredField<ElemT>::UnstructuredField (const GridFunction<ElemT::get_ncomp ()> &dof_data, int32 poly_order,
+                                             const std::string &name)
+: m_dof_data (dof_data), m_poly_order (poly_order)
 {
+  static_assert (ElemT::get_ncomp () == 1, "the GPU path evaluates scalar fields");
+  check_fixed_order (ElemT::get_P (), poly_order, "UnstructuredField");
+  m_name = name;
   const int32 npe = (poly_order + 1) * (poly_order + 1) * (poly_order + 1);
   if (m_dof_data.m_el_dofs == 0) m_dof_data.m_el_dofs = npe;
   if (m_dof_data.m_size_el == 0) m_dof_data.m_size_el = (int32)(m_dof_data.m_ctrl_idx.size () / npe);
 }
 
-Range UnstructuredField::range () const
+template <class ElemT> std::string UnstructuredField<ElemT>::type_name () const { return element_name<ElemT> (); }
+
+template <class ElemT> std::vector<Range> UnstructuredField<ElemT>::range () const
 {
   const Vec<Float, 1> *v = m_dof_data.m_values.get_host_ptr_const ();
   Range r = { INFINITY, -INFINITY };
@@ -227,10 +278,10 @@ Range UnstructuredField::range () const
     r.m_min = std::min (r.m_min, v[i][0]);
     r.m_max = std::max (r.m_max, v[i][0]);
   }
-  return r;
+  return std::vector<Range> { r };
 }
 
-drx_mesh UnstructuredField::device_view ()
+template <class ElemT> drx_mesh UnstructuredField<ElemT>::device_view ()
 {
   const int32 stride = drx_mesh_stride (m_poly_order, 1);
   if (m_packed.size () == 0)
@@ -241,7 +292,7 @@ drx_mesh UnstructuredField::device_view ()
                               nullptr, m_poly_order, m_dof_data.m_size_el, m_packed.get_device_ptr (), nullptr),
                "UnstructuredField: drx_mesh_pack");
   }
-  drx_mesh v;
+  drx_mesh v {};
   v.data = m_packed.get_device_ptr_const ();
   v.elem_offset = nullptr;
   v.elem_order = nullptr;
@@ -252,7 +303,7 @@ drx_mesh UnstructuredField::device_view ()
   return v;
 }
 
-void UnstructuredField::eval (const Array<Location> locs, Array<Float> &values)
+template <class ElemT> void UnstructuredField<ElemT>::eval (const Array<Location> locs, Array<Float> &values)
 {
   if (values.size () != locs.size ()) values.resize (locs.size ());
   drx_mesh fv = device_view ();
@@ -260,6 +311,10 @@ void UnstructuredField::eval (const Array<Location> locs, Array<Float> &values)
     drx_check (drx_field_eval (&fv, locs.get_device_ptr_const (), (int64_t)locs.size (), values.get_device_ptr (), nullptr),
                "Field::eval");
 }
+
+template class UnstructuredField<HexScalar>;
+template class UnstructuredField<HexScalar_P1>;
+template class UnstructuredField<HexScalar_P2>;
 
 // ---- ColorTable ------------------------------------------------------------------------------------------
 void ColorTable::add_point (float32 position, const Vec3f &c)
@@ -476,8 +531,35 @@ static float sample_distance (const float *bounds6, int32 samples)
   return std::sqrt (s) / float (samples);
 }
 
-// ---- DataSet / Collection ------------------------------------------------------------------------------------
-void DataSet::add_field (std::shared_ptr<UnstructuredField> field) { m_fields.push_back (std::move (field)); }
+// ---- DataSet (data_model/data_set.cpp) ----------------------------------------------------------------------
+bool DataSet::has_mesh (const std::string &name) const
+{
+  for (const auto &m : m_meshes)
+    if (m->name () == name) return true;
+  return false;
+}
+
+Mesh *DataSet::mesh (const int32 index)
+{
+  if (index < 0 || index >= number_of_meshes ())
+    throw DRayError ("DataSet: no mesh at index " + std::to_string (index) + " (" + std::to_string (number_of_meshes ()) +
+                     " meshes)");
+  return m_meshes[index].get ();
+}
+
+Mesh *DataSet::mesh (const std::string name)
+{
+  for (auto &m : m_meshes)
+    if (m->name () == name) return m.get ();
+  throw DRayError ("DataSet: no mesh named '" + name + "'");
+}
+
+std::vector<std::string> DataSet::meshes () const
+{
+  std::vector<std::string> out;
+  for (const auto &m : m_meshes) out.push_back (m->name ());
+  return out;
+}
 
 bool DataSet::has_field (const std::string &name) const
 {
@@ -486,11 +568,21 @@ bool DataSet::has_field (const std::string &name) const
   return false;
 }
 
-UnstructuredField *DataSet::field (const std::string &name)
+Field *DataSet::field (const std::string &name) { return field_shared (name).get (); }
+
+Field *DataSet::field (const int &index) { return field_shared (index).get (); }
+
+std::shared_ptr<Field> DataSet::field_shared (const std::string &name)
 {
   for (auto &f : m_fields)
-    if (f->name () == name) return f.get ();
+    if (f->name () == name) return f;
   throw DRayError ("No field named '" + name + "'");
+}
+
+std::shared_ptr<Field> DataSet::field_shared (const int &index)
+{
+  if (index < 0 || index >= number_of_fields ()) throw DRayError ("DataSet: bad field index " + std::to_string (index));
+  return m_fields[index];
 }
 
 std::vector<std::string> DataSet::fields () const
@@ -500,6 +592,14 @@ std::vector<std::string> DataSet::fields () const
   return out;
 }
 
+std::string DataSet::field_info ()
+{
+  std::string s;
+  for (auto &f : m_fields) s += f->name () + " " + f->type_name () + "\n";
+  return s;
+}
+
+// ---- Collection (data_model/collection.cpp) ----------------------------------------------------------------
 DataSet Collection::domain (int32 index)
 {
   if (index < 0 || index >= size ()) throw DRayError ("Collection: bad domain index " + std::to_string (index));
@@ -516,7 +616,7 @@ bool Collection::has_field (const std::string &name)
 Range Collection::range (const std::string &name)
 {
   Range r;
-  for (auto &d : m_domains) r.include (d.field (name)->range ());
+  for (auto &d : m_domains) r.include (d.field (name)->range ()[0]);
   return r;
 }
 
@@ -564,14 +664,22 @@ Array<VolumePartial> Volume::integrate (Array<Ray> &rays, Array<PointLight> &lig
   if (m_use_lighting && lights.size () > DRX_MAX_LIGHTS)
     throw DRayError ("Volume::integrate: at most " + std::to_string (DRX_MAX_LIGHTS) + " lights");
   DataSet ds = m_collection.domain (m_active_domain);
-  UnstructuredMesh &mesh = *ds.mesh ();
-  UnstructuredField &fld = *ds.field (m_field);
+  // the domain's mesh and field as their derived types (dray's IntegratePartialsFunctor through dispatch_3d)
+  drx_mesh mv {}, fv {};
+  drx_bvh bv {};
+  AABB<> mb;
+  auto views = [&] (auto &mesh, auto &field) {
+    mb = mesh.bounds ();   // builds the BVH on first use
+    mv = mesh.device_view ();
+    bv = mesh.bvh_view ();
+    fv = field.device_view ();
+  };
+  dispatch_3d (ds.mesh (), ds.field (m_field), views);
   drx_shading shading;
   shading.use_lighting = m_use_lighting ? 1 : 0;
   shading.n_lights = (int32)lights.size ();
   shading.lights = lights.size () ? lights.get_host_ptr_const () : nullptr;
   const Range range = m_color_map.scalar_range ();
-  const AABB<> mb = mesh.bounds ();
   const float mesh6[6] = { mb.m_ranges[0].m_min, mb.m_ranges[0].m_max, mb.m_ranges[1].m_min,
                            mb.m_ranges[1].m_max, mb.m_ranges[2].m_min, mb.m_ranges[2].m_max };
   const float bounds6[6] = { m_bounds.m_ranges[0].m_min, m_bounds.m_ranges[0].m_max, m_bounds.m_ranges[1].m_min,
@@ -598,9 +706,6 @@ Array<VolumePartial> Volume::integrate (Array<Ray> &rays, Array<PointLight> &lig
   }
   Array<VolumePartial> partials ((size_t)std::max (1, 5 * n_active));
   int32 count = 0;
-  drx_mesh mv = mesh.device_view ();
-  drx_mesh fv = fld.device_view ();
-  drx_bvh bv = mesh.bvh_view ();
   drx_locate_opts o {};
   o.cond_threshold = 1e4f;
   drx_check (drx_volume_integrate (&mv, &fv, &bv, active.get_device_ptr_const (), n_active,

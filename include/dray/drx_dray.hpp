@@ -3,8 +3,10 @@
 // Class and method names, argument meaning and error behaviour follow LLNL/devil_ray (src/dray/):
 //   Array<T>                 array.hpp:19-42 (host/device mirror, lazy copies, shallow shared copies)
 //   LinearBVHBuilder         linear_bvh_builder.hpp:16-22
-//   UnstructuredMesh         data_model/unstructured_mesh.hpp:117-135 (locate :84, get_bvh, bounds)
-//   UnstructuredField        data_model/unstructured_field.hpp (eval, range)
+//   Mesh, Field              data_model/mesh.hpp:18-36, data_model/field.hpp:19-54 (abstract bases)
+//   UnstructuredMesh<E>      data_model/unstructured_mesh.hpp:84-135 (locate, get_bvh, bounds; HexMesh[_P1|_P2])
+//   UnstructuredField<E>     data_model/unstructured_field.hpp:30-93 (eval, range; HexScalar[_P1|_P2])
+//   dispatch_3d              dispatcher.hpp:208-220 (Mesh* / Field* -> derived types)
 //   ColorTable               color_table.hpp (add_point, add_alpha, set_smooth, correct_opacity, sample)
 //   Camera                   rendering/camera.hpp (set_width/height/fov/pos/look_at/up, create_rays,
 //                            reset_to_bounds, elevate, azimuth)
@@ -175,7 +177,40 @@ public:
   BVH construct (Array<AABB<>> aabbs, Array<int32> prim_ids);
 };
 
-// ---- mesh / field --------------------------------------------------------------------------------------
+// ---- elements (data_model/element.hpp, elem_attr.hpp): the type tags meshes and fields are templated on ----
+enum class ElemType
+{
+  Tensor,
+  Simplex
+};
+namespace Order
+{
+enum : int32
+{
+  General = -1,   // any order, given at run time (dray's Order::General)
+  Constant = 0,
+  Linear = 1,
+  Quadratic = 2
+};
+}
+// Element<topological dims, components, shape, order>: dray's element type, here a tag only (the GPU
+// kernels evaluate every order 1..DRX_MAX_ORDER with one code path per order).
+template <int32 dim, int32 ncomp, ElemType etype, int32 P> struct Element
+{
+  static constexpr int32 get_dim () { return dim; }
+  static constexpr int32 get_ncomp () { return ncomp; }
+  static constexpr ElemType get_etype () { return etype; }
+  static constexpr int32 get_P () { return P; }   // Order::General (-1) or the fixed order
+};
+using Hex3 = Element<3, 3, ElemType::Tensor, Order::General>;          // unstructured_mesh.hpp:117-119
+using Hex_P1 = Element<3, 3, ElemType::Tensor, Order::Linear>;
+using Hex_P2 = Element<3, 3, ElemType::Tensor, Order::Quadratic>;
+using HexScalar = Element<3, 1, ElemType::Tensor, Order::General>;     // unstructured_field.hpp:90-93
+using HexScalar_P1 = Element<3, 1, ElemType::Tensor, Order::Linear>;
+using HexScalar_P2 = Element<3, 1, ElemType::Tensor, Order::Quadratic>;
+template <typename ElemT> std::string element_name ();   // "Hex_General", "Hex_P1", ... (dispatcher.hpp)
+
+// ---- grid function (data_model/grid_function.hpp) ------------------------------------------------------------
 template <int32 NC> struct GridFunction
 {
   Array<int32> m_ctrl_idx;          // (p+1)^3 per element, dof i + (p+1)(j + (p+1)k)
@@ -184,15 +219,62 @@ template <int32 NC> struct GridFunction
   int32 m_size_el = 0;
 };
 
-class UnstructuredMesh
+// ---- Mesh and Field: dray's abstract bases (data_model/mesh.hpp:18-36, data_model/field.hpp:19-54) -----------
+// The Conduit export (to_node) is left out: this build has no Conduit (I/O is outside the path).
+class Mesh
+{
+protected:
+  std::string m_name;
+
+public:
+  virtual ~Mesh () {}
+  std::string name () const { return m_name; }
+  void name (const std::string &name) { m_name = name; }
+  virtual std::string type_name () const = 0;
+  virtual int32 cells () const = 0;
+  virtual int32 order () const = 0;
+  virtual int32 dims () const = 0;
+  virtual AABB<3> bounds () = 0;
+  virtual Array<Location> locate (Array<Vec<Float, 3>> &wpoints) = 0;
+};
+
+class Field
+{
+protected:
+  std::string m_name;
+  std::string m_mesh;   // each field is associated with one mesh
+
+public:
+  virtual ~Field () {}
+  std::string name () const { return m_name; }
+  void name (const std::string &name) { m_name = name; }
+  void mesh_name (const std::string &name) { m_mesh = name; }
+  std::string mesh_name () const { return m_mesh; }
+  virtual std::vector<Range> range () const = 0;
+  virtual int32 order () const = 0;
+  virtual int32 components () const = 0;
+  virtual std::string type_name () const = 0;
+  virtual void eval (const Array<Location> locs, Array<Float> &values) = 0;
+};
+
+// ---- UnstructuredMesh<Element> (data_model/unstructured_mesh.hpp:84-135) ------------------------------------
+// Instantiated for Hex3 (any order 1..DRX_MAX_ORDER), Hex_P1 and Hex_P2 (the order must match).
+template <class ElemT> class UnstructuredMesh : public Mesh
 {
 public:
+  static constexpr int32 dim = ElemT::get_dim ();
+  using ElementType = ElemT;
+
+  UnstructuredMesh () = delete;
   UnstructuredMesh (const GridFunction<3> &dof_data, int32 poly_order);
-  int32 cells () const { return m_dof_data.m_size_el; }
-  int32 order () const { return m_poly_order; }
+  std::string type_name () const override;
+  int32 cells () const override { return m_dof_data.m_size_el; }
+  int32 order () const override { return m_poly_order; }
+  int32 dims () const override { return dim; }
+  AABB<3> bounds () override;                        // = get_bvh().m_bounds
+  Array<Location> locate (Array<Vec<Float, 3>> &wpoints) override;
   BVH &get_bvh ();                                   // built on first use (unstructured_mesh.cpp:23-31)
-  AABB<> bounds ();                                  // = get_bvh().m_bounds
-  Array<Location> locate (Array<Vec<Float, 3>> &wpoints);
+  GridFunction<3> get_dof_data () { return m_dof_data; }
   Array<AABB<>> &ref_aabbs () { return m_ref_aabbs; }
   // libdrx views
   drx_mesh device_view ();
@@ -206,22 +288,87 @@ private:
   Array<AABB<>> m_ref_aabbs;   // SubRef boxes (lo xyz, hi xyz), 6 floats like AABB
   Array<float32> m_packed;
 };
+using HexMesh = UnstructuredMesh<Hex3>;
+using HexMesh_P1 = UnstructuredMesh<Hex_P1>;
+using HexMesh_P2 = UnstructuredMesh<Hex_P2>;
 
-class UnstructuredField
+// ---- UnstructuredField<Element> (data_model/unstructured_field.hpp:30-88) -----------------------------------
+template <class ElemT> class UnstructuredField : public Field
 {
 public:
-  UnstructuredField (const GridFunction<1> &dof_data, int32 poly_order, const std::string &name);
-  const std::string &name () const { return m_name; }
-  Range range () const;                              // min/max of the control values
-  void eval (const Array<Location> locs, Array<Float> &values);
+  using ElementType = ElemT;
+
+  UnstructuredField () = delete;
+  UnstructuredField (const GridFunction<ElemT::get_ncomp ()> &dof_data, int32 poly_order, const std::string &name = "");
+  std::vector<Range> range () const override;        // min/max of the control values per component
+  int32 order () const override { return m_poly_order; }
+  int32 components () const override { return ElemT::get_ncomp (); }
+  std::string type_name () const override;
+  void eval (const Array<Location> locs, Array<Float> &values) override;
+  GridFunction<ElemT::get_ncomp ()> get_dof_data () { return m_dof_data; }
   drx_mesh device_view ();
 
 private:
-  GridFunction<1> m_dof_data;
+  GridFunction<ElemT::get_ncomp ()> m_dof_data;
   int32 m_poly_order;
-  std::string m_name;
   Array<float32> m_packed;
 };
+
+// ---- dispatch_3d (dispatcher.hpp:208-220): Mesh* / Field* -> the derived types, then func(mesh, field) -------
+namespace detail
+{
+[[noreturn]] void cast_mesh_failed (Mesh *mesh, const char *file, unsigned long long line);
+[[noreturn]] void cast_field_failed (Field *field, const char *file, unsigned long long line);
+
+template <typename DerivedMeshT, typename Functor> void dispatch_scalar_field (Field *field, DerivedMeshT *mesh, Functor &func)
+{
+  using MElemT = typename DerivedMeshT::ElementType;
+  using S = Element<MElemT::get_dim (), 1, MElemT::get_etype (), Order::General>;
+  using S1 = Element<MElemT::get_dim (), 1, MElemT::get_etype (), Order::Linear>;
+  using S2 = Element<MElemT::get_dim (), 1, MElemT::get_etype (), Order::Quadratic>;
+  if (auto *f = dynamic_cast<UnstructuredField<S> *> (field))
+    func (*mesh, *f);
+  else if (auto *f1 = dynamic_cast<UnstructuredField<S1> *> (field))
+    func (*mesh, *f1);
+  else if (auto *f2 = dynamic_cast<UnstructuredField<S2> *> (field))
+    func (*mesh, *f2);
+  else
+    cast_field_failed (field, __FILE__, __LINE__);
+}
+
+template <typename MeshGuessT, typename Functor> bool dispatch_mesh_field (MeshGuessT *, Mesh *mesh, Field *field, Functor &func)
+{
+  MeshGuessT *derived = dynamic_cast<MeshGuessT *> (mesh);
+  if (derived) dispatch_scalar_field (field, derived, func);
+  return derived != nullptr;
+}
+
+template <typename MeshGuessT, typename Functor> bool dispatch_mesh_only (MeshGuessT *, Mesh *mesh, Functor &func)
+{
+  MeshGuessT *derived = dynamic_cast<MeshGuessT *> (mesh);
+  if (derived) func (*derived);
+  return derived != nullptr;
+}
+} // namespace detail
+
+// Hexahedral meshes of any order (HexMesh, HexMesh_P1, HexMesh_P2) with a scalar field; dray's tet meshes
+// are outside this path (its GPU kernels are hex-only), so a tet mesh reaches cast_mesh_failed.
+template <typename Functor> bool dispatch_3d (Mesh *mesh, Field *field, Functor &func)
+{
+  if (!detail::dispatch_mesh_field ((HexMesh *)0, mesh, field, func) &&
+      !detail::dispatch_mesh_field ((HexMesh_P1 *)0, mesh, field, func) &&
+      !detail::dispatch_mesh_field ((HexMesh_P2 *)0, mesh, field, func))
+    detail::cast_mesh_failed (mesh, __FILE__, __LINE__);
+  return true;
+}
+
+template <typename Functor> bool dispatch_3d (Mesh *mesh, Functor &func)
+{
+  if (!detail::dispatch_mesh_only ((HexMesh *)0, mesh, func) && !detail::dispatch_mesh_only ((HexMesh_P1 *)0, mesh, func) &&
+      !detail::dispatch_mesh_only ((HexMesh_P2 *)0, mesh, func))
+    detail::cast_mesh_failed (mesh, __FILE__, __LINE__);
+  return true;
+}
 
 // ---- colour table / map ---------------------------------------------------------------------------------
 class ColorTable
@@ -285,17 +432,31 @@ class DataSet
 {
 public:
   DataSet () {}
-  explicit DataSet (std::shared_ptr<UnstructuredMesh> mesh) : m_mesh (std::move (mesh)) {}
-  void add_field (std::shared_ptr<UnstructuredField> field);
-  bool has_field (const std::string &name) const;
-  UnstructuredMesh *mesh () { return m_mesh.get (); }
-  UnstructuredField *field (const std::string &name);   // throws DRayError when missing
+  explicit DataSet (std::shared_ptr<Mesh> mesh) { add_mesh (std::move (mesh)); }
+  void domain_id (const int32 id) { m_domain_id = id; }
+  int32 domain_id () const { return m_domain_id; }
+  void clear_meshes () { m_meshes.clear (); }
+  int32 number_of_meshes () const { return (int32)m_meshes.size (); }
+  void add_mesh (std::shared_ptr<Mesh> mesh) { m_meshes.push_back (std::move (mesh)); }
+  bool has_mesh (const std::string &name) const;
+  Mesh *mesh (const int32 index = 0);                   // throws DRayError when out of range
+  Mesh *mesh (const std::string name);                  // throws DRayError when missing
+  std::vector<std::string> meshes () const;
   int32 number_of_fields () const { return (int32)m_fields.size (); }
+  void clear_fields () { m_fields.clear (); }
+  bool has_field (const std::string &name) const;
   std::vector<std::string> fields () const;
+  Field *field (const std::string &name);               // throws DRayError when missing
+  Field *field (const int &index);
+  std::shared_ptr<Field> field_shared (const int &index);
+  std::shared_ptr<Field> field_shared (const std::string &name);
+  void add_field (std::shared_ptr<Field> field) { m_fields.push_back (std::move (field)); }
+  std::string field_info ();
 
 private:
-  std::shared_ptr<UnstructuredMesh> m_mesh;
-  std::vector<std::shared_ptr<UnstructuredField>> m_fields;
+  std::vector<std::shared_ptr<Mesh>> m_meshes;
+  std::vector<std::shared_ptr<Field>> m_fields;
+  int32 m_domain_id = 0;
 };
 
 class Collection

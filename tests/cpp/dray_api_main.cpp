@@ -129,30 +129,59 @@ int main (int argc, char **argv)
     ff.m_values = to_array (field);
     ff.m_size_el = n_el;
 
-    auto mesh_p = std::make_shared<UnstructuredMesh> (gf, order);
-    auto fld_p = std::make_shared<UnstructuredField> (ff, order, "f");
-    UnstructuredMesh &mesh = *mesh_p;
-    UnstructuredField &fld = *fld_p;
-    DataSet ds (mesh_p);
+    // a data set as dray's readers build one: the mesh and the field behind their abstract bases
+    std::shared_ptr<Mesh> mesh_p = std::make_shared<HexMesh> (gf, order);
+    mesh_p->name ("mesh");
+    std::shared_ptr<Field> fld_p = std::make_shared<UnstructuredField<HexScalar>> (ff, order, "f");
+    fld_p->mesh_name ("mesh");
+    DataSet ds;
+    ds.add_mesh (mesh_p);
     ds.add_field (fld_p);
     Collection col;
     col.add_domain (ds);
+    Mesh *mesh = ds.mesh ();
+    Field *fld = ds.field ("f");
+    if (ds.number_of_meshes () != 1 || ds.mesh ("mesh") != mesh || mesh->cells () != n_el || mesh->order () != order ||
+        mesh->dims () != 3 || fld->order () != order || fld->components () != 1 || ds.field (0) != fld)
+      throw DRayError ("DataSet / Mesh / Field accessors");
+    {
+      std::ofstream tf (out + "/types.txt");
+      tf << mesh->type_name () << " " << fld->type_name () << "\n";
+    }
 
-    // BVH (LinearBVHBuilder through construct_bvh)
-    BVH &bvh = mesh.get_bvh ();
+    // BVH (LinearBVHBuilder through construct_bvh), reached from Mesh* through dispatch_3d
+    BVH *bvh_p = nullptr;
+    auto grab_bvh = [&bvh_p] (auto &m) { bvh_p = &m.get_bvh (); };
+    dispatch_3d (mesh, grab_bvh);
+    BVH &bvh = *bvh_p;
     write_array (out + "/inner.bin", bvh.m_inner_nodes);
     write_array (out + "/leaf.bin", bvh.m_leaf_nodes);
     write_array (out + "/ids.bin", bvh.m_aabb_ids);
-    AABB<> b = mesh.bounds ();
+    AABB<> b = mesh->bounds ();
     write_raw (out + "/bounds.bin", &b.m_ranges[0].m_min, 6);
 
-    // Mesh::locate + Field::eval
+    // Mesh::locate + Field::eval (virtual calls)
     Array<Vec<Float, 3>> wp = to_array (points);
-    Array<Location> locs = mesh.locate (wp);
+    Array<Location> locs = mesh->locate (wp);
     Array<Float> vals;
-    fld.eval (locs, vals);
+    fld->eval (locs, vals);
     write_array (out + "/loc.bin", locs);
     write_array (out + "/vals.bin", vals);
+    // the fixed-order mesh type of the same order (HexMesh_P1 / _P2) locates the same points identically
+    if (order == 1 || order == 2)
+    {
+      std::shared_ptr<Mesh> fixed;
+      if (order == 1)
+        fixed = std::make_shared<HexMesh_P1> (gf, order);
+      else
+        fixed = std::make_shared<HexMesh_P2> (gf, order);
+      Array<Location> l2 = fixed->locate (wp);
+      const Location *a = locs.get_host_ptr_const (), *c = l2.get_host_ptr_const ();
+      for (size_t i = 0; i < locs.size (); ++i)
+        if (a[i].m_cell_id != c[i].m_cell_id || a[i].m_ref_pt[0] != c[i].m_ref_pt[0] ||
+            a[i].m_ref_pt[1] != c[i].m_ref_pt[1] || a[i].m_ref_pt[2] != c[i].m_ref_pt[2])
+          throw DRayError ("fixed-order mesh type locates differently");
+    }
 
     // Camera -> Volume::integrate -> composite
     Camera cam;
@@ -207,8 +236,8 @@ int main (int argc, char **argv)
       gf2.m_values = to_array (shifted);
       GridFunction<1> ff2 = ff;
       ff2.m_values = to_array (f10);
-      DataSet ds2 (std::make_shared<UnstructuredMesh> (gf2, order));
-      ds2.add_field (std::make_shared<UnstructuredField> (ff2, order, "f"));
+      DataSet ds2 (std::make_shared<HexMesh> (gf2, order));
+      ds2.add_field (std::make_shared<UnstructuredField<HexScalar>> (ff2, order, "f"));
       Collection two;
       two.add_domain (ds);
       two.add_domain (ds2);
@@ -267,7 +296,22 @@ int main (int argc, char **argv)
     ok &= throws ([] { Camera c; c.set_width (0); });
     ok &= throws ([] { Camera c; c.set_fov (181.f); });
     ok &= throws ([] { Framebuffer f (0, 4); });
-    ok &= throws ([&] { UnstructuredMesh m (gf, 5); });
+    ok &= throws ([&] { HexMesh m (gf, 5); });
+    ok &= throws ([&] { HexMesh_P1 m (gf, 2); });              // fixed-order type, other order
+    ok &= throws ([&] { DataSet d; d.mesh (0); });             // no mesh
+    ok &= throws ([&] { ds.mesh ("nope"); });
+    ok &= throws ([&] {   // a field of no scalar element type dispatch_3d knows
+      struct OtherField : Field
+      {
+        std::vector<Range> range () const override { return {}; }
+        int32 order () const override { return 1; }
+        int32 components () const override { return 1; }
+        std::string type_name () const override { return "other"; }
+        void eval (const Array<Location>, Array<Float> &) override {}
+      } other;
+      auto noop = [] (auto &, auto &) {};
+      dispatch_3d (mesh, &other, noop);
+    });
     ok &= throws ([&] {   // more lights than the library takes
       Volume v (col);
       v.field ("f");

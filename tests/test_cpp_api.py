@@ -32,9 +32,12 @@ def test_cpp_program_links_and_reports_usage():
 
 def test_library_exports_dray_classes():
     out = subprocess.run(["nm", "-DC", "--defined-only", B.LIB], capture_output=True, text=True, check=True).stdout
-    for sym in ("dray::UnstructuredMesh::get_bvh()", "dray::UnstructuredMesh::locate(", "dray::Volume::integrate(",
+    hex3 = "dray::UnstructuredMesh<dray::Element<3, 3, (dray::ElemType)0, -1> >::"
+    for sym in (hex3 + "get_bvh()", hex3 + "locate(", "dray::UnstructuredMesh<dray::Element<3, 3, (dray::ElemType)0, 1> >::",
+                "dray::UnstructuredMesh<dray::Element<3, 3, (dray::ElemType)0, 2> >::", "dray::Volume::integrate(",
                 "dray::Volume::composite(", "dray::LinearBVHBuilder::construct(", "dray::Camera::create_rays(",
-                "dray::UnstructuredField::eval(", "dray::ColorTable::sample("):
+                "dray::UnstructuredField<dray::Element<3, 1, (dray::ElemType)0, -1> >::eval(", "dray::ColorTable::sample(",
+                "dray::DataSet::mesh(int)", "dray::DataSet::field(", "dray::detail::cast_mesh_failed("):
         assert sym in out, f"libdrx.so does not export {sym}"
 
 
@@ -54,11 +57,19 @@ def test_camera_orbits_match_oracle(orc, tmp_path):
     assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
 
 
+CPP_MESHES = [("p3_n5", lambda: syn.hex_mesh(5, 3, 0.1, 0.01, 2)),
+              ("p2_n4", lambda: syn.hex_mesh(4, 2, 0.05, 0.01, 7))]
+
+
 @pytest.mark.gpu
-def test_cpp_api_matches_c_abi(gpu, orc, tmp_path):
+@pytest.mark.parametrize("name,make", CPP_MESHES, ids=[c[0] for c in CPP_MESHES])
+def test_cpp_api_matches_c_abi(gpu, orc, tmp_path, name, make):
+    """The program holds its mesh and field as Mesh* / Field* in a DataSet (HexMesh and
+    UnstructuredField<HexScalar>), reaches the BVH through dispatch_3d, locates and evaluates through the
+    virtual calls, and for orders 1 and 2 checks HexMesh_P1 / HexMesh_P2 against HexMesh itself."""
     import torch
     _need_bin()
-    mesh = syn.hex_mesh(5, 3, 0.1, 0.01, 2)
+    mesh = make()
     w, h, samples, opacity = 40, 36, 96, 0.02
     alpha0 = syn.table_alpha_for_corrected(opacity, samples)
     pos, look = (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)
@@ -77,10 +88,12 @@ def test_cpp_api_matches_c_abi(gpu, orc, tmp_path):
     plp.tofile(ind / "pl_points.bin")
     (ind / "params.txt").write_text(" ".join(str(x) for x in (
         mesh.order, mesh.n_el, len(pts), w, h, samples, repr(float(alpha0)), 45.0, *pos, *look)))
-    print(f"[cpp api] p3 n5, {len(pts)} points, {w}x{h} @ {samples} spp", flush=True)
+    print(f"[cpp api] {name}, {len(pts)} points, {w}x{h} @ {samples} spp", flush=True)
     r = subprocess.run([BIN, str(ind), str(outd)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
     assert "errors ok" in r.stdout
+    # element_name (elem_attr.hpp:408-427) of the mesh (Hex3) and field (HexScalar) types
+    assert (outd / "types.txt").read_text().split() == ["3D_Tensor_C3_P-1", "3D_Tensor_C1_P-1"]
 
     # BVH: bit-exact with the oracle
     inner = np.fromfile(outd / "inner.bin", np.float32).reshape(-1, 16)
