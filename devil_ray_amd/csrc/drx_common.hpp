@@ -34,9 +34,10 @@ int fail (int status, const char *fmt, ...);
                           __FILE__, __LINE__);                                                  \
   } while (0)
 
-// ---- per-device arena of growable scratch slots ------------------------------------------------
-// Slots are independent buffers; a call may hold several at once.  Growing a slot synchronises the
-// device (hipFree).  Contents are not preserved across growth.
+// ---- per-(device, stream) arena of growable scratch slots -------------------------------------------
+// Slots are independent buffers; a call may hold several at once.  Each stream has its own set, so calls
+// on different streams never share scratch; growing a slot is stream-ordered.  Contents are not preserved
+// across growth.
 enum ArenaSlot
 {
   SLOT_SORT_KEYS_A = 0,
@@ -53,7 +54,7 @@ enum ArenaSlot
   SLOT_PYRAMID,
   SLOT_COUNT
 };
-void *arena (int slot, size_t bytes);   // nullptr + error set on failure
+void *arena (int slot, size_t bytes, hipStream_t stream);   // nullptr + error set on failure
 int arena_release ();
 
 inline hipStream_t as_stream (void *s) { return reinterpret_cast<hipStream_t> (s); }

@@ -413,9 +413,9 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   if ((int64_t)n_rays * 5 > (1ll << 31) - 1) return fail (DRX_ERR_ARG, "drx_volume_integrate: too many rays");
   hipStream_t st = as_stream (stream);
   const int64_t n_slots = 5 * (int64_t)n_rays;
-  Partial *slots = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n_slots);
-  int32_t *flags = (int32_t *)arena (SLOT_FLAGS, sizeof (int32_t) * 2 * (size_t)n_slots + 64);
-  unsigned long long *dstats = (unsigned long long *)arena (SLOT_STATS, 8 * sizeof (unsigned long long));
+  Partial *slots = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n_slots, st);
+  int32_t *flags = (int32_t *)arena (SLOT_FLAGS, sizeof (int32_t) * 2 * (size_t)n_slots + 64, st);
+  unsigned long long *dstats = (unsigned long long *)arena (SLOT_STATS, 8 * sizeof (unsigned long long), st);
   if (!slots || !flags || !dstats) return DRX_ERR_HIP;
   int32_t *pos = flags + n_slots;
   int32_t *err = (int32_t *)(dstats + 4);

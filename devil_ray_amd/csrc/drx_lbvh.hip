@@ -691,11 +691,11 @@ int morton_order_points (const float *pts, int n, const float *bounds6, hipStrea
   }
   else
   {
-    kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n);
-    kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n);
-    vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n);
-    vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n);
-    hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (size_t)radix_hist_len (n));
+    kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n, st);
+    kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n, st);
+    vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n, st);
+    vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n, st);
+    hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (size_t)radix_hist_len (n), st);
   }
   if (!kA || !kB || !vA || !vB || !hist) return DRX_ERR_HIP;
   float minc[3], inv[3];
@@ -773,7 +773,7 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
       DRX_HIP (hipStreamSynchronize (st));
       hp[1] = hp[0];
     }
-    float *pa = (float *)arena (SLOT_TREE2, 12 * sizeof (float) + 2 * sizeof (int32_t));
+    float *pa = (float *)arena (SLOT_TREE2, 12 * sizeof (float) + 2 * sizeof (int32_t), st);
     if (!pa) return DRX_ERR_HIP;
     DRX_HIP (hipMemcpyAsync (pa, hb, 12 * sizeof (float), hipMemcpyHostToDevice, st));
     DRX_HIP (hipMemcpyAsync (pa + 12, hp, 2 * sizeof (int32_t), hipMemcpyHostToDevice, st));
@@ -784,16 +784,16 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
   const int inner = n - 1;
   // scratch: keys/vals ping-pong, histogram, tree arrays
   const int n_tiles = (n + kSortTile - 1) / kSortTile;
-  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * n);
-  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * n);
-  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * n);
-  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * n);
-  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8));
+  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * n, st);
+  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * n, st);
+  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * n, st);
+  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * n, st);
+  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8), st);
   // tree slot: sorted aabbs (6n) | inner aabbs (6 inner) | left | right | parent (2n-1) | sorted prims | partial bounds
   const int red_blocks = std::min (1024, (int)nblk (n, kRedBlock));
   const size_t tree_floats = 6 * (size_t)n + 6 * (size_t)inner;
   const size_t tree_ints = (size_t)inner * 2 + (2 * (size_t)n - 1) + (size_t)n;
-  char *tree = (char *)arena (SLOT_TREE, tree_floats * 4 + tree_ints * 4 + 6 * 4 * (size_t)red_blocks + 64);
+  char *tree = (char *)arena (SLOT_TREE, tree_floats * 4 + tree_ints * 4 + 6 * 4 * (size_t)red_blocks + 64, st);
   if (!kA || !kB || !vA || !vB || !hist || !tree) return DRX_ERR_HIP;
   float *s_aabbs = (float *)tree;
   float *i_aabbs = s_aabbs + 6 * (size_t)n;
@@ -864,7 +864,7 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
     ++n_levels;
   }
   if (n_levels > kMaxLevels) return fail (DRX_ERR_ARG, "drx_bvh_build: too many boxes for the refit pyramid");
-  char *pyr_mem = (char *)arena (SLOT_PYRAMID, pyr_floats * sizeof (float) + 2 * sizeof (int32_t) * (size_t)inner + 64);
+  char *pyr_mem = (char *)arena (SLOT_PYRAMID, pyr_floats * sizeof (float) + 2 * sizeof (int32_t) * (size_t)inner + 64, st);
   if (!pyr_mem) return DRX_ERR_HIP;
   int32_t *range_lo = (int32_t *)pyr_mem;
   int32_t *range_hi = range_lo + inner;

@@ -543,11 +543,11 @@ int drx_mlp_train (void *state, const void *x, const float *target, int64_t n, c
   if (o->batch <= 0 || o->batch % 64 != 0) return fail (DRX_ERR_ARG, "drx_mlp_train: batch %d is not a positive multiple of 64", o->batch);
   if (o->steps < 0) return fail (DRX_ERR_ARG, "drx_mlp_train: negative step count");
   const int nw = o->batch / 64;
-  float *partials = (float *)arena (SLOT_TREE, (size_t)(nw + kGradGroups) * PSTRIDE * sizeof (float));
+  const hipStream_t st = as_stream (stream);
+  float *partials = (float *)arena (SLOT_TREE, (size_t)(nw + kGradGroups) * PSTRIDE * sizeof (float), st);
   if (!partials) return DRX_ERR_HIP;
   float *groups = partials + (size_t)nw * PSTRIDE;
   const MlpState s = state_view (state);
-  const hipStream_t st = as_stream (stream);
   const double b1 = o->beta1 > 0 ? o->beta1 : 0.9, b2 = o->beta2 > 0 ? o->beta2 : 0.999;
   const float eps = o->eps > 0 ? o->eps : 1e-8f;
   const float lr_final = o->lr_final >= 0 ? o->lr_final : o->lr;

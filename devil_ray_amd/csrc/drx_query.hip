@@ -792,7 +792,7 @@ static int locate_impl (const drx_mesh *mesh, const drx_bvh *bvh, const float *p
   if (launch < DRX_LAUNCH_AUTO || launch > DRX_LAUNCH_PERSISTENT || chunk < 0)
     return fail (DRX_ERR_ARG, "drx_locate: launch %d / chunk %d", launch, chunk);
   hipStream_t st = as_stream (stream);
-  int32_t *err = counters ? counters : (int32_t *)arena (SLOT_FLAGS, 64);
+  int32_t *err = counters ? counters : (int32_t *)arena (SLOT_FLAGS, 64, st);
   if (!err) return DRX_ERR_HIP;
   int32_t *queue = err + 1;   // chunk counter of the persistent launch
   hipLaunchKernelGGL (k_fill_i32q, dim3 (1), dim3 (64), 0, st, err, (int64_t)2, 0);
@@ -1076,7 +1076,7 @@ int drx_remove_missed_rays (const void *rays, int32_t n, const float *bounds6, v
     mb.b[2 * d + 1] = hi;
   }
   hipStream_t st = as_stream (stream);
-  int32_t *flags = (int32_t *)arena (SLOT_FLAGS, sizeof (int32_t) * 2 * (size_t)n + 64);
+  int32_t *flags = (int32_t *)arena (SLOT_FLAGS, sizeof (int32_t) * 2 * (size_t)n + 64, st);
   if (!flags) return DRX_ERR_HIP;
   int32_t *pos = flags + n;
   hipLaunchKernelGGL (k_mark_active, dim3 (nblk (n, 256)), dim3 (256), 0, st, (const Ray *)rays, n, mb, flags);
@@ -1101,7 +1101,7 @@ int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t heig
   const int tiles_x = (width + kTileW - 1) / kTileW, tiles_y = (height + kTileH - 1) / kTileH;
   const int64_t slots_n = (int64_t)tiles_x * tiles_y * 64;
   if (slots_n > (1ll << 30)) return fail (DRX_ERR_ARG, "drx_ray_tile_order: image too large");
-  int32_t *slots0 = (int32_t *)arena (SLOT_TREE2, sizeof (int32_t) * 4 * (size_t)slots_n + 64);
+  int32_t *slots0 = (int32_t *)arena (SLOT_TREE2, sizeof (int32_t) * 4 * (size_t)slots_n + 64, st);
   if (!slots0) return DRX_ERR_HIP;
   int32_t *slots = slots0 + slots_n, *flags = slots + slots_n, *pos = flags + slots_n;
   hipLaunchKernelGGL (k_fill_i32q, dim3 (nblk (slots_n, 256)), dim3 (256), 0, st, slots0, slots_n, -1);
@@ -1111,11 +1111,11 @@ int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t heig
   // dispatched last run on an emptying GPU, so they should be the short ones.
   const int n_img_tiles = tiles_x * tiles_y;
   const int n_rt = radix_tiles (n_img_tiles);
-  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n_img_tiles);
-  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n_img_tiles);
-  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n_img_tiles);
-  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n_img_tiles);
-  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_rt + 8));
+  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n_img_tiles, st);
+  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n_img_tiles, st);
+  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n_img_tiles, st);
+  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n_img_tiles, st);
+  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_rt + 8), st);
   if (!kA || !kB || !vA || !vB || !hist) return DRX_ERR_HIP;
   hipLaunchKernelGGL (k_tile_keys, dim3 (nblk (n_img_tiles, 256)), dim3 (256), 0, st, (const Ray *)rays, slots0,
                       n_img_tiles, kA, vA);
@@ -1142,12 +1142,12 @@ int drx_ray_tile_order (const void *rays, int32_t n, int32_t width, int32_t heig
 static int sort_partials_pixel_depth (const Partial *in, int32_t n, int64_t npix, hipStream_t st, Partial **out)
 {
   const int n_tiles = radix_tiles (n);
-  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n);
-  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n);
-  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n);
-  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n);
-  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8));
-  Partial *sorted = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n);
+  uint32_t *kA = (uint32_t *)arena (SLOT_SORT_KEYS_A, sizeof (uint32_t) * (size_t)n, st);
+  uint32_t *kB = (uint32_t *)arena (SLOT_SORT_KEYS_B, sizeof (uint32_t) * (size_t)n, st);
+  int32_t *vA = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * (size_t)n, st);
+  int32_t *vB = (int32_t *)arena (SLOT_SORT_VALS_B, sizeof (int32_t) * (size_t)n, st);
+  int32_t *hist = (int32_t *)arena (SLOT_SORT_HIST, sizeof (int32_t) * (2 * 256 * (size_t)n_tiles + 8), st);
+  Partial *sorted = (Partial *)arena (SLOT_PARTIALS, sizeof (Partial) * (size_t)n, st);
   if (!kA || !kB || !vA || !vB || !hist || !sorted) return DRX_ERR_HIP;
   hipLaunchKernelGGL (k_depth_keys, dim3 (nblk (n, 256)), dim3 (256), 0, st, in, n, kA, vA);
   DRX_LAUNCH_CHECK ();

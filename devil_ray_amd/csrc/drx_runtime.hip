@@ -35,6 +35,10 @@ int fail (int status, const char *fmt, ...)
 }
 
 // ---- arena ---------------------------------------------------------------------------------------
+// One set of growable scratch slots per (device, stream): work on different streams never shares scratch,
+// and work on one stream is ordered by the stream, so a slot may be reused by the next call on its stream
+// while the previous call's kernels are still queued.  Growing a slot frees the old buffer and allocates
+// the new one in stream order (hipFreeAsync / hipMallocAsync on the slot's stream): no device-wide wait.
 namespace
 {
 struct Slot
@@ -42,42 +46,54 @@ struct Slot
   void *ptr = nullptr;
   size_t bytes = 0;
 };
-constexpr int kMaxDevices = 16;
-Slot g_slots[kMaxDevices][SLOT_COUNT];
+struct SlotSet
+{
+  int dev;
+  hipStream_t stream;
+  Slot slots[SLOT_COUNT];
+};
+std::vector<SlotSet> g_sets;
 std::mutex g_arena_mutex;
 } // namespace
 
-void *arena (int slot, size_t bytes)
+void *arena (int slot, size_t bytes, hipStream_t stream)
 {
   int dev = 0;
-  if (hipGetDevice (&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+  if (hipGetDevice (&dev) != hipSuccess || dev < 0)
   {
     set_error ("arena: no current device");
     return nullptr;
   }
   std::lock_guard<std::mutex> lock (g_arena_mutex);
-  Slot &s = g_slots[dev][slot];
+  SlotSet *set = nullptr;
+  for (auto &x : g_sets)
+    if (x.dev == dev && x.stream == stream) set = &x;
+  if (!set)
+  {
+    g_sets.push_back (SlotSet { dev, stream, {} });
+    set = &g_sets.back ();
+  }
+  Slot &s = set->slots[slot];
   if (bytes == 0) bytes = 256;
   if (s.bytes >= bytes) return s.ptr;
   if (s.ptr)
   {
-    // the old buffer may still be in use by queued work: wait, then free it
-    const hipError_t es = hipDeviceSynchronize ();
-    const hipError_t ef = hipFree (s.ptr);
+    // only work queued on `stream` can still read the old buffer: free it in stream order
+    const hipError_t ef = hipFreeAsync (s.ptr, stream);
     s.ptr = nullptr;
     s.bytes = 0;
-    if (es != hipSuccess || ef != hipSuccess)
+    if (ef != hipSuccess)
     {
-      set_error ("arena: growing slot %d: %s", slot, hipGetErrorString (es != hipSuccess ? es : ef));
+      set_error ("arena: growing slot %d: hipFreeAsync: %s", slot, hipGetErrorString (ef));
       return nullptr;
     }
   }
   const size_t want = bytes + bytes / 8;   // grow with headroom
-  hipError_t e = hipMalloc (&s.ptr, want);
+  hipError_t e = hipMallocAsync (&s.ptr, want, stream);
   if (e != hipSuccess)
   {
     s.ptr = nullptr;
-    set_error ("arena: hipMalloc(%zu) failed: %s", want, hipGetErrorString (e));
+    set_error ("arena: hipMallocAsync(%zu) failed: %s", want, hipGetErrorString (e));
     return nullptr;
   }
   s.bytes = want;
@@ -91,14 +107,20 @@ int arena_release ()
   DRX_HIP (hipDeviceSynchronize ());
   std::lock_guard<std::mutex> lock (g_arena_mutex);
   hipError_t first = hipSuccess;
-  for (int i = 0; i < SLOT_COUNT; ++i)
+  for (auto it = g_sets.begin (); it != g_sets.end ();)
   {
-    if (g_slots[dev][i].ptr)
+    if (it->dev != dev)
     {
-      const hipError_t e = hipFree (g_slots[dev][i].ptr);
-      if (e != hipSuccess && first == hipSuccess) first = e;
+      ++it;
+      continue;
     }
-    g_slots[dev][i] = Slot ();
+    for (int i = 0; i < SLOT_COUNT; ++i)
+      if (it->slots[i].ptr)
+      {
+        const hipError_t e = hipFree (it->slots[i].ptr);
+        if (e != hipSuccess && first == hipSuccess) first = e;
+      }
+    it = g_sets.erase (it);
   }
   if (first != hipSuccess) return fail (DRX_ERR_HIP, "drx_release: hipFree: %s", hipGetErrorString (first));
   return DRX_OK;
@@ -191,7 +213,7 @@ int exclusive_scan_i32 (const int32_t *in, int32_t *out, int32_t n, int32_t *tot
     return DRX_OK;
   }
   const int32_t nb = (n + kScanTile - 1) / kScanTile;
-  if (!sums) sums = (int32_t *)arena (SLOT_SCAN, sizeof (int32_t) * (nb + 1));
+  if (!sums) sums = (int32_t *)arena (SLOT_SCAN, sizeof (int32_t) * (nb + 1), st);
   if (!sums) return DRX_ERR_HIP;
   hipLaunchKernelGGL (k_scan_reduce, dim3 (nb), dim3 (kScanBlock), 0, st, in, n, sums);
   hipLaunchKernelGGL (k_scan_sums, dim3 (1), dim3 (kScanBlock), 0, st, sums, nb);

@@ -145,18 +145,18 @@ int drx_bvh_widen (const drx_bvh *bvh, float *out_wide, int64_t capacity_nodes, 
   int32_t n_f = 1;
   int64_t total = 1;       // wide ids handed out so far (root = 0)
   std::vector<int32_t> h0 = { 0, 0 };
-  int32_t *fr = (int32_t *)arena (SLOT_TREE, 2 * sizeof (int32_t));
+  int32_t *fr = (int32_t *)arena (SLOT_TREE, 2 * sizeof (int32_t), st);
   if (!fr) return DRX_ERR_HIP;
   DRX_HIP (hipMemcpyAsync (fr, h0.data (), 2 * sizeof (int32_t), hipMemcpyHostToDevice, st));
   DRX_HIP (hipStreamSynchronize (st));
   int cur_slot = SLOT_TREE, nxt_slot = SLOT_TREE2;
   while (n_f > 0)
   {
-    int32_t *f_bin = (int32_t *)arena (cur_slot, 2 * sizeof (int32_t) * (size_t)n_f);
+    int32_t *f_bin = (int32_t *)arena (cur_slot, 2 * sizeof (int32_t) * (size_t)n_f, st);
     if (!f_bin) return DRX_ERR_HIP;
     int32_t *f_wide = f_bin + n_f;
-    WideItem *items = (WideItem *)arena (SLOT_SORT_KEYS_A, sizeof (WideItem) * kW * (size_t)n_f);
-    int32_t *cnts = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * 3 * (size_t)n_f + 64);
+    WideItem *items = (WideItem *)arena (SLOT_SORT_KEYS_A, sizeof (WideItem) * kW * (size_t)n_f, st);
+    int32_t *cnts = (int32_t *)arena (SLOT_SORT_VALS_A, sizeof (int32_t) * 3 * (size_t)n_f + 64, st);
     if (!items || !cnts) return DRX_ERR_HIP;
     int32_t *n_items = cnts, *n_inner = cnts + n_f, *base = cnts + 2 * n_f;
     hipLaunchKernelGGL (k_widen_expand, dim3 (nblk (n_f, 256)), dim3 (256), 0, st, nodes, f_bin, n_f, items, n_items,
@@ -173,7 +173,7 @@ int drx_bvh_widen (const drx_bvh *bvh, float *out_wide, int64_t capacity_nodes, 
     if (n_next > 0)
     {
       // the next frontier must not alias the current one: it lives in the other slot
-      nf = (int32_t *)arena (nxt_slot, 2 * sizeof (int32_t) * (size_t)n_next);
+      nf = (int32_t *)arena (nxt_slot, 2 * sizeof (int32_t) * (size_t)n_next, st);
       if (!nf) return DRX_ERR_HIP;
     }
     hipLaunchKernelGGL (k_widen_write, dim3 (nblk (n_f, 256)), dim3 (256), 0, st, f_wide, n_f, items, n_items, base,

@@ -6,7 +6,10 @@
  * message of the last failure on the calling host thread.  Functions enqueue on `stream` (a
  * hipStream_t passed as void*, NULL = the null stream) and, unless noted, return without waiting;
  * functions that hand back a host-visible count synchronise the stream first.  The library never frees
- * caller memory; temporaries live in a per-device arena owned by the library (drx_release frees it).
+ * caller memory; temporaries live in an arena owned by the library with one set of buffers per (device,
+ * stream): calls on different streams never share them, and calls on one stream are ordered by it, so
+ * any number of streams may be used at once (two host threads must not issue calls on the same stream
+ * at the same time).  drx_release frees the current device's arena.
  *
  * Reference interfaces replaced (LLNL/devil_ray, paths relative to its src/dray/):
  *   drx_bvh_split_elements  construct_bvh per-element split          data_model/mesh_utils.cpp:514-670

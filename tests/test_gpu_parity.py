@@ -732,10 +732,12 @@ def test_leaf_start_maps_same_results(gpu, orc):
     assert np.abs(x0 - exp).max() < 1e-5
 
 
-def test_point_location_collection(gpu):
+def test_point_location_collection(gpu, orc):
     """PointLocation over a two-domain Collection (a mesh and a copy shifted by (+0.6, +0.5, 0) whose
-    field is f + 10; the copies overlap and leave two corners of the union box uncovered): a point takes
-    the last domain that locates it, else empty_val; Collection range / bounds are the unions."""
+    field is f + 10; the copies overlap and leave two corners of the union box uncovered) against the
+    oracle's restatement of point_location.cpp:178-287 (values within 1e-5, apart from points whose
+    located id differs for a reason the parity bar excuses): a point takes the last domain that locates
+    it, else empty_val; Collection range / bounds are the unions."""
     a = syn.hex_mesh(4, 2, 0.05, 0.01, 7)
     b = syn.HexMesh(n=a.n, order=a.order, ctrl_idx=a.ctrl_idx,
                     values=(a.values + np.float32([0.6, 0.5, 0.0])).astype(np.float32),
@@ -757,10 +759,23 @@ def test_point_location_collection(gpu):
     pl.empty_val(-7.0)
     res = pl.execute(col, dp)
     got = _np(res["values"][0])
+    # the oracle's restatement of PointLocation::execute over the same two domains
+    obs = [orc.build_mesh_bvh(a), orc.build_mesh_bvh(b)]
+    exp_o, ocells = orc.point_location([(a, obs[0], [a.field]), (b, obs[1], [b.field])], pts, 1, -7.0)
     l0, l1 = meshes[0][0].locate(dp), meshes[1][0].locate(dp)
-    v0, v1 = _np(meshes[0][1].eval(l0)), _np(meshes[1][1].eval(l1))
     c0 = _np(l0, np.uint8).view(LOCATION_DTYPE)["cell"]
     c1 = _np(l1, np.uint8).view(LOCATION_DTYPE)["cell"]
+    excused = np.zeros(len(pts), bool)
+    for mesh, ob, l, oc in ((a, obs[0], l0, ocells[0]), (b, obs[1], l1, ocells[1])):
+        loc = _np(l, np.uint8).view(LOCATION_DTYPE)
+        orf = orc.locate(mesh, ob, pts)[1]
+        mism, explained, _, _ = explain_mismatches(orc, mesh, ob, pts, loc["cell"], loc["ref"], oc, orf)
+        assert explained.all()
+        excused[mism] = True
+    d = np.abs(got - exp_o[0])
+    assert d[~excused].max() <= 1e-5 and excused.sum() <= 10
+    # and the composition rule itself on the GPU's own per-domain results: the last domain that locates wins
+    v0, v1 = _np(meshes[0][1].eval(l0)), _np(meshes[1][1].eval(l1))
     exp = np.where(c1 >= 0, v1, np.where(c0 >= 0, v0, np.float32(-7.0))).astype(np.float32)
     assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
     assert (c0 >= 0).sum() > 1000 and (c1 >= 0).sum() > 1000 and (exp == -7).sum() > 100
@@ -771,3 +786,43 @@ def test_point_location_collection(gpu):
     assert bb[0] == ba[0] and bb[1] == bbm[1] and bb[2] == ba[2] and bb[3] == bbm[3]
     both = (c0 >= 0) & (c1 >= 0)
     assert both.sum() > 100 and np.all(got[both] == v1[both])
+
+
+def test_scratch_per_stream(gpu, orc):
+    """Library scratch is kept per stream: drx_partials_to_framebuffer (which returns before its kernels
+    have read its sorted partials) issued on one stream and a volume render (whose partial slots use the
+    same scratch slot) issued right after on another stream give exactly what each gives alone."""
+    rng = np.random.default_rng(12)
+    w, h, n = 64, 48, 200000
+    p = np.zeros(n, PARTIAL_DTYPE)
+    p["pixel_id"] = rng.integers(0, w * h, n)
+    p["depth"] = rng.integers(0, 50, n).astype(np.float32) * np.float32(0.5)
+    p["rgba"] = rng.uniform(0, 0.2, (n, 4)).astype(np.float32)
+    dp = gpu.as_bytes(p)
+    mesh = syn.hex_mesh(4, 2, 0.05, 0.01, 3)
+    m = gpu.UnstructuredMesh(mesh)
+    f = gpu.UnstructuredField(mesh)
+    vol = gpu.Volume(m, f)
+    vol.use_lighting = False
+    vol.samples(128)
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, 0.05)
+    rays = gpu.as_bytes(orc.camera_perspective(96, 96, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)))
+    # each alone, on the current stream
+    fb0 = gpu.Framebuffer(w, h)
+    gpu.partials_to_framebuffer([(dp, n)], fb0)
+    parts0, n0 = vol.integrate(rays)
+    torch.cuda.synchronize()
+    want_fb, want_parts = _np(fb0.colors).copy(), _np(parts0, np.uint8)[: n0 * 24].copy()
+    # back to back on two streams
+    fb1 = gpu.Framebuffer(w, h)
+    torch.cuda.synchronize()
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    with torch.cuda.stream(sa):
+        gpu.partials_to_framebuffer([(dp, n)], fb1)
+    with torch.cuda.stream(sb):
+        parts1, n1 = vol.integrate(rays)
+    torch.cuda.synchronize()
+    assert n1 == n0 and np.array_equal(_np(parts1, np.uint8)[: n1 * 24], want_parts)
+    assert np.array_equal(_np(fb1.colors).view(np.uint32), want_fb.view(np.uint32))
