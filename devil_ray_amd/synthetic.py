@@ -214,6 +214,23 @@ def mixed_hex_mesh(n: int, amp: float, jitter: float, seed: int, orders=(1, 2, 3
                    orders=el_order, offsets=offsets)
 
 
+def split_by_order(mesh: HexMesh):
+    """A mixed-order mesh as dray holds it, one uniform-order mesh per order (dray's UnstructuredMesh has
+    one order, data_model/unstructured_mesh.hpp:48; a mixed mesh is a Collection of such domains):
+    [(order, HexMesh of that order's elements in their original order, their original element ids)].  The
+    domains share the control-point arrays."""
+    if mesh.orders is None:
+        return [(mesh.order, mesh, np.arange(mesh.n_el, dtype=np.int32))]
+    out = []
+    for p in np.unique(mesh.orders):
+        els = np.nonzero(mesh.orders == p)[0].astype(np.int32)
+        npe = (int(p) + 1) ** 3
+        idx = (mesh.offsets[els][:, None] + np.arange(npe)[None, :]).reshape(-1)
+        out.append((int(p), HexMesh(n=mesh.n, order=int(p), ctrl_idx=mesh.ctrl_idx[idx].astype(np.int32),
+                                    values=mesh.values, field=mesh.field), els))
+    return out
+
+
 def query_points(n_pts: int, bounds6, seed: int = 0) -> np.ndarray:
     """Uniform points in the box [xmin,xmax]x[ymin,ymax]x[zmin,zmax] from the counter hash (float32 [n,3])."""
     b = np.asarray(bounds6, dtype=np.float64)

@@ -165,6 +165,38 @@ def locate(mesh, bvh, pts, dtype=np.float32, nthreads=0, want_steps=False):
     return (cells, refs, steps, stats) if want_steps else (cells, refs)
 
 
+def field_eval(mesh, cells, refs, values=None, out=None):
+    """Field::eval (unstructured_field.cpp:213-246) in float32: out[i] = the field at (cells[i], refs[i]);
+    points with cells[i] == -1 keep out[i].  values: per-control-point field (default mesh.field)."""
+    ctrl, _, orders, offs = _mesh_args(mesh)
+    vals = np.ascontiguousarray((mesh.field if values is None else values).astype(np.float32))
+    c = np.ascontiguousarray(np.asarray(cells, np.int32))
+    r = np.ascontiguousarray(np.asarray(refs, np.float32))
+    o = np.zeros(len(c), np.float32) if out is None else out
+    lib().orc_field_eval_f32(ptr(ctrl), ptr(vals), ptr(orders), ptr(offs), mesh.order, mesh.n_el, ptr(c), ptr(r),
+                             len(c), ptr(o))
+    return o
+
+
+def point_location(domains, pts, n_vars, empty_val):
+    """PointLocation::execute on one rank (queries/point_location.cpp:178-287): every value starts at
+    empty_val; the domains locate every point in collection order and, for a domain that located any
+    point, each variable's Field::eval overwrites the values of the points it located (misses are left
+    untouched), so a point keeps the last domain that located it.  domains: [(mesh, OracleBVH, [field
+    values per variable])].  Returns (values [n_vars, n], cells per domain)."""
+    n = int(pts.shape[0])
+    values = np.full((n_vars, n), np.float32(empty_val), np.float32)
+    cells_all = []
+    for mesh, ob, fields in domains:
+        cells, refs = locate(mesh, ob, pts)
+        cells_all.append(cells)
+        if not (cells != -1).any():
+            continue
+        for v in range(n_vars):
+            field_eval(mesh, cells, refs, values=fields[v], out=values[v])
+    return values, cells_all
+
+
 def locate_brute_f64(mesh, pts):
     npts = int(pts.shape[0])
     ctrl, vals, orders, offs = _mesh_args(mesh, np.float64)

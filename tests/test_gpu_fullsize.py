@@ -356,3 +356,48 @@ def test_config0_fp64_field_full_frame(gpu, orc):
     ref = np.array([orc.eval_d(mesh.field64[mesh.elem_dofs(int(l["cell"][i]))].reshape(-1, 1), 2,
                                l["ref"][i].astype(np.float64))[0][0] for i in hit])
     assert np.abs(vals[hit] - ref.astype(np.float32)).max() <= 2 * np.spacing(np.float32(np.abs(ref).max()))
+
+
+# ---------------------------------------------------------------------------------------------------
+# config 2 as dray holds a mixed-order mesh: one domain per order in a Collection, PointLocation
+# ---------------------------------------------------------------------------------------------------
+@pytest.mark.timeout(300)
+def test_config2_point_location_per_order_domains(gpu, orc):
+    """The 48^3 mixed-order mesh as four uniform-order domains (dray's UnstructuredMesh has one order) in a
+    Collection; PointLocation::execute over the first 2^20 furnace points against the oracle's restatement
+    of point_location.cpp:178-287 over the same domains: per domain, located ids as the parity bar allows;
+    every value equal to the oracle's to float rounding (1e-5 of the field's range) except points whose
+    winning domain's id differs for an excused reason; empty_val where no domain locates."""
+    mesh = syn.mixed_hex_mesh(48, 0.15, 0.01, 1234)
+    doms = syn.split_by_order(mesh)
+    col = gpu.Collection()
+    gdoms, odoms = [], []
+    for p, d, els in doms:
+        m, f = gpu.UnstructuredMesh(d), gpu.UnstructuredField(d)
+        ds = gpu.DataSet(m)
+        ds.add_field("f", f)
+        col.add_domain(ds)
+        gdoms.append((d, m))
+        odoms.append((d, _oracle_bvh(orc, m.get_bvh()), [d.field]))
+    bb = col.bounds()
+    pts = syn.furnace_points(1 << 20, bb)
+    dp = gpu.to_device(pts.reshape(-1))
+    pl = gpu.PointLocation()
+    pl.add_var("f")
+    pl.empty_val(-7.0)
+    got = _np(pl.execute(col, dp)["values"][0])
+    want, ocells = orc.point_location(odoms, pts, 1, -7.0)
+    excused = np.zeros(len(pts), bool)
+    for (d, m), (_, ob, _), oc in zip(gdoms, odoms, ocells):
+        loc = _np(m.locate(dp), LOCATION_DTYPE)
+        orf = orc.locate(d, ob, pts)[1]
+        mism, explained, _, _ = explain_mismatches(orc, d, ob, pts, loc["cell"], loc["ref"], oc, orf)
+        assert explained.all(), f"order-{d.order} domain: unexplained located-id mismatches at {mism[~explained][:8]}"
+        excused[mism] = True
+    span = float(mesh.field.max() - mesh.field.min())
+    diff = np.abs(got - want[0])
+    bad = (diff > 1e-5 * span) & ~excused
+    print(f"[config 2 domains] {len(doms)} domains, {len(pts)} points: located {int((got != -7).sum())}, "
+          f"excused {int(excused.sum())}, max |value - oracle| {diff[~excused].max():.3g}", flush=True)
+    assert not bad.any(), f"values differ from the oracle at {np.nonzero(bad)[0][:8]}"
+    assert np.array_equal(got == np.float32(-7.0), want[0] == np.float32(-7.0)) or excused.any()

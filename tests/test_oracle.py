@@ -772,3 +772,60 @@ def test_explain_mismatches_rule(orc):
         mism, ok, excess, by64 = explain_mismatches(orc, mesh, ob, p32, np.array([e0], np.int32), r0,
                                                     np.array([e1], np.int32), r1)
         assert list(mism) == [0] and bool(ok[0]) == want and not by64.any(), (shift, excess)
+
+
+def test_point_location_closed_form(orc):
+    """PointLocation over two overlapping affine domains with linear fields, where the answer is known in
+    closed form: an affine (lattice) net with a field sampled from a linear function reproduces it
+    exactly, so a point in the second domain (x in [0.5, 1.5]) takes f + 10, a point only in the first
+    takes f, and a point in neither keeps empty_val (the last domain that locates a point wins)."""
+    a = syn.hex_mesh(3, 2, 0.0, 0.0, 1)
+    b = syn.HexMesh(n=a.n, order=a.order, ctrl_idx=a.ctrl_idx,
+                    values=(a.values + np.float32([0.5, 0.0, 0.0])).astype(np.float32), field=a.field)
+    lin = lambda v: (v[:, 0] + 2 * v[:, 1] + 3 * v[:, 2]).astype(np.float32)
+    fa, fb = lin(a.values), (lin(b.values) + np.float32(10)).astype(np.float32)
+    rng = np.random.default_rng(8)
+    pts = (rng.random((4000, 3)) * [2.0, 1.2, 1.0]).astype(np.float32)
+    dom = [(a, orc.build_mesh_bvh(a), [fa]), (b, orc.build_mesh_bvh(b), [fb])]
+    vals, cells = orc.point_location(dom, pts, 1, -7.0)
+    x = pts.astype(np.float64)
+    f = x[:, 0] + 2 * x[:, 1] + 3 * x[:, 2]
+    # away from the domains' boundaries (the +-1e-4 acceptance band)
+    inside_a = np.all((x > 1e-3) & (x < [1 - 1e-3, 1 - 1e-3, 1 - 1e-3]), axis=1)
+    inside_b = np.all((x > [0.5 + 1e-3, 1e-3, 1e-3]) & (x < [1.5 - 1e-3, 1 - 1e-3, 1 - 1e-3]), axis=1)
+    near_a = np.all((x > -1e-3) & (x < 1 + 1e-3), axis=1)
+    near_b = np.all((x > [0.5 - 1e-3, -1e-3, -1e-3]) & (x < [1.5 + 1e-3, 1 + 1e-3, 1 + 1e-3]), axis=1)
+    outside = ~near_a & ~near_b
+    only_a = inside_a & (x[:, 0] < 0.5 - 1e-3)
+    assert inside_b.sum() > 500 and only_a.sum() > 500 and outside.sum() > 500
+    assert np.abs(vals[0][inside_b] - (f[inside_b] + 10)).max() < 1e-4
+    assert np.abs(vals[0][only_a] - f[only_a]).max() < 1e-4
+    assert np.all(vals[0][outside] == np.float32(-7.0))
+
+
+def test_split_by_order_domains(orc):
+    """The mixed-order mesh as one domain per order: the domains partition the elements, each domain's
+    element nets are the mixed mesh's, and PointLocation over them gives, for points well inside an
+    element, the element and value the mixed mesh's own locate + Field::eval give."""
+    mesh = syn.mixed_hex_mesh(4, 0.15, 0.05, 5)
+    doms = syn.split_by_order(mesh)
+    assert sorted(int(p) for p, _, _ in doms) == sorted(np.unique(mesh.orders).tolist())
+    ids = np.sort(np.concatenate([e for _, _, e in doms]))
+    assert np.array_equal(ids, np.arange(mesh.n_el))
+    for p, d, els in doms:
+        assert d.n_el == len(els) and np.all(mesh.orders[els] == p)
+        for k in (0, len(els) // 2, len(els) - 1):
+            assert np.array_equal(d.values[d.elem_dofs(k)], mesh.values[mesh.elem_dofs(int(els[k]))])
+    ob = orc.build_mesh_bvh(mesh)
+    pts = syn.query_points(6000, ob.bounds, seed=4)
+    cm, rm = orc.locate(mesh, ob, pts)
+    vm = orc.field_eval(mesh, cm, rm)
+    vals, cells = orc.point_location([(d, orc.build_mesh_bvh(d), [mesh.field]) for _, d, _ in doms], pts, 1, -1.0)
+    # the domain and element each point took (the last domain that located it)
+    glob = np.full(len(pts), -1)
+    for (_, _, els), c in zip(doms, cells):
+        glob = np.where(c >= 0, els[np.maximum(c, 0)], glob)
+    deep = (cm >= 0) & (np.abs(rm - 0.5).max(axis=1) < 0.45)
+    assert deep.sum() > 3000
+    assert np.array_equal(glob[deep], cm[deep])
+    assert np.abs(vals[0][deep] - vm[deep]).max() < 1e-6
