@@ -69,6 +69,9 @@ def main() -> int:
     ap.add_argument("--mlp-epochs", type=float, default=6.0, help="config 3: passes over the training pairs")
     ap.add_argument("--mlp-eval-points", type=int, default=1 << 22, help="config 3: queries per held-out mesh")
     ap.add_argument("--traversal", type=int, default=0, help="0 auto, 1 per lane (8-wide), 2 wave-coherent, 3 per lane (binary)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-initializer section of the 64^3 render line (trained and measured "
+                         "outside the timed region)")
     ap.add_argument("--error-sample", type=int, default=20000,
                     help="report ref-coordinate / RGBA error vs the FP64 oracle and Newton-step histograms on this "
                          "many points / rays/10, outside the timed region (0: off)")
@@ -196,6 +199,17 @@ def main() -> int:
 
     if args.error_sample:
         out["error_vs_fp64"] = error_report(args, api, syn, mesh, m, f, vol, bvh, centre, res, samples)
+
+    if args.config == 1 and not args.no_learned:
+        # learned Newton initializer vs the plain starts on this frame, FP64 polish on (outside the timed loop):
+        # trained on 16 seeds of 16^3 / 32^3 order-3 meshes (warp ~ U[0, 0.2]), 2^19 queries each, 4 epochs
+        li, X, T, n_pairs, tsteps, gen_s, train_ms, loss = train_initializer(api, syn, range(16), 1 << 19, (16, 32),
+                                                                             4.0)
+        del X, T
+        out["learned_init"] = {
+            "training": {"pairs": n_pairs, "steps": tsteps, "pair_generation_s": gen_s, "ms": train_ms,
+                         "mse_first": float(loss[:10].mean()), "mse_last": float(loss[-20:].mean())},
+            "render_fp64_polish": render_starts(api, syn, mesh, m, f, li.pack(), res, samples, 3, 1)}
 
     if not args.no_cpu:
         from oracle import orc
@@ -336,11 +350,16 @@ def bench_locate(args):
     m = api.UnstructuredMesh(mesh)
     bvh = m.get_bvh()
     torch.cuda.synchronize()
-    pts_h = syn.query_points(n_pts, bvh.bounds, seed=0)
+    # the furnace driver's points: std::minstd_rand seeded 0, uniform_real_distribution<float> per axis
+    # (utilities/furnace/point_location.cpp:49-68)
+    pts_h = syn.furnace_points(n_pts, bvh.bounds)
     pts = api.to_device(pts_h.reshape(-1))
-    variants = {"binary": api.LocateOptions(traversal=3), "morton+binary": api.LocateOptions(traversal=3, coherent_order=True),
-                "morton+wide": api.LocateOptions(traversal=1, coherent_order=True),
-                "morton+wave": api.LocateOptions(traversal=2, coherent_order=True)}
+    # FP32 Newton with the FP64 correction where cond(J) > 1e4, in every variant
+    pol = dict(fp64_polish=True, cond_threshold=1e4)
+    variants = {"binary": api.LocateOptions(traversal=3, **pol),
+                "morton+binary": api.LocateOptions(traversal=3, coherent_order=True, **pol),
+                "morton+wide": api.LocateOptions(traversal=1, coherent_order=True, **pol),
+                "morton+wave": api.LocateOptions(traversal=2, coherent_order=True, **pol)}
     # autotuner (untimed): chunk size x launch form of the streamed walks, recorded as their defaults
     tune = {k: m.locate_autotune(pts, variants[k]) for k in ("morton+binary", "morton+wide")}
     print(f"[config 2] autotune {json.dumps({k: (v['launch'], v['chunk']) for k, v in tune.items()})}",
@@ -363,9 +382,10 @@ def bench_locate(args):
     out = {
         "metric": "located queries/s (batched point location)", "value": value, "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000 * timings[best], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded mixed-order warped hex mesh, counter-hash uniform points)",
-        "config": {"workload": "batched locate, 2^26 points, 48^3 mixed-order (1..4) hex mesh, warp 0.15",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64polish",
+        "data": "synthetic (seeded mixed-order warped hex mesh, the furnace driver's minstd_rand points)",
+        "config": {"workload": "batched locate, 2^26 points, 48^3 mixed-order (1..4) hex mesh, warp 0.15, "
+                               "FP32 with FP64 polish",
                    "elements": 48 ** 3, "points": n_pts, "bvh_leaves": bvh.n_leaves, "best_variant": best},
         "ms_per_variant": {k: 1000 * v for k, v in timings.items()}, "variants_identical": same,
         "autotune": tune,
@@ -377,26 +397,35 @@ def bench_locate(args):
     if not args.no_cpu:
         from oracle import orc
         ob = _oracle_bvh(bvh)
-        sub = pts_h[:: 64].copy()
+        stride = 4   # a quarter of the queries: about 5 s on 16 EPYC threads
+        sub = pts_h[::stride].copy()
         t = time.perf_counter()
         oc, _ = orc.locate(mesh, ob, sub)
         tc = time.perf_counter() - t
         out["cpu_baseline"] = {"value": len(sub) / tc, "unit": "queries/s", "cores": orc.max_threads(), "kind": "port",
-                               "sample": f"every 64th query ({len(sub)} points, {tc:.1f} s)"}
-        gc = ref["cell"][::64]
+                               "sample": f"every {stride}th query ({len(sub)} points, {tc:.1f} s), FP32 without the "
+                                         f"polish (which never fires here); CPU {_cpu_model()}, nproc {os.cpu_count()}"}
+        gc = ref["cell"][::stride]
         mism = np.nonzero(gc != oc)[0]
         out["located_id_mismatch_vs_oracle_sample"] = int(len(mism))
         if len(mism):
-            # a mismatch is explained when the point lies within 1e-5 of the [-1e-4, 1+1e-4] acceptance band
-            # edge in the element either side chose (FP32 rounding decides acceptance there)
-            edge = np.full(len(mism), np.inf)
-            for cells in (oc[mism], gc[mism]):
-                okc = cells >= 0
-                if okc.any():
-                    ex = orc.refine_f64(mesh, sub[mism][okc], cells[okc], np.full((int(okc.sum()), 3), 0.5))
-                    e = np.minimum(np.abs(ex + 1e-4), np.abs(ex - (1 + 1e-4))).min(axis=1)
-                    edge[okc] = np.minimum(edge[okc], e)
-            out["mismatches_at_band_edge"] = int((edge < 1e-5).sum())
+            # the FP64 oracle (same walk in double) decides: how many of those the GPU has right
+            oc64, _ = orc.locate(mesh, ob, sub[mism], dtype=np.float64)
+            out["mismatches_where_gpu_agrees_with_fp64_oracle"] = int((gc[mism] == oc64).sum())
+        # error against the FP64 inverse map on a spread sample of located points, and the polish statistics
+        idx = np.nonzero(ref["cell"] >= 0)[0][:: max(1, n_pts // 20000)]
+        ex = orc.refine_f64(mesh, pts_h[idx], ref["cell"][idx], ref["ref"][idx].astype(np.float64))
+        err = np.abs(ref["ref"][idx].astype(np.float64) - ex).max(axis=1)
+        cond = orc.jacobian_cond_f64(mesh, ref["cell"][idx], ex)
+        _, steps, _, polished = m.locate(api.to_device(pts_h[idx].reshape(-1)), variants["binary"], stats=True,
+                                         polished=True)
+        out["error_vs_fp64"] = {"points": int(len(idx)), "max": float(err.max()), "mean": float(err.mean()),
+                                "n_over_1e-5": int((err > 1e-5).sum()),
+                                "cond_fp64": {"max": float(cond.max()), "p99.99": float(np.percentile(cond, 99.99)),
+                                              "n_over_1e4": int((cond > 1e4).sum())},
+                                "polished": int(polished.cpu().numpy().sum()),
+                                "newton_steps_hist_gpu": np.bincount(np.minimum(steps.cpu().numpy(), 40),
+                                                                     minlength=41).tolist()}
     print(json.dumps(out))
     return 0
 
@@ -404,6 +433,111 @@ def bench_locate(args):
 def _amp_of_seed(s: int) -> float:
     """Warp amplitude ~ U[0, 0.2] per seed (seeded PCG64)."""
     return float(np.random.default_rng(10_000 + s).uniform(0.0, 0.2))
+
+
+def train_initializer(api, syn, seeds, per_seed, sizes, epochs, batch=16384):
+    """Training pairs (query, converged reference point of the located element) from plain locate on
+    order-3 meshes (jitter 0.01 x spacing, warp ~ U[0, 0.2] per seed, sizes[s % len(sizes)]^3 elements),
+    shuffled, then `epochs` passes of Adam with the MFMA kernels.  Returns (initializer, n_pairs, steps,
+    generation s, training ms, loss)."""
+    import torch
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    xs, ts = [], []
+    for s in seeds:
+        n_el = sizes[s % len(sizes)]
+        mesh = syn.hex_mesh(n_el, 3, _amp_of_seed(s), 0.01, 1000 + s)
+        m = api.UnstructuredMesh(mesh)
+        pts = api.to_device(syn.query_points(per_seed, m.bounds(), seed=s).reshape(-1))
+        loc = m.locate(pts)
+        x, t, _ = api.LearnedInitializer.features(m, pts, loc)
+        keep = loc.view(torch.int32).view(-1, 4)[:per_seed, 0] >= 0
+        xs.append(x[keep])
+        ts.append(t[keep])
+        del m, pts, loc
+    X, T = torch.cat(xs), torch.cat(ts)
+    del xs, ts
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    perm = torch.randperm(X.shape[0], generator=g, device=dev)
+    X, T = X[perm].contiguous(), T[perm].contiguous()
+    gen_s = time.time() - t0
+    steps = int(epochs * X.shape[0] / batch)
+    li = api.LearnedInitializer(seed=0)
+    li.train(X, T, 1, batch=batch, lr=0.0, lr_final=-1.0, record_loss=False)   # untimed launch warm-up
+    li = api.LearnedInitializer(seed=0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    loss = li.train(X, T, steps, batch=batch, lr=2e-3, lr_final=2e-5)
+    e1.record()
+    torch.cuda.synchronize()
+    return li, X, T, int(X.shape[0]), steps, gen_s, e0.elapsed_time(e1), loss.cpu().numpy()
+
+
+def render_starts(api, syn, mesh, m, f, packed, res, samples, steps, warmup, polish=True, error_points=20000):
+    """One frame of the volume render (perspective, distance 2.5, fov 45, grey -> red, opacity 0.02) with
+    each Newton start: plain (corner-affine start; SubRef centre), learned (fallback to the SubRef centre,
+    as dray's subref_center; fallback to the corner-affine start) and, as the ceiling any initializer can
+    reach, the perfect start: every sample's converged location from a recording pass given as its start.
+    Per start: integrate kernel ms (mean of `steps` frames after `warmup`), Newton steps per located sample,
+    max RGBA difference from the plain frame; and, on `error_points` uniform points located with each start,
+    the max reference-coordinate error against the FP64 inverse map."""
+    import torch
+    from oracle import orc
+    from devil_ray_amd.synthetic import LOCATION_DTYPE
+    bvh = m.get_bvh()
+    vol, cam, _ = _volume_setup(api, syn, m, f, bvh, res, samples)
+    rays = cam.create_rays()
+    pol = dict(fp64_polish=polish, cond_threshold=1e4)
+    starts = {"plain_affine": api.LocateOptions(**pol), "plain_centre": api.LocateOptions(start="centre", **pol)}
+    if packed is not None:
+        starts["learned_centre_fallback"] = api.LocateOptions(mlp=packed, mlp_fallback="centre", **pol)
+        starts["learned_affine_fallback"] = api.LocateOptions(mlp=packed, mlp_fallback="affine", **pol)
+    # the perfect start: record every sample's location (plain start), then give it back as the start
+    n_act = api.remove_missed_rays(rays, bvh.bounds)[1]
+    spr = samples + 16
+    rec = torch.full((max(n_act, 1) * spr * 4,), -1, dtype=torch.int32, device="cuda")
+    vol.opts = api.LocateOptions(record=rec, samples_per_ray=spr, **pol)
+    vol.integrate(rays, tile_width=res, tile_height=res)
+    starts["recorded_plain"] = api.LocateOptions(record=rec, samples_per_ray=spr, **pol)
+    starts["perfect_start"] = api.LocateOptions(guess=rec, samples_per_ray=spr, **pol)
+    out, imgs = {}, {}
+    for k, o in starts.items():
+        vol.opts = o
+        for _ in range(max(warmup, 1)):
+            vol.integrate(rays, tile_width=res, tile_height=res)
+        ks = []
+        for _ in range(max(steps, 1)):
+            parts, cnt = vol.integrate(rays, tile_width=res, tile_height=res)
+            ks.append(vol.last_stats.kernel_ms)
+        st = vol.last_stats
+        imgs[k] = api.composite(parts, cnt, res, res).cpu().numpy()
+        k_ms = float(np.mean(ks))
+        out[k] = {"integrate_kernel_ms": k_ms, "located_samples": int(st.located_samples),
+                  "samples_per_s": st.located_samples / (k_ms / 1e3),
+                  "newton_steps_per_located": st.newton_steps / max(1, st.located_samples),
+                  "polished": int(st.polished)}
+    for k in out:
+        out[k]["rgba_max_diff_vs_plain_affine"] = float(np.abs(imgs[k] - imgs["plain_affine"]).max())
+    del rec
+    # reference-coordinate error of each start on uniform points (the perfect start: the plain answer given)
+    pts_h = syn.query_points(error_points, bvh.bounds, seed=99)
+    dp = api.to_device(pts_h.reshape(-1))
+    base = m.locate(dp, starts["plain_affine"])
+    for k, o in starts.items():
+        if k == "recorded_plain":
+            continue
+        if k == "perfect_start":
+            o = api.LocateOptions(guess=base, **pol)
+        l = m.locate(dp, o).cpu().numpy().view(LOCATION_DTYPE)
+        ok = l["cell"] >= 0
+        ex = orc.refine_f64(mesh, pts_h[ok], l["cell"][ok], l["ref"][ok].astype(np.float64))
+        out[k]["max_ref_err_vs_fp64"] = float(np.abs(l["ref"][ok].astype(np.float64) - ex).max())
+    ref_ms = out["plain_centre"]["integrate_kernel_ms"]
+    for k in out:
+        out[k]["speedup_vs_plain_centre"] = ref_ms / out[k]["integrate_kernel_ms"]
+    return out
 
 
 def bench_mlp(args):
@@ -430,42 +564,13 @@ def bench_mlp(args):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps, r
 
-    # ---- training pairs ----
-    t0 = time.time()
-    per_seed = -(-args.mlp_pairs // 32)
-    xs, ts = [], []
-    for s in range(32):
-        n_el = (16, 32, 64)[s % 3]
-        mesh = syn.hex_mesh(n_el, 3, _amp_of_seed(s), 0.01, 1000 + s)
-        m = api.UnstructuredMesh(mesh)
-        pts = api.to_device(syn.query_points(per_seed, m.bounds(), seed=s).reshape(-1))
-        loc = m.locate(pts)
-        x, t, _ = api.LearnedInitializer.features(m, pts, loc)
-        keep = loc.view(torch.int32).view(-1, 4)[:per_seed, 0] >= 0
-        xs.append(x[keep])
-        ts.append(t[keep])
-        del m, pts, loc
-    X, T = torch.cat(xs), torch.cat(ts)
-    del xs, ts
-    g = torch.Generator(device=dev)
-    g.manual_seed(0)
-    perm = torch.randperm(X.shape[0], generator=g, device=dev)
-    X, T = X[perm].contiguous(), T[perm].contiguous()
-    n_pairs = int(X.shape[0])
-    gen_s = time.time() - t0
-    print(f"[config 3] {n_pairs} training pairs in {gen_s:.1f} s", file=sys.stderr, flush=True)
-
-    # ---- training ----
+    # ---- training pairs and training ----
     batch = 16384
-    steps = int(args.mlp_epochs * n_pairs / batch)
-    li = api.LearnedInitializer(seed=0)
-    li.train(X, T, 1, batch=batch, lr=0.0, lr_final=-1.0, record_loss=False)   # untimed launch warm-up
-    li = api.LearnedInitializer(seed=0)
-    train_ms, loss = timed(lambda: li.train(X, T, steps, batch=batch, lr=2e-3, lr_final=2e-5), 1)
-    loss = loss.cpu().numpy()
+    li, X, T, n_pairs, steps, gen_s, train_ms, loss = train_initializer(api, syn, range(32), -(-args.mlp_pairs // 32),
+                                                                        (16, 32, 64), args.mlp_epochs, batch)
     train_rate = steps * batch / (train_ms / 1e3)
-    print(f"[config 3] trained {steps} steps in {train_ms / 1e3:.2f} s, mse {loss[:10].mean():.3g} -> "
-          f"{loss[-20:].mean():.3g}", file=sys.stderr, flush=True)
+    print(f"[config 3] {n_pairs} training pairs in {gen_s:.1f} s; trained {steps} steps in {train_ms / 1e3:.2f} s, "
+          f"mse {loss[:10].mean():.3g} -> {loss[-20:].mean():.3g}", file=sys.stderr, flush=True)
     packed = li.pack()
 
     # ---- inference throughput (standalone kernel over the training rows) ----
@@ -487,7 +592,7 @@ def bench_mlp(args):
     # ---- held-out evaluation ----
     hist_len = 41
     agg = {k: {"ms": 0.0, "steps": 0, "hist": np.zeros(hist_len, np.int64), "err": 0.0}
-           for k in ("plain", "plain_centre", "plain_wide", "learned")}
+           for k in ("plain", "plain_centre", "plain_wide", "learned", "learned_centre_fallback", "perfect_start")}
     mism_total, n_total, pred_err = 0, 0, []
     for s in range(32, 40):
         mesh = syn.hex_mesh(64, 3, _amp_of_seed(s), 0.01, 1000 + s)
@@ -497,8 +602,13 @@ def bench_mlp(args):
         # plain = the library default (binary walk, as the learned start uses, Newton from the corner-affine
         # start); plain_centre = the same from the SubRef centre (dray's subref_center start); plain_wide =
         # the 8-wide walk
+        # learned = fallback to the corner-affine start; learned_centre_fallback = to the SubRef centre;
+        # perfect_start = the plain answer given as the start (the ceiling of any initializer)
+        answer = m.locate(pts)
         opts = {"plain": api.LocateOptions(), "plain_centre": api.LocateOptions(start="centre"),
-                "plain_wide": api.LocateOptions(traversal=1), "learned": api.LocateOptions(mlp=packed)}
+                "plain_wide": api.LocateOptions(traversal=1), "learned": api.LocateOptions(mlp=packed),
+                "learned_centre_fallback": api.LocateOptions(mlp=packed, mlp_fallback="centre"),
+                "perfect_start": api.LocateOptions(guess=answer)}
         res = {}
         for k, o in opts.items():
             for _ in range(args.warmup):
@@ -528,45 +638,19 @@ def bench_mlp(args):
         pred_err.append(np.abs(guess - ex).max(axis=1))
         print(f"[config 3] held-out seed {s}: plain {agg['plain']['ms']:.1f} ms learned {agg['learned']['ms']:.1f} ms "
               f"(cumulative)", file=sys.stderr, flush=True)
-        del m, pts
+        del m, pts, answer
     pe = np.concatenate(pred_err)
     value = n_total / (agg["learned"]["ms"] / 1e3)
     plain_rate = n_total / (agg["plain"]["ms"] / 1e3)
     centre_rate = n_total / (agg["plain_centre"]["ms"] / 1e3)
 
     # ---- volume render of the 64^3 order-3 frame (1024^2, 512 samples/ray, FP64 polish on) per start ----
-    res_px, samples = 1024, 512
     mesh = syn.hex_mesh(64, 3, 0.1, 0.01, 1234)   # warp 0.1; seed outside the training and held-out seeds
     m, f = api.UnstructuredMesh(mesh), api.UnstructuredField(mesh)
-    vol, cam, _ = _volume_setup(api, syn, m, f, m.get_bvh(), res_px, samples)
-    rays = cam.create_rays()
-    render, imgs = {}, {}
-    starts = {"plain": api.LocateOptions(fp64_polish=True),
-              "plain_centre": api.LocateOptions(fp64_polish=True, start="centre"),
-              "learned": api.LocateOptions(fp64_polish=True, mlp=packed)}
-    for k, o in starts.items():
-        vol.opts = o
-        for _ in range(max(args.warmup, 1)):
-            vol.integrate(rays, tile_width=res_px, tile_height=res_px)
-        ks = []
-        for _ in range(max(args.steps, 1)):
-            parts, cnt = vol.integrate(rays, tile_width=res_px, tile_height=res_px)
-            ks.append(vol.last_stats.kernel_ms)
-        st = vol.last_stats
-        imgs[k] = api.composite(parts, cnt, res_px, res_px).cpu().numpy()
-        k_ms = float(np.mean(ks))
-        render[k] = {"integrate_kernel_ms": k_ms, "located_samples": int(st.located_samples),
-                     "samples_per_s": st.located_samples / (k_ms / 1e3),
-                     "newton_steps_per_located": st.newton_steps / max(1, st.located_samples)}
-        print(f"[config 3] render {k}: {k_ms:.2f} ms", file=sys.stderr, flush=True)
-    for k in render:
-        render[k]["rgba_max_diff_vs_plain"] = float(np.abs(imgs[k] - imgs["plain"]).max())
-    render["speedup_learned_vs_plain_centre"] = (render["plain_centre"]["integrate_kernel_ms"]
-                                                 / render["learned"]["integrate_kernel_ms"])
-    render["speedup_learned_vs_plain"] = render["plain"]["integrate_kernel_ms"] / render["learned"]["integrate_kernel_ms"]
-    render["speedup_plain_vs_plain_centre"] = (render["plain_centre"]["integrate_kernel_ms"]
-                                               / render["plain"]["integrate_kernel_ms"])
-    del m, f, vol
+    render = render_starts(api, syn, mesh, m, f, packed, 1024, 512, max(args.steps, 1), max(args.warmup, 1))
+    for k, v in render.items():
+        print(f"[config 3] render {k}: {v['integrate_kernel_ms']:.2f} ms", file=sys.stderr, flush=True)
+    del m, f
     out = {
         "metric": "learned-init vs plain locate @1e-5 err (held-out seeds)", "value": value, "unit": "queries/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": agg["learned"]["ms"] / 8,
@@ -577,6 +661,8 @@ def bench_mlp(args):
                    "training_pairs": n_pairs, "epochs": args.mlp_epochs, "batch": batch, "train_steps": steps,
                    "eval_points_per_mesh": args.mlp_eval_points, "newton_budget_from_guess": 4},
         "plain_queries_per_s": plain_rate, "speedup_vs_plain": value / plain_rate,
+        "learned_centre_fallback_queries_per_s": n_total / (agg["learned_centre_fallback"]["ms"] / 1e3),
+        "perfect_start_queries_per_s": n_total / (agg["perfect_start"]["ms"] / 1e3),
         "plain_centre_queries_per_s": centre_rate, "speedup_vs_plain_centre": value / centre_rate,
         "plain_wide_queries_per_s": n_total / (agg["plain_wide"]["ms"] / 1e3),
         "newton_steps_per_query": {k: agg[k]["steps"] / n_total for k in agg},

@@ -19,7 +19,7 @@ F64 = C.c_double
 
 DRX_OK = 0
 DRX_ERR_ARG = 1
-DRX_INIT_AFFINE, DRX_INIT_MLP, DRX_INIT_SUBREF_CENTRE = 0, 1, 2   # drx_init_mode
+DRX_INIT_AFFINE, DRX_INIT_MLP, DRX_INIT_SUBREF_CENTRE, DRX_INIT_GUESS = 0, 1, 2, 3   # drx_init_mode
 SPLITS_PER_ELEMENT = 33
 MAX_ORDER = 4
 LOCATE_NOALLOC_MAX_ORDERED = 262144   # DRX_LOCATE_NOALLOC_MAX_ORDERED
@@ -39,7 +39,8 @@ class DrxBVH(C.Structure):
 class DrxLocateOpts(C.Structure):
     _fields_ = [("init_mode", I32), ("fp64_polish", I32), ("cond_threshold", F32), ("newton_steps", P),
                 ("candidates", P), ("traversal", I32), ("coherent_order", I32), ("mlp", P),
-                ("mlp_newton_steps", I32), ("launch", I32), ("chunk", I32), ("polished", P)]
+                ("mlp_newton_steps", I32), ("launch", I32), ("chunk", I32), ("polished", P),
+                ("mlp_fallback", I32), ("guess", P), ("record", P), ("samples_per_ray", I32)]
 
 
 class DrxMlpTrainOpts(C.Structure):

@@ -123,6 +123,11 @@ class BVH:
         v = self.view()
         L.check(L.lib().drx_bvh_pack_leaves(C.byref(v), _ptr(self.leaf_pack), _stream()), "drx_bvh_pack_leaves")
 
+    def ensure_wide(self) -> None:
+        """The 8-wide copy, built on first use (the default walk does not read it)."""
+        if self.wide_nodes is None:
+            self.widen()
+
     def widen(self) -> None:
         """Build the 8-wide traversal copy (drx_bvh_widen): count, allocate, fill."""
         v = self.view()
@@ -223,6 +228,10 @@ class LocateOptions:
     start: str = "affine"   # plain Newton start: "affine" (corner-affine estimate) or "centre" (SubRef centre)
     launch: int = 0         # streamed walks: 0 tuned (drx_locate_autotune) else chunked, 1 chunked, 2 persistent
     chunk: int = 0          # points per wavefront chunk (0: tuned, else 1024)
+    mlp_fallback: str = "affine"   # learned start: the plain start that takes over, "affine" or "centre"
+    guess: Optional[torch.Tensor] = None    # given starts (Location records): DRX_INIT_GUESS
+    record: Optional[torch.Tensor] = None   # integrate: Location of every (ray, sample)
+    samples_per_ray: int = 0                # integrate: guess / record entries per ray
 
     def c(self, steps=None, cands=None, polished=None) -> L.DrxLocateOpts:
         o = L.DrxLocateOpts()
@@ -243,7 +252,21 @@ class LocateOptions:
             o.init_mode = 1
             o.mlp = self.mlp.data_ptr()
             o.mlp_newton_steps = int(self.mlp_newton_steps)
+            fb = {"affine": 0, "centre": 1}
+            if self.mlp_fallback not in fb:
+                raise ValueError(f"LocateOptions.mlp_fallback must be 'affine' or 'centre', not {self.mlp_fallback!r}")
+            o.mlp_fallback = fb[self.mlp_fallback]
+        if self.guess is not None:
+            o.init_mode = L.DRX_INIT_GUESS
+            o.guess = self.guess.data_ptr()
+        o.record = self.record.data_ptr() if self.record is not None else None
+        o.samples_per_ray = int(self.samples_per_ray)
         return o
+
+
+# BVHs of at most this many leaves get their 8-wide copy when they are built (the automatic walk of a render
+# may take the wave walk over it); larger ones build it when a walk asks for it.
+WIDE_EAGER_MAX_LEAVES = 1 << 16
 
 
 class UnstructuredMesh:
@@ -279,7 +302,8 @@ class UnstructuredMesh:
             bvh.pack_leaves()
             bvh.compute_leaf_starts(self.gf.view())
             bvh.quantize()
-            bvh.widen()
+            if bvh.n_leaves <= WIDE_EAGER_MAX_LEAVES:
+                bvh.widen()   # small BVHs: the automatic walk of a render may take the wave walk
             self._bvh = bvh
         return self._bvh
 
@@ -298,6 +322,8 @@ class UnstructuredMesh:
         steps = torch.zeros(max(n, 1), dtype=torch.int32, device=dev) if stats else None
         cands = torch.zeros(max(n, 1), dtype=torch.int32, device=dev) if stats else None
         pol = torch.zeros(max(n, 1), dtype=torch.int32, device=dev) if polished else None
+        if opts is not None and opts.traversal in (1, 2):
+            bvh.ensure_wide()
         o = (opts or LocateOptions()).c(steps, cands, pol)
         mv, bv = self.gf.view(), bvh.view()
         L.check(L.lib().drx_locate(C.byref(mv), C.byref(bv), _ptr(points), n, _ptr(out), C.byref(o), _stream()),
@@ -312,6 +338,8 @@ class UnstructuredMesh:
         options, and make the fastest the default (launch 0, chunk 0) for this device, order, polish and walk."""
         n = int(points.numel() // 3)
         out = torch.empty(max(n, 1) * 16, dtype=torch.uint8, device=_dev())
+        if opts is not None and opts.traversal in (1, 2):
+            self.get_bvh().ensure_wide()
         o = (opts or LocateOptions()).c()
         launch, chunk = C.c_int32(0), C.c_int32(0)
         ms = (C.c_float * 10)()
@@ -586,6 +614,8 @@ class Volume:
         out = torch.empty(max(5 * n_act, 1) * PARTIAL_DTYPE.itemsize, dtype=torch.uint8, device=_dev())
         cnt = C.c_int32(0)
         st = L.DrxIntegrateStats()
+        if self.opts.traversal in (1, 2):
+            self.mesh.get_bvh().ensure_wide()
         o = self.opts.c()
         mv, fv, bv = self.mesh.gf.view(), self.field_.gf.view(), self.mesh.get_bvh().view()
         L.check(L.lib().drx_volume_integrate(C.byref(mv), C.byref(fv), C.byref(bv), _ptr(active), n_act, _ptr(lut),

@@ -194,11 +194,15 @@ template <class ElemT> BVH &UnstructuredMesh<ElemT>::get_bvh ()
     drx_check (drx_bvh_quantize (&v, m_bvh.m_qnodes.get_device_ptr (), m_bvh.m_leaf_boxes.get_device_ptr (),
                                  m_bvh.m_qgrid, nullptr),
                "drx_bvh_quantize");
-    int64_t cnt = 0;
-    v = bvh_view ();
-    drx_check (drx_bvh_widen (&v, nullptr, 0, &cnt, nullptr), "drx_bvh_widen");
-    m_bvh.m_wide_nodes.resize (64 * (size_t)cnt);
-    drx_check (drx_bvh_widen (&v, m_bvh.m_wide_nodes.get_device_ptr (), cnt, &cnt, nullptr), "drx_bvh_widen");
+    // the 8-wide copy, read only by the wave walk that the automatic choice takes for small BVHs
+    if (m_bvh.m_leaf_nodes.size () <= (size_t (1) << 16))
+    {
+      int64_t cnt = 0;
+      v = bvh_view ();
+      drx_check (drx_bvh_widen (&v, nullptr, 0, &cnt, nullptr), "drx_bvh_widen");
+      m_bvh.m_wide_nodes.resize (64 * (size_t)cnt);
+      drx_check (drx_bvh_widen (&v, m_bvh.m_wide_nodes.get_device_ptr (), cnt, &cnt, nullptr), "drx_bvh_widen");
+    }
     m_is_constructed = true;
   }
   return m_bvh;

@@ -475,6 +475,9 @@ struct LocateCfg
   bool polish;
   float cond_threshold;
   int32_t *polished_out;    // optional, per point: 1 where the FP64 correction ran (batched locate only)
+  const Location *guess;    // DRX_INIT_GUESS: given starts (per point, or per ray and sample)
+  Location *record;         // integrate: the location of every (ray, sample)
+  int samples_per_ray;      // integrate: guess / record entries per ray
 };
 
 struct LocateStats
@@ -843,7 +846,7 @@ __device__ __forceinline__ int next_candidate_leaf (const LocateCfg &cfg, TravSt
 template <int P, bool POLISH>
 __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStack &todo, const float pt[3],
                                                   float ref[3], LocateStats &st, bool &overflow, int &cur, int &sp,
-                                                  int &cell, float *stage = nullptr)
+                                                  int &cell, float *stage = nullptr, const Location *guess = nullptr)
 {
   float4 l0, l1, map[3];
   const int leaf = next_candidate_leaf (cfg, todo, pt, cur, sp, l0, l1, map);
@@ -855,6 +858,12 @@ __device__ __forceinline__ int locate_point_step (const LocateCfg &cfg, TravStac
   const int el = __float_as_int (l0.x);
   float x[3];
   candidate_start_rows<P> (cfg, el, map, pt, l0, l1, x);
+  if (guess && guess->cell == el)   // a given start for this candidate's element (use_init_guess)
+  {
+    x[0] = guess->ref[0];
+    x[1] = guess->ref[1];
+    x[2] = guess->ref[2];
+  }
   bool found;
   if constexpr (P == 0)
     found = try_element_mixed<POLISH> (cfg, el, pt, x, st);

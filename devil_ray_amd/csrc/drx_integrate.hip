@@ -160,6 +160,10 @@ __device__ __forceinline__ float4 lit_color (const IntegrateArgs &a, int cell, c
 #define DRX_INTEGRATE_BLOCK 64
 #endif
 constexpr int kIntBlock = DRX_INTEGRATE_BLOCK;   // threads per workgroup of the integrate kernel
+// DRX_TRAVERSE_AUTO in drx_volume_integrate: the wave walk for BVHs of at most this many leaves and frames
+// of at least this many active rays per SIMD, else the binary walk (measured; see drx_volume_integrate).
+constexpr int kWaveWalkMaxLeaves = 1 << 16;
+constexpr int kWaveWalkMinRaysPerSimd = 48;
 constexpr int kIntWaves = kIntBlock / 64;
 
 // SHADE: 0 VolumeShader::color over an FP32 field, 1 shaded_color (lit), 2 color over an FP64 field.
@@ -178,7 +182,8 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
   // Element blocks staged in LDS for Newton: order 3 only.  Measured per order (binary walk): order 3
   // 15.5 -> 15.2 ms on the 64^3 frame; order 4 (two slots fit) 211 -> 234 ms and order 2 on a frame of one
   // wavefront per SIMD 4.3 -> 5.1 ms, where the copy and the wave synchronisation cost more than they save.
-  constexpr bool STAGE = MODE == 0 && P == 3;
+  constexpr bool STAGE = (MODE == 0 || MODE == 4) && P == 3;
+  constexpr bool GUIDED = MODE == 4;    // binary walk with given starts and / or the record of every sample
   __shared__ __attribute__ ((aligned (16))) float stage_lds[STAGE ? kIntWaves * Stage<(P >= 1 ? P : 1)>::FLOATS : 4];
   WaveScratch ws;
   const int wv = threadIdx.x >> 6;
@@ -235,6 +240,9 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
     // lanes go on with their next samples, so the wavefront never runs a Newton round for a few lanes.
     int tcur = 0, tsp = 0;
     bool resume = false;
+    int sample = 0;   // GUIDED: the number of locates this ray has finished
+    Location given;   // GUIDED: the given start of the current sample
+    given.cell = -1;
     for (;;)
     {
       if (alive && !inseg && !(distance < ray.far_)) alive = false;
@@ -267,13 +275,19 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
       }
       else if constexpr (WAVE)
         cell = locate_wave<P, POLISH> (a.cfg, ws, pt, ref, st, overflow);
-      else if constexpr (MODE <= 1)
+      else if constexpr (MODE <= 1 || GUIDED)
       {
         if (!resume)
         {
           tcur = 0;
           tsp = 0;
-          if constexpr (MODE == 0) todo.set (0, kBarrier);
+          if constexpr (MODE != 1) todo.set (0, kBarrier);
+          if constexpr (GUIDED)
+          {
+            given.cell = -1;
+            if (a.cfg.guess && sample < a.cfg.samples_per_ray)
+              given = a.cfg.guess[(int64_t)ri * a.cfg.samples_per_ray + sample];
+          }
         }
         int c = -1;
         int r;
@@ -281,10 +295,24 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
           r = locate_point_wide_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c);
         else
           r = locate_point_step<P, POLISH> (a.cfg, todo, pt, ref, st, overflow, tcur, tsp, c,
-                                            STAGE ? stage_lds + wv * Stage<(P >= 1 ? P : 1)>::FLOATS : nullptr);
+                                            STAGE ? stage_lds + wv * Stage<(P >= 1 ? P : 1)>::FLOATS : nullptr,
+                                            GUIDED ? &given : nullptr);
         resume = r == 0;
         if (resume) continue;
         cell = r > 0 ? c : -1;
+        if constexpr (GUIDED)
+        {
+          if (a.cfg.record && sample < a.cfg.samples_per_ray && !overflow)
+          {
+            Location rec;
+            rec.cell = cell;
+            rec.ref[0] = cell >= 0 ? ref[0] : -1.f;
+            rec.ref[1] = cell >= 0 ? ref[1] : -1.f;
+            rec.ref[2] = cell >= 0 ? ref[2] : -1.f;
+            a.cfg.record[(int64_t)ri * a.cfg.samples_per_ray + sample] = rec;
+          }
+          ++sample;
+        }
       }
       else
         cell = locate_any<P, POLISH, MODE == 1> (a.cfg, todo, pt, ref, st, overflow);
@@ -395,8 +423,13 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   if (n_rays < 0 || lut_size < 1 || !lut) return fail (DRX_ERR_ARG, "drx_volume_integrate: bad rays/lut");
   if (!(field_min <= field_max)) return fail (DRX_ERR_ARG, "ColorMap scalar range never set");
   const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
-  if (opts && opts->init_mode != DRX_INIT_AFFINE && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
+  const bool given = opts && opts->init_mode == DRX_INIT_GUESS;
+  if (opts && opts->init_mode != DRX_INIT_AFFINE && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned && !given)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: unknown init_mode %d", opts->init_mode);
+  const bool guided = given || (opts && opts->record);
+  if (guided && (opts->samples_per_ray <= 0 || (given && !opts->guess) || learned))
+    return fail (DRX_ERR_ARG, "drx_volume_integrate: guess / record need samples_per_ray > 0 (and opts->guess "
+                 "with DRX_INIT_GUESS), and not the learned start");
   if (learned && !opts->mlp)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
   const bool lit = shading && shading->use_lighting;
@@ -458,20 +491,25 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   int walk = trav;
   if (walk == DRX_TRAVERSE_AUTO && a.cfg.wide && !learned)
   {
-    // fewer wavefronts than SIMDs: nothing hides a sample's chain of dependent reads, and the wave walk's
-    // uniform node reads shorten it (64^3-mesh frames: binary; an 8^3 frame of 24,025 rays: 4.36 -> 3.97 ms)
+    // The wave walk (uniform node reads shared by a wavefront's coherent rays) pays on small BVHs with at
+    // least ~48 active rays per SIMD: an 8^3 order-2 mesh (16,896 leaves), orthographic frames of
+    // 224^2..384^2 rays run 1-13% faster than the binary walk, 192^2 (36 rays per SIMD) 11% slower.  On the
+    // 64^3 order-3 mesh (8.65 M leaves) it is 2.6-4.9x slower at every frame size from 128^2 to 512^2.
     int dev = 0;
-    if (hipGetDevice (&dev) == hipSuccess && (int64_t)n_rays <= (int64_t)64 * 4 * compute_units (dev))
+    if (bvh->n_leaves <= kWaveWalkMaxLeaves && hipGetDevice (&dev) == hipSuccess &&
+        (int64_t)n_rays >= (int64_t)kWaveWalkMinRaysPerSimd * 4 * compute_units (dev))
       walk = DRX_TRAVERSE_WAVE;
   }
-  const int mode = learned ? 3 : walk == DRX_TRAVERSE_WAVE ? 2 : (walk == DRX_TRAVERSE_LANE && a.cfg.wide) ? 1 : 0;
+  const int mode = guided ? 4 : learned ? 3 : walk == DRX_TRAVERSE_WAVE ? 2 : (walk == DRX_TRAVERSE_LANE && a.cfg.wide) ? 1 : 0;
   hipEvent_t ev[3];
   for (int q = 0; q < 3; ++q) DRX_HIP (hipEventCreate (&ev[q]));
   DRX_HIP (hipEventRecord (ev[0], st));
 #define DRX_INT2(PP, PO, LT)                                             \
   do                                                                     \
   {                                                                      \
-    if (mode == 3)                                                       \
+    if (mode == 4)                                                       \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 4, LT>), g, b, 0, st, a); \
+    else if (mode == 3)                                                  \
       hipLaunchKernelGGL ((k_integrate<PP, PO, 3, LT>), g, b, 0, st, a); \
     else if (mode == 2)                                                  \
       hipLaunchKernelGGL ((k_integrate<PP, PO, 2, LT>), g, b, 0, st, a); \
@@ -483,7 +521,9 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
 #define DRX_INT64(PP, PO)                                                \
   do                                                                     \
   {                                                                      \
-    if (mode == 2)                                                       \
+    if (mode == 4)                                                       \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 4, 2>), g, b, 0, st, a);  \
+    else if (mode == 2)                                                  \
       hipLaunchKernelGGL ((k_integrate<PP, PO, 2, 2>), g, b, 0, st, a);  \
     else if (mode == 1)                                                  \
       hipLaunchKernelGGL ((k_integrate<PP, PO, 1, 2>), g, b, 0, st, a);  \

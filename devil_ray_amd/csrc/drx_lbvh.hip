@@ -40,6 +40,57 @@ __device__ __forceinline__ float box_measure (const float *b)   // b: xmin,xmax,
   return v;
 }
 
+// de Casteljau halving of every line of a net along `axis` (order P, nets of NPE_ stride per component):
+// lane t handles (component t / lines, line t % lines), 64 lanes per pass.  Running min/max of both halves
+// per component go to a/b mn/mx.
+template <int P>
+__device__ __forceinline__ void split_lines (float *mom, float *dau, int NPE_, int axis, int lane, float amn[3],
+                                             float amx[3], float bmn[3], float bmx[3])
+{
+  constexpr int N1 = P + 1, LINES = N1 * N1;
+  const int stride = (axis == 0) ? 1 : ((axis == 1) ? N1 : N1 * N1);
+  for (int t = lane; t < 3 * LINES; t += 64)
+  {
+    const int c = t / LINES, l = t - c * LINES;
+    const int u = l % N1, v = l / N1;   // the two other indices, lowest axis first
+    const int base = (axis == 0) ? (N1 * u + LINES * v) : ((axis == 1) ? (u + LINES * v) : (u + N1 * v));
+    float *m = mom + c * NPE_ + base, *d = dau + c * NPE_ + base;
+    float a[N1], b[N1];
+#pragma unroll
+    for (int ii = 0; ii < N1; ++ii) a[ii] = b[ii] = m[ii * stride];
+#pragma unroll
+    for (int r = 1; r <= P; ++r)
+    {
+      // mother: new[ii] = .5 old[ii-1] + .5 old[ii] for ii >= r (descending, so old[ii-1] is still old)
+#pragma unroll
+      for (int ii = P; ii >= r; --ii) a[ii] = a[ii - 1] * (1.0f - 0.5f) + a[ii] * 0.5f;
+      // daughter: new[ii] = .5 old[ii] + .5 old[ii+1] for ii <= P - r (ascending)
+#pragma unroll
+      for (int ii = 0; ii <= P - r; ++ii) b[ii] = b[ii] * (1.0f - 0.5f) + b[ii + 1] * 0.5f;
+    }
+    float lo_a = INFINITY, hi_a = -INFINITY, lo_b = INFINITY, hi_b = -INFINITY;
+#pragma unroll
+    for (int ii = 0; ii < N1; ++ii)
+    {
+      m[ii * stride] = a[ii];
+      d[ii * stride] = b[ii];
+      lo_a = fminf (lo_a, a[ii]);
+      hi_a = fmaxf (hi_a, a[ii]);
+      lo_b = fminf (lo_b, b[ii]);
+      hi_b = fmaxf (hi_b, b[ii]);
+    }
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+      if (c == cc)
+      {
+        amn[cc] = fminf (amn[cc], lo_a);
+        amx[cc] = fmaxf (amx[cc], hi_a);
+        bmn[cc] = fminf (bmn[cc], lo_b);
+        bmx[cc] = fmaxf (bmx[cc], hi_b);
+      }
+  }
+}
+
 template <int PMAX>
 __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl_idx, const float *__restrict__ values,
                                                  const int32_t *__restrict__ orders, const int64_t *__restrict__ offsets,
@@ -139,74 +190,27 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
       for (int d = 0; d < 3; ++d)
         if (d == axis) my_ref[3 + d] = split_pt;
     }
-    // copy mother -> daughter
+    // split the mother's net along `axis`, one (line, component) per lane: a line is the p+1 dofs along
+    // the axis with the other two indices fixed; each lane reads its line, runs the de Casteljau rounds in
+    // registers (mother keeps [lo, mid], daughter [mid, hi]; the same operations in the same order as
+    // split_inplace, elem_ops.hpp:1346-1406) and writes both halves.  Lines are disjoint, so no lane
+    // reads what another lane writes in this count.
     float *mom = nets + max_id * 3 * NPE;
     float *dau = nets + count * 3 * NPE;
-#pragma unroll
-    for (int k = 0; k < DPL; ++k)
-    {
-      const int d = lane + 64 * k;
-      if (d < npe)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) dau[c * NPE + d] = mom[c * NPE + d];
-    }
-    __syncthreads ();
-    // de Casteljau rounds: mother keeps [lo, mid] (new[ii] = .5 old[ii-1] + .5 old[ii], ii >= r),
-    // daughter keeps [mid, hi] (new[ii] = .5 old[ii] + .5 old[ii+1], ii <= p - r)
-    const int stride = (axis == 0) ? 1 : ((axis == 1) ? (p + 1) : (p + 1) * (p + 1));
-    for (int r = 1; r <= p; ++r)
-    {
-      float nm[DPL][3], nd[DPL][3];
-#pragma unroll
-      for (int k = 0; k < DPL; ++k)
-      {
-        const int d = lane + 64 * k;
-        if (d < npe)
-        {
-          const int ii = (d / stride) % (p + 1);
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-          {
-            const float sm = mom[c * NPE + d];
-            nm[k][c] = (ii >= r) ? (mom[c * NPE + d - stride] * (1.0f - 0.5f) + sm * 0.5f) : sm;
-            const float sd = dau[c * NPE + d];
-            nd[k][c] = (ii <= p - r) ? (sd * (1.0f - 0.5f) + dau[c * NPE + d + stride] * 0.5f) : sd;
-          }
-        }
-      }
-      __syncthreads ();
-#pragma unroll
-      for (int k = 0; k < DPL; ++k)
-      {
-        const int d = lane + 64 * k;
-        if (d < npe)
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-          {
-            mom[c * NPE + d] = nm[k][c];
-            dau[c * NPE + d] = nd[k][c];
-          }
-      }
-      __syncthreads ();
-    }
-    // hulls
     float amn[3] = { INFINITY, INFINITY, INFINITY }, amx[3] = { -INFINITY, -INFINITY, -INFINITY };
     float bmn[3] = { INFINITY, INFINITY, INFINITY }, bmx[3] = { -INFINITY, -INFINITY, -INFINITY };
-#pragma unroll
-    for (int k = 0; k < DPL; ++k)
-    {
-      const int d = lane + 64 * k;
-      if (d < npe)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-        {
-          const float a = mom[c * NPE + d], b = dau[c * NPE + d];
-          amn[c] = fminf (amn[c], a);
-          amx[c] = fmaxf (amx[c], a);
-          bmn[c] = fminf (bmn[c], b);
-          bmx[c] = fmaxf (bmx[c], b);
-        }
-    }
+    if constexpr (PMAX == 0)
+      ;
+    else if (p == PMAX)
+      split_lines<PMAX> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx);
+    else
+      switch (p)   // mixed-order meshes
+      {
+      case 1: split_lines<1> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx); break;
+      case 2: split_lines<(PMAX >= 2 ? 2 : 1)> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx); break;
+      case 3: split_lines<(PMAX >= 3 ? 3 : 1)> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx); break;
+      default: split_lines<(PMAX >= 4 ? 4 : 1)> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx); break;
+      }
     float ba[6], bb[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c)

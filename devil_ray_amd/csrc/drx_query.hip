@@ -216,6 +216,8 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
   float pt[3] = { 0.f, 0.f, 0.f };
   int cur = 0, sp = 0;
   LocateStats st { 0, 0, 0, 0 };
+  Location given;   // MODE 4: the point's given start
+  given.cell = -1;
   auto start = [&] () {
     i = order ? (int64_t)order[k] : k;
     pt[0] = pts[3 * i];
@@ -223,7 +225,8 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
     pt[2] = pts[3 * i + 2];
     cur = 0;
     sp = 0;
-    if constexpr (MODE == 0) todo.set (0, kBarrier);
+    if constexpr (MODE != 1) todo.set (0, kBarrier);
+    if constexpr (MODE == 4) given = cfg.guess[i];
     st = LocateStats { 0, 0, 0, 0 };
   };
   bool done = true;   // every lane starts without a point
@@ -246,7 +249,8 @@ k_locate_stream (LocateCfg cfg, const float *__restrict__ pts, int64_t n, const 
       if constexpr (MODE == 1)
         r = locate_point_wide_step<P, POLISH> (cfg, todo, pt, ref, st, overflow, cur, sp, cell);
       else
-        r = locate_point_step<P, POLISH> (cfg, todo, pt, ref, st, overflow, cur, sp, cell);
+        r = locate_point_step<P, POLISH> (cfg, todo, pt, ref, st, overflow, cur, sp, cell, nullptr,
+                                          MODE == 4 ? &given : nullptr);
       if (r != 0)
       {
         done = true;
@@ -784,8 +788,10 @@ static int locate_impl (const drx_mesh *mesh, const drx_bvh *bvh, const float *p
                         const SortScratch *ws)
 {
   const bool learned = opts && opts->init_mode == DRX_INIT_MLP;
-  if (opts && opts->init_mode != DRX_INIT_AFFINE && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned)
+  const bool given = opts && opts->init_mode == DRX_INIT_GUESS;
+  if (opts && opts->init_mode != DRX_INIT_AFFINE && opts->init_mode != DRX_INIT_SUBREF_CENTRE && !learned && !given)
     return fail (DRX_ERR_ARG, "drx_locate: unknown init_mode %d", opts->init_mode);
+  if (given && !opts->guess) return fail (DRX_ERR_ARG, "drx_locate: DRX_INIT_GUESS needs opts->guess");
   if (learned && !opts->mlp) return fail (DRX_ERR_ARG, "drx_locate: DRX_INIT_MLP needs opts->mlp (drx_mlp_pack)");
   int launch = opts ? opts->launch : DRX_LAUNCH_AUTO;
   int chunk = opts ? opts->chunk : 0;
@@ -832,7 +838,8 @@ static int locate_impl (const drx_mesh *mesh, const drx_bvh *bvh, const float *p
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !cfg.wide)
     return fail (DRX_ERR_ARG, "drx_locate: wave traversal needs bvh->wide_nodes");
-  const int mode = trav == DRX_TRAVERSE_WAVE ? 2 : (trav == DRX_TRAVERSE_LANE && cfg.wide) ? 1 : 0;
+  // given starts ride on the binary walk
+  const int mode = given ? 4 : trav == DRX_TRAVERSE_WAVE ? 2 : (trav == DRX_TRAVERSE_LANE && cfg.wide) ? 1 : 0;
   const int32_t *order = nullptr;
   if (opts && opts->coherent_order)
   {
@@ -852,6 +859,9 @@ static int locate_impl (const drx_mesh *mesh, const drx_bvh *bvh, const float *p
       hipLaunchKernelGGL ((k_locate<PP, PO, 2>), g, b, 0, st, cfg, points, n, order, out, steps, cands, err); \
     else if (mode == 1)                                                                             \
       hipLaunchKernelGGL ((k_locate_stream<PP, PO, 1>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err, \
+                          chunk, queue);                                                            \
+    else if (mode == 4)                                                                             \
+      hipLaunchKernelGGL ((k_locate_stream<PP, PO, 4>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err, \
                           chunk, queue);                                                            \
     else                                                                                            \
       hipLaunchKernelGGL ((k_locate_stream<PP, PO, 0>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err, \

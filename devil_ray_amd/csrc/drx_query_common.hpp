@@ -143,7 +143,7 @@ static inline LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx
     c.q_inv[d] = b->qgrid[3 + d];
   }
   if (c.qnodes && !c.leaf_boxes) c.qnodes = nullptr;
-  c.centre = o && o->init_mode == DRX_INIT_SUBREF_CENTRE;
+  c.centre = o && (o->init_mode == DRX_INIT_SUBREF_CENTRE || (o->init_mode == DRX_INIT_MLP && o->mlp_fallback == 1));
   c.starts = c.centre ? nullptr : (const float4 *)b->leaf_starts;
   c.mesh = m->data;
   c.offsets = m->elem_offset;
@@ -152,6 +152,9 @@ static inline LocateCfg make_cfg (const drx_mesh *m, const drx_bvh *b, const drx
   c.polish = o ? (o->fp64_polish != 0) : false;
   c.cond_threshold = o ? o->cond_threshold : 1e4f;
   c.polished_out = o ? o->polished : nullptr;
+  c.guess = (o && o->init_mode == DRX_INIT_GUESS) ? (const Location *)o->guess : nullptr;
+  c.record = o ? (Location *)o->record : nullptr;
+  c.samples_per_ray = o ? o->samples_per_ray : 0;
   return c;
 }
 

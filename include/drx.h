@@ -115,8 +115,13 @@ enum drx_init_mode
                                  drx_bvh_leaf_starts); the SubRef centre where that estimate is singular */
   DRX_INIT_MLP = 1,           /* learned initial guess (opts->mlp); plain Newton from the DRX_INIT_AFFINE
                                  start decides acceptance when the guess has not converged */
-  DRX_INIT_SUBREF_CENTRE = 2  /* Newton starts at the candidate's SubRef centre, subref_center
+  DRX_INIT_SUBREF_CENTRE = 2, /* Newton starts at the candidate's SubRef centre, subref_center
                                  (data_model/subref.hpp:52-56): dray's start when use_init_guess is false */
+  DRX_INIT_GUESS = 3          /* given initial guesses (LocateHack::eval_inverse with use_init_guess true,
+                                 device_mesh.hpp:131-165): the candidate whose element is the guess's cell
+                                 starts at the guess's reference point, the others at the DRX_INIT_AFFINE
+                                 start; opts->guess: one Location per point (drx_locate) or per (ray,
+                                 sample) (drx_volume_integrate, binary walk) */
 };
 
 typedef struct drx_locate_opts
@@ -136,6 +141,16 @@ typedef struct drx_locate_opts
   int32_t chunk;              /* points per wavefront chunk for those walks (0: tuned, else 1024) */
   int32_t *polished;          /* device, nullable: drx_locate writes 1 where the point's accepted solution took
                                  the FP64 correction (condition estimate > cond_threshold), else 0 */
+  int32_t mlp_fallback;       /* DRX_INIT_MLP: the plain start that takes over (and decides acceptance) when
+                                 the learned guess has not converged: 0 the DRX_INIT_AFFINE start, 1 the
+                                 SubRef centre (dray's subref_center) */
+  const void *guess;          /* DRX_INIT_GUESS: device Locations (see drx_init_mode) */
+  void *record;               /* drx_volume_integrate, nullable: device Locations, the location found for
+                                 every (ray, sample) (cell -1 where none), entry ray * samples_per_ray +
+                                 sample in the order the march locates them; with opts->guess it reads the
+                                 same entries (the binary walk then runs for every frame size) */
+  int32_t samples_per_ray;    /* entries per ray of guess / record in drx_volume_integrate (samples beyond
+                                 it are neither guessed nor recorded) */
 } drx_locate_opts;
 
 enum drx_launch
@@ -161,8 +176,8 @@ typedef struct drx_mlp_train_opts
 enum drx_traversal
 {
   DRX_TRAVERSE_AUTO = 0,      /* library default: DRX_TRAVERSE_BINARY (fastest measured on MI355X), except
-                                 drx_volume_integrate with fewer active rays than 64 per SIMD and an 8-wide
-                                 collapse: DRX_TRAVERSE_WAVE (latency-bound frames) */
+                                 drx_volume_integrate over a BVH of at most 65,536 leaves with its 8-wide
+                                 collapse and at least 48 active rays per SIMD: DRX_TRAVERSE_WAVE */
   DRX_TRAVERSE_LANE = 1,      /* every lane walks the 8-wide collapse (bvh->wide_nodes) depth-first */
   DRX_TRAVERSE_WAVE = 2,      /* the wavefront walks the union of its lanes' paths over the 8-wide
                                  collapse with uniform node reads */

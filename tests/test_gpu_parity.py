@@ -422,8 +422,8 @@ def test_locate_autotune(gpu):
 @pytest.mark.parametrize("name,make", [MESHES[1], MESHES[2], MESHES[4]], ids=["p2_n4", "p3_n5", "mixed_n4"])
 def test_render_walks_identical(gpu, orc, name, make):
     """Volume::integrate gives bit-identical partials and the same located-sample, Newton-step and
-    candidate counts with every walk: automatic (the wave walk for a frame this small), binary over the
-    compact nodes, 8-wide per lane, wave-coherent."""
+    candidate counts with every walk: automatic, binary over the compact nodes, 8-wide per lane,
+    wave-coherent."""
     mesh = make()
     m, bvh = _gpu_bvh(gpu, mesh)
     f = gpu.UnstructuredField(mesh)
@@ -826,3 +826,103 @@ def test_scratch_per_stream(gpu, orc):
     torch.cuda.synchronize()
     assert n1 == n0 and np.array_equal(_np(parts1, np.uint8)[: n1 * 24], want_parts)
     assert np.array_equal(_np(fb1.colors).view(np.uint32), want_fb.view(np.uint32))
+
+
+def test_auto_walk_choice(gpu):
+    """DRX_TRAVERSE_AUTO in a render: the wave walk for a BVH of at most 65,536 leaves (whose 8-wide copy is
+    built with it) and at least 48 active rays per SIMD, else the binary walk; a larger BVH gets no 8-wide
+    copy until a walk asks for one."""
+    simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+    small = syn.hex_mesh(8, 2, 0.05, 0.005, 1234, jitter_scaled=False)
+    m = gpu.UnstructuredMesh(small)
+    assert m.get_bvh().n_leaves <= 1 << 16 and m.get_bvh().wide_nodes is not None
+    vol = gpu.Volume(m, gpu.UnstructuredField(small))
+    vol.use_lighting = False
+    vol.samples(100)
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, 0.05)
+    side_big = int(np.ceil(np.sqrt(48 * simds))) + 8
+    for side, want in ((side_big, 2), (96, 0)):
+        vol.integrate(gpu.as_bytes(syn.orthographic_rays(side, side)))
+        assert vol.last_stats.walk == want, (side, vol.last_stats.walk)
+    big = syn.hex_mesh(20, 2, 0.05, 0.005, 3, jitter_scaled=False)
+    mb = gpu.UnstructuredMesh(big)
+    bvh = mb.get_bvh()
+    assert bvh.n_leaves > 1 << 16 and bvh.wide_nodes is None
+    vb = gpu.Volume(mb, gpu.UnstructuredField(big))
+    vb.use_lighting = False
+    vb.samples(100)
+    vb.table = vol.table
+    parts, n = vb.integrate(gpu.as_bytes(syn.orthographic_rays(side_big, side_big)))
+    assert vb.last_stats.walk == 0 and bvh.wide_nodes is None
+    vb.opts = gpu.LocateOptions(traversal=2)
+    parts2, n2 = vb.integrate(gpu.as_bytes(syn.orthographic_rays(side_big, side_big)))
+    assert vb.last_stats.walk == 2 and bvh.wide_nodes is not None
+    assert n2 == n and np.array_equal(_np(parts2, np.uint8)[: n * 24], _np(parts, np.uint8)[: n * 24])
+
+
+def test_given_guess_locate(gpu, orc):
+    """DRX_INIT_GUESS (dray's use_init_guess): located with the converged locations of a first pass as
+    guesses, every point lands in the same cell, its reference point moves by float rounding only, and
+    the candidate that accepts takes exactly one Newton step (the start is already converged)."""
+    mesh = syn.hex_mesh(5, 3, 0.1, 0.01, 2)
+    m = gpu.UnstructuredMesh(mesh)
+    pts = syn.query_points(40000, m.bounds(), seed=17)
+    dp = gpu.to_device(pts.reshape(-1))
+    first, s1, c1 = m.locate(dp, gpu.LocateOptions(), stats=True)
+    again, s2, c2 = m.locate(dp, gpu.LocateOptions(guess=first), stats=True)
+    a, b = _np(first, LOCATION_DTYPE), _np(again, LOCATION_DTYPE)
+    assert np.array_equal(a["cell"], b["cell"]) and np.array_equal(_np(c1), _np(c2))
+    hit = a["cell"] >= 0
+    assert np.abs(a["ref"][hit] - b["ref"][hit]).max() <= 2e-6
+    one = hit & (_np(c1) == 1)
+    assert one.sum() > 10000 and np.all(_np(s2)[one] == 1)
+    print(f"[given guess] Newton steps per located point {_np(s1)[hit].mean():.3f} -> {_np(s2)[hit].mean():.3f}",
+          flush=True)
+    # DRX_INIT_GUESS without guesses is refused
+    import ctypes as C
+    o = gpu.LocateOptions().c()
+    o.init_mode = gpu.L.DRX_INIT_GUESS
+    mv, bv = m.gf.view(), m.get_bvh().view()
+    out = torch.empty(16 * len(pts), dtype=torch.uint8, device="cuda")
+    rc = gpu.L.lib().drx_locate(C.byref(mv), C.byref(bv), C.c_void_p(dp.data_ptr()), len(pts),
+                                C.c_void_p(out.data_ptr()), C.byref(o), None)
+    assert rc == gpu.L.DRX_ERR_ARG
+
+
+def test_render_record_replay(gpu, orc):
+    """drx_volume_integrate with opts.record writes the location of every (ray, sample) and renders the
+    same partials as the plain binary walk; replayed as given starts (DRX_INIT_GUESS), the render is the
+    same to float rounding with fewer Newton steps (one per located sample on the accepting candidate)."""
+    mesh = syn.hex_mesh(5, 3, 0.1, 0.01, 2)
+    m, f = gpu.UnstructuredMesh(mesh), gpu.UnstructuredField(mesh)
+    vol = gpu.Volume(m, f)
+    vol.use_lighting = False
+    vol.samples(128)
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, 0.05)
+    w = h = 48
+    rays = gpu.as_bytes(orc.camera_perspective(w, h, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)))
+    vol.opts = gpu.LocateOptions(traversal=3)
+    p0, n0 = vol.integrate(rays, tile_width=w, tile_height=h)
+    s0 = vol.last_stats
+    spr = 300
+    rec = torch.full((w * h * spr * 4,), -1, dtype=torch.int32, device="cuda")
+    vol.opts = gpu.LocateOptions(record=rec, samples_per_ray=spr)
+    p1, n1 = vol.integrate(rays, tile_width=w, tile_height=h)
+    s1 = vol.last_stats
+    assert s1.walk == 4 and n1 == n0 and np.array_equal(_np(p1, np.uint8)[: n1 * 24], _np(p0, np.uint8)[: n0 * 24])
+    assert (s1.located_samples, s1.newton_steps) == (s0.located_samples, s0.newton_steps)
+    r = _np(rec).view(LOCATION_DTYPE)
+    assert (r["cell"] >= 0).sum() == s0.located_samples
+    vol.opts = gpu.LocateOptions(guess=rec, record=None, samples_per_ray=spr)
+    p2, n2 = vol.integrate(rays, tile_width=w, tile_height=h)
+    s2 = vol.last_stats
+    g0 = _np(p0, np.uint8)[: n0 * 24].view(PARTIAL_DTYPE)
+    g2 = _np(p2, np.uint8)[: n2 * 24].view(PARTIAL_DTYPE)
+    print(f"[record/replay] located {s0.located_samples}/{s2.located_samples} newton {s0.newton_steps} -> "
+          f"{s2.newton_steps}", flush=True)
+    assert n2 == n0 and s2.located_samples == s0.located_samples and s2.newton_steps < s0.newton_steps
+    assert np.abs(g2["rgba"] - g0["rgba"]).max() <= 1e-4
