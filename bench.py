@@ -259,6 +259,7 @@ def _volume_setup(api, syn, m, f, bvh, res, samples):
     vol = api.Volume(m, f)
     vol.use_lighting = False
     vol.samples(samples)
+    vol.table = api.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, syn.table_alpha_for_corrected(0.02, samples))

@@ -103,6 +103,10 @@ _SIGS = {
     "drx_framebuffer_composite_background": (I32, [P, I32, P, P]),
     "drx_save_png": (I32, [P, I32, I32, C.c_char_p]),
     "drx_color_table_sample": (I32, [P, I32, P, I32, I32, F32, I32, F32, P]),
+    "drx_color_table_preset": (I32, [C.c_char_p, P, I32, C.POINTER(I32), C.POINTER(I32)]),
+    "drx_camera_reset_to_bounds": (I32, [C.POINTER(DrxCamera), P, P]),
+    "drx_camera_elevate": (I32, [C.POINTER(DrxCamera), F32]),
+    "drx_camera_azimuth": (I32, [C.POINTER(DrxCamera), F32]),
     "drx_mlp_param_count": (I64, []),
     "drx_mlp_state_bytes": (I64, []),
     "drx_mlp_packed_bytes": (I64, []),

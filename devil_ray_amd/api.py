@@ -356,9 +356,10 @@ class UnstructuredMesh:
 class UnstructuredField:
     """UnstructuredField<Hex scalar> (data_model/unstructured_field.cpp)."""
 
-    def __init__(self, mesh: HexMesh, values: Optional[np.ndarray] = None, fp64: bool = False):
+    def __init__(self, mesh: HexMesh, values: Optional[np.ndarray] = None, fp64: bool = False, value_range=None):
         """fp64: an FP64 field (dray's Float = double): values default to mesh.field64, kept and evaluated
-        in double; its colour range is the double min/max."""
+        in double; its colour range is the double min/max.  value_range: the range to report instead of the
+        values' min/max (a field degree-elevated at import keeps the range of its original coefficients)."""
         if values is None:
             values = mesh.field64 if fp64 else mesh.field
             if values is None:
@@ -370,6 +371,8 @@ class UnstructuredField:
         # the device copy is a snapshot of vals, so its range is fixed here (not recomputed per render)
         v = np.asarray(vals, np.float64 if fp64 else np.float32)
         self._range = (float(v.min()), float(v.max())) if v.size else (float("inf"), float("-inf"))
+        if value_range is not None:
+            self._range = (float(value_range[0]), float(value_range[1]))
 
     def range(self):
         """get_range (unstructured_field.cpp:35-90): min/max over the control values (in the field's
@@ -445,6 +448,41 @@ class Collection:
         return "[" + " ".join(names) + "]"
 
 
+class BlueprintReader:
+    """dray::BlueprintReader (io/blueprint_reader.cpp:280-440) for Conduit-Blueprint HDF5 data sets of
+    high-order hex meshes: every domain the root file names becomes a DataSet of an UnstructuredMesh and its
+    scalar fields (converted to Bernstein form and to one order per domain on the host)."""
+
+    @staticmethod
+    def load(root_file: str) -> Collection:
+        from . import blueprint
+        col = Collection()
+        for d in blueprint.load(root_file):
+            ds = DataSet(UnstructuredMesh(d.mesh))
+            ds.domain_id = d.domain_id
+            for name in d.fields:
+                ds.add_field(name, UnstructuredField(d.mesh, values=d.fields[name], value_range=d.ranges[name]))
+            col.add_domain(ds)
+        return col
+
+
+def collection_volumes(collection: Collection, field: str) -> list:
+    """dray::Volume (Collection&) + field(name) as one Volume per domain: every domain samples with the
+    Collection's bounds (volume.cpp:89-91) and colours with the Collection's field range; set the colour
+    table, samples and lighting on each (or share one ColorTable object)."""
+    if not collection.has_field(field):
+        raise L.DrxError(f"Volume: field '{field}' is not in every domain {collection.field_list()}")
+    bounds = collection.bounds()
+    rng = collection.range(field)
+    vols = []
+    for d in collection.domains:
+        v = Volume(d.mesh, d.field(field))
+        v.sample_bounds = bounds
+        v.range = rng
+        vols.append(v)
+    return vols
+
+
 class PointLocation:
     """dray::PointLocation (queries/point_location.cpp:176-287) on one rank: every domain locates every
     point; a point located in a domain takes that domain's field values (later domains overwrite
@@ -485,12 +523,23 @@ class PointLocation:
 
 # ---------------------------------------------------------------------------------------------
 class ColorTable:
-    """dray::ColorTable (color_table.cpp): rgb + alpha control points, smooth flag (default False)."""
+    """dray::ColorTable (color_table.cpp): rgb + alpha control points and a smooth flag.  ColorTable() is
+    empty and not smooth (:67-71); ColorTable(name) holds the named table's rgb points, smooth except
+    "sharp", "" or "default" naming cool2warm and an unknown name falling back to YlOrRd (:378-1330)."""
 
-    def __init__(self):
+    def __init__(self, name: Optional[str] = None):
         self.rgb = []      # (pos, r, g, b, a)
         self.alpha = []    # (pos, alpha)
         self.smooth = False
+        if name is not None:
+            n, sm = C.c_int32(0), C.c_int32(0)
+            L.check(L.lib().drx_color_table_preset(name.encode(), None, 0, C.byref(n), C.byref(sm)),
+                    "drx_color_table_preset")
+            pts = np.zeros((n.value, 5), np.float32)
+            L.check(L.lib().drx_color_table_preset(name.encode(), pts.ctypes.data, n.value, C.byref(n),
+                                                   C.byref(sm)), "drx_color_table_preset")
+            self.rgb = pts.tolist()
+            self.smooth = bool(sm.value)
 
     def add_point(self, pos, rgb):
         c = list(rgb) + ([1.0] if len(rgb) == 3 else [])
@@ -527,6 +576,32 @@ class Camera:
         for q in range(3):
             c.pos[q], c.look_at[q], c.up[q] = self.pos[q], self.look_at[q], self.up[q]
         return c
+
+    def _take(self, cc: L.DrxCamera):
+        self.pos = tuple(float(cc.pos[q]) for q in range(3))
+        self.look_at = tuple(float(cc.look_at[q]) for q in range(3))
+        self.fov = float(cc.fov_y)
+
+    def reset_to_bounds(self, bounds6, xpad: float = 0.0, ypad: float = 0.0, zpad: float = 0.0):
+        """Camera::reset_to_bounds (camera.cpp:485-514): bounds (xmin, xmax, ymin, ymax, zmin, zmax)."""
+        cc = self.c()
+        b = np.ascontiguousarray(np.asarray(bounds6, np.float32).reshape(6))
+        pads = np.ascontiguousarray(np.array([xpad, ypad, zpad], np.float64))
+        L.check(L.lib().drx_camera_reset_to_bounds(C.byref(cc), b.ctypes.data, pads.ctypes.data),
+                "drx_camera_reset_to_bounds")
+        self._take(cc)
+
+    def elevate(self, degrees: float):
+        """Camera::elevate (camera.cpp:516-534)."""
+        cc = self.c()
+        L.check(L.lib().drx_camera_elevate(C.byref(cc), float(degrees)), "drx_camera_elevate")
+        self._take(cc)
+
+    def azimuth(self, degrees: float):
+        """Camera::azimuth (camera.cpp:536-552)."""
+        cc = self.c()
+        L.check(L.lib().drx_camera_azimuth(C.byref(cc), float(degrees)), "drx_camera_azimuth")
+        self._take(cc)
 
     def create_rays(self) -> torch.Tensor:
         out = torch.empty(self.width * self.height * RAY_DTYPE.itemsize, dtype=torch.uint8, device=_dev())
@@ -570,7 +645,7 @@ class Volume:
         self.field_ = field
         self.m_samples = 100
         self.use_lighting = True   # dray default (VolumeShader::shaded_color with the given lights)
-        self.table = ColorTable()
+        self.table = ColorTable("cool2warm")   # ColorMap's default table (color_map.cpp:11-18)
         self.range = None
         self.opts = LocateOptions()
         self.last_stats = None

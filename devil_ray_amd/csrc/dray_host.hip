@@ -329,6 +329,14 @@ void ColorTable::add_point (float32 position, const Vec4f &c)
 {
   m_rgb.insert (m_rgb.end (), { position, c[0], c[1], c[2], c[3] });
 }
+ColorTable::ColorTable (const std::string &name)
+{
+  int32 n = 0, smooth = 0;
+  drx_check (drx_color_table_preset (name.c_str (), nullptr, 0, &n, &smooth), "ColorTable");
+  m_rgb.resize ((size_t)n * 5);
+  drx_check (drx_color_table_preset (name.c_str (), m_rgb.data (), n, &n, &smooth), "ColorTable");
+  m_smooth = smooth != 0;
+}
 void ColorTable::add_alpha (float32 position, float32 alpha) { m_alpha.insert (m_alpha.end (), { position, alpha }); }
 
 ColorTable ColorTable::correct_opacity (const float32 &factor) const
@@ -520,6 +528,66 @@ void Camera::azimuth (const float32 degrees)
   look = rotate_vector (degrees, up, look);
   m_pos = v_sub (m_look_at, v_scale (look, distance));
 }
+
+} // namespace dray
+
+// The camera orbits as C entry points (drx_camera in, moved drx_camera out), for callers without the C++ layer.
+namespace drx
+{
+int fail (int status, const char *fmt, ...);
+}
+namespace
+{
+template <typename Op> int camera_op (drx_camera *cam, const char *what, Op op)
+{
+  if (!cam) return drx::fail (DRX_ERR_ARG, "%s: null camera", what);
+  try
+  {
+    dray::Camera k;
+    k.set_width (cam->width);
+    k.set_height (cam->height);
+    k.set_fov (cam->fov_y);
+    k.set_zoom (cam->zoom);
+    k.set_pos ({ { cam->pos[0], cam->pos[1], cam->pos[2] } });
+    k.set_look_at ({ { cam->look_at[0], cam->look_at[1], cam->look_at[2] } });
+    k.set_up ({ { cam->up[0], cam->up[1], cam->up[2] } });
+    op (k);
+    *cam = k.view ();
+    return DRX_OK;
+  }
+  catch (const std::exception &e)
+  {
+    return drx::fail (DRX_ERR_ARG, "%s: %s", what, e.what ());
+  }
+}
+} // namespace
+
+extern "C" {
+int drx_camera_reset_to_bounds (drx_camera *cam, const float *bounds6, const double *pads)
+{
+  if (!bounds6) return drx::fail (DRX_ERR_ARG, "drx_camera_reset_to_bounds: null bounds");
+  return camera_op (cam, "drx_camera_reset_to_bounds", [&] (dray::Camera &k) {
+    dray::AABB<3> b;
+    for (int d = 0; d < 3; ++d)
+    {
+      b.m_ranges[d].include (bounds6[2 * d]);
+      b.m_ranges[d].include (bounds6[2 * d + 1]);
+    }
+    k.reset_to_bounds (b, pads ? pads[0] : 0., pads ? pads[1] : 0., pads ? pads[2] : 0.);
+  });
+}
+int drx_camera_elevate (drx_camera *cam, float degrees)
+{
+  return camera_op (cam, "drx_camera_elevate", [&] (dray::Camera &k) { k.elevate (degrees); });
+}
+int drx_camera_azimuth (drx_camera *cam, float degrees)
+{
+  return camera_op (cam, "drx_camera_azimuth", [&] (dray::Camera &k) { k.azimuth (degrees); });
+}
+} // extern "C"
+
+namespace dray
+{
 
 // ---- Volume ----------------------------------------------------------------------------------------------
 // |bounds diagonal| / samples in float32, unfused like dray's (volume.cpp:89-91)

@@ -4,9 +4,13 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
+
+#include "drx_color_presets.inc"
 
 namespace drx
 {
@@ -431,6 +435,41 @@ int drx_color_table_sample (const float *rgb, int32_t n_rgb, const float *alpha,
     c[3] = al * alpha_scale;
     for (int k = 0; k < 4; ++k) out[4 * i + k] = c[k];
   }
+  return DRX_OK;
+}
+
+// ColorTable (const std::string &name) (color_table.cpp:378-1330): the named table's rgb control points
+// (alpha 1) as n x (pos, r, g, b, a).  "" and "default" name cool2warm; named tables are smooth except
+// "sharp"; an unknown name prints a notice and takes YlOrRd, as dray does.  With rgb_pts null (or capacity
+// below the count) only *n_pts and *smooth are written.
+int drx_color_table_preset (const char *name, float *rgb_pts, int32_t capacity, int32_t *n_pts, int32_t *smooth)
+{
+  if (!name || !n_pts || !smooth) return fail (DRX_ERR_ARG, "drx_color_table_preset: null argument");
+  std::string nm (name);
+  if (nm.empty () || nm == "default") nm = "cool2warm";
+  const PresetTable *t = nullptr, *fallback = nullptr;
+  for (const PresetTable &e : kPresetTables)
+  {
+    if (nm == e.name) t = &e;
+    if (std::string (e.name) == "YlOrRd") fallback = &e;
+  }
+  if (!t)
+  {
+    std::fprintf (stderr, "Unknown Color Table: '%s' defaulting\n", name);
+    t = fallback;
+  }
+  *n_pts = t->count;
+  *smooth = t->smooth;
+  if (rgb_pts && capacity >= t->count)
+    for (int i = 0; i < t->count; ++i)
+    {
+      const float *q = kPresetPoints[t->first + i];
+      rgb_pts[5 * i + 0] = q[0];
+      rgb_pts[5 * i + 1] = q[1];
+      rgb_pts[5 * i + 2] = q[2];
+      rgb_pts[5 * i + 3] = q[3];
+      rgb_pts[5 * i + 4] = 1.f;
+    }
   return DRX_OK;
 }
 

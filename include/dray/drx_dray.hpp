@@ -375,6 +375,9 @@ class ColorTable
 {
 public:
   ColorTable () {}
+  // ColorTable (name) (color_table.cpp:378-1330): the named table's rgb points, smooth except "sharp";
+  // "" and "default" name cool2warm; an unknown name prints a notice and takes YlOrRd.
+  explicit ColorTable (const std::string &name);
   void add_point (float32 position, const Vec3f &color);
   void add_point (float32 position, const Vec4f &color);
   void add_alpha (float32 position, float32 alpha);
@@ -490,7 +493,7 @@ public:
   bool range_set () const { return m_range_set; }
 
 private:
-  ColorTable m_table;
+  ColorTable m_table { "cool2warm" };   // ColorMap () (color_map.cpp:11-18)
   Range m_range;
   bool m_range_set = false;
 };

@@ -36,6 +36,9 @@
  *   drx_save_png            Framebuffer::save -> PNGEncoder           rendering/framebuffer.cpp:44-51,
  *                                                                     utils/png_encoder.cpp:56-116
  *   drx_color_table_sample  ColorTable::correct_opacity + sample      color_table.cpp:152-324
+ *   drx_color_table_preset  ColorTable (const std::string &name)      color_table.cpp:378-1330
+ *   drx_camera_reset_to_bounds, drx_camera_elevate, drx_camera_azimuth
+ *                           Camera::reset_to_bounds / elevate / azimuth  rendering/camera.cpp:485-552
  *   drx_mesh_pack           (no counterpart: GridFunction -> element-local SoA conversion done once at
  *                           upload; replaces DeviceMesh::get_elem's indirect reads, device_mesh.hpp:79-96)
  *   drx_mlp_*               (no counterpart: learned Newton initial guess; dray starts Newton from
@@ -395,6 +398,18 @@ int drx_mlp_infer (const void *packed, const void *x, const float *base, int64_t
 int drx_color_table_sample (const float *rgb_pts, int32_t n_rgb, const float *alpha_pts, int32_t n_alpha,
                             int32_t smooth, float factor, int32_t num_samples, float alpha_scale,
                             float *out_rgba_host);
+/* The rgb control points (pos, r, g, b, 1) of a named table, host.  "" and "default" name cool2warm (the
+ * ColorMap default); named tables are smooth except "sharp"; an unknown name prints a notice and takes
+ * YlOrRd.  rgb_pts null or capacity < *n_pts: only *n_pts and *smooth are written. */
+int drx_color_table_preset (const char *name, float *rgb_pts, int32_t capacity, int32_t *n_pts, int32_t *smooth);
+
+/* ---- camera orbits (host, float32 arithmetic in dray's operation order) ------------------------ */
+/* Look at the centre of bounds (xmin, xmax, ymin, ymax, zmin, zmax) padded by pads[d] x extent from one
+ * padded diagonal away along the current viewing direction; fov_y becomes 60 degrees. */
+int drx_camera_reset_to_bounds (drx_camera *cam, const float *bounds6, const double *pads);
+/* Orbit the camera position about look_at by degrees about its right (elevate) or up (azimuth) axis. */
+int drx_camera_elevate (drx_camera *cam, float degrees);
+int drx_camera_azimuth (drx_camera *cam, float degrees);
 
 #ifdef __cplusplus
 }

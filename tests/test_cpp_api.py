@@ -116,6 +116,7 @@ def test_cpp_api_matches_c_abi(gpu, orc, tmp_path, name, make):
     vol = gpu.Volume(m, f)
     vol.use_lighting = False
     vol.samples(samples)
+    vol.table = gpu.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, alpha0)

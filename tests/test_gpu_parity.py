@@ -142,6 +142,7 @@ def _render_both(gpu, orc, mesh, rays_np, w, h, samples, opacity=0.02, tile=True
     vol = gpu.Volume(m, f)
     vol.use_lighting = lights is not None
     vol.samples(samples)
+    vol.table = gpu.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, syn.table_alpha_for_corrected(opacity, samples))
@@ -218,6 +219,7 @@ def test_fp64_polish_fires(gpu, orc):
     vol = gpu.Volume(m, f)
     vol.use_lighting = False
     vol.samples(64)
+    vol.table = gpu.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)
@@ -352,6 +354,7 @@ def test_compact_nodes_render_identical(gpu, orc):
     vol = gpu.Volume(m, f)
     vol.use_lighting = False
     vol.samples(128)
+    vol.table = gpu.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)
@@ -430,6 +433,7 @@ def test_render_walks_identical(gpu, orc, name, make):
     vol = gpu.Volume(m, f)
     vol.use_lighting = False
     vol.samples(96)
+    vol.table = gpu.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)
@@ -508,6 +512,7 @@ def test_multi_domain_render(gpu, orc):
         vol.samples(96)
         vol.sample_bounds = cb
         vol.color_map_range(lo, hi)
+        vol.table = gpu.ColorTable()
         vol.table.add_point(0.0, (0.1, 0.3, 0.9))
         vol.table.add_point(1.0, (0.9, 0.2, 0.1))
         vol.table.add_alpha(0.0, 0.3)
@@ -577,6 +582,7 @@ def test_renderer_two_domains(gpu, orc, tmp_path):
         v = gpu.Volume(gpu.UnstructuredMesh(mesh), gpu.UnstructuredField(mesh))
         v.use_lighting = False
         v.samples(96)
+        v.table = gpu.ColorTable()
         v.table.add_point(0.0, (0.1, 0.3, 0.9))
         v.table.add_point(1.0, (0.9, 0.2, 0.1))
         v.table.add_alpha(0.0, 0.3)
@@ -805,6 +811,7 @@ def test_scratch_per_stream(gpu, orc):
     vol = gpu.Volume(m, f)
     vol.use_lighting = False
     vol.samples(128)
+    vol.table = gpu.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)
@@ -839,6 +846,7 @@ def test_auto_walk_choice(gpu):
     vol = gpu.Volume(m, gpu.UnstructuredField(small))
     vol.use_lighting = False
     vol.samples(100)
+    vol.table = gpu.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)
@@ -900,6 +908,7 @@ def test_render_record_replay(gpu, orc):
     vol = gpu.Volume(m, f)
     vol.use_lighting = False
     vol.samples(128)
+    vol.table = gpu.ColorTable()
     vol.table.add_point(0.0, (0.5, 0.5, 0.5))
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)

@@ -258,6 +258,7 @@ def test_learned_integrate_matches_plain(gpu, trained):
         vol = gpu.Volume(m, f)
         vol.use_lighting = False
         vol.samples(128)
+        vol.table = gpu.ColorTable()
         vol.table.add_point(0.0, (0.5, 0.5, 0.5))
         vol.table.add_point(1.0, (1.0, 0.0, 0.0))
         vol.table.add_alpha(0.0, syn.table_alpha_for_corrected(0.02, 128))
