@@ -44,14 +44,17 @@ __device__ __forceinline__ float box_measure (const float *b)   // b: xmin,xmax,
 // lane t handles (component t / lines, line t % lines), 64 lanes per pass.  Running min/max of both halves
 // per component go to a/b mn/mx.
 template <int P>
-__device__ __forceinline__ void split_lines (float *mom, float *dau, int NPE_, int axis, int lane, float amn[3],
-                                             float amx[3], float bmn[3], float bmx[3])
+__device__ __forceinline__ void split_lines (float *mom, float *dau, int NPE_, int axis, int lane, float &lo_a,
+                                             float &hi_a, float &lo_b, float &hi_b)
 {
+  // Row c of the wavefront (lanes 16c .. 16c+15) takes component c's lines, 16 at a time; row 3 idles.
+  // Each lane's hulls of its lines then reduce within its row.
   constexpr int N1 = P + 1, LINES = N1 * N1;
   const int stride = (axis == 0) ? 1 : ((axis == 1) ? N1 : N1 * N1);
-  for (int t = lane; t < 3 * LINES; t += 64)
+  const int c = lane >> 4;
+  if (c > 2) return;
+  for (int l = lane & 15; l < LINES; l += 16)
   {
-    const int c = t / LINES, l = t - c * LINES;
     const int u = l % N1, v = l / N1;   // the two other indices, lowest axis first
     const int base = (axis == 0) ? (N1 * u + LINES * v) : ((axis == 1) ? (u + LINES * v) : (u + N1 * v));
     float *m = mom + c * NPE_ + base, *d = dau + c * NPE_ + base;
@@ -68,7 +71,6 @@ __device__ __forceinline__ void split_lines (float *mom, float *dau, int NPE_, i
 #pragma unroll
       for (int ii = 0; ii <= P - r; ++ii) b[ii] = b[ii] * (1.0f - 0.5f) + b[ii + 1] * 0.5f;
     }
-    float lo_a = INFINITY, hi_a = -INFINITY, lo_b = INFINITY, hi_b = -INFINITY;
 #pragma unroll
     for (int ii = 0; ii < N1; ++ii)
     {
@@ -79,15 +81,6 @@ __device__ __forceinline__ void split_lines (float *mom, float *dau, int NPE_, i
       lo_b = fminf (lo_b, b[ii]);
       hi_b = fmaxf (hi_b, b[ii]);
     }
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc)
-      if (c == cc)
-      {
-        amn[cc] = fminf (amn[cc], lo_a);
-        amx[cc] = fmaxf (amx[cc], hi_a);
-        bmn[cc] = fminf (bmn[cc], lo_b);
-        bmx[cc] = fmaxf (bmx[cc], hi_b);
-      }
   }
 }
 
@@ -190,35 +183,38 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
       for (int d = 0; d < 3; ++d)
         if (d == axis) my_ref[3 + d] = split_pt;
     }
-    // split the mother's net along `axis`, one (line, component) per lane: a line is the p+1 dofs along
+    // split the mother's net along `axis`, one (line, component) per lane (component c in row c): a line is the p+1 dofs along
     // the axis with the other two indices fixed; each lane reads its line, runs the de Casteljau rounds in
     // registers (mother keeps [lo, mid], daughter [mid, hi]; the same operations in the same order as
     // split_inplace, elem_ops.hpp:1346-1406) and writes both halves.  Lines are disjoint, so no lane
     // reads what another lane writes in this count.
     float *mom = nets + max_id * 3 * NPE;
     float *dau = nets + count * 3 * NPE;
-    float amn[3] = { INFINITY, INFINITY, INFINITY }, amx[3] = { -INFINITY, -INFINITY, -INFINITY };
-    float bmn[3] = { INFINITY, INFINITY, INFINITY }, bmx[3] = { -INFINITY, -INFINITY, -INFINITY };
+    float lo_a = INFINITY, hi_a = -INFINITY, lo_b = INFINITY, hi_b = -INFINITY;
     if constexpr (PMAX == 0)
       ;
     else if (p == PMAX)
-      split_lines<PMAX> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx);
+      split_lines<PMAX> (mom, dau, NPE, axis, lane, lo_a, hi_a, lo_b, hi_b);
     else
       switch (p)   // mixed-order meshes
       {
-      case 1: split_lines<1> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx); break;
-      case 2: split_lines<(PMAX >= 2 ? 2 : 1)> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx); break;
-      case 3: split_lines<(PMAX >= 3 ? 3 : 1)> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx); break;
-      default: split_lines<(PMAX >= 4 ? 4 : 1)> (mom, dau, NPE, axis, lane, amn, amx, bmn, bmx); break;
+      case 1: split_lines<1> (mom, dau, NPE, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
+      case 2: split_lines<(PMAX >= 2 ? 2 : 1)> (mom, dau, NPE, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
+      case 3: split_lines<(PMAX >= 3 ? 3 : 1)> (mom, dau, NPE, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
+      default: split_lines<(PMAX >= 4 ? 4 : 1)> (mom, dau, NPE, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
       }
+    lo_a = row_reduce_min (lo_a);
+    hi_a = row_reduce_max (hi_a);
+    lo_b = row_reduce_min (lo_b);
+    hi_b = row_reduce_max (hi_b);
     float ba[6], bb[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
     {
-      ba[2 * c] = wave_reduce_min (amn[c]);
-      ba[2 * c + 1] = wave_reduce_max (amx[c]);
-      bb[2 * c] = wave_reduce_min (bmn[c]);
-      bb[2 * c + 1] = wave_reduce_max (bmx[c]);
+      ba[2 * c] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (lo_a), 16 * c));
+      ba[2 * c + 1] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (hi_a), 16 * c));
+      bb[2 * c] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (lo_b), 16 * c));
+      bb[2 * c + 1] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (hi_b), 16 * c));
     }
     if (lane == max_id)
     {

@@ -54,6 +54,26 @@ __device__ __forceinline__ float wave_reduce_max (float v)
   return __int_as_float (__builtin_amdgcn_readlane (__float_as_int (v), 63));
 }
 
+// Min / max within each 16-lane row (quad swaps, half-row and row mirrors): every lane of a row ends
+// holding its row's result.
+__device__ __forceinline__ float row_reduce_min (float v)
+{
+  v = fminf (v, dpp_move<0xB1> (v, v));
+  v = fminf (v, dpp_move<0x4E> (v, v));
+  v = fminf (v, dpp_move<0x141> (v, v));
+  v = fminf (v, dpp_move<0x140> (v, v));
+  return v;
+}
+
+__device__ __forceinline__ float row_reduce_max (float v)
+{
+  v = fmaxf (v, dpp_move<0xB1> (v, v));
+  v = fmaxf (v, dpp_move<0x4E> (v, v));
+  v = fmaxf (v, dpp_move<0x141> (v, v));
+  v = fmaxf (v, dpp_move<0x140> (v, v));
+  return v;
+}
+
 // Inclusive prefix sum across the 64 lanes.
 __device__ __forceinline__ int32_t wave_scan_incl_add (int32_t v)
 {
