@@ -151,8 +151,10 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
       const float m0 = __int_as_float (__builtin_amdgcn_readfirstlane (__float_as_int (my_meas)));
       if (m0 == m0)
       {
-        const float cand = (lane < count) ? my_meas : -INFINITY;
-        const float M = wave_reduce_max (cand);
+        // integer keys: NaN measures and lanes beyond count can never be the maximum (box 0's key is a
+        // number here, so the maximum is at least that)
+        const int32_t cand = (lane < count && my_meas == my_meas) ? float_key (my_meas) : INT32_MIN;
+        const int32_t M = wave_reduce_max (cand);
         const uint64_t hit = __ballot (lane < count && cand == M);
         max_id = __ffsll ((unsigned long long)hit) - 1;
       }
@@ -203,18 +205,18 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
       case 3: split_lines<(PMAX >= 3 ? 3 : 1)> (mom, dau, NPE, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
       default: split_lines<(PMAX >= 4 ? 4 : 1)> (mom, dau, NPE, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
       }
-    lo_a = row_reduce_min (lo_a);
-    hi_a = row_reduce_max (hi_a);
-    lo_b = row_reduce_min (lo_b);
-    hi_b = row_reduce_max (hi_b);
+    // the lanes' hulls are never NaN (fminf / fmaxf from +-inf skip NaN values), so their integer keys
+    // reduce exactly like the floats
+    const int32_t klo_a = row_reduce_min (float_key (lo_a)), khi_a = row_reduce_max (float_key (hi_a));
+    const int32_t klo_b = row_reduce_min (float_key (lo_b)), khi_b = row_reduce_max (float_key (hi_b));
     float ba[6], bb[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
     {
-      ba[2 * c] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (lo_a), 16 * c));
-      ba[2 * c + 1] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (hi_a), 16 * c));
-      bb[2 * c] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (lo_b), 16 * c));
-      bb[2 * c + 1] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (hi_b), 16 * c));
+      ba[2 * c] = key_float (__builtin_amdgcn_readlane (klo_a, 16 * c));
+      ba[2 * c + 1] = key_float (__builtin_amdgcn_readlane (khi_a, 16 * c));
+      bb[2 * c] = key_float (__builtin_amdgcn_readlane (klo_b, 16 * c));
+      bb[2 * c + 1] = key_float (__builtin_amdgcn_readlane (khi_b, 16 * c));
     }
     if (lane == max_id)
     {
@@ -305,6 +307,59 @@ __global__ void __launch_bounds__ (kRedBlock) k_bounds_partial (const float *__r
   }
 }
 
+// The partial bounds of n_partial blocks -> the bounds (res.include(mins); res.include(maxs), as
+// linear_bvh_builder.cpp:49-57) in out[0..5] and the Morton frame (mins, 1 / extent or 0) in out[6..11].
+// One block; n_partial 0 leaves the empty box (+inf, -inf).
+__global__ void __launch_bounds__ (kRedBlock) k_bounds_final (const float *__restrict__ partial, int n_partial,
+                                                              float *__restrict__ out)
+{
+  __shared__ float s[kRedBlock / 64][6];
+  float mn[3] = { INFINITY, INFINITY, INFINITY }, mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+  for (int b = threadIdx.x; b < n_partial; b += blockDim.x)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      mn[d] = fminf (mn[d], partial[6 * b + 2 * d]);
+      mx[d] = fmaxf (mx[d], partial[6 * b + 2 * d + 1]);
+    }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    mn[d] = wave_reduce_min (mn[d]);
+    mx[d] = wave_reduce_max (mx[d]);
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 3; ++d)
+    {
+      s[threadIdx.x >> 6][2 * d] = mn[d];
+      s[threadIdx.x >> 6][2 * d + 1] = mx[d];
+    }
+  __syncthreads ();
+  if (threadIdx.x < 3)
+  {
+    const int d = threadIdx.x;
+    float a = s[0][2 * d], b = s[0][2 * d + 1];
+    for (int w = 1; w < kRedBlock / 64; ++w)
+    {
+      a = fminf (a, s[w][2 * d]);
+      b = fmaxf (b, s[w][2 * d + 1]);
+    }
+    float lo = INFINITY, hi = -INFINITY;   // no boxes: the empty box as it stands
+    if (n_partial > 0)
+    {
+      lo = fminf (lo, a);
+      hi = fmaxf (hi, a);
+      lo = fminf (lo, b);
+      hi = fmaxf (hi, b);
+    }
+    out[2 * d] = lo;
+    out[2 * d + 1] = hi;
+    const float ext = hi - lo;
+    out[6 + d] = lo;
+    out[9 + d] = (ext == .0f) ? 0.f : 1.f / ext;
+  }
+}
+
 // =============================================================================================
 // Morton codes (linear_bvh_builder.cpp:60-90, morton_codes.hpp:16-51)
 // =============================================================================================
@@ -317,13 +372,13 @@ __device__ __forceinline__ uint32_t expand_bits32 (uint32_t x)
   return x;
 }
 
-__global__ void k_morton (const float *__restrict__ aabbs, int n, float minx, float miny, float minz, float ix,
-                          float iy, float iz, uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+__global__ void k_morton (const float *__restrict__ aabbs, int n, const float *__restrict__ frame,
+                          uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float *a = aabbs + 6 * (int64_t)i;
-  const float mins[3] = { minx, miny, minz }, inv[3] = { ix, iy, iz };
+  const float mins[3] = { frame[0], frame[1], frame[2] }, inv[3] = { frame[3], frame[4], frame[5] };
   uint32_t q[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
@@ -793,7 +848,7 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
   const int red_blocks = std::min (1024, (int)nblk (n, kRedBlock));
   const size_t tree_floats = 6 * (size_t)n + 6 * (size_t)inner;
   const size_t tree_ints = (size_t)inner * 2 + (2 * (size_t)n - 1) + (size_t)n;
-  char *tree = (char *)arena (SLOT_TREE, tree_floats * 4 + tree_ints * 4 + 6 * 4 * (size_t)red_blocks + 64, st);
+  char *tree = (char *)arena (SLOT_TREE, tree_floats * 4 + tree_ints * 4 + 6 * 4 * (size_t)red_blocks + 128, st);
   if (!kA || !kB || !vA || !vB || !hist || !tree) return DRX_ERR_HIP;
   float *s_aabbs = (float *)tree;
   float *i_aabbs = s_aabbs + 6 * (size_t)n;
@@ -803,44 +858,16 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
   int32_t *s_prims = parent + (2 * (size_t)n - 1);
   float *partial = (float *)(s_prims + n);
 
-  // bounds
-  float bounds[6] = { INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY };
+  // bounds, reduced on the device (no host round trip before the Morton codes); copied out at the end
+  float *dbounds = partial + 6 * (size_t)red_blocks;   // bounds[6], Morton frame (mins, 1 / extent)[6]
   if (!zero_case)
   {
     hipLaunchKernelGGL (k_bounds_partial, dim3 (red_blocks), dim3 (kRedBlock), 0, st, aabbs, (int64_t)n, partial);
     DRX_LAUNCH_CHECK ();
-    std::vector<float> hpart (6 * (size_t)red_blocks);
-    DRX_HIP (hipMemcpyAsync (hpart.data (), partial, sizeof (float) * hpart.size (), hipMemcpyDeviceToHost, st));
-    DRX_HIP (hipStreamSynchronize (st));
-    float mn[3] = { INFINITY, INFINITY, INFINITY }, mx[3] = { -INFINITY, -INFINITY, -INFINITY };
-    for (int b = 0; b < red_blocks; ++b)
-      for (int d = 0; d < 3; ++d)
-      {
-        mn[d] = std::min (mn[d], hpart[6 * b + 2 * d]);
-        mx[d] = std::max (mx[d], hpart[6 * b + 2 * d + 1]);
-      }
-    // res.include(mins); res.include(maxs)
-    for (int d = 0; d < 3; ++d)
-    {
-      float lo = INFINITY, hi = -INFINITY;
-      lo = std::fmin (lo, mn[d]);
-      hi = std::fmax (hi, mn[d]);
-      lo = std::fmin (lo, mx[d]);
-      hi = std::fmax (hi, mx[d]);
-      bounds[2 * d] = lo;
-      bounds[2 * d + 1] = hi;
-    }
   }
-  for (int q = 0; q < 6; ++q) out_bounds[q] = bounds[q];
-  float minc[3], inv[3];
-  for (int d = 0; d < 3; ++d)
-  {
-    minc[d] = bounds[2 * d];
-    const float ext = bounds[2 * d + 1] - bounds[2 * d];
-    inv[d] = (ext == .0f) ? 0.f : 1.f / ext;
-  }
-  hipLaunchKernelGGL (k_morton, dim3 (nblk (n, 256)), dim3 (256), 0, st, aabbs, n, minc[0], minc[1], minc[2], inv[0],
-                      inv[1], inv[2], kA, vA);
+  hipLaunchKernelGGL (k_bounds_final, dim3 (1), dim3 (kRedBlock), 0, st, partial, zero_case ? 0 : red_blocks, dbounds);
+  DRX_LAUNCH_CHECK ();
+  hipLaunchKernelGGL (k_morton, dim3 (nblk (n, 256)), dim3 (256), 0, st, aabbs, n, dbounds + 6, kA, vA);
   DRX_LAUNCH_CHECK ();
   // stable radix sort, 8-bit digits over 30-bit keys
   uint32_t *ks = nullptr;
@@ -888,6 +915,7 @@ int drx_bvh_build (const float *aabbs_in, const int32_t *prims_in, int32_t n_in,
   hipLaunchKernelGGL (k_emit, dim3 (nblk (inner, 256)), dim3 (256), 0, st, left, right, s_aabbs, i_aabbs, inner,
                       (float4 *)out_inner);
   DRX_LAUNCH_CHECK ();
+  DRX_HIP (hipMemcpyAsync (out_bounds, dbounds, 6 * sizeof (float), hipMemcpyDeviceToHost, st));
   DRX_HIP (hipStreamSynchronize (st));
   return DRX_OK;
 }

@@ -54,24 +54,57 @@ __device__ __forceinline__ float wave_reduce_max (float v)
   return __int_as_float (__builtin_amdgcn_readlane (__float_as_int (v), 63));
 }
 
-// Min / max within each 16-lane row (quad swaps, half-row and row mirrors): every lane of a row ends
-// holding its row's result.
-__device__ __forceinline__ float row_reduce_min (float v)
+// Order-preserving integer key of a float (any non-NaN value): the sign bit kept, the magnitude bits of a
+// negative value flipped, so signed integer order is float order (-0 below +0).  Its own inverse.
+__device__ __forceinline__ int32_t float_key (float f)
 {
-  v = fminf (v, dpp_move<0xB1> (v, v));
-  v = fminf (v, dpp_move<0x4E> (v, v));
-  v = fminf (v, dpp_move<0x141> (v, v));
-  v = fminf (v, dpp_move<0x140> (v, v));
+  const int32_t b = __float_as_int (f);
+  return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+__device__ __forceinline__ float key_float (int32_t k) { return __int_as_float (k ^ ((k >> 31) & 0x7FFFFFFF)); }
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int32_t dpp_move_i (int32_t old, int32_t v)
+{
+  return __builtin_amdgcn_update_dpp (old, v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// DPP permutation of v within each row where every lane has a source lane (quad_perm, mirrors): no `old`
+// value is involved, so the move can fold into the instruction that consumes it.
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_perm_i (int32_t v)
+{
+  return __builtin_amdgcn_mov_dpp (v, CTRL, 0xF, 0xF, true);
+}
+
+// Min / max of integer keys within each 16-lane row (quad swaps, half-row and row mirrors): every lane of
+// a row ends holding its row's result.  Integer min/max need no NaN canonicalisation, so each step can be
+// one DPP-modified instruction.
+__device__ __forceinline__ int32_t row_reduce_min (int32_t v)
+{
+  v = min (v, dpp_perm_i<0xB1> (v));
+  v = min (v, dpp_perm_i<0x4E> (v));
+  v = min (v, dpp_perm_i<0x141> (v));
+  v = min (v, dpp_perm_i<0x140> (v));
   return v;
 }
 
-__device__ __forceinline__ float row_reduce_max (float v)
+__device__ __forceinline__ int32_t row_reduce_max (int32_t v)
 {
-  v = fmaxf (v, dpp_move<0xB1> (v, v));
-  v = fmaxf (v, dpp_move<0x4E> (v, v));
-  v = fmaxf (v, dpp_move<0x141> (v, v));
-  v = fmaxf (v, dpp_move<0x140> (v, v));
+  v = max (v, dpp_perm_i<0xB1> (v));
+  v = max (v, dpp_perm_i<0x4E> (v));
+  v = max (v, dpp_perm_i<0x141> (v));
+  v = max (v, dpp_perm_i<0x140> (v));
   return v;
+}
+
+// Max of integer keys over the 64 lanes, read back to every lane.
+__device__ __forceinline__ int32_t wave_reduce_max (int32_t v)
+{
+  v = row_reduce_max (v);
+  v = max (v, dpp_move_i<0x142, 0xA> (v, v));   // row_bcast15 into rows 1, 3
+  v = max (v, dpp_move_i<0x143, 0xC> (v, v));   // row_bcast31 into rows 2, 3
+  return __builtin_amdgcn_readlane (v, 63);
 }
 
 // Inclusive prefix sum across the 64 lanes.
