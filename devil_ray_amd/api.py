@@ -645,7 +645,11 @@ class Volume:
         self.field_ = field
         self.m_samples = 100
         self.use_lighting = True   # dray default (VolumeShader::shaded_color with the given lights)
-        self.table = ColorTable("cool2warm")   # ColorMap's default table (color_map.cpp:11-18)
+        # ColorMap's default table (color_map.cpp:11-18) with the alpha ramp Volume's constructor adds
+        # (volume.cpp:274-286)
+        self.table = ColorTable("cool2warm")
+        self.table.add_alpha(0.1, 0.0)
+        self.table.add_alpha(1.0, 0.7)
         self.range = None
         self.opts = LocateOptions()
         self.last_stats = None
