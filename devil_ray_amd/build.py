@@ -26,10 +26,13 @@ SOURCES = {
     "drx_mlp.hip": [],
     "drx_image.hip": [],
     "dray_host.hip": [],
+    "dray_blueprint.hip": [],
 }
 ROOT = os.path.dirname(HERE)
 CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "dray_api_main.cpp")
 CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "dray_api_test")
+CPP_BP_SRC = os.path.join(ROOT, "tests", "cpp", "dray_blueprint_main.cpp")
+CPP_BP_BIN = os.path.join(ROOT, "tests", "cpp", "dray_blueprint_test")
 
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -66,14 +69,16 @@ def build(verbose: bool = False, defines=(), out: str = LIB, obj_dir: str = OBJ)
 
 
 def build_cpp_test(lib: str = LIB, out: str = CPP_TEST_BIN) -> str:
-    """The C++ program that drives the path through the dray-named classes (linked against libdrx.so)."""
-    tmp = out + ".tmp"
+    """The C++ programs that drive the path through the dray-named classes (linked against libdrx.so):
+    tests/cpp/dray_api_test and tests/cpp/dray_blueprint_test (beside `out`)."""
     rpath = os.path.relpath(os.path.dirname(lib), os.path.dirname(out))
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC, "-o", tmp,
-           "-L", os.path.dirname(lib), "-l:" + os.path.basename(lib), f"-Wl,-rpath,$ORIGIN/{rpath}",
-           "-Wl,-rpath-link,/opt/rocm/lib"]
-    subprocess.check_call(cmd)
-    shutil.move(tmp, out)
+    for src, dst in ((CPP_TEST_SRC, out), (CPP_BP_SRC, os.path.join(os.path.dirname(out), os.path.basename(CPP_BP_BIN)))):
+        tmp = dst + ".tmp"
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o", tmp,
+               "-L", os.path.dirname(lib), "-l:" + os.path.basename(lib), f"-Wl,-rpath,$ORIGIN/{rpath}",
+               "-Wl,-rpath-link,/opt/rocm/lib"]
+        subprocess.check_call(cmd)
+        shutil.move(tmp, dst)
     return out
 
 

@@ -1,0 +1,720 @@
+// BlueprintReader (io/blueprint_reader.cpp:280-440) for Conduit-Blueprint HDF5 data sets of hexahedral MFEM
+// meshes, host code.  The HDF5 subset follows the published HDF5 File Format Specification 3.0; the MFEM dof
+// numbering and the projection onto the positive (Bernstein) basis follow mfem2dray.cpp:392-1160 and MFEM's
+// published conventions.  The arithmetic of the conversion is written in the same order as
+// devil_ray_amd/blueprint.py, so both readers give the same float32 control points.
+#include "../../include/dray/drx_dray.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+
+namespace dray
+{
+namespace
+{
+#pragma clang fp contract(off)
+
+constexpr double kPi = 3.14159265358979323846;
+
+[[noreturn]] void bp_error (const std::string &what) { throw DRayError ("BlueprintReader: " + what); }
+
+// ---- HDF5 subset ---------------------------------------------------------------------------------------------
+constexpr uint64_t kUndef = ~0ull;
+
+struct H5Value
+{
+  bool is_string = false;
+  std::string str;
+  std::vector<double> num;        // every numeric element, converted to double
+  std::vector<int64_t> inum;      // integer elements exactly (integer datasets only)
+  bool is_int = false;
+};
+
+class H5File
+{
+public:
+  explicit H5File (const std::string &path) : m_path (path)
+  {
+    std::ifstream f (path, std::ios::binary);
+    if (!f) bp_error ("cannot open " + path);
+    m_buf.assign (std::istreambuf_iterator<char> (f), std::istreambuf_iterator<char> ());
+    static const unsigned char sig[8] = { 0x89, 'H', 'D', 'F', '\r', '\n', 0x1a, '\n' };
+    if (m_buf.size () < 48 || std::memcmp (m_buf.data (), sig, 8) != 0) bp_error (path + ": not an HDF5 file");
+    const int ver = u8 (8);
+    if (ver != 2 && ver != 3) bp_error (path + ": superblock version " + std::to_string (ver) + " is not supported");
+    if (u8 (9) != 8 || u8 (10) != 8) bp_error (path + ": offset/length sizes other than 8 are not supported");
+    m_root = u64 (36);
+  }
+
+  std::vector<std::string> keys (const std::string &path) const
+  {
+    std::vector<std::string> out;
+    for (const auto &l : links (addr (path))) out.push_back (l.first);
+    return out;
+  }
+  bool is_group (const std::string &path) const
+  {
+    for (const auto &m : messages (addr (path)))
+      if (m.type == 0x08) return false;
+    return true;
+  }
+  H5Value read (const std::string &path) const;
+
+private:
+  struct Msg
+  {
+    int type;
+    size_t off, size;
+  };
+  std::string m_path;
+  std::vector<char> m_buf;
+  uint64_t m_root = 0;
+
+  uint8_t u8 (size_t o) const
+  {
+    if (o >= m_buf.size ()) bp_error (m_path + ": truncated file");
+    return (uint8_t)m_buf[o];
+  }
+  uint64_t uint_le (size_t o, int w) const
+  {
+    uint64_t v = 0;
+    for (int i = 0; i < w; ++i) v |= (uint64_t)u8 (o + i) << (8 * i);
+    return v;
+  }
+  uint16_t u16 (size_t o) const { return (uint16_t)uint_le (o, 2); }
+  uint32_t u32 (size_t o) const { return (uint32_t)uint_le (o, 4); }
+  uint64_t u64 (size_t o) const { return uint_le (o, 8); }
+
+  void parse_block (size_t start, size_t end, bool track_order, std::vector<Msg> &out) const
+  {
+    size_t q = start;
+    while (q + 4 <= end)
+    {
+      const int type = u8 (q);
+      const size_t size = u16 (q + 1);
+      q += 4 + (track_order ? 2 : 0);
+      out.push_back ({ type, q, size });
+      if (type == 0x10)   // continuation: an "OCHK" block
+      {
+        const uint64_t caddr = u64 (q), clen = u64 (q + 8);
+        if (caddr + 4 > m_buf.size () || std::memcmp (&m_buf[caddr], "OCHK", 4) != 0)
+          bp_error (m_path + ": continuation block without OCHK signature");
+        parse_block (caddr + 4, caddr + clen - 4, track_order, out);   // minus the trailing checksum
+      }
+      q += size;
+    }
+  }
+
+  std::vector<Msg> messages (uint64_t a) const
+  {
+    if (a + 6 > m_buf.size () || std::memcmp (&m_buf[a], "OHDR", 4) != 0)
+      bp_error (m_path + ": only version-2 object headers are supported");
+    const int flags = u8 (a + 5);
+    size_t p = a + 6;
+    if (flags & 0x20) p += 16;   // times
+    if (flags & 0x10) p += 4;    // attribute phase change
+    const int w = 1 << (flags & 3);
+    const size_t size0 = uint_le (p, w);
+    p += w;
+    std::vector<Msg> out;
+    parse_block (p, p + size0, (flags & 0x04) != 0, out);
+    return out;
+  }
+
+  std::vector<std::pair<std::string, uint64_t>> links (uint64_t a) const
+  {
+    std::vector<std::pair<std::string, uint64_t>> out;
+    for (const Msg &m : messages (a))
+    {
+      const size_t d = m.off;
+      if (m.type == 0x02)   // link info
+      {
+        const int fl = u8 (d + 1);
+        const uint64_t heap = u64 (d + 2 + ((fl & 1) ? 8 : 0));
+        if (heap != kUndef) bp_error (m_path + ": groups with dense link storage are not supported");
+      }
+      else if (m.type == 0x06)   // link
+      {
+        const int fl = u8 (d + 1);
+        size_t q = d + 2;
+        int ltype = 0;
+        if (fl & 0x08) ltype = u8 (q++);
+        if (fl & 0x04) q += 8;
+        if (fl & 0x10) q += 1;
+        const int w = 1 << (fl & 3);
+        const size_t n = uint_le (q, w);
+        q += w;
+        if (q + n + 8 > m_buf.size ()) bp_error (m_path + ": truncated link message");
+        std::string name (&m_buf[q], n);
+        q += n;
+        if (ltype != 0) bp_error (m_path + ": link '" + name + "' is not a hard link");
+        out.emplace_back (name, u64 (q));
+      }
+    }
+    return out;
+  }
+
+  uint64_t addr (const std::string &path) const
+  {
+    uint64_t a = m_root;
+    size_t i = 0;
+    while (i < path.size ())
+    {
+      while (i < path.size () && path[i] == '/') ++i;
+      if (i >= path.size ()) break;
+      size_t j = path.find ('/', i);
+      if (j == std::string::npos) j = path.size ();
+      const std::string part = path.substr (i, j - i);
+      bool found = false;
+      for (const auto &l : links (a))
+        if (l.first == part)
+        {
+          a = l.second;
+          found = true;
+          break;
+        }
+      if (!found) bp_error (m_path + ": no node '" + path + "'");
+      i = j;
+    }
+    return a;
+  }
+};
+
+H5Value H5File::read (const std::string &path) const
+{
+  std::vector<uint64_t> dims;
+  bool have_space = false, have_type = false, have_data = false;
+  int cls = -1, size = 0;
+  bool is_signed = false;
+  size_t data_off = 0, data_len = 0;
+  for (const Msg &m : messages (addr (path)))
+  {
+    const size_t d = m.off;
+    if (m.type == 0x01)   // dataspace
+    {
+      const int ver = u8 (d), nd = u8 (d + 1);
+      const size_t q = (ver == 2) ? 4 : 8;
+      dims.clear ();
+      for (int i = 0; i < nd; ++i) dims.push_back (u64 (d + q + 8 * i));
+      if (ver == 2 && u8 (d + 3) == 2) dims.assign (1, 0);   // null dataspace
+      have_space = true;
+    }
+    else if (m.type == 0x03)   // datatype
+    {
+      cls = u8 (d) & 0x0F;
+      const int bits0 = u8 (d + 1);
+      size = (int)u32 (d + 4);
+      if ((bits0 & 1) && (cls == 0 || cls == 1)) bp_error (m_path + ":" + path + ": big-endian data is not supported");
+      if (cls != 0 && cls != 1 && cls != 3) bp_error (m_path + ":" + path + ": datatype class " + std::to_string (cls));
+      is_signed = (cls == 0) && (bits0 & 0x08);
+      have_type = true;
+    }
+    else if (m.type == 0x0B)
+      bp_error (m_path + ":" + path + ": filtered (compressed) datasets are not supported");
+    else if (m.type == 0x08)   // data layout version 3
+    {
+      if (u8 (d) != 3) bp_error (m_path + ":" + path + ": data layout version " + std::to_string (u8 (d)));
+      const int lc = u8 (d + 1);
+      if (lc == 0)
+      {
+        data_len = u16 (d + 2);
+        data_off = d + 4;
+      }
+      else if (lc == 1)
+      {
+        const uint64_t a = u64 (d + 2), n = u64 (d + 10);
+        data_off = (a == kUndef) ? 0 : a;
+        data_len = (a == kUndef) ? 0 : n;
+      }
+      else
+        bp_error (m_path + ":" + path + ": chunked layout is not supported");
+      have_data = true;
+    }
+  }
+  if (!have_space || !have_type || !have_data) bp_error (m_path + ":" + path + " is not a dataset");
+  uint64_t count = 1;
+  for (uint64_t x : dims) count *= x;
+  if (data_off + std::min<uint64_t> (data_len, count * size) > m_buf.size ()) bp_error (m_path + ": truncated dataset");
+  H5Value v;
+  if (cls == 3)
+  {
+    v.is_string = true;
+    const size_t n = std::min<size_t> (data_len, count * size);
+    std::string s (&m_buf[data_off], n);
+    v.str = s.substr (0, s.find ('\0'));
+    return v;
+  }
+  if (count * size > data_len) bp_error (m_path + ":" + path + ": dataset shorter than its dataspace");
+  v.num.resize (count);
+  v.is_int = (cls == 0);
+  if (v.is_int) v.inum.resize (count);
+  for (uint64_t i = 0; i < count; ++i)
+  {
+    const size_t o = data_off + i * size;
+    if (cls == 1)
+    {
+      if (size == 4)
+      {
+        float f;
+        std::memcpy (&f, &m_buf[o], 4);
+        v.num[i] = f;
+      }
+      else if (size == 8)
+        std::memcpy (&v.num[i], &m_buf[o], 8);
+      else
+        bp_error (m_path + ":" + path + ": float of " + std::to_string (size) + " bytes");
+    }
+    else
+    {
+      uint64_t raw = uint_le (o, size);
+      int64_t x;
+      if (is_signed && size < 8 && (raw >> (8 * size - 1)) & 1) raw |= ~0ull << (8 * size);
+      std::memcpy (&x, &raw, 8);
+      v.inum[i] = x;
+      v.num[i] = is_signed ? (double)x : (double)raw;
+    }
+  }
+  return v;
+}
+
+// ---- MFEM conventions and the Bernstein conversion ---------------------------------------------------------
+const int kHexV[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 1, 1, 0 }, { 0, 1, 0 },
+                          { 0, 0, 1 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 1, 1 } };
+const int kHexE[12][2] = { { 0, 1 }, { 1, 2 }, { 3, 2 }, { 0, 3 }, { 4, 5 }, { 5, 6 },
+                           { 7, 6 }, { 4, 7 }, { 0, 4 }, { 1, 5 }, { 2, 6 }, { 3, 7 } };
+const int kHexF[6][4] = { { 3, 2, 1, 0 }, { 0, 1, 5, 4 }, { 1, 2, 6, 5 }, { 2, 3, 7, 6 }, { 3, 0, 4, 7 }, { 4, 5, 6, 7 } };
+
+enum BasisType { GaussLegendre = 0, GaussLobatto = 1, Positive = 2 };
+
+struct Basis
+{
+  bool h1;
+  int btype;
+  int order;
+};
+
+Basis parse_basis (const std::string &name, const std::string &what)
+{
+  if (name == "Quadratic") return { true, GaussLobatto, 2 };
+  if (name == "Linear") return { true, GaussLobatto, 1 };
+  int p = 0, t = -1;
+  char tail;
+  if (std::sscanf (name.c_str (), "H1Pos_3D_P%d%c", &p, &tail) == 1) return { true, Positive, p };
+  if (std::sscanf (name.c_str (), "H1_T%d_3D_P%d%c", &t, &p, &tail) == 2) return { true, t, p };
+  if (std::sscanf (name.c_str (), "L2_T%d_3D_P%d%c", &t, &p, &tail) == 2) return { false, t, p };
+  if (std::sscanf (name.c_str (), "H1_3D_P%d%c", &p, &tail) == 1) return { true, GaussLobatto, p };
+  if (std::sscanf (name.c_str (), "L2_3D_P%d%c", &p, &tail) == 1) return { false, GaussLegendre, p };
+  bp_error (what + ": basis '" + name + "' is not supported");
+}
+
+// lattice index i + (p+1)(j + (p+1)k) of each local H1 dof of an order-1 or order-2 hex
+std::vector<int> local_lattice (int order)
+{
+  std::vector<int> lat;
+  if (order == 1)
+  {
+    for (const auto &v : kHexV) lat.push_back (v[0] + 2 * (v[1] + 2 * v[2]));
+    return lat;
+  }
+  if (order != 2) bp_error ("H1 order " + std::to_string (order) + " (orders 1 and 2 are supported)");
+  for (const auto &v : kHexV) lat.push_back (2 * v[0] + 3 * (2 * v[1] + 3 * 2 * v[2]));
+  for (const auto &e : kHexE)
+  {
+    int c[3];
+    for (int d = 0; d < 3; ++d) c[d] = kHexV[e[0]][d] + kHexV[e[1]][d];
+    lat.push_back (c[0] + 3 * (c[1] + 3 * c[2]));
+  }
+  for (const auto &f : kHexF)
+  {
+    int c[3];
+    for (int d = 0; d < 3; ++d) c[d] = (kHexV[f[0]][d] + kHexV[f[1]][d] + kHexV[f[2]][d] + kHexV[f[3]][d]) / 2;
+    lat.push_back (c[0] + 3 * (c[1] + 3 * c[2]));
+  }
+  lat.push_back (1 + 3 * (1 + 3 * 1));
+  return lat;
+}
+
+// global H1 dofs per element: vertices, then edges and faces numbered in first-encounter order (keyed by
+// the vertex pair / the three lowest vertex ids), then interiors
+std::vector<int64_t> h1_element_dofs (const std::vector<int64_t> &conn, int64_t n_vertices, int order, int64_t &ndofs)
+{
+  const int64_t n_el = (int64_t)conn.size () / 8;
+  if (order == 1)
+  {
+    ndofs = n_vertices;
+    return conn;
+  }
+  if (order != 2) bp_error ("H1 order " + std::to_string (order) + " (orders 1 and 2 are supported)");
+  std::map<std::pair<int64_t, int64_t>, int64_t> edge_id;
+  std::map<std::array<int64_t, 3>, int64_t> face_id;
+  std::vector<int64_t> el_edges (12 * n_el), el_faces (6 * n_el);
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int j = 0; j < 12; ++j)
+    {
+      int64_t a = conn[8 * e + kHexE[j][0]], b = conn[8 * e + kHexE[j][1]];
+      if (b < a) std::swap (a, b);
+      auto it = edge_id.emplace (std::make_pair (a, b), (int64_t)edge_id.size ()).first;
+      el_edges[12 * e + j] = it->second;
+    }
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int j = 0; j < 6; ++j)
+    {
+      int64_t v[4];
+      for (int q = 0; q < 4; ++q) v[q] = conn[8 * e + kHexF[j][q]];
+      std::sort (v, v + 4);
+      auto it = face_id.emplace (std::array<int64_t, 3> { v[0], v[1], v[2] }, (int64_t)face_id.size ()).first;
+      el_faces[6 * e + j] = it->second;
+    }
+  const int64_t ne = (int64_t)edge_id.size (), nf = (int64_t)face_id.size ();
+  std::vector<int64_t> dofs (27 * n_el);
+  for (int64_t e = 0; e < n_el; ++e)
+  {
+    for (int q = 0; q < 8; ++q) dofs[27 * e + q] = conn[8 * e + q];
+    for (int q = 0; q < 12; ++q) dofs[27 * e + 8 + q] = n_vertices + el_edges[12 * e + q];
+    for (int q = 0; q < 6; ++q) dofs[27 * e + 20 + q] = n_vertices + ne + el_faces[6 * e + q];
+    dofs[27 * e + 26] = n_vertices + ne + nf + e;
+  }
+  ndofs = n_vertices + ne + nf + n_el;
+  return dofs;
+}
+
+// the 1D nodes on [0, 1] of MFEM's nodal bases
+std::vector<double> nodes_1d (int btype, int p)
+{
+  std::vector<double> x;
+  if (p == 0) return { 0.5 };
+  if (btype == GaussLobatto)
+  {
+    // endpoints and the roots of P_p'(t), by Newton from Chebyshev-Gauss-Lobatto starts
+    x.push_back (0.0);
+    for (int i = 1; i < p; ++i)
+    {
+      double t = -std::cos (kPi * i / p);
+      for (int it = 0; it < 100; ++it)
+      {
+        // P_p and its derivatives by the three-term recurrence
+        double p0 = 1, p1 = t;
+        for (int k = 2; k <= p; ++k)
+        {
+          const double pk = ((2 * k - 1) * t * p1 - (k - 1) * p0) / k;
+          p0 = p1;
+          p1 = pk;
+        }
+        const double dp = p * (t * p1 - p0) / (t * t - 1);        // P_p'
+        const double d2p = (2 * t * dp - p * (p + 1) * p1) / (1 - t * t);   // P_p''
+        const double step = dp / d2p;
+        t -= step;
+        if (std::fabs (step) < 1e-16) break;
+      }
+      x.push_back (0.5 * (t + 1.0));
+    }
+    x.push_back (1.0);
+    return x;
+  }
+  if (btype == GaussLegendre)
+  {
+    for (int i = 0; i < p + 1; ++i)
+    {
+      double t = -std::cos (kPi * (i + 0.75) / (p + 1.5));
+      for (int it = 0; it < 100; ++it)
+      {
+        double p0 = 1, p1 = t;
+        for (int k = 2; k <= p + 1; ++k)
+        {
+          const double pk = ((2 * k - 1) * t * p1 - (k - 1) * p0) / k;
+          p0 = p1;
+          p1 = pk;
+        }
+        const double dp = (p + 1) * (t * p1 - p0) / (t * t - 1);
+        const double step = p1 / dp;
+        t -= step;
+        if (std::fabs (step) < 1e-16) break;
+      }
+      x.push_back (0.5 * (t + 1.0));
+    }
+    return x;
+  }
+  bp_error ("basis type " + std::to_string (btype) + " has no nodes");
+}
+
+// one element's values at the tensor nodes (lexicographic) -> Bernstein coefficients, axis by axis
+void to_bernstein (double *c, int btype, int p)
+{
+  const int n1 = p + 1;
+  if (btype == Positive || (btype == GaussLobatto && p == 1) || p == 0) return;
+  auto line = [&] (int base, int stride, auto &&fn) {
+    double f[16];
+    for (int i = 0; i < n1; ++i) f[i] = c[base + i * stride];
+    fn (f);
+    for (int i = 0; i < n1; ++i) c[base + i * stride] = f[i];
+  };
+  const int strides[3] = { 1, n1, n1 * n1 };
+  if (btype == GaussLobatto && p == 2)
+  {
+    for (int ax = 0; ax < 3; ++ax)
+      for (int l = 0; l < 9; ++l)
+      {
+        const int u = l % 3, v = l / 3;
+        const int base = ax == 0 ? 3 * u + 9 * v : (ax == 1 ? u + 9 * v : u + 3 * v);
+        line (base, strides[ax], [] (double *f) { f[1] = 2.0 * f[1] - 0.5 * (f[0] + f[2]); });
+      }
+    return;
+  }
+  // general nodes: invert the Bernstein collocation matrix B_j(x_i) once (Gauss-Jordan, partial pivoting)
+  const std::vector<double> x = nodes_1d (btype, p);
+  double m[16][32];
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j)
+    {
+      double b = 1;
+      for (int k = 0; k < j; ++k) b *= x[i];
+      for (int k = 0; k < p - j; ++k) b *= 1.0 - x[i];
+      double binom = 1;
+      for (int k = 1; k <= j; ++k) binom = binom * (p - j + k) / k;
+      m[i][j] = binom * b;
+      m[i][n1 + j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int col = 0; col < n1; ++col)
+  {
+    int piv = col;
+    for (int r = col + 1; r < n1; ++r)
+      if (std::fabs (m[r][col]) > std::fabs (m[piv][col])) piv = r;
+    for (int k = 0; k < 2 * n1; ++k) std::swap (m[col][k], m[piv][k]);
+    const double d = m[col][col];
+    for (int k = 0; k < 2 * n1; ++k) m[col][k] /= d;
+    for (int r = 0; r < n1; ++r)
+      if (r != col)
+      {
+        const double f = m[r][col];
+        for (int k = 0; k < 2 * n1; ++k) m[r][k] -= f * m[col][k];
+      }
+  }
+  for (int ax = 0; ax < 3; ++ax)
+    for (int l = 0; l < n1 * n1; ++l)
+    {
+      const int u = l % n1, v = l / n1;
+      const int base = ax == 0 ? n1 * u + n1 * n1 * v : (ax == 1 ? u + n1 * n1 * v : u + n1 * v);
+      line (base, strides[ax], [&] (double *f) {
+        double g[16];
+        for (int i = 0; i < n1; ++i)
+        {
+          double s = 0;
+          for (int j = 0; j < n1; ++j) s += m[i][n1 + j] * f[j];
+          g[i] = s;
+        }
+        for (int i = 0; i < n1; ++i) f[i] = g[i];
+      });
+    }
+}
+
+// degree elevation of one element's tensor Bernstein coefficients from order p to q
+std::vector<double> elevate (const double *c, int p, int q)
+{
+  std::vector<double> a (c, c + (p + 1) * (p + 1) * (p + 1));
+  int n[3] = { p, p, p };   // current degree along i, j, k
+  for (int r = p; r < q; ++r)
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      int m[3] = { n[0], n[1], n[2] };
+      m[ax] += 1;
+      std::vector<double> b ((size_t)(m[0] + 1) * (m[1] + 1) * (m[2] + 1));
+      for (int k = 0; k <= m[2]; ++k)
+        for (int j = 0; j <= m[1]; ++j)
+          for (int i = 0; i <= m[0]; ++i)
+          {
+            const int idx[3] = { i, j, k };
+            const int t = idx[ax], d = n[ax];
+            auto at = [&] (int tt) -> double {
+              int s[3] = { i, j, k };
+              s[ax] = tt;
+              return a[s[0] + (n[0] + 1) * (s[1] + (n[1] + 1) * s[2])];
+            };
+            const double lo = t >= 1 ? at (t - 1) : 0.0, hi = t <= d ? at (t) : 0.0;
+            const double w = (double)t / (double)(d + 1);
+            b[i + (m[0] + 1) * (j + (m[1] + 1) * k)] = w * lo + (1 - w) * hi;
+          }
+      a.swap (b);
+      n[ax] += 1;
+    }
+  return a;
+}
+
+std::vector<std::vector<double>> components (const H5File &h, const std::string &path)
+{
+  std::vector<std::vector<double>> out;
+  if (h.is_group (path + "/values"))
+    for (const auto &c : h.keys (path + "/values")) out.push_back (h.read (path + "/values/" + c).num);
+  else
+    out.push_back (h.read (path + "/values").num);
+  return out;
+}
+
+} // namespace
+
+DataSet BlueprintReader::load_domain (const std::string &file)
+{
+  H5File h (file);
+  const std::vector<std::string> topos = h.keys ("/topologies");
+  if (topos.empty ()) bp_error (file + ": no topologies");
+  const std::string tname = topos[0];
+  const std::string topo = "/topologies/" + tname;
+  if (h.read (topo + "/type").str != "unstructured" || h.read (topo + "/elements/shape").str != "hex")
+    bp_error (file + ": only unstructured hex topologies are supported");
+  const H5Value cv = h.read (topo + "/elements/connectivity");
+  std::vector<int64_t> conn (cv.num.size ());
+  for (size_t i = 0; i < conn.size (); ++i) conn[i] = cv.is_int ? cv.inum[i] : (int64_t)cv.num[i];
+  if (conn.size () % 8) bp_error (file + ": connectivity is not a multiple of 8");
+  const int64_t n_el = (int64_t)conn.size () / 8;
+  const std::string cset = h.read (topo + "/coordset").str;
+  const auto verts = components (h, "/coordsets/" + cset);
+  const int64_t n_vertices = (int64_t)verts[0].size ();
+  std::string gf;
+  for (const auto &k : h.keys (topo))
+    if (k == "grid_function") gf = h.read (topo + "/grid_function").str;
+
+  Basis mb = { true, GaussLobatto, 1 };
+  std::vector<std::vector<double>> comps = verts;
+  if (!gf.empty ())
+  {
+    mb = parse_basis (h.read ("/fields/" + gf + "/basis").str, file + ":" + gf);
+    if (!mb.h1) bp_error (file + ": mesh nodes outside an H1 space are not supported");
+    comps = components (h, "/fields/" + gf);
+  }
+  if (comps.size () != 3) bp_error (file + ": " + std::to_string (comps.size ()) + " node components (3 supported)");
+  int64_t ndofs = 0;
+  const std::vector<int64_t> dofs = h1_element_dofs (conn, n_vertices, mb.order, ndofs);
+  for (const auto &c : comps)
+    if ((int64_t)c.size () != ndofs)
+      bp_error (file + ": " + std::to_string (c.size ()) + " node dofs, the numbering gives " + std::to_string (ndofs));
+  const std::vector<int> lat = local_lattice (mb.order);
+  const int mp = mb.order, mnpe = (mp + 1) * (mp + 1) * (mp + 1);
+  std::vector<double> geom[3];
+  for (int c = 0; c < 3; ++c)
+  {
+    geom[c].resize ((size_t)n_el * mnpe);
+    for (int64_t e = 0; e < n_el; ++e)
+    {
+      double *el = &geom[c][(size_t)e * mnpe];
+      for (int l = 0; l < mnpe; ++l) el[lat[l]] = comps[c][dofs[(size_t)e * mnpe + l]];
+      to_bernstein (el, mb.btype, mp);
+    }
+  }
+
+  struct RawField
+  {
+    std::string name;
+    std::vector<double> coef;
+    int p;
+  };
+  std::vector<RawField> raw;
+  for (const auto &name : h.keys ("/fields"))
+  {
+    if (name == gf || name.find ("_attribute") != std::string::npos || !h.is_group ("/fields/" + name)) continue;
+    bool has_basis = false;
+    for (const auto &k : h.keys ("/fields/" + name)) has_basis |= (k == "basis");
+    if (!has_basis) continue;
+    const auto fc = components (h, "/fields/" + name);
+    if (fc.size () != 1) continue;   // vectors
+    const Basis fb = parse_basis (h.read ("/fields/" + name + "/basis").str, file + ":" + name);
+    const int fp = fb.order, fnpe = (fp + 1) * (fp + 1) * (fp + 1);
+    RawField f { name, std::vector<double> ((size_t)n_el * fnpe), fp };
+    if (!fb.h1)
+    {
+      if ((int64_t)fc[0].size () != n_el * fnpe)
+        bp_error (file + ":" + name + ": " + std::to_string (fc[0].size ()) + " values for " + std::to_string (n_el) +
+                  " order-" + std::to_string (fp) + " elements");
+      f.coef = fc[0];
+    }
+    else
+    {
+      int64_t nd2 = 0;
+      const std::vector<int64_t> d2 = h1_element_dofs (conn, n_vertices, fp, nd2);
+      if ((int64_t)fc[0].size () != nd2) bp_error (file + ":" + name + ": dof count does not match the numbering");
+      const std::vector<int> lat2 = local_lattice (fp);
+      for (int64_t e = 0; e < n_el; ++e)
+        for (int l = 0; l < fnpe; ++l) f.coef[(size_t)e * fnpe + lat2[l]] = fc[0][d2[(size_t)e * fnpe + l]];
+    }
+    for (int64_t e = 0; e < n_el; ++e) to_bernstein (&f.coef[(size_t)e * fnpe], fb.btype, fp);
+    raw.push_back (std::move (f));
+  }
+
+  int q = mp;
+  for (const auto &f : raw) q = std::max (q, f.p);
+  const int npq = (q + 1) * (q + 1) * (q + 1);
+  GridFunction<3> g;
+  g.m_ctrl_idx.resize ((size_t)n_el * npq);
+  g.m_values.resize ((size_t)n_el * npq);
+  g.m_size_el = (int32)n_el;
+  g.m_el_dofs = npq;
+  {
+    int32 *ci = g.m_ctrl_idx.get_host_ptr ();
+    Vec<Float, 3> *vv = g.m_values.get_host_ptr ();
+    for (int64_t i = 0; i < n_el * npq; ++i) ci[i] = (int32)i;
+    for (int c = 0; c < 3; ++c)
+      for (int64_t e = 0; e < n_el; ++e)
+      {
+        const std::vector<double> el = elevate (&geom[c][(size_t)e * mnpe], mp, q);
+        for (int l = 0; l < npq; ++l) vv[(size_t)e * npq + l][c] = (Float)el[l];
+      }
+  }
+  DataSet ds;
+  std::shared_ptr<Mesh> mesh = std::make_shared<HexMesh> (g, q);
+  mesh->name (tname);
+  ds.add_mesh (mesh);
+  for (const auto &f : raw)
+  {
+    const int fnpe = (f.p + 1) * (f.p + 1) * (f.p + 1);
+    Range r = { INFINITY, -INFINITY };
+    for (double v : f.coef)
+    {
+      const float x = (float)v;
+      r.m_min = std::min (r.m_min, x);
+      r.m_max = std::max (r.m_max, x);
+    }
+    GridFunction<1> fg;
+    fg.m_ctrl_idx = g.m_ctrl_idx;
+    fg.m_values.resize ((size_t)n_el * npq);
+    fg.m_size_el = (int32)n_el;
+    fg.m_el_dofs = npq;
+    Vec<Float, 1> *fv = fg.m_values.get_host_ptr ();
+    for (int64_t e = 0; e < n_el; ++e)
+    {
+      const std::vector<double> el = elevate (&f.coef[(size_t)e * fnpe], f.p, q);
+      for (int l = 0; l < npq; ++l) fv[(size_t)e * npq + l][0] = (Float)el[l];
+    }
+    auto field = std::make_shared<UnstructuredField<HexScalar>> (fg, q, f.name);
+    field->range_override ({ r });
+    field->mesh_name (tname);
+    ds.add_field (field);
+  }
+  for (const auto &k : h.keys ("/"))
+    if (k == "state")
+      for (const auto &s : h.keys ("/state"))
+        if (s == "domain_id") ds.domain_id ((int32)h.read ("/state/domain_id").num.at (0));
+  return ds;
+}
+
+Collection BlueprintReader::load (const std::string &root_file)
+{
+  H5File r (root_file);
+  const std::string pattern = r.read ("/file_pattern").str;
+  const int n_files = (int)r.read ("/number_of_files").num.at (0);
+  std::string base = root_file;
+  const size_t slash = base.find_last_of ('/');
+  base = (slash == std::string::npos) ? std::string (".") : base.substr (0, slash);
+  Collection col;
+  for (int i = 0; i < n_files; ++i)
+  {
+    char name[4096];
+    std::snprintf (name, sizeof name, pattern.c_str (), i);
+    col.add_domain (load_domain (base + "/" + name));
+  }
+  return col;
+}
+
+} // namespace dray

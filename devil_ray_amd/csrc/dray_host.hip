@@ -275,6 +275,7 @@ template <class ElemT> std::string UnstructuredField<ElemT>::type_name () const 
 
 template <class ElemT> std::vector<Range> UnstructuredField<ElemT>::range () const
 {
+  if (!m_range_override.empty ()) return m_range_override;
   const Vec<Float, 1> *v = m_dof_data.m_values.get_host_ptr_const ();
   Range r = { INFINITY, -INFINITY };
   for (size_t i = 0; i < m_dof_data.m_values.size (); ++i)

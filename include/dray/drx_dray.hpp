@@ -301,6 +301,9 @@ public:
   UnstructuredField () = delete;
   UnstructuredField (const GridFunction<ElemT::get_ncomp ()> &dof_data, int32 poly_order, const std::string &name = "");
   std::vector<Range> range () const override;        // min/max of the control values per component
+  // Fix the range to report (not in dray): a field whose coefficients were degree-elevated on import keeps
+  // the range of the coefficients it was read with, as dray's range of the unelevated field.
+  void range_override (const std::vector<Range> &r) { m_range_override = r; }
   int32 order () const override { return m_poly_order; }
   int32 components () const override { return ElemT::get_ncomp (); }
   std::string type_name () const override;
@@ -312,6 +315,7 @@ private:
   GridFunction<ElemT::get_ncomp ()> m_dof_data;
   int32 m_poly_order;
   Array<float32> m_packed;
+  std::vector<Range> m_range_override;
 };
 
 // ---- dispatch_3d (dispatcher.hpp:208-220): Mesh* / Field* -> the derived types, then func(mesh, field) -------
@@ -476,6 +480,19 @@ public:
 
 private:
   std::vector<DataSet> m_domains;
+};
+
+// ---- Conduit-Blueprint reader (io/blueprint_reader.cpp:280-440) ---------------------------------------------
+// HDF5 files as Conduit's relay writes them (superblock 2/3, compact groups, contiguous or compact datasets);
+// one unstructured hex topology per domain with an MFEM H1 node field of order <= 2 and scalar H1 (order <= 2)
+// or L2 fields.  Data are converted to Bernstein form on the host (MFEM dof numbering, nodal-to-Bernstein
+// interpolation) and each domain's mesh and fields are degree-elevated to one order, the GPU kernels' form.
+// Unsupported content throws DRayError.
+class BlueprintReader
+{
+public:
+  static Collection load (const std::string &root_file);
+  static DataSet load_domain (const std::string &domain_file);
 };
 
 // ---- colour map (color_map.hpp) ----------------------------------------------------------------------------

@@ -1,13 +1,19 @@
 """Host side of the data-set path: the HDF5 subset reader, the Blueprint/MFEM -> Bernstein conversion, named
 colour tables and camera orbits; and the CPU oracle's renders of dray's two shipped-data volume tests against
 dray's own baseline images (the reader, conversion, colour table, camera and render chain pinned together)."""
+import os
+import subprocess
+from math import comb
+
 import numpy as np
 import pytest
-from math import comb
 
 import blueprint_cases as bc
 from devil_ray_amd import blueprint as bp
+from devil_ray_amd import build as B
 from devil_ray_amd.hdf5_lite import H5File
+
+BP_BIN = B.CPP_BP_BIN
 
 
 def _bern_eval_hex(coef, p, u, v, w):
@@ -202,3 +208,41 @@ def test_oracle_render_matches_dray_baseline(orc, case):
     assert outside <= 0.01
     # the picture is not blank: a planted mistake in the table (no colour) or the camera (turned away) fails
     assert (img[..., :3] < 250).any(-1).mean() > 0.2
+
+
+# ---- the C++ BlueprintReader (include/dray/drx_dray.hpp, host code) ---------------------------------------------------
+@pytest.mark.parametrize("case", list(bc.CASES))
+def test_cpp_reader_matches_python_reader(case, tmp_path):
+    """BlueprintReader::load through the C++ classes gives the Python reader's domains bit for bit: domain ids,
+    topology names, orders, element counts, float32 control points, field coefficients and ranges."""
+    if not os.path.exists(BP_BIN):
+        pytest.fail(f"{BP_BIN} is not built (run devil_ray_amd/build.py)")
+    r = subprocess.run([BP_BIN, "load", bc.root(case), str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    doms = bp.load(bc.root(case))
+    lines = (tmp_path / "summary.txt").read_text().splitlines()
+    assert len(lines) == len(doms)
+    for i, (d, line) in enumerate(zip(doms, lines)):
+        parts = line.split()
+        assert (int(parts[0]), parts[1], int(parts[2]), int(parts[3])) == (d.domain_id, d.topology, d.mesh.order,
+                                                                          d.mesh.n_el)
+        fields = {f.split(":")[0]: (float(f.split(":")[1]), float(f.split(":")[2])) for f in parts[4:]}
+        assert set(fields) == set(d.fields)
+        got = np.fromfile(tmp_path / f"d{i}_values.bin", np.float32).reshape(-1, 3)
+        assert np.array_equal(got.view(np.uint32), d.mesh.values.view(np.uint32))
+        for name in d.fields:
+            gf = np.fromfile(tmp_path / f"d{i}_{name}.bin", np.float32)
+            assert np.array_equal(gf.view(np.uint32), d.fields[name].view(np.uint32))
+            assert np.allclose(fields[name], d.ranges[name], rtol=1e-5)
+
+
+def test_cpp_reader_errors(tmp_path):
+    if not os.path.exists(BP_BIN):
+        pytest.fail(f"{BP_BIN} is not built (run devil_ray_amd/build.py)")
+    bad = tmp_path / "x.root"
+    bad.write_bytes(b"not hdf5")
+    r = subprocess.run([BP_BIN, "load", str(bad), str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "not an HDF5 file" in r.stderr
+    r = subprocess.run([BP_BIN, "load", str(tmp_path / "missing.root"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1 and "cannot open" in r.stderr

@@ -1,12 +1,16 @@
 """dray's two shipped-data volume tests rendered on the GPU through the Python mirror of dray's API
 (BlueprintReader -> Collection -> Volume per domain -> Renderer -> Framebuffer -> composite_background -> PNG),
 checked against dray's baseline images under dray's rule and against the CPU oracle's render."""
+import os
+import subprocess
+
 import numpy as np
 import pytest
 import torch
 
 import blueprint_cases as bc
 from devil_ray_amd import blueprint as bp
+from devil_ray_amd import build as B
 from pngread import read_png
 
 pytestmark = pytest.mark.gpu
@@ -54,3 +58,24 @@ def test_render_matches_dray_baseline(gpu, orc, case, tmp_path):
     # unlit renders equal the oracle byte for byte; lit ones may differ by one byte step where the shading's
     # float rounding (contracted multiply-adds on the GPU) crosses a quantisation boundary
     assert vs_oracle <= 0.005 and dmax <= 1
+
+
+@pytest.mark.parametrize("case", list(bc.CASES))
+def test_cpp_render_matches_python_render(gpu, case, tmp_path):
+    """The same render through the C++ classes (BlueprintReader::load, ColorTable("Spectral"), Camera::azimuth /
+    elevate / reset_to_bounds, Volume(Collection&), Renderer::render, Framebuffer::save), as dray's test
+    programs write it, in a process of its own: the PNG equals the Python mirror's byte for byte."""
+    print(f"case {case}", flush=True)
+    c = bc.CASES[case]
+    az = [d for m, d in c["orbit"] if m == "azimuth"][0]
+    el = ([d for m, d in c["orbit"] if m == "elevate"] or [0.0])[0]
+    args = [B.CPP_BP_BIN, "render", bc.root(case), c["field"], "1" if c["lit"] else "0", str(az), str(el),
+            str(tmp_path / "cpp")]
+    for pos, a in c["alphas"]:
+        args += [str(pos), str(a)]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    cpp = read_png(str(tmp_path / "cpp.png"))
+    py, _ = _gpu_render(gpu, case, tmp_path)
+    assert np.array_equal(cpp, py)
+    assert bc.diff_fraction(cpp, bc.baseline(case)) <= 0.05
