@@ -265,6 +265,217 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
   }
 }
 
+// The same split with one wavefront per component (a 192-thread workgroup per element), for order 4, where
+// the 33 nets take 49.5 KB of LDS and allow only 3 elements per CU: each wavefront splits only its own
+// component's lines (one line per lane, one pass), reduces that component's two hulls over the wave, and the
+// three wavefronts exchange the six values through LDS once per split.  A wavefront reads and writes only its
+// own component's part of the nets, so the exchange is the only synchronisation.  Every wavefront keeps the
+// same box state (hulls, SubRefs, measures) and takes the same decisions; wavefront 0 writes the result.
+template <int P>
+__device__ __forceinline__ void split_lines_comp (float *mom, float *dau, int axis, int lane, float &lo_a,
+                                                  float &hi_a, float &lo_b, float &hi_b)
+{
+  constexpr int N1 = P + 1, LINES = N1 * N1;
+  const int stride = (axis == 0) ? 1 : ((axis == 1) ? N1 : N1 * N1);
+  if (lane >= LINES) return;
+  const int u = lane % N1, v = lane / N1;
+  const int base = (axis == 0) ? (N1 * u + LINES * v) : ((axis == 1) ? (u + LINES * v) : (u + N1 * v));
+  float *m = mom + base, *d = dau + base;
+  float a[N1], b[N1];
+#pragma unroll
+  for (int ii = 0; ii < N1; ++ii) a[ii] = b[ii] = m[ii * stride];
+#pragma unroll
+  for (int r = 1; r <= P; ++r)
+  {
+#pragma unroll
+    for (int ii = P; ii >= r; --ii) a[ii] = a[ii - 1] * (1.0f - 0.5f) + a[ii] * 0.5f;
+#pragma unroll
+    for (int ii = 0; ii <= P - r; ++ii) b[ii] = b[ii] * (1.0f - 0.5f) + b[ii + 1] * 0.5f;
+  }
+#pragma unroll
+  for (int ii = 0; ii < N1; ++ii)
+  {
+    m[ii * stride] = a[ii];
+    d[ii * stride] = b[ii];
+    lo_a = fminf (lo_a, a[ii]);
+    hi_a = fmaxf (hi_a, a[ii]);
+    lo_b = fminf (lo_b, b[ii]);
+    hi_b = fmaxf (hi_b, b[ii]);
+  }
+}
+
+template <int PMAX>
+__global__ void __launch_bounds__ (192) k_split_cw (const int32_t *__restrict__ ctrl_idx, const float *__restrict__ values,
+                                                    const int32_t *__restrict__ orders, const int64_t *__restrict__ offsets,
+                                                    int order, int n_el, float *__restrict__ out_aabbs,
+                                                    int32_t *__restrict__ out_prims, float *__restrict__ out_subrefs)
+{
+  constexpr int NPE = (PMAX + 1) * (PMAX + 1) * (PMAX + 1);
+  __shared__ float nets[kSplitBoxes * 3 * NPE];   // box b, component c at (b * 3 + c) * NPE
+  __shared__ float xch[2][3][4];                   // per split parity: each component's (lo_a, hi_a, lo_b, hi_b)
+
+  const int e = blockIdx.x;
+  if (e >= n_el) return;
+  const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p = orders ? orders[e] : order;
+  const int npe = (p + 1) * (p + 1) * (p + 1);
+  const int64_t off = offsets ? offsets[e] : (int64_t)e * npe;
+
+  // net 0 <- component c of the element's control points
+  float mn = INFINITY, mx = -INFINITY;
+  for (int d = lane; d < npe; d += 64)
+  {
+    const float v = values[(int64_t)ctrl_idx[off + d] * 3 + c];
+    nets[c * NPE + d] = v;
+    mn = fminf (mn, v);
+    mx = fmaxf (mx, v);
+  }
+  const int32_t kmn = wave_reduce_min (float_key (mn)), kmx = wave_reduce_max (float_key (mx));
+  if (lane == 0)   // buffer 0: the first split (count 1) writes buffer 1, and count 2 writes buffer 0 only
+  {                // after every wavefront has passed count 1's barrier, so after these reads
+    xch[0][c][0] = key_float (kmn);
+    xch[0][c][1] = key_float (kmx);
+  }
+  __syncthreads ();
+  float my_box[6], my_ref[6], my_meas = -INFINITY;
+  {
+    float b0[6];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      b0[2 * d] = xch[0][d][0];
+      b0[2 * d + 1] = xch[0][d][1];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) my_box[q] = b0[q];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      my_ref[d] = 0.f;
+      my_ref[3 + d] = 1.f;
+    }
+    if (lane == 0) my_meas = box_measure (b0);
+  }
+
+  for (int count = 1; count <= kSplitBoxes - 1; ++count)
+  {
+    int max_id = 0;
+    {
+      const float m0 = __int_as_float (__builtin_amdgcn_readfirstlane (__float_as_int (my_meas)));
+      if (m0 == m0)
+      {
+        const int32_t cand = (lane < count && my_meas == my_meas) ? float_key (my_meas) : INT32_MIN;
+        const int32_t M = wave_reduce_max (cand);
+        const uint64_t hit = __ballot (lane < count && cand == M);
+        max_id = __ffsll ((unsigned long long)hit) - 1;
+      }
+    }
+    float sr[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sr[q] = __int_as_float (__builtin_amdgcn_readlane (__float_as_int (my_ref[q]), max_id));
+    int axis = 0;
+    float maxlen = sr[3] - sr[0];
+    for (int d = 1; d < 3; ++d)
+    {
+      const float len = sr[3 + d] - sr[d];
+      if (len > maxlen)
+      {
+        axis = d;
+        maxlen = len;
+      }
+    }
+    const float split_pt = sr[axis] * (1.0f - 0.5f) + sr[3 + axis] * 0.5f;
+    if (lane == count)
+    {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) my_ref[q] = (q == axis) ? split_pt : sr[q];
+    }
+    if (lane == max_id)
+    {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (d == axis) my_ref[3 + d] = split_pt;
+    }
+    float *mom = nets + (max_id * 3 + c) * NPE;
+    float *dau = nets + (count * 3 + c) * NPE;
+    float lo_a = INFINITY, hi_a = -INFINITY, lo_b = INFINITY, hi_b = -INFINITY;
+    if (p == PMAX)
+      split_lines_comp<PMAX> (mom, dau, axis, lane, lo_a, hi_a, lo_b, hi_b);
+    else
+      switch (p)   // mixed-order meshes
+      {
+      case 1: split_lines_comp<1> (mom, dau, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
+      case 2: split_lines_comp<(PMAX >= 2 ? 2 : 1)> (mom, dau, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
+      case 3: split_lines_comp<(PMAX >= 3 ? 3 : 1)> (mom, dau, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
+      default: split_lines_comp<(PMAX >= 4 ? 4 : 1)> (mom, dau, axis, lane, lo_a, hi_a, lo_b, hi_b); break;
+      }
+    // the lanes' hulls are never NaN (fminf / fmaxf from +-inf skip NaN values): integer keys reduce them
+    const int32_t klo_a = wave_reduce_min (float_key (lo_a)), khi_a = wave_reduce_max (float_key (hi_a));
+    const int32_t klo_b = wave_reduce_min (float_key (lo_b)), khi_b = wave_reduce_max (float_key (hi_b));
+    const int par = count & 1;
+    if (lane == 0)
+    {
+      xch[par][c][0] = key_float (klo_a);
+      xch[par][c][1] = key_float (khi_a);
+      xch[par][c][2] = key_float (klo_b);
+      xch[par][c][3] = key_float (khi_b);
+    }
+    __syncthreads ();
+    float ba[6], bb[6];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      ba[2 * d] = xch[par][d][0];
+      ba[2 * d + 1] = xch[par][d][1];
+      bb[2 * d] = xch[par][d][2];
+      bb[2 * d + 1] = xch[par][d][3];
+    }
+    if (lane == max_id)
+    {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) my_box[q] = ba[q];
+      my_meas = box_measure (ba);
+    }
+    if (lane == count)
+    {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) my_box[q] = bb[q];
+      my_meas = box_measure (bb);
+    }
+  }
+
+  if (c == 0 && lane < kSplitBoxes)
+  {
+    float b[6];
+    for (int q = 0; q < 6; ++q) b[q] = my_box[q];
+    const float s = (float)1.000001;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      float lo = b[2 * d], hi = b[2 * d + 1];
+      if (!(lo > hi))
+      {
+        const float cc = 0.5f * (lo + hi);
+        const float delta = s * 0.5f * (hi - lo);
+        const float v0 = cc - delta, v1 = cc + delta;
+        lo = fminf (lo, v0);
+        hi = fmaxf (hi, v0);
+        lo = fminf (lo, v1);
+        hi = fmaxf (hi, v1);
+      }
+      b[2 * d] = lo;
+      b[2 * d + 1] = hi;
+    }
+    const int64_t k = (int64_t)e * kSplitBoxes + lane;
+    for (int q = 0; q < 6; ++q)
+    {
+      out_aabbs[6 * k + q] = b[q];
+      out_subrefs[6 * k + q] = my_ref[q];
+    }
+    out_prims[k] = e;
+  }
+}
+
 // =============================================================================================
 // Bounds reduction (linear_bvh_builder.cpp:19-58)
 // =============================================================================================
@@ -796,7 +1007,7 @@ int drx_bvh_split_elements (const int32_t *ctrl_idx, const float *values, const 
   case 1: hipLaunchKernelGGL (k_split<1>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
   case 2: hipLaunchKernelGGL (k_split<2>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
   case 3: hipLaunchKernelGGL (k_split<3>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
-  default: hipLaunchKernelGGL (k_split<4>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
+  default: hipLaunchKernelGGL (k_split_cw<4>, dim3 (n_el), dim3 (192), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
   }
   DRX_LAUNCH_CHECK ();
   return DRX_OK;

@@ -98,6 +98,15 @@ __device__ __forceinline__ int32_t row_reduce_max (int32_t v)
   return v;
 }
 
+// Min of integer keys over the 64 lanes, read back to every lane.
+__device__ __forceinline__ int32_t wave_reduce_min (int32_t v)
+{
+  v = row_reduce_min (v);
+  v = min (v, dpp_move_i<0x142, 0xA> (v, v));   // row_bcast15 into rows 1, 3
+  v = min (v, dpp_move_i<0x143, 0xC> (v, v));   // row_bcast31 into rows 2, 3
+  return __builtin_amdgcn_readlane (v, 63);
+}
+
 // Max of integer keys over the 64 lanes, read back to every lane.
 __device__ __forceinline__ int32_t wave_reduce_max (int32_t v)
 {
