@@ -788,6 +788,16 @@ class Renderer:
     def __init__(self):
         self.volumes = []
         self.lights = []
+        # dray draws a colour bar (and optionally world annotations) over the image (renderer.cpp:166-186);
+        # the switches are kept, nothing is drawn
+        self.m_color_bar = True
+        self.m_world_annotations = False
+
+    def color_bar(self, on: bool):
+        self.m_color_bar = bool(on)
+
+    def world_annotations(self, on: bool):
+        self.m_world_annotations = bool(on)
 
     def volume(self, volumes):
         self.volumes = list(volumes) if isinstance(volumes, (list, tuple)) else [volumes]

@@ -951,4 +951,11 @@ Array<Vec<float32, 4>> Volume::composite (Array<VolumePartial> &partials, int32 
   return img;
 }
 
+void Volume::save (const std::string name, Array<VolumePartial> partials, const int32 width, const int32 height)
+{
+  Array<Vec<float32, 4>> img = composite (partials, width, height);
+  const std::string path = name + ".png";
+  drx_check (drx_save_png ((const float *)img.get_host_ptr_const (), width, height, path.c_str ()), "Volume::save");
+}
+
 } // namespace dray

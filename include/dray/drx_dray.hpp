@@ -541,6 +541,8 @@ public:
   Array<VolumePartial> integrate (Array<Ray> &rays, Array<PointLight> &lights);
   // Volume::save's compositing + white background (volume.cpp:369-416), RGBA per pixel (no PNG)
   Array<Vec<float32, 4>> composite (Array<VolumePartial> &partials, int32 width, int32 height);
+  // Volume::save (volume.cpp:368-413): the compositing above written as name + ".png"
+  void save (const std::string name, Array<VolumePartial> partials, const int32 width, const int32 height);
   drx_integrate_stats last_stats () const { return m_stats; }
 
 private:
@@ -621,12 +623,19 @@ public:
   void add_light (const PointLight &light) { m_lights.push_back (light); }
   void clear_lights () { m_lights.clear (); }
   void use_lighting (bool use_it) { m_use_lighting = use_it; }   // surfaces only in dray; kept for the API
+  // dray draws a colour bar (and, optionally, world annotations) over the composited image
+  // (renderer.cpp:178-186,284-357); this build keeps the switches and draws neither.
+  void color_bar (bool on) { m_color_bar = on; }
+  bool color_bar () const { return m_color_bar; }
+  void world_annotations (bool on) { m_world_annotations = on; }
+  bool world_annotations () const { return m_world_annotations; }
   Framebuffer render (Camera &camera);
 
 private:
   std::shared_ptr<Volume> m_volume;
   std::vector<PointLight> m_lights;
   bool m_use_lighting = true;
+  bool m_color_bar = true, m_world_annotations = false;   // renderer.cpp:166-167
 };
 
 // Renderer::render's cross-domain volume compositing (rendering/renderer.cpp:307-330): the partials of

@@ -10,7 +10,7 @@
 //          pl_points.bin (float32 xyz, PointLocation queries)
 // out_dir: inner.bin leaf.bin ids.bin bounds.bin loc.bin vals.bin rays.bin partials.bin image.bin
 //          lit_image.bin light.bin bounds2.bin range2.bin pl_values.bin two_image.bin two_parts0.bin
-//          two_parts1.bin fb_colors.bin fb_depths.bin fb_final.bin render.png stats.txt, and "errors ok" on
+//          two_parts1.bin fb_colors.bin fb_depths.bin fb_final.bin render.png vol_save.png stats.txt, and "errors ok" on
 //          stdout once the error paths have thrown as expected.  --camera writes camera.bin: pos, look_at,
 //          fov after reset_to_bounds, then pos after azimuth(-60), then pos after elevate(25).
 #include "dray/drx_dray.hpp"
@@ -210,6 +210,7 @@ int main (int argc, char **argv)
     Array<Vec<float32, 4>> img = vol.composite (parts, width, height);
     write_array (out + "/partials.bin", parts);
     write_array (out + "/image.bin", img);
+    vol.save (out + "/vol_save", parts, width, height);   // Volume::save -> vol_save.png
     // lit render with the Renderer's default light (dray's default shading)
     {
       Volume lit (col);

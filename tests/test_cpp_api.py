@@ -129,6 +129,9 @@ def test_cpp_api_matches_c_abi(gpu, orc, tmp_path, name, make):
     assert np.array_equal(cp, parts.cpu().numpy()[: n * PARTIAL_DTYPE.itemsize])
     assert np.array_equal(np.fromfile(outd / "image.bin", np.float32).view(np.uint32),
                           img.cpu().numpy().view(np.uint32))
+    # Volume::save: the composited image as an 8-bit PNG, row 0 of the framebuffer at the bottom
+    vs = read_png(outd / "vol_save.png")
+    assert np.array_equal(vs, (img.cpu().numpy().reshape(h, w, 4) * np.float32(255)).astype(np.uint8)[::-1])
     # lit render with the default light: the same as the Python call
     light = np.fromfile(outd / "light.bin", np.float32)
     assert np.allclose(light, gpu.default_light(gpu.Camera(w, h, 45.0, pos=pos, look_at=look)))
