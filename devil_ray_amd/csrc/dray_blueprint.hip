@@ -90,8 +90,9 @@ private:
   uint32_t u32 (size_t o) const { return (uint32_t)uint_le (o, 4); }
   uint64_t u64 (size_t o) const { return uint_le (o, 8); }
 
-  void parse_block (size_t start, size_t end, bool track_order, std::vector<Msg> &out) const
+  void parse_block (size_t start, size_t end, bool track_order, std::vector<Msg> &out, int depth = 0) const
   {
+    if (depth > 64) bp_error (m_path + ": object header continuation chain too long");
     size_t q = start;
     while (q + 4 <= end)
     {
@@ -104,7 +105,8 @@ private:
         const uint64_t caddr = u64 (q), clen = u64 (q + 8);
         if (caddr + 4 > m_buf.size () || std::memcmp (&m_buf[caddr], "OCHK", 4) != 0)
           bp_error (m_path + ": continuation block without OCHK signature");
-        parse_block (caddr + 4, caddr + clen - 4, track_order, out);   // minus the trailing checksum
+        if (clen < 8) bp_error (m_path + ": continuation block too short");
+        parse_block (caddr + 4, caddr + clen - 4, track_order, out, depth + 1);   // minus the trailing checksum
       }
       q += size;
     }

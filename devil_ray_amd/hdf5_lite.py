@@ -51,7 +51,9 @@ class H5File:
         track_order = bool(flags & 0x04)
         out = []
 
-        def parse(start, end):
+        def parse(start, end, depth=0):
+            if depth > 64:
+                raise ValueError(f"object header at {addr:#x}: continuation chain too long")
             q = start
             while q + 4 <= end:
                 mtype = b[q]
@@ -64,7 +66,9 @@ class H5File:
                     caddr, clen = struct.unpack_from("<QQ", data, 0)
                     if b[caddr:caddr + 4] != b"OCHK":
                         raise ValueError(f"continuation block at {caddr:#x} has no OCHK signature")
-                    parse(caddr + 4, caddr + clen - 4)   # minus the trailing checksum
+                    if clen < 8:
+                        raise ValueError(f"continuation block at {caddr:#x} is too short")
+                    parse(caddr + 4, caddr + clen - 4, depth + 1)   # minus the trailing checksum
 
         parse(p, p + size0)
         self._cache[addr] = out
