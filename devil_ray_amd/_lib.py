@@ -21,7 +21,8 @@ DRX_OK = 0
 DRX_ERR_ARG = 1
 DRX_INIT_AFFINE, DRX_INIT_MLP, DRX_INIT_SUBREF_CENTRE, DRX_INIT_GUESS = 0, 1, 2, 3   # drx_init_mode
 SPLITS_PER_ELEMENT = 33
-MAX_ORDER = 4
+MAX_ORDER = 8          # DRX_MAX_ORDER: one order per mesh
+MAX_MIXED_ORDER = 4    # DRX_MAX_MIXED_ORDER: mixed-order meshes, learned start
 LOCATE_NOALLOC_MAX_ORDERED = 262144   # DRX_LOCATE_NOALLOC_MAX_ORDERED
 
 

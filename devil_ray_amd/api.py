@@ -188,6 +188,8 @@ class DeviceGridFunction:
             h_off = np.zeros(self.n_el, np.int64)
             o32 = np.ascontiguousarray(orders.astype(np.int32))
             total = lib.drx_mesh_pack_layout(o32.ctypes.data, self.n_el, ncomp, h_off.ctypes.data)
+            if total < 0:
+                raise L.DrxError(L.lib().drx_last_error().decode(errors="replace"))
             self.stride = 0
             self.pack_offsets = to_device(h_off)
         if fp64:

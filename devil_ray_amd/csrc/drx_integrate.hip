@@ -489,7 +489,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   if (trav == DRX_TRAVERSE_WAVE && !a.cfg.wide)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: wave traversal needs bvh->wide_nodes");
   int walk = trav;
-  if (walk == DRX_TRAVERSE_AUTO && a.cfg.wide && !learned)
+  if (walk == DRX_TRAVERSE_AUTO && a.cfg.wide && !learned && P <= DRX_MAX_MIXED_ORDER)
   {
     // The wave walk (uniform node reads shared by a wavefront's coherent rays) pays on small BVHs with at
     // least ~48 active rays per SIMD: an 8^3 order-2 mesh (16,896 leaves), orthographic frames of
@@ -501,6 +501,9 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
       walk = DRX_TRAVERSE_WAVE;
   }
   const int mode = guided ? 4 : learned ? 3 : walk == DRX_TRAVERSE_WAVE ? 2 : (walk == DRX_TRAVERSE_LANE && a.cfg.wide) ? 1 : 0;
+  if (P > DRX_MAX_MIXED_ORDER && mode != 0)
+    return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: order %d runs the binary walk with the plain start "
+                 "only (orders up to %d take the other forms)", P, DRX_MAX_MIXED_ORDER);
   hipEvent_t ev[3];
   for (int q = 0; q < 3; ++q) DRX_HIP (hipEventCreate (&ev[q]));
   DRX_HIP (hipEventRecord (ev[0], st));
@@ -540,6 +543,16 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     else                         \
       DRX_INT2 (PP, PO, 0);      \
   } while (0)
+#define DRX_INT_HI(PP, PO)                                                   \
+  do                                                                         \
+  {                                                                          \
+    if (f64)                                                                 \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 0, 2>), g, b, 0, st, a);      \
+    else if (lit)                                                            \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 0, 1>), g, b, 0, st, a);      \
+    else                                                                     \
+      hipLaunchKernelGGL ((k_integrate<PP, PO, 0, 0>), g, b, 0, st, a);      \
+  } while (0)
   if (a.cfg.polish)
     switch (P)
     {
@@ -547,7 +560,11 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     case 1: DRX_INT (1, true); break;
     case 2: DRX_INT (2, true); break;
     case 3: DRX_INT (3, true); break;
-    default: DRX_INT (4, true); break;
+    case 4: DRX_INT (4, true); break;
+    case 5: DRX_INT_HI (5, true); break;
+    case 6: DRX_INT_HI (6, true); break;
+    case 7: DRX_INT_HI (7, true); break;
+    default: DRX_INT_HI (8, true); break;
     }
   else
     switch (P)
@@ -556,8 +573,13 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     case 1: DRX_INT (1, false); break;
     case 2: DRX_INT (2, false); break;
     case 3: DRX_INT (3, false); break;
-    default: DRX_INT (4, false); break;
+    case 4: DRX_INT (4, false); break;
+    case 5: DRX_INT_HI (5, false); break;
+    case 6: DRX_INT_HI (6, false); break;
+    case 7: DRX_INT_HI (7, false); break;
+    default: DRX_INT_HI (8, false); break;
     }
+#undef DRX_INT_HI
 #undef DRX_INT
 #undef DRX_INT64
 #undef DRX_INT2

@@ -22,6 +22,7 @@ namespace drx
 // Then scale every box by 1.000001f about its centre (range.hpp:161-173).
 // =============================================================================================
 constexpr int kSplitBoxes = 33;
+constexpr int kSplitGlobalBlocks = 512;   // workgroups of the order-7/8 split, each with its nets in global memory
 
 __device__ __forceinline__ float box_measure (const float *b)   // b: xmin,xmax,ymin,ymax,zmin,zmax
 {
@@ -265,8 +266,8 @@ __global__ void __launch_bounds__ (64) k_split (const int32_t *__restrict__ ctrl
   }
 }
 
-// The same split with one wavefront per component (a 192-thread workgroup per element), for order 4, where
-// the 33 nets take 49.5 KB of LDS and allow only 3 elements per CU: each wavefront splits only its own
+// The same split with one wavefront per component (a 192-thread workgroup per element), for orders >= 4, where
+// the 33 nets take 49.5 KB of LDS (order 4) and allow only 3 elements per CU: each wavefront splits only its own
 // component's lines (one line per lane, one pass), reduces that component's two hulls over the wave, and the
 // three wavefronts exchange the six values through LDS once per split.  A wavefront reads and writes only its
 // own component's part of the nets, so the exchange is the only synchronisation.  Every wavefront keeps the
@@ -277,46 +278,51 @@ __device__ __forceinline__ void split_lines_comp (float *mom, float *dau, int ax
 {
   constexpr int N1 = P + 1, LINES = N1 * N1;
   const int stride = (axis == 0) ? 1 : ((axis == 1) ? N1 : N1 * N1);
-  if (lane >= LINES) return;
-  const int u = lane % N1, v = lane / N1;
-  const int base = (axis == 0) ? (N1 * u + LINES * v) : ((axis == 1) ? (u + LINES * v) : (u + N1 * v));
-  float *m = mom + base, *d = dau + base;
-  float a[N1], b[N1];
-#pragma unroll
-  for (int ii = 0; ii < N1; ++ii) a[ii] = b[ii] = m[ii * stride];
-#pragma unroll
-  for (int r = 1; r <= P; ++r)
+  for (int l = lane; l < LINES; l += 64)   // one pass up to order 7, two at order 8
   {
+    const int u = l % N1, v = l / N1;
+    const int base = (axis == 0) ? (N1 * u + LINES * v) : ((axis == 1) ? (u + LINES * v) : (u + N1 * v));
+    float *m = mom + base, *d = dau + base;
+    float a[N1], b[N1];
 #pragma unroll
-    for (int ii = P; ii >= r; --ii) a[ii] = a[ii - 1] * (1.0f - 0.5f) + a[ii] * 0.5f;
+    for (int ii = 0; ii < N1; ++ii) a[ii] = b[ii] = m[ii * stride];
 #pragma unroll
-    for (int ii = 0; ii <= P - r; ++ii) b[ii] = b[ii] * (1.0f - 0.5f) + b[ii + 1] * 0.5f;
-  }
+    for (int r = 1; r <= P; ++r)
+    {
 #pragma unroll
-  for (int ii = 0; ii < N1; ++ii)
-  {
-    m[ii * stride] = a[ii];
-    d[ii * stride] = b[ii];
-    lo_a = fminf (lo_a, a[ii]);
-    hi_a = fmaxf (hi_a, a[ii]);
-    lo_b = fminf (lo_b, b[ii]);
-    hi_b = fmaxf (hi_b, b[ii]);
+      for (int ii = P; ii >= r; --ii) a[ii] = a[ii - 1] * (1.0f - 0.5f) + a[ii] * 0.5f;
+#pragma unroll
+      for (int ii = 0; ii <= P - r; ++ii) b[ii] = b[ii] * (1.0f - 0.5f) + b[ii + 1] * 0.5f;
+    }
+#pragma unroll
+    for (int ii = 0; ii < N1; ++ii)
+    {
+      m[ii * stride] = a[ii];
+      d[ii * stride] = b[ii];
+      lo_a = fminf (lo_a, a[ii]);
+      hi_a = fmaxf (hi_a, a[ii]);
+      lo_b = fminf (lo_b, b[ii]);
+      hi_b = fmaxf (hi_b, b[ii]);
+    }
   }
 }
 
-template <int PMAX>
+// GNETS (orders 7 and 8, whose nets exceed the LDS): the nets live in a global scratch area, one per
+// workgroup, and each workgroup takes elements blockIdx.x, blockIdx.x + gridDim.x, ...
+template <int PMAX, bool GNETS>
 __global__ void __launch_bounds__ (192) k_split_cw (const int32_t *__restrict__ ctrl_idx, const float *__restrict__ values,
                                                     const int32_t *__restrict__ orders, const int64_t *__restrict__ offsets,
                                                     int order, int n_el, float *__restrict__ out_aabbs,
-                                                    int32_t *__restrict__ out_prims, float *__restrict__ out_subrefs)
+                                                    int32_t *__restrict__ out_prims, float *__restrict__ out_subrefs,
+                                                    float *__restrict__ gnets)
 {
   constexpr int NPE = (PMAX + 1) * (PMAX + 1) * (PMAX + 1);
-  __shared__ float nets[kSplitBoxes * 3 * NPE];   // box b, component c at (b * 3 + c) * NPE
-  __shared__ float xch[2][3][4];                   // per split parity: each component's (lo_a, hi_a, lo_b, hi_b)
-
-  const int e = blockIdx.x;
-  if (e >= n_el) return;
+  __shared__ float lnets[GNETS ? 1 : kSplitBoxes * 3 * NPE];   // box b, component c at (b * 3 + c) * NPE
+  __shared__ float xch[2][3][4];                                // per split parity: each component's hull pair
+  float *nets = GNETS ? gnets + (size_t)blockIdx.x * kSplitBoxes * 3 * NPE : lnets;
   const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int e = blockIdx.x; e < n_el; e += gridDim.x)
+  {
   const int p = orders ? orders[e] : order;
   const int npe = (p + 1) * (p + 1) * (p + 1);
   const int64_t off = offsets ? offsets[e] : (int64_t)e * npe;
@@ -473,6 +479,8 @@ __global__ void __launch_bounds__ (192) k_split_cw (const int32_t *__restrict__ 
       out_subrefs[6 * k + q] = my_ref[q];
     }
     out_prims[k] = e;
+  }
+  __syncthreads ();   // the next element rewrites the nets and the exchange buffers
   }
 }
 
@@ -998,16 +1006,28 @@ int drx_bvh_split_elements (const int32_t *ctrl_idx, const float *values, const 
     return fail (DRX_ERR_ARG, "drx_bvh_split_elements: null pointer");
   if ((orders == nullptr) != (offsets == nullptr))
     return fail (DRX_ERR_ARG, "drx_bvh_split_elements: orders and offsets go together");
-  const int pmax = orders ? DRX_MAX_ORDER : order;
+  const int pmax = orders ? DRX_MAX_MIXED_ORDER : order;
   if (pmax < 1 || pmax > DRX_MAX_ORDER)
     return fail (DRX_ERR_ARG, "drx_bvh_split_elements: order %d outside 1..%d", order, DRX_MAX_ORDER);
   hipStream_t st = as_stream (stream);
+  float *gnets = nullptr;
+  const int gn_blocks = n_el < kSplitGlobalBlocks ? n_el : kSplitGlobalBlocks;
+  if (pmax >= 7)
+  {
+    const size_t npe = (size_t)(pmax + 1) * (pmax + 1) * (pmax + 1);
+    gnets = (float *)arena (SLOT_TREE, sizeof (float) * kSplitBoxes * 3 * npe * gn_blocks, st);
+    if (!gnets) return DRX_ERR_HIP;
+  }
   switch (pmax)
   {
   case 1: hipLaunchKernelGGL (k_split<1>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
   case 2: hipLaunchKernelGGL (k_split<2>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
   case 3: hipLaunchKernelGGL (k_split<3>, dim3 (n_el), dim3 (64), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
-  default: hipLaunchKernelGGL (k_split_cw<4>, dim3 (n_el), dim3 (192), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs); break;
+  case 4: hipLaunchKernelGGL ((k_split_cw<4, false>), dim3 (n_el), dim3 (192), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs, nullptr); break;
+  case 5: hipLaunchKernelGGL ((k_split_cw<5, false>), dim3 (n_el), dim3 (192), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs, nullptr); break;
+  case 6: hipLaunchKernelGGL ((k_split_cw<6, false>), dim3 (n_el), dim3 (192), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs, nullptr); break;
+  case 7: hipLaunchKernelGGL ((k_split_cw<7, true>), dim3 (gn_blocks), dim3 (192), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs, gnets); break;
+  default: hipLaunchKernelGGL ((k_split_cw<8, true>), dim3 (gn_blocks), dim3 (192), 0, st, ctrl_idx, values, orders, offsets, order, n_el, out_aabbs, out_prims, out_subrefs, gnets); break;
   }
   DRX_LAUNCH_CHECK ();
   return DRX_OK;

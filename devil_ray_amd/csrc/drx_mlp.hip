@@ -516,7 +516,7 @@ int drx_mlp_features (const drx_mesh *mesh, const float *points, const void *loc
   if (n == 0) return DRX_OK;
   const bool mixed = mesh->elem_offset != nullptr;
   if (mixed != (mesh->elem_order != nullptr)) return fail (DRX_ERR_ARG, "drx_mlp_features: mixed mesh needs orders and offsets");
-  if (!mixed && (mesh->order < 1 || mesh->order > DRX_MAX_ORDER))
+  if (!mixed && (mesh->order < 1 || mesh->order > DRX_MAX_MIXED_ORDER))
     return fail (DRX_ERR_ARG, "drx_mlp_features: order %d unsupported", mesh->order);
   const dim3 grid ((unsigned)((n + 255) / 256)), blk (256);
   const int p = mixed ? 0 : mesh->order;

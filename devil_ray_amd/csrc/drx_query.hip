@@ -684,7 +684,7 @@ struct LocateTune
   int chunk;
 };
 constexpr int kTuneDevices = 16;
-LocateTune g_tune[kTuneDevices][5][2][3];
+LocateTune g_tune[kTuneDevices][DRX_MAX_ORDER + 1][2][3];
 
 int stream_walk (const drx_bvh *bvh, const drx_locate_opts *opts)
 {
@@ -849,6 +849,12 @@ static int locate_impl (const drx_mesh *mesh, const drx_bvh *bvh, const float *p
   }
   const MlpNet net = mlp_net (learned ? opts->mlp : nullptr);
   const int budget = (learned && opts->mlp_newton_steps > 0) ? opts->mlp_newton_steps : 4;
+  if (P > DRX_MAX_MIXED_ORDER && (learned || mode != 0))
+    return fail (DRX_ERR_UNSUPPORTED, "drx_locate: order %d runs the binary walk with the plain start only "
+                 "(orders up to %d take the wide, wave, learned and given-start forms)", P, DRX_MAX_MIXED_ORDER);
+#define DRX_LOC_HI(PP, PO)                                                                          \
+  hipLaunchKernelGGL ((k_locate_stream<PP, PO, 0>), gs, bs, 0, st, cfg, points, n, order, out, steps, cands, err, \
+                      chunk, queue)
 #define DRX_LOC(PP, PO)                                                                             \
   do                                                                                                \
   {                                                                                                 \
@@ -874,7 +880,11 @@ static int locate_impl (const drx_mesh *mesh, const drx_bvh *bvh, const float *p
     case 1: DRX_LOC (1, true); break;
     case 2: DRX_LOC (2, true); break;
     case 3: DRX_LOC (3, true); break;
-    default: DRX_LOC (4, true); break;
+    case 4: DRX_LOC (4, true); break;
+    case 5: DRX_LOC_HI (5, true); break;
+    case 6: DRX_LOC_HI (6, true); break;
+    case 7: DRX_LOC_HI (7, true); break;
+    default: DRX_LOC_HI (8, true); break;
     }
   else
     switch (P)
@@ -883,9 +893,14 @@ static int locate_impl (const drx_mesh *mesh, const drx_bvh *bvh, const float *p
     case 1: DRX_LOC (1, false); break;
     case 2: DRX_LOC (2, false); break;
     case 3: DRX_LOC (3, false); break;
-    default: DRX_LOC (4, false); break;
+    case 4: DRX_LOC (4, false); break;
+    case 5: DRX_LOC_HI (5, false); break;
+    case 6: DRX_LOC_HI (6, false); break;
+    case 7: DRX_LOC_HI (7, false); break;
+    default: DRX_LOC_HI (8, false); break;
     }
 #undef DRX_LOC
+#undef DRX_LOC_HI
   DRX_LAUNCH_CHECK ();
   int32_t herr = 0;
   DRX_HIP (hipMemcpyAsync (&herr, err, sizeof (int32_t), hipMemcpyDeviceToHost, st));
@@ -1027,7 +1042,11 @@ int drx_field_eval (const drx_mesh *field, const void *locations, int64_t n, flo
   case 1: hipLaunchKernelGGL ((k_field_eval<1, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
   case 2: hipLaunchKernelGGL ((k_field_eval<2, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
   case 3: hipLaunchKernelGGL ((k_field_eval<3, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
-  default: hipLaunchKernelGGL ((k_field_eval<4, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;  \
+  case 4: hipLaunchKernelGGL ((k_field_eval<4, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
+  case 5: hipLaunchKernelGGL ((k_field_eval<5, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
+  case 6: hipLaunchKernelGGL ((k_field_eval<6, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
+  case 7: hipLaunchKernelGGL ((k_field_eval<7, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;   \
+  default: hipLaunchKernelGGL ((k_field_eval<8, FT>), g, b, 0, st, PTR, nullptr, nullptr, l, n, out_values); break;  \
   }
   if (field->data64)
   {

@@ -341,6 +341,12 @@ int64_t drx_mesh_pack_layout (const int32_t *h_orders, int32_t n_el, int32_t nco
   int64_t total = 0;
   for (int32_t e = 0; e < n_el; ++e)
   {
+    if (h_orders[e] < 1 || h_orders[e] > DRX_MAX_MIXED_ORDER)
+    {
+      fail (DRX_ERR_ARG, "drx_mesh_pack_layout: element %d has order %d (mixed-order meshes take 1..%d)", e,
+            h_orders[e], DRX_MAX_MIXED_ORDER);
+      return -1;
+    }
     h_offsets[e] = total;
     total += drx_mesh_stride (h_orders[e], ncomp);
   }

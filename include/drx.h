@@ -63,8 +63,11 @@ enum drx_status
   DRX_ERR_STATE = 5        /* library not initialised / wrong device */
 };
 
-/* Highest element order the GPU kernels accept. */
-#define DRX_MAX_ORDER 4
+/* Highest element order the GPU kernels accept for a mesh of one order (dray allows up to 20,
+ * constants.hpp:14).  Orders above DRX_MAX_MIXED_ORDER run the binary walk with the plain Newton start
+ * only; mixed-order meshes and the learned start take orders up to DRX_MAX_MIXED_ORDER. */
+#define DRX_MAX_ORDER 8
+#define DRX_MAX_MIXED_ORDER 4
 /* BVH leaves per element: 32 binary de Casteljau splits -> 33 sub-boxes (mesh_utils.cpp:528). */
 #define DRX_SPLITS_PER_ELEMENT 33
 
@@ -238,7 +241,8 @@ int drx_release (void);                    /* free the library's arena on the cu
 /* ---- mesh packing -------------------------------------------------------------------------- */
 /* Host helper: floats per element block for order p and ncomp. */
 int32_t drx_mesh_stride (int32_t order, int32_t ncomp);
-/* Host helper for mixed-order meshes: fills h_offsets[n_el] (host) and returns total floats. */
+/* Host helper for mixed-order meshes: fills h_offsets[n_el] (host) and returns total floats; -1 (with the
+ * message in drx_last_error) when an element's order is outside 1..DRX_MAX_MIXED_ORDER. */
 int64_t drx_mesh_pack_layout (const int32_t *h_orders, int32_t n_el, int32_t ncomp, int64_t *h_offsets);
 /* dray GridFunction (ctrl_idx + ncomp values per control point) -> packed blocks in `out`.
  * orders/offsets (device) are NULL for uniform order; offsets index ctrl_idx per element and

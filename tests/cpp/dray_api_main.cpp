@@ -297,7 +297,7 @@ int main (int argc, char **argv)
     ok &= throws ([] { Camera c; c.set_width (0); });
     ok &= throws ([] { Camera c; c.set_fov (181.f); });
     ok &= throws ([] { Framebuffer f (0, 4); });
-    ok &= throws ([&] { HexMesh m (gf, 5); });
+    ok &= throws ([&] { HexMesh m (gf, DRX_MAX_ORDER + 1); });   // above the highest order
     ok &= throws ([&] { HexMesh_P1 m (gf, 2); });              // fixed-order type, other order
     ok &= throws ([&] { DataSet d; d.mesh (0); });             // no mesh
     ok &= throws ([&] { ds.mesh ("nope"); });
