@@ -163,39 +163,43 @@ __device__ __forceinline__ float group16_sum (float v)
   return v;
 }
 
-// One training step's forward + backward for 64 samples (one wavefront per block).  Writes this
-// wave's gradient (and squared-error sum at [NPARAM]) to partials[blockIdx.x * PSTRIDE ...].
-__global__ void __launch_bounds__ (64) k_mlp_train (const bf16x8 *__restrict__ x, const float *__restrict__ tgt,
-                                                    int64_t n, int64_t start, const float *__restrict__ params,
-                                                    const __bf16 *__restrict__ fw, const __bf16 *__restrict__ tw,
-                                                    float *__restrict__ partials, float grad_scale)
+// One training step's forward + backward for 64 samples, four wavefronts per block.  Wave w owns sample
+// block w (16 samples) in the forward pass, the loss and the deltas, and the weight-gradient tiles of
+// input block w (K = all 64 samples, read from the block's feature-major LDS copies), so each gradient
+// entry is written by exactly one wave.  Bias gradients and the loss are summed over the four waves in
+// wave order (deterministic).  Writes the block's gradient (and squared-error sum at [NPARAM]) to
+// partials[blockIdx.x * PSTRIDE ...].
+constexpr int kTrainRed = 4 * H + OUT_PAD + 1;   // bias gradients of layers 1-4, layer 5, loss
+__global__ void __launch_bounds__ (256) k_mlp_train (const bf16x8 *__restrict__ x, const float *__restrict__ tgt,
+                                                     int64_t n, int64_t start, const float *__restrict__ params,
+                                                     const __bf16 *__restrict__ fw, const __bf16 *__restrict__ tw,
+                                                     float *__restrict__ partials, float grad_scale)
 {
   __shared__ __attribute__ ((aligned (16))) __bf16 xs[64 * IN_PAD];   // sample-major inputs
   __shared__ __attribute__ ((aligned (16))) __bf16 xf[IN_PAD * 64];   // feature-major inputs
   __shared__ __attribute__ ((aligned (16))) __bf16 hs[2][64 * H];     // sample-major activations (ping-pong)
   __shared__ __attribute__ ((aligned (16))) __bf16 hf[4][H * 64];     // feature-major activations H1..H4
-  __shared__ __attribute__ ((aligned (16))) __bf16 gp[4][64 * 64];    // GELU' per lane, [value][lane]
+  __shared__ __attribute__ ((aligned (16))) __bf16 gp[4][4][16 * 64]; // GELU' per wave and lane, [value][lane]
   __shared__ __attribute__ ((aligned (16))) __bf16 dsm[2][64 * H];    // sample-major deltas (ping-pong)
   __shared__ __attribute__ ((aligned (16))) __bf16 dfm[2][H * 64];    // feature-major deltas (ping-pong)
+  __shared__ float red[4][kTrainRed];                                 // per-wave bias gradients + loss
 
-  const int lane = threadIdx.x, j = lane & 15, g = lane >> 4;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
+  const int sb = wv;   // this wave's sample block
   float *part = partials + (int64_t)blockIdx.x * PSTRIDE;
   const int64_t base_idx = start + (int64_t)blockIdx.x * 64;
 
   {
-    const int64_t idx = (base_idx + lane) % n;
+    const int smp = threadIdx.x >> 2, v = threadIdx.x & 3;
+    const int64_t idx = (base_idx + smp) % n;
+    const bf16x8 t = x[4 * idx + v];
+    *(bf16x8 *)(xs + smp * IN_PAD + 8 * v) = t;
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-    {
-      const bf16x8 t = x[4 * idx + v];
-      *(bf16x8 *)(xs + lane * IN_PAD + 8 * v) = t;
-#pragma unroll
-      for (int s = 0; s < 8; ++s) xf[(8 * v + s) * 64 + lane] = t[s];
-    }
+    for (int s = 0; s < 8; ++s) xf[(8 * v + s) * 64 + smp] = t[s];
   }
   __syncthreads ();
 
-  // ---- forward, hidden layers ----
+  // ---- forward, hidden layers (sample block sb) ----
 #pragma unroll
   for (int l = 1; l <= 4; ++l)
   {
@@ -204,78 +208,66 @@ __global__ void __launch_bounds__ (64) k_mlp_train (const bf16x8 *__restrict__ x
     __bf16 *out_s = hs[(l - 1) & 1];
     const __bf16 *W = fw + f_off (l);
 #pragma unroll
-    for (int sb = 0; sb < 4; ++sb)
+    for (int ob = 0; ob < 4; ++ob)
+    {
+      floatx4 acc = *(const floatx4 *)(params + b_off (l) + 16 * ob + 4 * g);
 #pragma unroll
-      for (int ob = 0; ob < 4; ++ob)
+      for (int kb = 0; kb < ip / 32; ++kb)
       {
-        floatx4 acc = *(const floatx4 *)(params + b_off (l) + 16 * ob + 4 * g);
-#pragma unroll
-        for (int kb = 0; kb < ip / 32; ++kb)
-        {
-          const bf16x8 a = *(const bf16x8 *)(W + (16 * ob + j) * ip + 32 * kb + 8 * g);
-          const bf16x8 b = *(const bf16x8 *)(in_s + (16 * sb + j) * ip + 32 * kb + 8 * g);
-          acc = mfma_bf16 (a, b, acc);
-        }
-        bf16x4 hv;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-        {
-          const float z = acc[r];
-          const float h = gelu (z);
-          const float dg = gelu_grad (z);
-          hv[r] = (__bf16)h;
-          hf[l - 1][(16 * ob + 4 * g + r) * 64 + 16 * sb + j] = hv[r];
-          gp[l - 1][(16 * sb + 4 * ob + r) * 64 + lane] = (__bf16)dg;
-        }
-        *(bf16x4 *)(out_s + (16 * sb + j) * H + 16 * ob + 4 * g) = hv;
+        const bf16x8 a = *(const bf16x8 *)(W + (16 * ob + j) * ip + 32 * kb + 8 * g);
+        const bf16x8 b = *(const bf16x8 *)(in_s + (16 * sb + j) * ip + 32 * kb + 8 * g);
+        acc = mfma_bf16 (a, b, acc);
       }
+      bf16x4 hv;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        const float z = acc[r];
+        hv[r] = (__bf16)gelu (z);
+        hf[l - 1][(16 * ob + 4 * g + r) * 64 + 16 * sb + j] = hv[r];
+        gp[l - 1][wv][(4 * ob + r) * 64 + lane] = (__bf16)gelu_grad (z);
+      }
+      *(bf16x4 *)(out_s + (16 * sb + j) * H + 16 * ob + 4 * g) = hv;
+    }
     __syncthreads ();
   }
 
   // ---- output layer + loss gradient ----
-  float loss = 0.f;
   {
-    float db[4] = { 0.f, 0.f, 0.f, 0.f };
     const __bf16 *W = fw + F5;
     const bf16x4 zero4 = { (__bf16)0.f, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f };
+    floatx4 acc = *(const floatx4 *)(params + B5 + 4 * g);
 #pragma unroll
-    for (int sb = 0; sb < 4; ++sb)
+    for (int kb = 0; kb < 2; ++kb)
     {
-      floatx4 acc = *(const floatx4 *)(params + B5 + 4 * g);
-#pragma unroll
-      for (int kb = 0; kb < 2; ++kb)
-      {
-        const bf16x8 a = *(const bf16x8 *)(W + j * H + 32 * kb + 8 * g);
-        const bf16x8 b = *(const bf16x8 *)(hs[1] + (16 * sb + j) * H + 32 * kb + 8 * g);
-        acc = mfma_bf16 (a, b, acc);
-      }
-      const int smp = 16 * sb + j;
-      const int64_t idx = (base_idx + smp) % n;
-      bf16x4 dv;
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-      {
-        const int f = 4 * g + r;
-        float d = 0.f;
-        if (f < OUT)
-        {
-          const float e = acc[r] - tgt[3 * idx + f];
-          loss += e * e;
-          d = 2.f * e * grad_scale;
-        }
-        dv[r] = (__bf16)d;
-        dfm[0][f * 64 + smp] = dv[r];
-        db[r] += d;
-      }
-      *(bf16x4 *)(dsm[0] + smp * H + 4 * g) = dv;
-      *(bf16x4 *)(dsm[0] + smp * H + 16 + 4 * g) = zero4;   // k padding of the 16-wide output layer
+      const bf16x8 a = *(const bf16x8 *)(W + j * H + 32 * kb + 8 * g);
+      const bf16x8 b = *(const bf16x8 *)(hs[1] + (16 * sb + j) * H + 32 * kb + 8 * g);
+      acc = mfma_bf16 (a, b, acc);
     }
+    const int smp = 16 * sb + j;
+    const int64_t idx = (base_idx + smp) % n;
+    float loss = 0.f;
+    bf16x4 dv;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
     {
-      const float t = group16_sum (db[r]);
-      if (j == 0) part[B5 + 4 * g + r] = t;
+      const int f = 4 * g + r;
+      float d = 0.f;
+      if (f < OUT)
+      {
+        const float e = acc[r] - tgt[3 * idx + f];
+        loss += e * e;
+        d = 2.f * e * grad_scale;
+      }
+      dv[r] = (__bf16)d;
+      dfm[0][f * 64 + smp] = dv[r];
+      const float t = group16_sum (d);
+      if (j == 0) red[wv][4 * H + f] = t;
     }
+    *(bf16x4 *)(dsm[0] + smp * H + 4 * g) = dv;
+    *(bf16x4 *)(dsm[0] + smp * H + 16 + 4 * g) = zero4;   // k padding of the 16-wide output layer
+    loss = wave_reduce_add (loss);
+    if (lane == 0) red[wv][4 * H + OUT_PAD] = loss;
   }
   __syncthreads ();
 
@@ -284,12 +276,12 @@ __global__ void __launch_bounds__ (64) k_mlp_train (const bf16x8 *__restrict__ x
 #pragma unroll
   for (int L = 5; L >= 2; --L)
   {
-    // dW_L = delta_L . H_{L-1}^T   (K = 64 samples)
-    const __bf16 *hT = hf[L - 2];
+    // dW_L = delta_L . H_{L-1}^T   (K = 64 samples), input block ib = wv
+    {
+      const int ib = wv;
+      const __bf16 *hT = hf[L - 2];
 #pragma unroll
-    for (int ob = 0; ob < out_pad (L) / 16; ++ob)
-#pragma unroll
-      for (int ib = 0; ib < 4; ++ib)
+      for (int ob = 0; ob < out_pad (L) / 16; ++ob)
       {
         floatx4 acc = { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
@@ -302,52 +294,39 @@ __global__ void __launch_bounds__ (64) k_mlp_train (const bf16x8 *__restrict__ x
 #pragma unroll
         for (int r = 0; r < 4; ++r) part[w_off (L) + (16 * ob + 4 * g + r) * H + 16 * ib + j] = acc[r];
       }
-    // delta_{L-1} = (W_L^T delta_L) * GELU'(z_{L-1})
+    }
+    // delta_{L-1} = (W_L^T delta_L) * GELU'(z_{L-1}), sample block sb
     const int kp = t_pad (L);
     const __bf16 *WT = tw + t_off (L);
-    float db[4][4];
 #pragma unroll
     for (int ib = 0; ib < 4; ++ib)
+    {
+      floatx4 acc = { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
-      for (int r = 0; r < 4; ++r) db[ib][r] = 0.f;
-#pragma unroll
-    for (int sb = 0; sb < 4; ++sb)
-#pragma unroll
-      for (int ib = 0; ib < 4; ++ib)
+      for (int kb = 0; kb < kp / 32; ++kb)
       {
-        floatx4 acc = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-        for (int kb = 0; kb < kp / 32; ++kb)
-        {
-          const bf16x8 a = *(const bf16x8 *)(WT + (16 * ib + j) * kp + 32 * kb + 8 * g);
-          const bf16x8 b = *(const bf16x8 *)(dsm[cur] + (16 * sb + j) * H + 32 * kb + 8 * g);
-          acc = mfma_bf16 (a, b, acc);
-        }
-        bf16x4 dv;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-        {
-          const float d = acc[r] * (float)gp[L - 2][(16 * sb + 4 * ib + r) * 64 + lane];
-          dv[r] = (__bf16)d;
-          dfm[cur ^ 1][(16 * ib + 4 * g + r) * 64 + 16 * sb + j] = dv[r];
-          db[ib][r] += d;
-        }
-        *(bf16x4 *)(dsm[cur ^ 1] + (16 * sb + j) * H + 16 * ib + 4 * g) = dv;
+        const bf16x8 a = *(const bf16x8 *)(WT + (16 * ib + j) * kp + 32 * kb + 8 * g);
+        const bf16x8 b = *(const bf16x8 *)(dsm[cur] + (16 * sb + j) * H + 32 * kb + 8 * g);
+        acc = mfma_bf16 (a, b, acc);
       }
-#pragma unroll
-    for (int ib = 0; ib < 4; ++ib)
+      bf16x4 dv;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
       {
-        const float t = group16_sum (db[ib][r]);
-        if (j == 0) part[b_off (L - 1) + 16 * ib + 4 * g + r] = t;
+        const float d = acc[r] * (float)gp[L - 2][wv][(4 * ib + r) * 64 + lane];
+        dv[r] = (__bf16)d;
+        dfm[cur ^ 1][(16 * ib + 4 * g + r) * 64 + 16 * sb + j] = dv[r];
+        const float t = group16_sum (d);
+        if (j == 0) red[wv][(L - 2) * H + 16 * ib + 4 * g + r] = t;
       }
+      *(bf16x4 *)(dsm[cur ^ 1] + (16 * sb + j) * H + 16 * ib + 4 * g) = dv;
+    }
     __syncthreads ();
     cur ^= 1;
   }
-  // dW_1 = delta_1 . X^T
-#pragma unroll
-  for (int ob = 0; ob < 4; ++ob)
+  // dW_1 = delta_1 . X^T, output block ob = wv
+  {
+    const int ob = wv;
 #pragma unroll
     for (int ib = 0; ib < 2; ++ib)
     {
@@ -362,8 +341,13 @@ __global__ void __launch_bounds__ (64) k_mlp_train (const bf16x8 *__restrict__ x
 #pragma unroll
       for (int r = 0; r < 4; ++r) part[W1 + (16 * ob + 4 * g + r) * IN_PAD + 16 * ib + j] = acc[r];
     }
-  loss = wave_reduce_add (loss);
-  if (lane == 0) part[NPARAM] = loss;
+  }
+  // bias gradients and loss: the four waves' values added in wave order
+  for (int t = threadIdx.x; t < kTrainRed; t += 256)
+  {
+    const float s = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+    part[t < 4 * H ? b_off (t / H + 1) + t % H : t < 4 * H + OUT_PAD ? B5 + (t - 4 * H) : NPARAM] = s;
+  }
 }
 
 // First level of the gradient sum: group g adds its share of the per-wave rows, in order.
@@ -556,7 +540,7 @@ int drx_mlp_train (void *state, const void *x, const float *target, int64_t n, c
   {
     const int t = o->step0 + k + 1;   // Adam step number (1-based)
     const int64_t start = (o->sample0 + (int64_t)k * o->batch) % n;
-    hipLaunchKernelGGL (k_mlp_train, dim3 (nw), dim3 (64), 0, st, (const bf16x8 *)x, target, n, start, s.params, s.fw,
+    hipLaunchKernelGGL (k_mlp_train, dim3 (nw), dim3 (256), 0, st, (const bf16x8 *)x, target, n, start, s.params, s.fw,
                         s.tw, partials, 1.f / (float)o->batch);
     DRX_LAUNCH_CHECK ();
     // cosine decay from lr to lr_final over total steps
