@@ -153,15 +153,9 @@ __global__ void __launch_bounds__ (256) k_mlp_features (const float *__restrict_
     for (int d = 0; d < 3; ++d) out_base[3 * i + d] = base[d];
 }
 
-// Sum over the 16 lanes that share a lane group (lanes j = 0..15 of group g); result in lane j == 0.
-__device__ __forceinline__ float group16_sum (float v)
-{
-  v += __shfl_xor (v, 1);
-  v += __shfl_xor (v, 2);
-  v += __shfl_xor (v, 4);
-  v += __shfl_xor (v, 8);
-  return v;
-}
+// Sum over the 16 lanes that share a lane group (lanes j = 0..15 of group g, one DPP row); every lane of
+// the group receives it.
+__device__ __forceinline__ float group16_sum (float v) { return row_reduce_add (v); }
 
 // One training step's forward + backward for 64 samples, four wavefronts per block.  Wave w owns sample
 // block w (16 samples) in the forward pass, the loss and the deltas, and the weight-gradient tiles of
@@ -350,7 +344,25 @@ __global__ void __launch_bounds__ (256) k_mlp_train (const bf16x8 *__restrict__ 
   }
 }
 
-// First level of the gradient sum: group g adds its share of the per-wave rows, in order.
+// partials[w * PSTRIDE + p] summed over rows w0..w1-1 in row order; the loads of eight rows are issued
+// before they are added, so the rows cost one memory latency per eight instead of one each.
+__device__ __forceinline__ float ordered_row_sum (const float *__restrict__ partials, int w0, int w1, int p)
+{
+  float sum = 0.f;
+  int w = w0;
+  for (; w + 8 <= w1; w += 8)
+  {
+    float t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = partials[(int64_t)(w + k) * PSTRIDE + p];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum += t[k];
+  }
+  for (; w < w1; ++w) sum += partials[(int64_t)w * PSTRIDE + p];
+  return sum;
+}
+
+// First level of the gradient sum: group g adds its share of the per-block rows, in order.
 constexpr int kGradGroups = 16;
 __global__ void __launch_bounds__ (256) k_mlp_grad_groups (const float *__restrict__ partials, int nwaves,
                                                            float *__restrict__ groups)
@@ -359,9 +371,7 @@ __global__ void __launch_bounds__ (256) k_mlp_grad_groups (const float *__restri
   if (p > NPARAM) return;
   const int g = blockIdx.y;
   const int w0 = (int)((int64_t)nwaves * g / kGradGroups), w1 = (int)((int64_t)nwaves * (g + 1) / kGradGroups);
-  float sum = 0.f;
-  for (int w = w0; w < w1; ++w) sum += partials[(int64_t)w * PSTRIDE + p];
-  groups[(int64_t)g * PSTRIDE + p] = sum;
+  groups[(int64_t)g * PSTRIDE + p] = ordered_row_sum (partials, w0, w1, p);
 }
 
 // Adam over the gradient (the group sums added in a fixed order: deterministic), then the bf16 copies.
@@ -372,8 +382,7 @@ __global__ void __launch_bounds__ (256) k_mlp_adam (MlpState s, const float *__r
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > NPARAM) return;
-  float gsum = 0.f;
-  for (int w = 0; w < nwaves; ++w) gsum += partials[(int64_t)w * PSTRIDE + p];
+  const float gsum = ordered_row_sum (partials, 0, nwaves, p);
   if (p == NPARAM)
   {
     if (loss_out) *loss_out = gsum * loss_scale;

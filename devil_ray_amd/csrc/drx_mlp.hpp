@@ -48,7 +48,7 @@ constexpr int64_t PACKED_BYTES = (int64_t)NFRAG * 64 * 16 + NBIAS * 4;
 // training state blob: params | adam m | adam v (fp32, NPARAM_PAD each) | fwd bf16 | transposed bf16
 constexpr int NPARAM_PAD = (NPARAM + 63) & ~63;
 constexpr int64_t STATE_BYTES = 3ll * NPARAM_PAD * 4 + (int64_t)NFWD * 2 + (int64_t)NTRN * 2;
-constexpr int PSTRIDE = NPARAM_PAD + 64;   // per-wave gradient partial row (+ loss at NPARAM)
+constexpr int PSTRIDE = NPARAM_PAD + 64;   // per-block gradient partial row (one block = 64 training rows) (+ loss at NPARAM)
 
 __host__ __device__ constexpr int in_pad (int l) { return l == 1 ? IN_PAD : H; }
 __host__ __device__ constexpr int out_pad (int l) { return l == 5 ? OUT_PAD : H; }

@@ -98,6 +98,18 @@ __device__ __forceinline__ int32_t row_reduce_max (int32_t v)
   return v;
 }
 
+// Float sum within each 16-lane row by the same four DPP steps: quad sums, then the two quads of each half
+// row, then the two halves.  Each step adds values that are equal across the lanes being combined, and
+// addition commutes, so every lane of a row ends with the bit-identical sum.
+__device__ __forceinline__ float row_reduce_add (float v)
+{
+  v += __int_as_float (dpp_perm_i<0xB1> (__float_as_int (v)));
+  v += __int_as_float (dpp_perm_i<0x4E> (__float_as_int (v)));
+  v += __int_as_float (dpp_perm_i<0x141> (__float_as_int (v)));
+  v += __int_as_float (dpp_perm_i<0x140> (__float_as_int (v)));
+  return v;
+}
+
 // Min of integer keys over the 64 lanes, read back to every lane.
 __device__ __forceinline__ int32_t wave_reduce_min (int32_t v)
 {
