@@ -161,7 +161,9 @@ def main() -> int:
     bytes_per_sample = (12 + (8 if fp64_field else 4)) * (p + 1) ** 3
     achieved_gbs = located_per_frame * bytes_per_sample / (k_ms * 1e-3) / 1e9
     walk = int(st.walk)   # the walk the library ran (the automatic choice depends on the frame's size)
-    kernel_name = f"k_integrate<{p},{'true' if args.polish else 'false'},{walk},{2 if fp64_field else 0}>"
+    pol, shade = 'true' if args.polish else 'false', 2 if fp64_field else 0
+    # walk 5: the sample-parallel kernel the library takes for frames of few rays per SIMD
+    kernel_name = (f"k_integrate_split<{p},{pol},{shade}>" if walk == 5 else f"k_integrate<{p},{pol},{walk},{shade}>")
 
     out = {
         "metric": "located samples/s (volume render; also Mrays/s)",

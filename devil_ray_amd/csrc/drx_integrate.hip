@@ -164,6 +164,13 @@ constexpr int kIntBlock = DRX_INTEGRATE_BLOCK;   // threads per workgroup of the
 // of at least this many active rays per SIMD, else the binary walk (measured; see drx_volume_integrate).
 constexpr int kWaveWalkMaxLeaves = 1 << 16;
 constexpr int kWaveWalkMinRaysPerSimd = 48;
+// DRX_LAUNCH_AUTO in drx_volume_integrate: the sample-parallel kernel below this many active rays per SIMD
+// (measured on MI355X: 2.5-3.5x faster than one lane per ray at 16-150 rays per SIMD, level at ~290,
+// 1.17x slower at ~375, the 1024^2 frame of a 64^3 order-3 mesh).
+#ifndef DRX_SPLIT_MAX_RAYS_PER_SIMD
+#define DRX_SPLIT_MAX_RAYS_PER_SIMD 256
+#endif
+constexpr int kSplitMaxRaysPerSimd = DRX_SPLIT_MAX_RAYS_PER_SIMD;
 constexpr int kIntWaves = kIntBlock / 64;
 
 // SHADE: 0 VolumeShader::color over an FP32 field, 1 shaded_color (lit), 2 color over an FP64 field.
@@ -387,6 +394,204 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
   }
 }
 
+// ---- Sample-parallel form for frames with too few rays to fill the GPU at one lane per ray ----
+// A block holds 64 rays (one per lane) and kSplitWaves wavefronts.  One round covers kSplitR consecutive
+// samples of every ray: wave w locates and shades samples [base + w*T, base + (w+1)*T) of its lane's ray
+// into LDS (a sample's location and colour depend on its point alone), then wave 0 advances each ray
+// through k_integrate's march state machine over the round's samples, and the block stops when its rays
+// are done.  The sample distances come from the same float additions as the march's, and every locate the
+// march makes (including the repeated one after a segment closes) is replayed from its sample's record,
+// so partials and statistics equal k_integrate's bit for bit; samples taken past a ray's end are wasted
+// work (under kSplitR per ray), never results.
+#ifndef DRX_SPLIT_WAVES
+#define DRX_SPLIT_WAVES 4   // wavefronts per 64 rays
+#endif
+#ifndef DRX_SPLIT_T
+#define DRX_SPLIT_T 4       // consecutive samples per wavefront and round
+#endif
+constexpr int kSplitWaves = DRX_SPLIT_WAVES, kSplitT = DRX_SPLIT_T, kSplitR = kSplitWaves * kSplitT;
+constexpr int32_t kRecFound = 1 << 30, kRecPolished = 1 << 29, kRecOverflow = 1 << 28, kRecCand = (1 << 28) - 1;
+
+template <int P, bool POLISH, int SHADE>
+__global__ void __launch_bounds__ (64 * kSplitWaves, waves_per_simd (P, POLISH, 0)) k_integrate_split (IntegrateArgs a)
+{
+  __shared__ int32_t stack_lds[kStackLds * 64 * kSplitWaves];
+  __shared__ float4 rcol[kSplitR * 64];   // sample colours, [sample of the round][lane]
+  __shared__ int2 rcnt[kSplitR * 64];     // Newton steps; candidates | found / polished / overflow bits
+  __shared__ int32_t ray_live[64];
+  __shared__ int32_t more;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  TravStack todo;
+  todo.lds = stack_lds;
+  todo.bdim = blockDim.x;
+  todo.tid = threadIdx.x;
+  const int i = blockIdx.x * 64 + lane;
+  const bool has_ray = i < a.n_rays;
+  const int ri = has_ray ? (a.order ? a.order[i] : i) : 0;
+  Ray ray;
+  if (has_ray)
+    ray = a.rays[ri];
+  else
+  {
+    ray.dir[0] = ray.dir[1] = ray.dir[2] = 0.f;
+    ray.orig[0] = ray.orig[1] = ray.orig[2] = 0.f;
+    ray.near_ = ray.far_ = 0.f;
+    ray.pixel_id = -1;
+  }
+  Partial *out5 = a.slots + 5 * (int64_t)ri;
+  if (wv == 0 && has_ray)
+#pragma unroll
+    for (int s = 0; s < 5; ++s)
+    {
+      Partial p0;
+      p0.pixel_id = -1;
+      p0.depth = INFINITY;
+      p0.rgba[0] = p0.rgba[1] = p0.rgba[2] = p0.rgba[3] = 0.f;
+      out5[s] = p0;
+    }
+  if (wv == 0) ray_live[lane] = has_ray;
+  // march state (wave 0): the distance of sample k
+  float distance = ray.near_ + a.sample_dist;
+  int k = 0, segment = 0;
+  bool inseg = false, alive = has_ray, overflow = false;
+  float depth = 0.f;
+  float rgba[4] = { 0.f, 0.f, 0.f, 0.f };
+  unsigned long long v[4] = { 0, 0, 0, 0 };   // located, Newton steps, candidates, polished
+  // sampling state: the distance of this wave's next sample and of the sample before it (the march only
+  // locates sample k when sample k - 1 lies before `far`)
+  float sd = ray.near_ + a.sample_dist, sprev = -INFINITY;
+  for (int q = 0; q < wv * kSplitT; ++q)
+  {
+    sprev = sd;
+    sd += a.sample_dist;
+  }
+  __syncthreads ();
+  for (int base = 0;; base += kSplitR)
+  {
+    const bool live = ray_live[lane] != 0;
+#pragma unroll 1
+    for (int s = 0; s < kSplitT; ++s)
+    {
+      const int slot = (wv * kSplitT + s) * 64 + lane;
+      int2 cn = { 0, 0 };
+      float4 col = { 0.f, 0.f, 0.f, 0.f };
+      if (live && sprev < ray.far_)
+      {
+        const float pt[3] = { ray_point (ray.orig[0], sd, ray.dir[0]), ray_point (ray.orig[1], sd, ray.dir[1]),
+                              ray_point (ray.orig[2], sd, ray.dir[2]) };
+        LocateStats st { 0, 0, 0, 0 };
+        bool ovf = false;
+        float ref[3];
+        const int cell = locate_point<P, POLISH> (a.cfg, todo, pt, ref, st, ovf);
+        cn.x = st.newton;
+        cn.y = min (st.candidates, kRecCand) | (cell >= 0 ? kRecFound : 0) | (st.polished ? kRecPolished : 0) |
+               (ovf ? kRecOverflow : 0);
+        if (cell >= 0 && !ovf)
+        {
+          if constexpr (SHADE == 1)
+            col = lit_color<P> (a, cell, ref, ray);
+          else if constexpr (SHADE == 2)
+            col = color_lookup64 (a.lut, a.lut_size, a.fmin64, a.fmax64, a.inv_range64,
+                                  field_at<P, double> (a, a.field64, cell, ref));
+          else
+            col = color_lookup (a.lut, a.lut_size, a.fmin, a.fmax, a.inv_range, field_at<P> (a, a.field, cell, ref));
+        }
+      }
+      rcol[slot] = col;
+      rcnt[slot] = cn;
+      sprev = sd;
+      sd += a.sample_dist;
+    }
+    for (int q = 0; q < kSplitR - kSplitT; ++q)
+    {
+      sprev = sd;
+      sd += a.sample_dist;
+    }
+    __syncthreads ();
+    if (wv == 0)
+    {
+      while (alive)
+      {
+        if (!inseg && !(distance < ray.far_))
+        {
+          alive = false;
+          break;
+        }
+        if (k >= base + kSplitR) break;   // the next round holds this sample
+        const int slot = (k - base) * 64 + lane;
+        const int2 cn = rcnt[slot];
+        v[1] += (unsigned)cn.x;
+        v[2] += (unsigned)(cn.y & kRecCand);
+        v[3] += (cn.y & kRecPolished) ? 1 : 0;
+        if (cn.y & kRecOverflow)
+        {
+          overflow = true;
+          alive = false;
+          break;
+        }
+        const bool found = (cn.y & kRecFound) != 0;
+        if (found) ++v[0];
+        bool shade = false;
+        if (!inseg)
+        {
+          if (found)
+          {
+            inseg = true;
+            depth = distance;
+            rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
+            shade = true;
+          }
+          else
+          {
+            distance += a.sample_dist;
+            ++k;
+          }
+        }
+        else if (distance < ray.far_ && found && rgba[3] < 0.95f)
+          shade = true;
+        else
+        {
+          Partial pp;
+          pp.pixel_id = ray.pixel_id;
+          pp.depth = depth;
+          pp.rgba[0] = rgba[0];
+          pp.rgba[1] = rgba[1];
+          pp.rgba[2] = rgba[2];
+          pp.rgba[3] = rgba[3];
+          out5[segment] = pp;
+          ++segment;
+          if (distance >= ray.far_ || rgba[3] > 0.95f || segment == 5)
+          {
+            alive = false;
+            break;
+          }
+          inseg = false;
+        }
+        if (shade)
+        {
+          blend_nc (rgba, rcol[slot]);
+          distance += a.sample_dist;
+          ++k;
+        }
+      }
+      ray_live[lane] = alive;
+      const bool any = __ballot (alive) != 0;
+      if (lane == 0) more = any;
+    }
+    __syncthreads ();
+    if (!more) break;
+  }
+  if (overflow) atomicOr (a.err, 1);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+  {
+    unsigned long long x = v[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor (x, o, 64);
+    if (lane == 0 && x) atomicAdd (&a.stats[q], x);
+  }
+}
+
 __global__ void k_flag_partials (const Partial *__restrict__ p, int64_t n, int32_t *__restrict__ flags)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -488,7 +693,25 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
   const int trav = opts ? opts->traversal : DRX_TRAVERSE_AUTO;
   if (trav == DRX_TRAVERSE_WAVE && !a.cfg.wide)
     return fail (DRX_ERR_ARG, "drx_volume_integrate: wave traversal needs bvh->wide_nodes");
-  int walk = trav;
+  // Sample-parallel launch (k_integrate_split): asked for, or picked for frames of fewer than
+  // kSplitMaxRaysPerSimd active rays per SIMD under the automatic walk and launch.
+  const int launch = opts ? opts->launch : DRX_LAUNCH_AUTO;
+  const bool plain = !guided && !learned && P <= DRX_MAX_MIXED_ORDER;
+  bool split = false;
+  if (launch == DRX_LAUNCH_SAMPLES)
+  {
+    if (!plain || (trav != DRX_TRAVERSE_AUTO && trav != DRX_TRAVERSE_BINARY))
+      return fail (DRX_ERR_UNSUPPORTED, "drx_volume_integrate: the sample-parallel launch runs the binary walk with "
+                   "the plain start, orders up to %d", DRX_MAX_MIXED_ORDER);
+    split = true;
+  }
+  else if (launch == DRX_LAUNCH_AUTO && trav == DRX_TRAVERSE_AUTO && plain)
+  {
+    int dev = 0;
+    split = hipGetDevice (&dev) == hipSuccess &&
+            (int64_t)n_rays < (int64_t)kSplitMaxRaysPerSimd * 4 * compute_units (dev);
+  }
+  int walk = split ? DRX_TRAVERSE_BINARY : trav;
   if (walk == DRX_TRAVERSE_AUTO && a.cfg.wide && !learned && P <= DRX_MAX_MIXED_ORDER)
   {
     // The wave walk (uniform node reads shared by a wavefront's coherent rays) pays on small BVHs with at
@@ -543,6 +766,17 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     else                         \
       DRX_INT2 (PP, PO, 0);      \
   } while (0)
+#define DRX_SPLIT(PP, PO)                                                                  \
+  do                                                                                       \
+  {                                                                                        \
+    const dim3 gs (nblk (n_rays, 64)), bs (64 * kSplitWaves);                              \
+    if (f64)                                                                               \
+      hipLaunchKernelGGL ((k_integrate_split<PP, PO, 2>), gs, bs, 0, st, a);               \
+    else if (lit)                                                                          \
+      hipLaunchKernelGGL ((k_integrate_split<PP, PO, 1>), gs, bs, 0, st, a);               \
+    else                                                                                   \
+      hipLaunchKernelGGL ((k_integrate_split<PP, PO, 0>), gs, bs, 0, st, a);               \
+  } while (0)
 #define DRX_INT_HI(PP, PO)                                                   \
   do                                                                         \
   {                                                                          \
@@ -553,7 +787,28 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     else                                                                     \
       hipLaunchKernelGGL ((k_integrate<PP, PO, 0, 0>), g, b, 0, st, a);      \
   } while (0)
-  if (a.cfg.polish)
+  if (split)
+  {
+    if (a.cfg.polish)
+      switch (P)
+      {
+      case 0: DRX_SPLIT (0, true); break;
+      case 1: DRX_SPLIT (1, true); break;
+      case 2: DRX_SPLIT (2, true); break;
+      case 3: DRX_SPLIT (3, true); break;
+      default: DRX_SPLIT (4, true); break;
+      }
+    else
+      switch (P)
+      {
+      case 0: DRX_SPLIT (0, false); break;
+      case 1: DRX_SPLIT (1, false); break;
+      case 2: DRX_SPLIT (2, false); break;
+      case 3: DRX_SPLIT (3, false); break;
+      default: DRX_SPLIT (4, false); break;
+      }
+  }
+  else if (a.cfg.polish)
     switch (P)
     {
     case 0: DRX_INT (0, true); break;
@@ -580,6 +835,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     default: DRX_INT_HI (8, false); break;
     }
 #undef DRX_INT_HI
+#undef DRX_SPLIT
 #undef DRX_INT
 #undef DRX_INT64
 #undef DRX_INT2
@@ -613,7 +869,7 @@ int drx_volume_integrate (const drx_mesh *mesh, const drx_mesh *field, const drx
     stats->polished = (int64_t)hs[3];
     stats->kernel_ms = k_ms;
     stats->total_ms = t_ms;
-    stats->walk = mode;
+    stats->walk = split ? 5 : mode;
   }
   return DRX_OK;
 }

@@ -143,7 +143,8 @@ typedef struct drx_locate_opts
   int32_t mlp_newton_steps;   /* DRX_INIT_MLP: Newton budget from the learned guess before the plain
                                  fallback (<= 0: 4) */
   int32_t launch;             /* drx_launch: how drx_locate's streamed walks (binary, 8-wide per lane,
-                                 learned start) are launched; the results are the same either way */
+                                 learned start) are launched, and whether drx_volume_integrate takes the
+                                 sample-parallel kernel; the results are the same either way */
   int32_t chunk;              /* points per wavefront chunk for those walks (0: tuned, else 1024) */
   int32_t *polished;          /* device, nullable: drx_locate writes 1 where the point's accepted solution took
                                  the FP64 correction (condition estimate > cond_threshold), else 0 */
@@ -163,7 +164,13 @@ enum drx_launch
 {
   DRX_LAUNCH_AUTO = 0,        /* the form drx_locate_autotune picked for this device, order and walk, else chunked */
   DRX_LAUNCH_CHUNKED = 1,     /* one wavefront per chunk of points */
-  DRX_LAUNCH_PERSISTENT = 2   /* about one wavefront per resident slot; wavefronts take chunks from a queue */
+  DRX_LAUNCH_PERSISTENT = 2,  /* about one wavefront per resident slot; wavefronts take chunks from a queue */
+  DRX_LAUNCH_SAMPLES = 3      /* drx_volume_integrate: several wavefronts per 64 rays locate and shade
+                                 consecutive samples in parallel and one of them runs each ray's march over
+                                 them (same partials and statistics; binary walk, plain start, orders up to
+                                 DRX_MAX_MIXED_ORDER).  Under DRX_LAUNCH_AUTO and DRX_TRAVERSE_AUTO a render
+                                 takes it for frames of fewer than 256 active rays per SIMD; DRX_LAUNCH_CHUNKED
+                                 or DRX_LAUNCH_PERSISTENT keep one lane per ray */
 };
 
 /* Training schedule of the learned initializer (drx_mlp_train): Adam, cosine decay lr -> lr_final over
@@ -198,7 +205,8 @@ typedef struct drx_integrate_stats
   int64_t polished;           /* samples that took the FP64 correction */
   float kernel_ms;            /* duration of the integrate kernel alone (HIP events on `stream`) */
   float total_ms;             /* the whole call: integrate + partial compaction (HIP events) */
-  int32_t walk;               /* the walk that ran: 0 binary, 1 8-wide per lane, 2 wave-coherent, 3 learned start */
+  int32_t walk;               /* the walk that ran: 0 binary, 1 8-wide per lane, 2 wave-coherent, 3 learned start,
+                                 4 binary with given starts / record, 5 binary, sample-parallel launch */
 } drx_integrate_stats;
 
 /* PointLight (rendering/point_light.hpp:15-33); dray's defaults: amb 0.4, diff 0.75, spec 0.8, spec_pow 15. */

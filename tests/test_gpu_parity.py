@@ -836,9 +836,11 @@ def test_scratch_per_stream(gpu, orc):
 
 
 def test_auto_walk_choice(gpu):
-    """DRX_TRAVERSE_AUTO in a render: the wave walk for a BVH of at most 65,536 leaves (whose 8-wide copy is
-    built with it) and at least 48 active rays per SIMD, else the binary walk; a larger BVH gets no 8-wide
-    copy until a walk asks for one."""
+    """DRX_TRAVERSE_AUTO with DRX_LAUNCH_AUTO in a render: the sample-parallel kernel (walk 5, binary walk)
+    for frames of fewer than 256 active rays per SIMD; above that, and with one lane per ray asked for
+    (DRX_LAUNCH_CHUNKED), the wave walk for a BVH of at most 65,536 leaves (whose 8-wide copy is built
+    with it) and at least 48 active rays per SIMD, else the binary walk; a larger BVH gets no 8-wide copy
+    until a walk asks for one.  Every form renders the same partials."""
     simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
     small = syn.hex_mesh(8, 2, 0.05, 0.005, 1234, jitter_scaled=False)
     m = gpu.UnstructuredMesh(small)
@@ -851,9 +853,17 @@ def test_auto_walk_choice(gpu):
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)
     side_big = int(np.ceil(np.sqrt(48 * simds))) + 8
-    for side, want in ((side_big, 2), (96, 0)):
-        vol.integrate(gpu.as_bytes(syn.orthographic_rays(side, side)))
-        assert vol.last_stats.walk == want, (side, vol.last_stats.walk)
+    side_huge = int(np.ceil(np.sqrt(256 * simds))) + 8
+    for side, launch, want in ((side_big, 0, 5), (96, 0, 5), (side_huge, 0, 2), (side_big, 1, 2), (96, 1, 0)):
+        vol.opts = gpu.LocateOptions(launch=launch)
+        parts, n = vol.integrate(gpu.as_bytes(syn.orthographic_rays(side, side)))
+        assert vol.last_stats.walk == want, (side, launch, vol.last_stats.walk)
+        if side == side_big:
+            got = _np(parts, np.uint8)[: n * 24].copy()
+            if launch == 0:
+                first = got
+            else:
+                assert np.array_equal(got, first)
     big = syn.hex_mesh(20, 2, 0.05, 0.005, 3, jitter_scaled=False)
     mb = gpu.UnstructuredMesh(big)
     bvh = mb.get_bvh()
@@ -863,11 +873,71 @@ def test_auto_walk_choice(gpu):
     vb.samples(100)
     vb.table = vol.table
     parts, n = vb.integrate(gpu.as_bytes(syn.orthographic_rays(side_big, side_big)))
+    assert vb.last_stats.walk == 5 and bvh.wide_nodes is None
+    vb.opts = gpu.LocateOptions(launch=1)
+    parts1, n1 = vb.integrate(gpu.as_bytes(syn.orthographic_rays(side_big, side_big)))
     assert vb.last_stats.walk == 0 and bvh.wide_nodes is None
     vb.opts = gpu.LocateOptions(traversal=2)
     parts2, n2 = vb.integrate(gpu.as_bytes(syn.orthographic_rays(side_big, side_big)))
     assert vb.last_stats.walk == 2 and bvh.wide_nodes is not None
-    assert n2 == n and np.array_equal(_np(parts2, np.uint8)[: n * 24], _np(parts, np.uint8)[: n * 24])
+    assert n1 == n and n2 == n
+    assert np.array_equal(_np(parts2, np.uint8)[: n * 24], _np(parts, np.uint8)[: n * 24])
+    assert np.array_equal(_np(parts1, np.uint8)[: n * 24], _np(parts, np.uint8)[: n * 24])
+
+
+@pytest.mark.parametrize("name,make", MESHES, ids=[m[0] for m in MESHES])
+def test_render_sample_parallel_identical(gpu, orc, name, make):
+    """The sample-parallel launch (DRX_LAUNCH_SAMPLES: four wavefronts per 64 rays locate and shade
+    consecutive samples, one replays each ray's march over them) renders the partials of one lane per ray
+    (DRX_LAUNCH_CHUNKED, binary walk) bit for bit, with the same located-sample, Newton-step, candidate
+    and polished counts: unlit, lit, with FP64 polish on every accepted point, and with fewer samples than
+    one round (the march ends inside the first round)."""
+    mesh = make()
+    m, bvh = _gpu_bvh(gpu, mesh)
+    f = gpu.UnstructuredField(mesh)
+    vol = gpu.Volume(m, f)
+    vol.table = gpu.ColorTable()
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, 0.05)
+    vol.table.add_alpha(1.0, 0.6)
+    w = h = 40
+    rays = gpu.as_bytes(orc.camera_perspective(w, h, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)))
+    for lit, polish, samples in ((False, False, 96), (True, False, 96), (False, True, 96), (False, False, 7)):
+        vol.use_lighting = lit
+        vol.samples(samples)
+        res = {}
+        for launch in (1, 3):
+            vol.opts = gpu.LocateOptions(launch=launch, traversal=3, fp64_polish=polish,
+                                         cond_threshold=0.5 if polish else 1e4)
+            parts, n = vol.integrate(rays, tile_width=w, tile_height=h)
+            st = vol.last_stats
+            res[launch] = (_np(parts, np.uint8)[: n * 24].copy(),
+                           (st.located_samples, st.newton_steps, st.candidates, st.polished), st.walk)
+        print(f"[sample-parallel] {name} lit={lit} polish={polish} samples={samples}: partials "
+              f"{len(res[1][0]) // 24}, counts {res[1][1]}", flush=True)
+        assert res[1][2] == 0 and res[3][2] == 5
+        assert res[1][1][0] > 0 and (not polish or res[1][1][3] > 0)
+        assert np.array_equal(res[3][0], res[1][0])
+        assert res[3][1] == res[1][1]
+
+
+def test_render_sample_parallel_refuses_other_forms(gpu):
+    """DRX_LAUNCH_SAMPLES runs the binary walk with the plain start: the 8-wide and wave walks and the
+    learned start are refused with DRX_ERR_UNSUPPORTED."""
+    mesh = syn.hex_mesh(3, 2, 0.05, 0.005, 1, jitter_scaled=False)
+    m, bvh = _gpu_bvh(gpu, mesh)
+    bvh.ensure_wide()
+    vol = gpu.Volume(m, gpu.UnstructuredField(mesh))
+    vol.use_lighting = False
+    vol.samples(32)
+    vol.table = gpu.ColorTable()
+    vol.table.add_alpha(0.0, 0.05)
+    rays = gpu.as_bytes(syn.orthographic_rays(16, 16))
+    for trav in (1, 2):
+        vol.opts = gpu.LocateOptions(launch=3, traversal=trav)
+        with pytest.raises(gpu.L.DrxError, match="sample-parallel"):
+            vol.integrate(rays)
 
 
 def test_given_guess_locate(gpu, orc):
