@@ -229,7 +229,7 @@ class LocateOptions:
     mlp_newton_steps: int = 4            # Newton budget from the learned guess before the plain fallback
     start: str = "affine"   # plain Newton start: "affine" (corner-affine estimate) or "centre" (SubRef centre)
     launch: int = 0         # streamed walks: 0 tuned (drx_locate_autotune) else chunked, 1 chunked, 2 persistent;
-                            # renders: 0 sample-parallel below 256 active rays per SIMD, 3 always, 1/2 never
+                            # renders: 0 sample-parallel below 300 active rays per SIMD, 3 always, 1/2 never
     chunk: int = 0          # points per wavefront chunk (0: tuned, else 1024)
     mlp_fallback: str = "affine"   # learned start: the plain start that takes over, "affine" or "centre"
     guess: Optional[torch.Tensor] = None    # given starts (Location records): DRX_INIT_GUESS

@@ -165,10 +165,10 @@ constexpr int kIntBlock = DRX_INTEGRATE_BLOCK;   // threads per workgroup of the
 constexpr int kWaveWalkMaxLeaves = 1 << 16;
 constexpr int kWaveWalkMinRaysPerSimd = 48;
 // DRX_LAUNCH_AUTO in drx_volume_integrate: the sample-parallel kernel below this many active rays per SIMD
-// (measured on MI355X: 2.5-3.5x faster than one lane per ray at 16-150 rays per SIMD, level at ~290,
-// 1.17x slower at ~375, the 1024^2 frame of a 64^3 order-3 mesh).
+// (measured on MI355X: 1.9-3.5x faster than one lane per ray at 6-150 rays per SIMD, 3% faster at 286,
+// 10% slower at 374, the 1024^2 frame of a 64^3 order-3 mesh).
 #ifndef DRX_SPLIT_MAX_RAYS_PER_SIMD
-#define DRX_SPLIT_MAX_RAYS_PER_SIMD 256
+#define DRX_SPLIT_MAX_RAYS_PER_SIMD 300
 #endif
 constexpr int kSplitMaxRaysPerSimd = DRX_SPLIT_MAX_RAYS_PER_SIMD;
 constexpr int kIntWaves = kIntBlock / 64;
@@ -407,13 +407,20 @@ __global__ void __launch_bounds__ (kIntBlock, waves_per_simd (P, POLISH, MODE)) 
 #define DRX_SPLIT_WAVES 4   // wavefronts per 64 rays
 #endif
 #ifndef DRX_SPLIT_T
-#define DRX_SPLIT_T 4       // consecutive samples per wavefront and round
+#define DRX_SPLIT_T 2       // consecutive samples per wavefront and round
+#endif
+#ifndef DRX_SPLIT_WAVES_PER_SIMD
+#define DRX_SPLIT_WAVES_PER_SIMD 4   // occupancy target of the split kernels (0: the integrate kernels' choice)
 #endif
 constexpr int kSplitWaves = DRX_SPLIT_WAVES, kSplitT = DRX_SPLIT_T, kSplitR = kSplitWaves * kSplitT;
+__host__ __device__ constexpr int split_waves_per_simd (int P, bool polish)
+{
+  return DRX_SPLIT_WAVES_PER_SIMD > 0 ? DRX_SPLIT_WAVES_PER_SIMD : waves_per_simd (P, polish, 0);
+}
 constexpr int32_t kRecFound = 1 << 30, kRecPolished = 1 << 29, kRecOverflow = 1 << 28, kRecCand = (1 << 28) - 1;
 
 template <int P, bool POLISH, int SHADE>
-__global__ void __launch_bounds__ (64 * kSplitWaves, waves_per_simd (P, POLISH, 0)) k_integrate_split (IntegrateArgs a)
+__global__ void __launch_bounds__ (64 * kSplitWaves, split_waves_per_simd (P, POLISH)) k_integrate_split (IntegrateArgs a)
 {
   __shared__ int32_t stack_lds[kStackLds * 64 * kSplitWaves];
   __shared__ float4 rcol[kSplitR * 64];   // sample colours, [sample of the round][lane]

@@ -169,7 +169,7 @@ enum drx_launch
                                  consecutive samples in parallel and one of them runs each ray's march over
                                  them (same partials and statistics; binary walk, plain start, orders up to
                                  DRX_MAX_MIXED_ORDER).  Under DRX_LAUNCH_AUTO and DRX_TRAVERSE_AUTO a render
-                                 takes it for frames of fewer than 256 active rays per SIMD; DRX_LAUNCH_CHUNKED
+                                 takes it for frames of fewer than 300 active rays per SIMD; DRX_LAUNCH_CHUNKED
                                  or DRX_LAUNCH_PERSISTENT keep one lane per ray */
 };
 

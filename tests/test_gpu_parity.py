@@ -837,7 +837,7 @@ def test_scratch_per_stream(gpu, orc):
 
 def test_auto_walk_choice(gpu):
     """DRX_TRAVERSE_AUTO with DRX_LAUNCH_AUTO in a render: the sample-parallel kernel (walk 5, binary walk)
-    for frames of fewer than 256 active rays per SIMD; above that, and with one lane per ray asked for
+    for frames of fewer than 300 active rays per SIMD; above that, and with one lane per ray asked for
     (DRX_LAUNCH_CHUNKED), the wave walk for a BVH of at most 65,536 leaves (whose 8-wide copy is built
     with it) and at least 48 active rays per SIMD, else the binary walk; a larger BVH gets no 8-wide copy
     until a walk asks for one.  Every form renders the same partials."""
@@ -853,7 +853,7 @@ def test_auto_walk_choice(gpu):
     vol.table.add_point(1.0, (1.0, 0.0, 0.0))
     vol.table.add_alpha(0.0, 0.05)
     side_big = int(np.ceil(np.sqrt(48 * simds))) + 8
-    side_huge = int(np.ceil(np.sqrt(256 * simds))) + 8
+    side_huge = int(np.ceil(np.sqrt(300 * simds))) + 8
     for side, launch, want in ((side_big, 0, 5), (96, 0, 5), (side_huge, 0, 2), (side_big, 1, 2), (96, 1, 0)):
         vol.opts = gpu.LocateOptions(launch=launch)
         parts, n = vol.integrate(gpu.as_bytes(syn.orthographic_rays(side, side)))
@@ -903,7 +903,7 @@ def test_render_sample_parallel_identical(gpu, orc, name, make):
     vol.table.add_alpha(1.0, 0.6)
     w = h = 40
     rays = gpu.as_bytes(orc.camera_perspective(w, h, 45.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5)))
-    for lit, polish, samples in ((False, False, 96), (True, False, 96), (False, True, 96), (False, False, 7)):
+    for lit, polish, samples in ((False, False, 96), (True, False, 96), (False, True, 96), (False, False, 5)):
         vol.use_lighting = lit
         vol.samples(samples)
         res = {}
