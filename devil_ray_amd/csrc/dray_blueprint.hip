@@ -86,6 +86,8 @@ private:
     for (int i = 0; i < w; ++i) v |= (uint64_t)u8 (o + i) << (8 * i);
     return v;
   }
+  // [off, off + len) lies inside the file (written so that no sum can wrap around)
+  bool span_ok (uint64_t off, uint64_t len) const { return off <= m_buf.size () && len <= m_buf.size () - off; }
   uint16_t u16 (size_t o) const { return (uint16_t)uint_le (o, 2); }
   uint32_t u32 (size_t o) const { return (uint32_t)uint_le (o, 4); }
   uint64_t u64 (size_t o) const { return uint_le (o, 8); }
@@ -103,9 +105,9 @@ private:
       if (type == 0x10)   // continuation: an "OCHK" block
       {
         const uint64_t caddr = u64 (q), clen = u64 (q + 8);
-        if (caddr + 4 > m_buf.size () || std::memcmp (&m_buf[caddr], "OCHK", 4) != 0)
+        if (!span_ok (caddr, 4) || std::memcmp (&m_buf[caddr], "OCHK", 4) != 0)
           bp_error (m_path + ": continuation block without OCHK signature");
-        if (clen < 8) bp_error (m_path + ": continuation block too short");
+        if (clen < 8 || !span_ok (caddr, clen)) bp_error (m_path + ": continuation block too short or truncated");
         parse_block (caddr + 4, caddr + clen - 4, track_order, out, depth + 1);   // minus the trailing checksum
       }
       q += size;
@@ -114,7 +116,7 @@ private:
 
   std::vector<Msg> messages (uint64_t a) const
   {
-    if (a + 6 > m_buf.size () || std::memcmp (&m_buf[a], "OHDR", 4) != 0)
+    if (!span_ok (a, 6) || std::memcmp (&m_buf[a], "OHDR", 4) != 0)
       bp_error (m_path + ": only version-2 object headers are supported");
     const int flags = u8 (a + 5);
     size_t p = a + 6;
@@ -151,7 +153,7 @@ private:
         const int w = 1 << (fl & 3);
         const size_t n = uint_le (q, w);
         q += w;
-        if (q + n + 8 > m_buf.size ()) bp_error (m_path + ": truncated link message");
+        if (n > m_buf.size () || !span_ok (q, n + 8)) bp_error (m_path + ": truncated link message");
         std::string name (&m_buf[q], n);
         q += n;
         if (ltype != 0) bp_error (m_path + ": link '" + name + "' is not a hard link");
@@ -213,6 +215,8 @@ H5Value H5File::read (const std::string &path) const
       size = (int)u32 (d + 4);
       if ((bits0 & 1) && (cls == 0 || cls == 1)) bp_error (m_path + ":" + path + ": big-endian data is not supported");
       if (cls != 0 && cls != 1 && cls != 3) bp_error (m_path + ":" + path + ": datatype class " + std::to_string (cls));
+      if (cls == 0 && size != 1 && size != 2 && size != 4 && size != 8)
+        bp_error (m_path + ":" + path + ": integer of " + std::to_string (size) + " bytes");
       is_signed = (cls == 0) && (bits0 & 0x08);
       have_type = true;
     }
@@ -239,9 +243,14 @@ H5Value H5File::read (const std::string &path) const
     }
   }
   if (!have_space || !have_type || !have_data) bp_error (m_path + ":" + path + " is not a dataset");
+  if (size <= 0) bp_error (m_path + ":" + path + ": datatype size " + std::to_string (size));
   uint64_t count = 1;
-  for (uint64_t x : dims) count *= x;
-  if (data_off + std::min<uint64_t> (data_len, count * size) > m_buf.size ()) bp_error (m_path + ": truncated dataset");
+  for (uint64_t x : dims)
+  {
+    if (x != 0 && count > (uint64_t)m_buf.size () / size / x) bp_error (m_path + ":" + path + ": dataspace larger than the file");
+    count *= x;
+  }
+  if (!span_ok (data_off, std::min<uint64_t> (data_len, count * size))) bp_error (m_path + ": truncated dataset");
   H5Value v;
   if (cls == 3)
   {

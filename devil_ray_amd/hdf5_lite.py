@@ -160,7 +160,11 @@ class H5File:
                     raise ValueError(f"{path}: chunked layout is not supported")
         if raw is None or dtype is None:
             raise ValueError(f"{path} is not a dataset")
-        count = int(np.prod(dims)) if dims else 1
+        count = 1
+        for x in dims or ():
+            count *= int(x)
+        if not is_str and count * dtype.itemsize > len(raw):
+            raise ValueError(f"{path}: dataset shorter than its dataspace")
         if is_str:
             s = bytes(raw[: count * dtype.itemsize])
             return s.split(b"\x00", 1)[0].decode("utf-8")

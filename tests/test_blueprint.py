@@ -246,3 +246,45 @@ def test_cpp_reader_errors(tmp_path):
     r = subprocess.run([BP_BIN, "load", str(tmp_path / "missing.root"), str(tmp_path)], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def test_readers_reject_corrupted_files(tmp_path):
+    """Seeded byte corruption of dray's Laghos root and domain files (lengths, offsets and counts set to random
+    or all-ones bytes): the C++ BlueprintReader either loads the data set or exits with an error message
+    (status 1), never with a signal, and the Python reader either loads it or raises an exception."""
+    import shutil
+    if not os.path.exists(BP_BIN):
+        pytest.fail(f"{BP_BIN} is not built (run devil_ray_amd/build.py)")
+    src_root = bc.root("tg_mpi_volume")
+    src_dir = src_root[: -len(".root")]
+    root = tmp_path / os.path.basename(src_root)
+    dom_dir = tmp_path / os.path.basename(src_dir)
+    shutil.copytree(src_dir, dom_dir)
+    originals = {root: open(src_root, "rb").read(), dom_dir / "domain_000000.hdf5":
+                 open(os.path.join(src_dir, "domain_000000.hdf5"), "rb").read()}
+    out = tmp_path / "out"
+    out.mkdir()
+    rng = np.random.default_rng(2024)
+    outcomes = {0: 0, 1: 0}
+    for trial in range(120):
+        for path, data in originals.items():
+            path.write_bytes(data)
+        target = root if trial % 3 == 0 else dom_dir / "domain_000000.hdf5"
+        buf = bytearray(originals[target])
+        for _ in range(int(rng.integers(1, 7))):
+            pos = int(rng.integers(0, min(len(buf), 8192) if rng.random() < 0.8 else len(buf)))
+            buf[pos] = 0xFF if rng.random() < 0.3 else int(rng.integers(0, 256))
+        target.write_bytes(bytes(buf))
+        r = subprocess.run([BP_BIN, "load", str(root), str(out)], capture_output=True, text=True, errors="replace",
+                           timeout=60)
+        assert r.returncode in (0, 1), (trial, r.returncode, r.stderr[-300:])
+        if r.returncode == 1:
+            assert r.stderr.startswith("error:"), (trial, r.stderr[-300:])
+        outcomes[r.returncode] += 1
+        try:
+            with np.errstate(over="ignore"):   # corrupted coordinates may overflow float32: data, not a fault
+                bp.load(str(root))
+        except Exception:
+            pass
+    print(f"[corrupted files] loaded {outcomes[0]}, rejected {outcomes[1]} of 120")
+    assert outcomes[1] > 0
