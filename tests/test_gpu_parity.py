@@ -1005,3 +1005,56 @@ def test_render_record_replay(gpu, orc):
           f"{s2.newton_steps}", flush=True)
     assert n2 == n0 and s2.located_samples == s0.located_samples and s2.newton_steps < s0.newton_steps
     assert np.abs(g2["rgba"] - g0["rgba"]).max() <= 1e-4
+
+
+def _slab_stack(n_slabs=6, gap=0.06):
+    """Order-2 slabs of 3 x 3 x 3 elements stacked along z with empty gaps between them: a ray along z
+    crosses n_slabs segments, more than the five partials a ray keeps."""
+    thick = (1.0 - gap * (n_slabs - 1)) / n_slabs
+    idx, vals, fld, off = [], [], [], 0
+    for k in range(n_slabs):
+        m = syn.hex_mesh(3, 2, 0.05, 0.005, 40 + k, jitter_scaled=False)
+        v = m.values.astype(np.float64).copy()
+        v[:, 2] = v[:, 2] * thick + k * (thick + gap)
+        idx.append(m.ctrl_idx + off)
+        vals.append(v.astype(np.float32))
+        fld.append(m.field)
+        off += len(m.values)
+    return syn.HexMesh(n=0, order=2, ctrl_idx=np.concatenate(idx).astype(np.int32), values=np.concatenate(vals),
+                       field=np.concatenate(fld).astype(np.float32))
+
+
+def test_render_segments_and_partial_limit(gpu, orc):
+    """Rays through six separated slabs: each segment closes at the first miss, the march re-locates at the
+    same distance, and a ray keeps its first five partials.  Both launches (one lane per ray, sample-parallel)
+    give the same partials and counts bit for bit, and both match the oracle's integrate (partial count,
+    pixel ids, located samples, RGBA within RGBA_TOL)."""
+    mesh = _slab_stack()
+    w = h = 32
+    rays = syn.orthographic_rays(w, h)
+    gp, img, op, oimg, gst, ost = _render_both(gpu, orc, mesh, rays, w, h, samples=200)
+    assert gst.walk == 5
+    per_ray = np.bincount(gp["pixel_id"], minlength=w * h)
+    print(f"[segments] partials {len(gp)} (oracle {len(op)}), rays with 5 partials {(per_ray == 5).sum()}, "
+          f"located {gst.located_samples} (oracle {ost[0]})", flush=True)
+    assert (per_ray == 5).sum() > w * h // 2 and per_ray.max() == 5
+    assert len(gp) == len(op) and np.array_equal(gp["pixel_id"], op["pixel_id"])
+    assert gst.located_samples == ost[0]
+    assert np.abs(img - oimg).max() <= RGBA_TOL
+    m, bvh = _gpu_bvh(gpu, mesh)
+    vol = gpu.Volume(m, gpu.UnstructuredField(mesh))
+    vol.use_lighting = False
+    vol.samples(200)
+    vol.table = gpu.ColorTable()
+    vol.table.add_point(0.0, (0.5, 0.5, 0.5))
+    vol.table.add_point(1.0, (1.0, 0.0, 0.0))
+    vol.table.add_alpha(0.0, syn.table_alpha_for_corrected(0.02, 200))
+    vol.table.set_smooth(True)
+    res = {}
+    for launch in (1, 3):
+        vol.opts = gpu.LocateOptions(launch=launch)
+        parts, n = vol.integrate(gpu.as_bytes(rays), tile_width=w, tile_height=h)
+        st = vol.last_stats
+        res[launch] = (_np(parts, np.uint8)[: n * 24].copy(), (st.located_samples, st.newton_steps, st.candidates))
+    assert np.array_equal(res[1][0], res[3][0]) and res[1][1] == res[3][1]
+    assert np.array_equal(res[3][0].view(PARTIAL_DTYPE)["pixel_id"], gp["pixel_id"])
