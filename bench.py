@@ -33,7 +33,8 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counters read for a 1 GiB device copy (x2.0, x1.0); the summary with its commands is
 # profiles/k_integrate_p3_64cubed_1024sq_512spp.md.  Keyed by (config, kernel) so that other workloads or
 # kernel variants report null rather than a stale figure.
-PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,0>"): 29.99e9 + 3.34e9}
+PMC_TRAFFIC_BYTES = {(1, "k_integrate<3,false,0,0>"): 29.99e9 + 3.34e9,
+                     (0, "k_integrate_split<2,false,2>"): 26.0e6 + 45.7e6}
 
 
 def _cpu_model() -> str:
