@@ -400,7 +400,8 @@ __global__ void __launch_bounds__ (256) k_mlp_adam (MlpState s, const float *__r
   write_copies (s, l, row, col, w);
 }
 
-// Inference fragments in MFMA A-operand order (permuted k for layers fed from registers) + fp32 biases.
+// Inference fragments in MFMA A-operand order (permuted k for layers fed from registers) + fp32 biases,
+// scaled for the halved-activation form of the hidden layers (gelu_from_half).
 __global__ void k_mlp_pack (const float *__restrict__ params, bf16x8 *__restrict__ frag, float *__restrict__ bias)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -408,7 +409,7 @@ __global__ void k_mlp_pack (const float *__restrict__ params, bf16x8 *__restrict
   {
     float b;
     if (t < 4 * H)
-      b = params[b_off (t / H + 1) + t % H];
+      b = 0.5f * params[b_off (t / H + 1) + t % H];   // hidden layers yield half their pre-activation
     else
       b = params[B5 + (t - 4 * H)];
     bias[t] = b;
@@ -435,12 +436,14 @@ __global__ void k_mlp_pack (const float *__restrict__ params, bf16x8 *__restrict
     kb = fr - FR5;
   }
   const int ip = in_pad (l), row = 16 * ob + j;
+  // weight scale of the halved-activation form (gelu_from_half): layer 1 x 0.5, layers 2-4 x 1, output x 2
+  const float sc = (l == 1) ? 0.5f : (l == 5) ? 2.f : 1.f;
   bf16x8 v;
 #pragma unroll
   for (int s = 0; s < 8; ++s)
   {
     const int k = (l == 1) ? 8 * g + s : 32 * kb + ((s < 4) ? 4 * g + s : 16 + 4 * g + (s - 4));
-    v[s] = (__bf16)params[w_off (l) + row * ip + k];
+    v[s] = (__bf16)(sc * params[w_off (l) + row * ip + k]);
   }
   frag[t] = v;
 }
