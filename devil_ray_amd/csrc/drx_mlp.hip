@@ -401,7 +401,7 @@ __global__ void __launch_bounds__ (256) k_mlp_adam (MlpState s, const float *__r
 }
 
 // Inference fragments in MFMA A-operand order (permuted k for layers fed from registers) + fp32 biases,
-// scaled for the halved-activation form of the hidden layers (gelu_from_half).
+// scaled for the scaled-activation form of the hidden layers (gelu_from_quarter).
 __global__ void k_mlp_pack (const float *__restrict__ params, bf16x8 *__restrict__ frag, float *__restrict__ bias)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -409,7 +409,7 @@ __global__ void k_mlp_pack (const float *__restrict__ params, bf16x8 *__restrict
   {
     float b;
     if (t < 4 * H)
-      b = 0.5f * params[b_off (t / H + 1) + t % H];   // hidden layers yield half their pre-activation
+      b = 0.25f * params[b_off (t / H + 1) + t % H];   // hidden layers yield a quarter of their pre-activation
     else
       b = params[B5 + (t - 4 * H)];
     bias[t] = b;
@@ -436,8 +436,8 @@ __global__ void k_mlp_pack (const float *__restrict__ params, bf16x8 *__restrict
     kb = fr - FR5;
   }
   const int ip = in_pad (l), row = 16 * ob + j;
-  // weight scale of the halved-activation form (gelu_from_half): layer 1 x 0.5, layers 2-4 x 1, output x 2
-  const float sc = (l == 1) ? 0.5f : (l == 5) ? 2.f : 1.f;
+  // weight scale of the scaled-activation form (gelu_from_quarter): layer 1 x 0.25, layers 2-4 x 2, output x 8
+  const float sc = (l == 1) ? 0.25f : (l == 5) ? 8.f : 2.f;
   bf16x8 v;
 #pragma unroll
   for (int s = 0; s < 8; ++s)

@@ -109,15 +109,16 @@ __device__ __forceinline__ const T *opaque_ptr (const T *p)
   return reinterpret_cast<const T *> (v);
 }
 
-// Inference form of GELU: x Phi~(x) / 2 from zh = x / 2.  drx_mlp_pack scales the packed weights so that
-// every hidden layer's MFMA yields half its pre-activation (layer-1 weights and hidden biases x 0.5) and
-// the layer after takes the halved activation (output-layer weights x 2; hidden layers 2-4 combine x 2
-// and x 0.5).  Powers of two scale every product, sum and bf16 rounding exactly, so the outputs equal the
-// unscaled network's, and the clamp's argument needs no multiply: five VALU operations per value, not six.
-__device__ __forceinline__ float gelu_from_half (float zh)
+// Inference form of GELU: x Phi~(x) / 8 from zq = x / 4.  drx_mlp_pack scales the packed weights so that
+// every hidden layer's MFMA yields a quarter of its pre-activation (layer-1 weights and hidden biases
+// x 0.25) and the layer after takes the activation divided by 8 (hidden weights x 8 x 0.25 = x 2, output
+// weights x 8).  With w = clamp(zq, -1/2, 1/2) = u / 2: 0.75 - u^2/4 = fma(-w, w, 0.75) and
+// Phi~ / 2 = fma(w, t, 0.25).  Powers of two scale every product, sum and bf16 rounding exactly, so the
+// outputs equal the unscaled network's: four VALU operations per value instead of six.
+__device__ __forceinline__ float gelu_from_quarter (float zq)
 {
-  const float u = __builtin_amdgcn_fmed3f (zh, -1.f, 1.f);
-  return zh * fmaf (u, fmaf (-0.25f * u, u, 0.75f), 0.5f);
+  const float w = __builtin_amdgcn_fmed3f (zq, -0.5f, 0.5f);
+  return zq * fmaf (w, fmaf (-w, w, 0.75f), 0.25f);
 }
 
 // GELU of two 16-row C tiles (one 32-wide k block of the next layer) packed into its B operand.
@@ -127,8 +128,8 @@ __device__ __forceinline__ bf16x8 gelu_pack (const floatx4 &lo, const floatx4 &h
 #pragma unroll
   for (int r = 0; r < 4; ++r)
   {
-    v[r] = (__bf16)gelu_from_half (lo[r]);
-    v[4 + r] = (__bf16)gelu_from_half (hi[r]);
+    v[r] = (__bf16)gelu_from_quarter (lo[r]);
+    v[4 + r] = (__bf16)gelu_from_quarter (hi[r]);
   }
   return v;
 }
