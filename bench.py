@@ -546,7 +546,7 @@ def render_starts(api, syn, mesh, m, f, packed, res, samples, steps, warmup, pol
 
 def bench_mlp(args):
     """Config 3: learned Newton initializer.  Training pairs (query, converged ref coords of the located
-    element) come from plain locate on config-1-style meshes (order 3, jitter 0.01 x spacing, warp ~ U[0,0.2])
+    element) come from plain locate on order-3 meshes (jitter 0.01 x spacing, warp ~ U[0,0.2])
     of 32 training seeds (16^3, 32^3 and 64^3 elements in turn); the network trains with the MFMA kernels;
     8 held-out seeds (64^3) compare learned-start against plain locate: time, Newton steps, located ids and
     error against the FP64 refinement.  value = learned-start located queries/s on the held-out meshes."""

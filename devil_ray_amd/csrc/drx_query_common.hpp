@@ -64,9 +64,10 @@ static inline unsigned nblk (int64_t n, int b) { return (unsigned)((n + b - 1) /
 #endif
 
 // Minimum waves per SIMD for the locate / integrate kernels (sets the VGPR budget: 5 -> 96, 4 -> 128).
-// Measured on MI355X: order <= 3 walks run fastest at 5 (config 1: 19.97 ms vs 22.37 ms at 4, some spills
-// included; with FP64 polish, whose correction is an out-of-line call, 14.36 ms vs 16.40 ms at 4); order 4
-// and the learned start need the 128-register budget (config 4: 297.5 ms at 4 vs 323 ms at 5).
+// Measured on MI355X: order <= 3 walks run fastest at 5 (1024^2 render of a 64^3 order-3 mesh, 512
+// samples per ray: 19.97 ms vs 22.37 ms at 4, some spills included; with FP64 polish, whose correction is
+// an out-of-line call, 14.36 ms vs 16.40 ms at 4); order 4 and the learned start need the 128-register
+// budget (2048^2 render of a 128^3 order-4 mesh, 1000 samples per ray: 297.5 ms at 4 vs 323 ms at 5).
 #ifndef DRX_LEARNED_WAVES
 #define DRX_LEARNED_WAVES 4   // learned-start integrate kernels (mode 3); 3 (no spills) measured slower
 #endif
