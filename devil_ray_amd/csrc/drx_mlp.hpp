@@ -21,7 +21,8 @@
                             // 2: 13.4, 1: 9.2 G rows/s with the four-operation GELU)
 #endif
 #ifndef DRX_MLP_WAVE_SB
-#define DRX_MLP_WAVE_SB 1   // the same inside point location, where registers are scarcer
+#define DRX_MLP_WAVE_SB 2   // the same inside point location (learned locate 1.11 G/s vs 1.08 at 1 and
+                            // 1.09 at 4; learned render 19.8 ms vs 22.8 at 1)
 #endif
 
 namespace drx
